@@ -1,0 +1,141 @@
+/*
+ * wt2.h -- C ABI of libwt2.so: MI355X (gfx950) kernels for the 3D SBP-SAT
+ * scalar-wave right-hand side and the classical RK4 step.
+ *
+ * Each entry point names the reference (eschnett/WaveToySecondOrder.jl) call it
+ * replaces.  All functions are extern "C", take plain pointers and sizes,
+ * return 0 on success or a negative WT2_ERR_* code (never abort), and leave a
+ * message for wt2_last_error() (thread-local).
+ *
+ * Field layout (every double* / float* field argument): Julia column-major
+ * (N,N,N,Ne), i.e. C index i + N*(j + N*(k + N*e)), elements x-fastest
+ * e = ex + Mx*(ey + My*ez).  Faces 1..6 are (-x,+x,-y,+y,-z,+z); an outer face
+ * f carries boundary tag f and Dirichlet value bdry_values[f-1].
+ *
+ * Pointers are device pointers unless WT2_HOST_PTRS is passed in flags, in
+ * which case they are host arrays copied in and out around the call.
+ * Contexts are not thread-safe; separate contexts are independent.  Calls are
+ * synchronous with respect to the host (they end with a stream sync).
+ */
+#ifndef WT2_H
+#define WT2_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ---- */
+#define WT2_OK                   0
+#define WT2_ERR_ARG             -1   /* null pointer / bad size / bad flag      */
+#define WT2_ERR_SHAPE           -2   /* N or M out of the supported range       */
+#define WT2_ERR_TAU             -3   /* SIPG penalty below coercivity           */
+#define WT2_ERR_UNSUPPORTED     -4   /* feature not built (e.g. M > 64 per axis) */
+#define WT2_ERR_HIP             -5   /* HIP runtime error                       */
+#define WT2_ERR_OOM             -6   /* device allocation failed                */
+#define WT2_ERR_NONFINITE       -7   /* non-finite value detected in the state  */
+#define WT2_ERR_STATE           -8   /* call not valid in this context state    */
+
+/* ---- precision ---- */
+#define WT2_FP64   0   /* everything in FP64                                      */
+#define WT2_MIXED  1   /* FP32 RHS bulk; FP64 state, RK sums, geometry and energy */
+
+/* ---- flags ---- */
+#define WT2_HOST_PTRS         1   /* field pointers are host arrays               */
+#define WT2_ALLOW_NONCOERCIVE 2   /* ctx_create: accept tau below coercivity      */
+#define WT2_NO_GRAPH          4   /* rk4_steps: launch kernels one by one         */
+
+/* Mesh / operator description (HexMeshes make_uniform_hex /
+ * make_warped_uniform_hex, src/evolve.jl:915, test/dirichlet_warped_tests.jl:23;
+ * Params3d tau/bdry_values, src/wave.jl:67-95; eps_KO,
+ * src/wave3d_curved.jl:205-213). */
+typedef struct wt2_mesh_desc {
+    int32_t N;               /* GLL nodes per direction, 3..10               */
+    int32_t M[3];            /* elements per axis, 1..64                     */
+    double  x0[3], x1[3];    /* domain per axis                              */
+    int32_t periodic[3];     /* 1 = periodic along that axis                 */
+    double  warp_A;          /* x = x0 + L (xi + A sin sin sin); 0 = affine  */
+    double  tau;             /* SIPG penalty in unit-element units           */
+    double  eps_ko;          /* Kreiss-Oliger epsilon; 0 disables KO         */
+    double  bdry_values[6];  /* Dirichlet value per tag 1..6 (RK steps)      */
+    int32_t precision;       /* WT2_FP64 or WT2_MIXED                        */
+    int32_t flags;           /* WT2_ALLOW_NONCOERCIVE                        */
+} wt2_mesh_desc;
+
+typedef struct wt2_ctx wt2_ctx;
+
+typedef struct wt2_info {
+    int64_t npoints;         /* N^3 * Ne                                     */
+    int32_t Ne;
+    int32_t curved;          /* 1 if warp or background set                  */
+    double  mu;              /* spectral radius of apply_D! along axis 1     */
+    double  h_min;           /* smallest node spacing (src/evolve.jl:60-69)  */
+    double  tau;             /* penalty in use                               */
+    double  max_sym_eig;     /* max eigenvalue of sym(H L) of the 1D operator */
+    int64_t device_bytes;    /* device memory held by the context            */
+} wt2_info;
+
+/* GLL nodes xi on [0,1], weights w (sum 1) and differentiation matrix
+ * G[i*N+p] = l_p'(xi_i).  Replaces HexSBPSAT make_element/make_operators
+ * (src/wave.jl:603; test/kernels3d_tests.jl:69-70).  Host only. */
+int wt2_gll(int32_t N, double* xi, double* w, double* G);
+
+/* Create a context: mesh, operators, geometry, workspace and a HIP stream on
+ * the current device.  Replaces make_uniform_hex + make_element +
+ * make_operators + make_geometry + make_workspace + to_device
+ * (src/evolve.jl:910-936) and make_wave3d_workspace's mu
+ * (src/wave3d_curved.jl:158-178).  Fails with WT2_ERR_TAU when the 1D SIPG
+ * operator is indefinite at desc->tau unless WT2_ALLOW_NONCOERCIVE is set. */
+int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out);
+void wt2_ctx_destroy(wt2_ctx* ctx);
+int wt2_ctx_info(const wt2_ctx* ctx, wt2_info* info);
+
+/* Static ADM background per node (host FP64, 10 values per node, node-major:
+ * alpha, beta1..3, gamma11, gamma12, gamma13, gamma22, gamma23, gamma33).
+ * Replaces sample_background3d! + make_coef3d (src/wave3d_curved.jl:106-124),
+ * sampled once because the background is static.  Switches the context to the
+ * curved operator. */
+int wt2_ctx_set_background(wt2_ctx* ctx, const double* bg10);
+
+/* Node coordinates (3,N,N,N,Ne), host array (geom.coords). */
+int wt2_coords(const wt2_ctx* ctx, double* xyz);
+
+/* rhs_wave3d!(udd, u, ud, bdry_values; geom, ops, work, tau)
+ * (src/wave.jl:583-611): udd = L_SIPG u with Dirichlet values bdry_values[tag-1]
+ * on outer faces.  On a curved/background context udd also carries the u_t
+ * advection terms of the ADM operator.  No KO (the reference RHS has none). */
+int wt2_rhs3d(wt2_ctx* ctx, double* udd, const double* u, const double* ud,
+              const double bdry_values[6], int32_t flags);
+
+/* Full first-order right-hand side with KO, the (u, u_t) analogue of
+ * wave3d_curved_rhs! (src/wave3d_curved.jl:205-258):
+ *   du = ud + eps mu^-5 sum_d D_d^6 u,  dud = rhs_wave3d!(u, ud) + eps mu^-5 sum_d D_d^6 ud. */
+int wt2_rhs_full(wt2_ctx* ctx, double* du, double* dud, const double* u,
+                 const double* ud, int32_t flags);
+
+/* nsteps classical RK4 steps of size dt on (u, ud) in place (the integrator
+ * step! loop of src/evolve.jl:1028-1031 with RK4(), stage order of
+ * test/wave3d_tests.jl:32-44).  Each stage is the RHS fused with the stage
+ * update.  If energy_out is non-null and energy_every > 0 it receives
+ * 1 + nsteps/energy_every energies (host array): before the first step and
+ * after every energy_every-th step. */
+int wt2_rk4_steps(wt2_ctx* ctx, double* u, double* ud, double dt, int32_t nsteps,
+                  int32_t energy_every, double* energy_out, int32_t flags);
+
+/* Discrete energy E = 1/2<ud,ud>_W + 1/2<u,-L u>_H (zero boundary data)
+ * (test/kernels3d_tests.jl:32-47; src/wave3d_curved.jl:268-287), FP64,
+ * deterministic reduction order. */
+int wt2_energy(wt2_ctx* ctx, const double* u, const double* ud, double* E, int32_t flags);
+
+/* Use an external HIP stream (hipStream_t passed as void*); NULL restores the
+ * context's own stream. */
+int wt2_set_stream(wt2_ctx* ctx, void* stream);
+
+const char* wt2_last_error(void);
+const char* wt2_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WT2_H */

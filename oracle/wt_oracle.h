@@ -1,0 +1,86 @@
+/*
+ * wt_oracle.h -- CPU FP64 oracle for the 3D SBP-SAT wave RHS and RK4 step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or the timed CPU baseline).  The product path (libwt2.so) never calls it.
+ *
+ * It restates, element by element, the reference's hot path:
+ *   - GLL element operators (HexSBPSAT make_element/make_operators, used at
+ *     src/wave.jl:603; README.md:10-20),
+ *   - the SIPG Laplacian of rhs_wave3d! / apply_laplacian! (src/wave.jl:573-611,
+ *     formula README.md:57-81),
+ *   - the ADM background coefficients of sample_background3d!
+ *     (src/wave3d_curved.jl:75-124) folded into a second-order (u, u_t) form,
+ *   - Kreiss-Oliger eps*mu^-5*D^6 per axis with the SAT-coupled apply_D!
+ *     (src/wave3d_curved.jl:158-192; METHODS.md:256-298), applied to u_t
+ *     (see wto_rhs for why), outer faces closed with a zero ghost value,
+ *   - classical RK4 (test/wave3d_tests.jl:32-44),
+ *   - the H-norm energy (test/kernels3d_tests.jl:32-47).
+ * Layout: fields are (N,N,N,Ne) column-major (i fastest), elements x-fastest
+ * e = ex + Mx*(ey + My*ez); faces 1..6 = (-x,+x,-y,+y,-z,+z).
+ */
+#ifndef WT_ORACLE_H
+#define WT_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WTO_NMAX 16
+
+typedef struct wto_desc {
+    int N;                 /* GLL nodes per direction                      */
+    int M[3];              /* elements per axis                            */
+    double x0[3], x1[3];   /* domain extent per axis                       */
+    int periodic[3];       /* 1 = periodic along that axis                 */
+    double warp_A;         /* warp amplitude (0 = affine)                  */
+    double tau;            /* SIPG penalty, unit-element units             */
+    double eps_ko;         /* Kreiss-Oliger epsilon (0 disables)           */
+    double bdry_values[6]; /* Dirichlet value per boundary tag 1..6        */
+} wto_desc;
+
+typedef struct wto_ctx wto_ctx;
+
+/* GLL nodes xi on [0,1], weights w (sum 1), differentiation matrix
+ * G[i*N+p] = l_p'(xi_i).  Returns 0 on success. */
+int wto_gll(int N, double* xi, double* w, double* G);
+
+/* bg: NULL for Minkowski, else 10 doubles per node, node-major
+ * (alpha, beta1, beta2, beta3, g11, g12, g13, g22, g23, g33). */
+wto_ctx* wto_create(const wto_desc* d, const double* bg);
+void wto_destroy(wto_ctx* c);
+
+long wto_npoints(const wto_ctx* c);
+int wto_is_curved(const wto_ctx* c);
+double wto_mu(const wto_ctx* c);
+double wto_hmin(const wto_ctx* c);
+void wto_coords(const wto_ctx* c, double* xyz);    /* (3,N,N,N,Ne) */
+void wto_hphys(const wto_ctx* c, double* hw);      /* H3*|J| per node */
+/* curved coefficients per node: K11 K12 K13 K22 K23 K33 c1 c2 c3 wgt kap1 kap2 kap3 */
+int wto_coefs(const wto_ctx* c, double* out13);
+
+/* Drop-in RHS (rhs_wave3d! semantics): udd = L u (flat; ud unused) or
+ * wgt*(Ltilde u + A ud) on curved/background meshes.  No KO. */
+void wto_lap(const wto_ctx* c, const double* u, const double* ud, double* udd);
+/* Full first-order RHS: du = v, dv = lap(u,v) + KO(v). */
+void wto_rhs(const wto_ctx* c, const double* u, const double* v, double* du, double* dv);
+/* out = D_axis in (SAT-coupled first derivative, apply_D! semantics:
+ * one-sided at outer faces). */
+void wto_apply_D(const wto_ctx* c, int axis, const double* in, double* out);
+/* The KO derivative: as wto_apply_D but outer faces see a zero ghost value. */
+void wto_apply_Dko(const wto_ctx* c, int axis, const double* in, double* out);
+/* Classical RK4, nsteps steps of dt, in place on (u, v).  energy_out (nullable)
+ * receives nsteps+1 energies (before the first step and after each). */
+void wto_rk4(const wto_ctx* c, double* u, double* v, double dt, int nsteps, double* energy_out);
+double wto_energy(const wto_ctx* c, const double* u, const double* v);
+
+/* Dense 1D operators on a line of M elements (for spectrum / symmetry pins).
+ * kind 0: SIPG Laplacian with homogeneous Dirichlet (or periodic) ends,
+ * kind 1: SAT-coupled first derivative D (one-sided ends), kind 2: the KO
+ * derivative (zero-ghost ends).  A is (M*N)x(M*N) row-major. */
+int wto_assemble_1d(int N, int M, int periodic, double tau, double h, int kind, double* A);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

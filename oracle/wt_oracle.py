@@ -1,0 +1,188 @@
+"""ctypes binding of the CPU FP64 oracle (libwt_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product library never loads
+it.  The C source (wt_oracle.c) cites the reference file:line each function
+restates.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libwt_oracle.so")
+_lib = None
+
+NMAX = 16
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [
+        ("N", ctypes.c_int),
+        ("M", ctypes.c_int * 3),
+        ("x0", ctypes.c_double * 3),
+        ("x1", ctypes.c_double * 3),
+        ("periodic", ctypes.c_int * 3),
+        ("warp_A", ctypes.c_double),
+        ("tau", ctypes.c_double),
+        ("eps_ko", ctypes.c_double),
+        ("bdry_values", ctypes.c_double * 6),
+    ]
+
+
+def build():
+    """Compile libwt_oracle.so with the Makefile beside this file."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        vp = ctypes.c_void_p
+        L.wto_gll.argtypes = [ctypes.c_int, dp, dp, dp]
+        L.wto_create.argtypes = [ctypes.POINTER(Desc), dp]
+        L.wto_create.restype = vp
+        L.wto_destroy.argtypes = [vp]
+        L.wto_npoints.argtypes = [vp]
+        L.wto_npoints.restype = ctypes.c_long
+        L.wto_is_curved.argtypes = [vp]
+        L.wto_mu.argtypes = [vp]
+        L.wto_mu.restype = ctypes.c_double
+        L.wto_hmin.argtypes = [vp]
+        L.wto_hmin.restype = ctypes.c_double
+        L.wto_coords.argtypes = [vp, dp]
+        L.wto_hphys.argtypes = [vp, dp]
+        L.wto_coefs.argtypes = [vp, dp]
+        L.wto_lap.argtypes = [vp, dp, dp, dp]
+        L.wto_rhs.argtypes = [vp, dp, dp, dp, dp]
+        L.wto_apply_D.argtypes = [vp, ctypes.c_int, dp, dp]
+        L.wto_apply_Dko.argtypes = [vp, ctypes.c_int, dp, dp]
+        L.wto_rk4.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_int, dp]
+        L.wto_energy.argtypes = [vp, dp, dp]
+        L.wto_energy.restype = ctypes.c_double
+        L.wto_assemble_1d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_int, dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def gll(N):
+    xi = np.zeros(N); w = np.zeros(N); G = np.zeros((N, N))
+    if lib().wto_gll(N, _p(xi), _p(w), _p(G)) != 0:
+        raise ValueError("N out of range")
+    return xi, w, G
+
+
+def assemble_1d(N, M, periodic, tau, h, kind):
+    A = np.zeros((M * N, M * N))
+    lib().wto_assemble_1d(N, M, int(periodic), tau, h, kind, _p(A))
+    return A
+
+
+class Oracle:
+    """One mesh + operator set; arrays are flat float64 (N,N,N,Ne) Fortran-order
+    fields stored as 1-D numpy arrays of length N^3*Ne."""
+
+    def __init__(self, N, M, x0=0.0, x1=1.0, periodic=False, warp_A=0.0, tau=None,
+                 eps_ko=0.0, bdry_values=(0.0,) * 6, bg=None):
+        if isinstance(M, int):
+            M = (M, M, M)
+        if not isinstance(x0, (tuple, list)):
+            x0 = (x0,) * 3
+        if not isinstance(x1, (tuple, list)):
+            x1 = (x1,) * 3
+        if not isinstance(periodic, (tuple, list)):
+            periodic = (periodic,) * 3
+        if tau is None:
+            tau = 1.5 * (N - 1) ** 2
+        d = Desc()
+        d.N = N
+        for a in range(3):
+            d.M[a] = M[a]; d.x0[a] = x0[a]; d.x1[a] = x1[a]; d.periodic[a] = int(periodic[a])
+        d.warp_A = warp_A; d.tau = tau; d.eps_ko = eps_ko
+        for f in range(6):
+            d.bdry_values[f] = bdry_values[f]
+        self.desc = d
+        self.N, self.M = N, tuple(M)
+        self.Ne = M[0] * M[1] * M[2]
+        self.n = N ** 3 * self.Ne
+        self._bg = None if bg is None else np.ascontiguousarray(bg, dtype=np.float64)
+        self.h = self._bg  # keep alive
+        self.ctx = lib().wto_create(ctypes.byref(d), _p(self._bg))
+        if not self.ctx:
+            raise ValueError("wto_create failed")
+
+    def __del__(self):
+        if getattr(self, "ctx", None):
+            lib().wto_destroy(self.ctx)
+            self.ctx = None
+
+    def _new(self):
+        return np.zeros(self.n)
+
+    @property
+    def mu(self):
+        return lib().wto_mu(self.ctx)
+
+    @property
+    def hmin(self):
+        return lib().wto_hmin(self.ctx)
+
+    def coords(self):
+        x = np.zeros(3 * self.n)
+        lib().wto_coords(self.ctx, _p(x))
+        return x.reshape(self.n, 3)
+
+    def hphys(self):
+        h = self._new()
+        lib().wto_hphys(self.ctx, _p(h))
+        return h
+
+    def coefs(self):
+        c = np.zeros(13 * self.n)
+        if lib().wto_coefs(self.ctx, _p(c)) != 0:
+            return None
+        return c.reshape(self.n, 13)
+
+    def lap(self, u, ud=None):
+        y = self._new()
+        lib().wto_lap(self.ctx, _p(u), _p(ud), _p(y))
+        return y
+
+    def rhs(self, u, v):
+        du = self._new(); dv = self._new()
+        lib().wto_rhs(self.ctx, _p(u), _p(v), _p(du), _p(dv))
+        return du, dv
+
+    def apply_D(self, axis, x):
+        y = self._new()
+        lib().wto_apply_D(self.ctx, axis, _p(x), _p(y))
+        return y
+
+    def apply_Dko(self, axis, x):
+        y = self._new()
+        lib().wto_apply_Dko(self.ctx, axis, _p(x), _p(y))
+        return y
+
+    def rk4(self, u, v, dt, nsteps, energy=False):
+        u = np.array(u, dtype=np.float64, copy=True)
+        v = np.array(v, dtype=np.float64, copy=True)
+        E = np.zeros(nsteps + 1) if energy else None
+        lib().wto_rk4(self.ctx, _p(u), _p(v), dt, nsteps, _p(E))
+        return (u, v, E) if energy else (u, v)
+
+    def energy(self, u, v):
+        return lib().wto_energy(self.ctx, _p(u), _p(v))

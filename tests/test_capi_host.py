@@ -1,0 +1,109 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol
+include/wt2.h declares, and rejects bad arguments with the documented codes
+before touching a device."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import wt_oracle as wo
+import wavetoy_hip as wh
+from wavetoy_hip import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "wt2.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(wt2_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    declared = _declared()
+    assert declared, "no declarations parsed"
+    for name in declared:
+        assert hasattr(L, name), f"{name} declared in include/wt2.h but not exported"
+    assert sorted(_lib.EXPORTS) == declared
+
+
+def test_version_string():
+    assert b"gfx950" in _lib.lib().wt2_version()
+
+
+@pytest.mark.parametrize("N", [3, 4, 6, 8, 10])
+def test_gll_matches_oracle(N):
+    e = wh.make_element(N)
+    xi, w, G = wo.gll(N)
+    assert np.max(np.abs(e.xs - xi)) < 1e-15
+    assert np.max(np.abs(e.w - w)) < 1e-15
+    assert np.max(np.abs(e.G - G)) < 1e-12 * np.max(np.abs(G))
+
+
+def _desc(**kw):
+    d = _lib.MeshDesc()
+    d.N = kw.get("N", 4)
+    for a in range(3):
+        d.M[a] = kw.get("M", 2)
+        d.x0[a] = 0.0
+        d.x1[a] = 1.0
+        d.periodic[a] = 0
+    d.tau = kw.get("tau", 1.5 * (d.N - 1) ** 2)
+    d.eps_ko = kw.get("eps", 0.0)
+    d.precision = kw.get("precision", 0)
+    d.flags = kw.get("flags", 0)
+    return d
+
+
+def _create(d):
+    ctx = ctypes.c_void_p()
+    rc = _lib.lib().wt2_ctx_create(ctypes.byref(d), ctypes.byref(ctx))
+    return rc, ctx
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(N=2), _lib.WT2_ERR_SHAPE),
+    (dict(N=11), _lib.WT2_ERR_SHAPE),
+    (dict(M=0), _lib.WT2_ERR_SHAPE),
+    (dict(M=65), _lib.WT2_ERR_UNSUPPORTED),
+    (dict(precision=7), _lib.WT2_ERR_ARG),
+    (dict(tau=-1.0), _lib.WT2_ERR_ARG),
+    (dict(eps=-0.1), _lib.WT2_ERR_ARG),
+    (dict(N=8, tau=20.0), _lib.WT2_ERR_TAU),   # literal tau = 20 is not coercive at N = 8
+    (dict(N=10, tau=20.0), _lib.WT2_ERR_TAU),
+])
+def test_ctx_create_rejects(kw, code):
+    rc, ctx = _create(_desc(**kw))
+    assert rc == code
+    assert not ctx.value
+    assert len(_lib.lib().wt2_last_error()) > 0
+
+
+def test_null_pointers_rejected():
+    L = _lib.lib()
+    assert L.wt2_ctx_create(None, None) == _lib.WT2_ERR_ARG
+    assert L.wt2_gll(4, None, None, None) == _lib.WT2_ERR_ARG
+    assert L.wt2_rhs3d(None, None, None, None, None, 0) == _lib.WT2_ERR_ARG
+    assert L.wt2_rk4_steps(None, None, None, 0.1, 1, 0, None, 0) == _lib.WT2_ERR_ARG
+    assert L.wt2_energy(None, None, None, None, 0) == _lib.WT2_ERR_ARG
+    assert L.wt2_ctx_info(None, None) == _lib.WT2_ERR_ARG
+
+
+def test_host_coords_match_oracle():
+    for A in (0.0, 0.05):
+        mesh = (wh.make_warped_uniform_hex(3, 2, 2, 0.0, 1.0, A) if A
+                else wh.make_uniform_hex((3, 2, 2), 0.0, 1.0))
+        e = wh.make_element(5)
+        x = wh.host_coords(mesh, e)
+        o = wo.Oracle(5, (3, 2, 2), warp_A=A)
+        assert np.max(np.abs(x - o.coords())) < 1e-14
+
+
+def test_tau_effective_matches_reference_driver_scaling():
+    # src/evolve.jl:964-968: tau = 1.5 (N-1)^2 on cubical meshes
+    assert wh.tau_effective(20.0, 8) == 73.5
+    assert wh.tau_effective(20.0, 4) == 20.0
+    assert wh.tau_effective(None, 10) == 121.5

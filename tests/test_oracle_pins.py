@@ -1,0 +1,423 @@
+"""Pins of the CPU oracle against what the reference's tests and the mathematics fix.
+
+The reference ships no golden vectors (its tests are properties, closed forms
+and tolerances), and its arithmetic (HexSBPSAT/HexMeshes) is not in the
+reference tree, so every oracle function is tied down here by a check a
+plausible mistake (a dropped term, a wrong sign or index) would fail.
+"""
+import math
+
+import numpy as np
+import pytest
+from numpy.polynomial import legendre as npleg
+
+import wt_oracle as wo
+
+
+# ------------------------------------------------------------------ element
+
+@pytest.mark.parametrize("N", [3, 4, 5, 6, 8, 10])
+def test_gll_matches_numpy_and_sbp(N):
+    """GLL nodes = endpoints + roots of P'_{N-1} (numpy.polynomial.legendre),
+    weights 2/(n(n+1)P_n^2) sum to 1, G exact on degree <= N-1, HG+(HG)^T = B
+    (README.md:13-15), G 1 = 0."""
+    xi, w, G = wo.gll(N)
+    c = np.zeros(N); c[-1] = 1.0
+    r = np.sort(npleg.legroots(npleg.legder(c)))
+    x = np.concatenate([[-1.0], r, [1.0]])
+    assert np.max(np.abs((x + 1) / 2 - xi)) < 1e-14
+    pn = npleg.legval(x, c)
+    w_ref = 2.0 / ((N - 1) * N * pn ** 2) / 2.0
+    assert np.max(np.abs(w - w_ref)) < 1e-14
+    assert abs(w.sum() - 1.0) < 1e-14
+    H = np.diag(w)
+    B = np.zeros((N, N)); B[0, 0] = -1; B[-1, -1] = 1
+    assert np.max(np.abs(H @ G + (H @ G).T - B)) < 1e-13
+    for p in range(1, N):
+        assert np.max(np.abs(G @ xi ** p - p * xi ** (p - 1))) < 1e-12
+    assert np.max(np.abs(G @ np.ones(N))) < 1e-13
+    # quadrature exact to degree 2N-3
+    for p in range(0, 2 * N - 2):
+        assert abs(w @ xi ** p - 1.0 / (p + 1)) < 1e-14
+
+
+# --------------------------------------------------------------- 1D SIPG
+
+@pytest.mark.parametrize("N", [4, 6, 8, 10])
+def test_sipg_1d_symmetric_nsd_at_tau_eff(N):
+    """H L symmetric; NSD at tau = 1.5 (N-1)^2 (src/evolve.jl:964-968); smallest
+    Dirichlet eigenvalue -> -pi^2 on [0,1] (test/dirichlet_cube_tests.jl:110-130)."""
+    M = 4
+    h = 1.0 / M
+    _, w, _ = wo.gll(N)
+    A = wo.assemble_1d(N, M, 0, 1.5 * (N - 1) ** 2, h, 0)
+    S = np.diag(np.tile(w, M) * h) @ A
+    assert np.max(np.abs(S - S.T)) <= 1e-12 * np.max(np.abs(S))
+    ev = np.linalg.eigvalsh(0.5 * (S + S.T))
+    assert ev.max() < 0
+    lam = np.linalg.eigvals(A)
+    assert np.max(np.abs(lam.imag)) < 1e-8 * np.max(np.abs(lam))
+    assert abs(np.max(lam.real) + math.pi ** 2) < 1e-2 * math.pi ** 2
+
+
+def test_sipg_literal_tau20_indefinite_for_n8():
+    """The configs' literal tau = 20 is below SIPG coercivity at N = 8 (a positive
+    eigenvalue appears), while N = 4 is fine (README.md:77-81 'tau >~ 17')."""
+    _, w8, _ = wo.gll(8)
+    A8 = wo.assemble_1d(8, 4, 0, 20.0, 0.25, 0)
+    assert np.max(np.linalg.eigvals(A8).real) > 1.0
+    A4 = wo.assemble_1d(4, 4, 0, 20.0, 0.25, 0)
+    assert np.max(np.linalg.eigvals(A4).real) < 0
+
+
+def test_sipg_periodic_annihilates_constants_and_kinks_removed():
+    """Periodic: L 1 = 0 (test/wave_lap_strong_mesh_tests.jl:45-53); the kink mode
+    (continuous piecewise linear) is NOT in the kernel once the gradient jump is
+    penalised (README.md:35-55)."""
+    N, M = 5, 4
+    A = wo.assemble_1d(N, M, 1, 1.5 * (N - 1) ** 2, 1.0 / M, 0)
+    assert np.max(np.abs(A @ np.ones(M * N))) < 1e-10
+    xi, _, _ = wo.gll(N)
+    x = np.concatenate([(e + xi) / M for e in range(M)])
+    kink = np.abs(x - 0.5)   # continuous, kinked at 0.5 (an element face for M=4)
+    assert np.max(np.abs(wo.assemble_1d(N, M, 0, 1.5 * (N - 1) ** 2, 1.0 / M, 0) @ kink)) > 1.0
+
+
+def test_sipg_second_derivative_accuracy():
+    """On a smooth function vanishing at the ends, L u -> u'' at spectral rate."""
+    errs = []
+    for M in (2, 4, 8):
+        N = 6
+        xi, _, _ = wo.gll(N)
+        x = np.concatenate([(e + xi) / M for e in range(M)])
+        A = wo.assemble_1d(N, M, 0, 1.5 * (N - 1) ** 2, 1.0 / M, 0)
+        u = np.sin(math.pi * x)
+        errs.append(np.max(np.abs(A @ u + math.pi ** 2 * u)))
+    assert errs[1] < errs[0] / 8 and errs[2] < errs[1] / 8
+
+
+# ------------------------------------------------------------------ D / KO
+
+@pytest.mark.parametrize("N", [4, 8])
+def test_apply_D_skew_periodic_and_exact(N):
+    """H D + (H D)^T = 0 on a periodic line (METHODS.md:259-262); D exact on
+    global polynomials of degree <= N-1 away from SAT faces (continuous data:
+    SAT jump terms vanish)."""
+    M = 4
+    h = 1.0 / M
+    _, w, _ = wo.gll(N)
+    D = wo.assemble_1d(N, M, 1, 0.0, h, 1)
+    HD = np.diag(np.tile(w, M) * h) @ D
+    assert np.max(np.abs(HD + HD.T)) < 1e-12 * np.max(np.abs(HD))
+    Dn = wo.assemble_1d(N, M, 0, 0.0, h, 1)
+    xi, _, _ = wo.gll(N)
+    x = np.concatenate([(e + xi) / M for e in range(M)])
+    for p in range(1, N):
+        assert np.max(np.abs(Dn @ x ** p - p * x ** (p - 1))) < 1e-9
+
+
+def test_mu_power_iteration_matches_dense_spectral_radius():
+    """mu (30 power iterations on D^2 from the alternating vector,
+    src/wave3d_curved.jl:158-178) is within a few percent of rho(D)
+    (METHODS.md: 'accurate to a few %')."""
+    for N, M in ((4, 4), (8, 4), (6, 3)):
+        o = wo.Oracle(N, M, eps_ko=0.1)
+        D = wo.assemble_1d(N, M, 0, 0.0, 1.0 / M, 1)
+        rho = np.max(np.abs(np.linalg.eigvals(D)))
+        assert abs(o.mu / rho - 1) < 0.05
+
+
+def test_ko_derivative_skew_on_bounded_line():
+    """The KO derivative (apply_D! with outer faces closed against a zero ghost)
+    is exactly H-skew on a bounded line, so H D^6 <= 0 there; apply_D! itself
+    (one-sided ends) satisfies H D + (H D)^T = B instead (SBP)."""
+    for N in (4, 8):
+        M = 3
+        _, w, _ = wo.gll(N)
+        H = np.diag(np.tile(w, M) / M)
+        Dk = wo.assemble_1d(N, M, 0, 0.0, 1.0 / M, 2)
+        HD = H @ Dk
+        assert np.max(np.abs(HD + HD.T)) < 1e-12 * np.max(np.abs(HD))
+        K = H @ np.linalg.matrix_power(Dk, 6)
+        assert np.max(np.linalg.eigvalsh(0.5 * (K + K.T))) < 1e-9 * np.max(np.abs(K))
+        D1 = wo.assemble_1d(N, M, 0, 0.0, 1.0 / M, 1)
+        B = np.zeros((M * N, M * N)); B[0, 0] = -1; B[-1, -1] = 1
+        assert np.max(np.abs(H @ D1 + (H @ D1).T - B)) < 1e-12 * np.max(np.abs(H @ D1))
+
+
+def test_ko_is_dissipative_in_H():
+    """D^6 is NSD in the H norm on a periodic line (D skew => D^6 = -(D^3)^T H D^3)."""
+    N, M = 6, 4
+    _, w, _ = wo.gll(N)
+    D = wo.assemble_1d(N, M, 1, 0.0, 1.0 / M, 1)
+    H = np.diag(np.tile(w, M) / M)
+    K = H @ np.linalg.matrix_power(D, 6)
+    assert np.max(np.linalg.eigvalsh(0.5 * (K + K.T))) < 1e-8 * np.max(np.abs(K))
+
+
+# ------------------------------------------------------------------- 3D flat
+
+def _dense(o, f, n):
+    A = np.zeros((n, n))
+    x = np.zeros(n)
+    for j in range(n):
+        x[:] = 0; x[j] = 1
+        A[:, j] = f(x)
+    return A
+
+
+def test_flat_3d_is_kronecker_sum():
+    """The flat 3D operator equals L1 (x) I (x) I + I (x) L2 (x) I + I (x) I (x) L3 once
+    the element-major layout is mapped to global lexicographic node order."""
+    N, M = 3, (2, 3, 2)
+    o = wo.Oracle(N, M, x1=(1.0, 1.5, 0.8))
+    A = _dense(o, o.lap, o.n)
+    L = [wo.assemble_1d(N, M[a], 0, 1.5 * (N - 1) ** 2, [1.0, 1.5, 0.8][a] / M[a], 0) for a in range(3)]
+    # global index g = gx + nx*(gy + ny*gz), gx = ex*N + i
+    nx, ny, nz = M[0] * N, M[1] * N, M[2] * N
+    perm = np.zeros(o.n, dtype=int)
+    q = 0
+    for ez in range(M[2]):
+        for ey in range(M[1]):
+            for ex in range(M[0]):
+                for k in range(N):
+                    for j in range(N):
+                        for i in range(N):
+                            gx, gy, gz = ex * N + i, ey * N + j, ez * N + k
+                            perm[q] = gx + nx * (gy + ny * gz)
+                            q += 1
+    K = (np.kron(np.eye(nz), np.kron(np.eye(ny), L[0])) + np.kron(np.eye(nz), np.kron(L[1], np.eye(nx)))
+         + np.kron(L[2], np.eye(nx * ny)))
+    Kp = K[np.ix_(perm, perm)]
+    assert np.max(np.abs(A - Kp)) < 1e-10 * np.max(np.abs(A))
+
+
+def _sinmode(o):
+    x = o.coords()
+    return np.sin(math.pi * x[:, 0]) * np.sin(math.pi * x[:, 1]) * np.sin(math.pi * x[:, 2])
+
+
+def test_dirichlet_rayleigh_quotient_converges_to_3pi2():
+    """test/dirichlet_cube_tests.jl:58-108: Rayleigh quotient -> 3 pi^2, < 1e-3 at
+    M = 4 and >= 4x per halving (N = 4)."""
+    errs = []
+    for M in (2, 4, 8):
+        o = wo.Oracle(4, M)
+        u = _sinmode(o)
+        H = o.hphys()
+        lam = -np.dot(u, H * o.lap(u)) / np.dot(u, H * u)
+        errs.append(abs(lam - 3 * math.pi ** 2) / (3 * math.pi ** 2))
+    assert errs[1] < 1e-3
+    assert errs[1] < errs[0] / 4 and errs[2] < errs[1] / 4
+
+
+def test_dirichlet_spectrum_real_nonpositive():
+    """test/dirichlet_cube_tests.jl:110-130 (N = 3, M = 2)."""
+    o = wo.Oracle(3, 2, tau=1.5 * 4)
+    A = _dense(o, o.lap, o.n)
+    lam = np.linalg.eigvals(A)
+    assert np.max(lam.real) < 1e-10 * np.max(np.abs(lam))
+    assert np.max(np.abs(lam.imag)) < 1e-8 * np.max(np.abs(lam))
+    assert np.min(lam.real) < -1.0
+
+
+def test_dirichlet_values_enter_by_tag():
+    """bdry_values[tag-1] is the Dirichlet value of outer face tag f = 1..6 =
+    (-x,+x,-y,+y,-z,+z) (src/wave.jl:567-571): with u = 0 and data 1 on tag f
+    only, L u is non-zero exactly on the elements touching face f and the
+    Nitsche penalty pulls u towards the data (positive at the face nodes); a
+    constant u with matching data on every face gives L u = 0."""
+    N, M = 4, 3
+    for f in range(6):
+        bv = [0.0] * 6
+        bv[f] = 1.0
+        o = wo.Oracle(N, M, bdry_values=tuple(bv))
+        x = o.coords()
+        y = o.lap(np.zeros(o.n))
+        a, hi = f // 2, f % 2
+        e = np.arange(o.n) // N ** 3
+        ec = (e % M, (e // M) % M, e // (M * M))[a]
+        near = ec == (M - 1 if hi else 0)
+        on_face = np.abs(x[:, a] - (1.0 if hi else 0.0)) < 1e-12
+        assert np.max(np.abs(y[~near])) == 0.0
+        assert np.all(y[on_face] > 0)
+    c = 0.7
+    o = wo.Oracle(N, M, bdry_values=(c,) * 6)
+    assert np.max(np.abs(o.lap(np.full(o.n, c)))) < 1e-9
+
+
+def test_full_rhs_spectrum_stable_with_ko():
+    """test/wave3d_tests.jl:72-89: max Re(lambda) <= 1e-5 |lambda|_max with eps_KO = 0.1."""
+    o = wo.Oracle(3, 2, eps_ko=0.1)
+    n = o.n
+
+    def f(x):
+        du, dv = o.rhs(x[:n].copy(), x[n:].copy())
+        return np.concatenate([du, dv])
+    A = _dense(o, f, 2 * n)
+    lam = np.linalg.eigvals(A)
+    assert np.max(lam.real) <= 1e-5 * np.max(np.abs(lam))
+
+
+def test_rk4_step_is_fourth_order_taylor_polynomial():
+    """For y' = A y one classical RK4 step is sum_{k<=4} (dt A)^k/k! y0 exactly."""
+    o = wo.Oracle(3, 2, eps_ko=0.1)
+    n = o.n
+
+    def f(x):
+        du, dv = o.rhs(x[:n].copy(), x[n:].copy())
+        return np.concatenate([du, dv])
+    A = _dense(o, f, 2 * n)
+    rng = np.random.default_rng(7)
+    y0 = rng.standard_normal(2 * n)
+    dt = 0.2 * o.hmin
+    P = np.eye(2 * n); T = np.eye(2 * n)
+    for k in range(1, 5):
+        T = T @ (dt * A) / k
+        P = P + T
+    y1 = P @ y0
+    u1, v1 = o.rk4(y0[:n], y0[n:], dt, 1)
+    assert np.max(np.abs(np.concatenate([u1, v1]) - y1)) < 1e-12 * np.max(np.abs(y1))
+
+
+def test_eigenmode_evolution_converges_and_conserves_energy():
+    """Exact standing wave cos(sqrt(3) pi t) sin sin sin (the plain-method CPU
+    configuration, at reduced size): error ratio > 4 per M doubling
+    (test/evolve3d_tests.jl:14-21) and energy drift < 1e-3 without KO
+    (test/wave3d_tests.jl:50-69)."""
+    errs = []
+    t1 = 0.1
+    for M in (2, 4):
+        o = wo.Oracle(4, M)
+        u0 = _sinmode(o)
+        v0 = np.zeros_like(u0)
+        dt = 0.2 * o.hmin
+        ns = int(math.ceil(t1 / dt))
+        dt = t1 / ns
+        u, v, E = o.rk4(u0, v0, dt, ns, energy=True)
+        ex = math.cos(math.sqrt(3) * math.pi * t1) * u0
+        H = o.hphys()
+        errs.append(math.sqrt(np.sum((u - ex) ** 2 * H)))
+        assert abs(E[-1] / E[0] - 1) < 1e-3
+    assert errs[0] / errs[1] > 4
+
+
+def test_energy_nonincreasing_with_ko_dirichlet_noise():
+    """test/evolve3d_tests.jl:31-36 pattern: noise IC, KO on, closed Dirichlet box:
+    the discrete energy does not grow."""
+    o = wo.Oracle(4, 2, eps_ko=0.1)
+    rng = np.random.default_rng(20260604)
+    amp = math.sqrt(np.finfo(float).eps)
+    u0 = amp * rng.standard_normal(o.n)
+    v0 = amp * rng.standard_normal(o.n)
+    dt = 0.2 * o.hmin
+    _, _, E = o.rk4(u0, v0, dt, 60, energy=True)
+    assert np.all(np.diff(E) <= 1e-12 * E[0])
+    assert E[-1] < E[0]
+
+
+# ------------------------------------------------------------------ curved
+
+def _flat_bg(n):
+    bg = np.zeros((n, 10))
+    bg[:, 0] = 1.0
+    bg[:, 4] = bg[:, 7] = bg[:, 9] = 1.0
+    return bg
+
+
+def _bump_bg(x):
+    r2 = np.sum((x - 0.5) ** 2, axis=1)
+    e = np.exp(-r2 / 0.1)
+    bg = np.zeros((len(x), 10))
+    bg[:, 0] = 1 - 0.2 * e
+    bg[:, 1:4] = 0.05 * e[:, None] * (x - 0.5)
+    g = 1 + 0.2 * e
+    bg[:, 4] = bg[:, 7] = bg[:, 9] = g
+    return bg
+
+
+def test_curved_reduces_to_flat():
+    """Warp A = 0 with alpha = 1, beta = 0, gamma = delta: the curved operator
+    equals the flat one (test/dirichlet_warped_tests.jl:19-36 reduction)."""
+    N, M = 4, (2, 3, 2)
+    bv = (0.1, -0.2, 0.3, 0.0, 0.5, -0.4)
+    of = wo.Oracle(N, M, x1=(1.0, 1.2, 0.9), bdry_values=bv)
+    oc = wo.Oracle(N, M, x1=(1.0, 1.2, 0.9), bdry_values=bv, bg=_flat_bg(of.n))
+    rng = np.random.default_rng(2026_06_05)
+    u = rng.standard_normal(of.n)
+    v = rng.standard_normal(of.n)
+    a, b = of.lap(u), oc.lap(u, v)
+    assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(a))
+
+
+def test_curved_symmetric_nsd_and_free_stream():
+    """Warped mesh + ADM bump background: H3 Ltilde symmetric (SIPG), NSD at the
+    effective tau, and a constant is annihilated on a periodic warped mesh
+    (free-stream, test/wave3d_tests.jl:121-125)."""
+    N, M = 3, 2
+    o0 = wo.Oracle(N, M, warp_A=0.05)
+    x = o0.coords()
+    o = wo.Oracle(N, M, warp_A=0.05, bg=_bump_bg(x))
+    n = o.n
+    A = _dense(o, lambda z: o.lap(z, np.zeros(n)), n)
+    co = o.coefs()
+    wgt = co[:, 9]
+    _, w, _ = wo.gll(N)
+    H3 = np.tile(np.einsum("k,j,i->kji", w, w, w).reshape(-1), o.Ne)
+    S = np.diag(H3 / wgt) @ A    # H3 Ltilde = (H3/wgt)(wgt Ltilde)
+    assert np.max(np.abs(S - S.T)) < 1e-11 * np.max(np.abs(S))
+    assert np.max(np.linalg.eigvalsh(0.5 * (S + S.T))) < 0
+    op = wo.Oracle(N, M, warp_A=0.05, periodic=True, bg=_bump_bg(wo.Oracle(N, M, warp_A=0.05, periodic=True).coords()))
+    c = np.full(op.n, 2.5)
+    assert np.max(np.abs(op.lap(c, np.zeros(op.n)))) < 1e-9
+
+
+def test_curved_advection_is_skew_in_energy_norm():
+    """The u_t advection part A (split form + SAT) is skew in the H3/wgt norm, so
+    with KO off the energy is conserved up to time-integration error."""
+    N, M = 3, 2
+    x = wo.Oracle(N, M, warp_A=0.05).coords()
+    o = wo.Oracle(N, M, warp_A=0.05, bg=_bump_bg(x))
+    n = o.n
+    Av = _dense(o, lambda z: o.lap(np.zeros(n), z), n)
+    co = o.coefs()
+    _, w, _ = wo.gll(N)
+    H3 = np.tile(np.einsum("k,j,i->kji", w, w, w).reshape(-1), o.Ne)
+    S = np.diag(H3 / co[:, 9]) @ Av
+    assert np.max(np.abs(S + S.T)) < 1e-11 * max(np.max(np.abs(S)), 1e-300)
+    assert np.max(np.abs(S)) > 0
+
+
+def test_warped_rayleigh_quotient():
+    """test/dirichlet_warped_tests.jl:59-78: A = 0.05, Rayleigh error < 5% at
+    M = 4; geometric-mean ratio > 4; < 1e-3 at M = 8 (N = 4)."""
+    errs = []
+    for M in (2, 4, 8):
+        o = wo.Oracle(4, M, warp_A=0.05)
+        x = o.coords()
+        u = np.sin(math.pi * x[:, 0]) * np.sin(math.pi * x[:, 1]) * np.sin(math.pi * x[:, 2])
+        H = o.hphys()
+        lam = -np.dot(u, H * o.lap(u, np.zeros(o.n))) / np.dot(u, H * u)
+        errs.append(abs(lam - 3 * math.pi ** 2) / (3 * math.pi ** 2))
+    assert errs[1] < 0.05
+    assert math.sqrt((errs[0] / errs[1]) * (errs[1] / errs[2])) > 4
+    assert errs[2] < 1e-3
+
+
+def test_curved_energy_conserved_without_ko():
+    """Warped mesh + ADM background, KO off: the semi-discrete energy is
+    conserved (H3 Ltilde symmetric, advection skew), so the drift over a fixed
+    time is RK4's own error: non-increasing and shrinking at >= 4th order in dt."""
+    N, M = 4, 2
+    x = wo.Oracle(N, M, warp_A=0.05).coords()
+    o = wo.Oracle(N, M, warp_A=0.05, bg=_bump_bg(x))
+    xx = o.coords()
+    u0 = np.exp(-np.sum((xx - 0.45) ** 2, axis=1) / 0.02)
+    v0 = np.zeros(o.n)
+    drift = []
+    for f in (1, 2):
+        dt = 0.2 * o.hmin / f
+        _, _, E = o.rk4(u0, v0, dt, 20 * f, energy=True)
+        assert np.all(np.diff(E) <= 1e-13 * E[0])
+        drift.append(abs(E[-1] / E[0] - 1))
+    assert drift[1] < drift[0] / 16

@@ -1,0 +1,780 @@
+// wt2_capi.hip -- the extern "C" entry points of libwt2.so (declared in
+// include/wt2.h): context set-up (operators, mu, coercivity check, device
+// workspace) and the host-side sequencing of the pencil passes.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "wt2_curved.hpp"
+#include "wt2_internal.hpp"
+
+namespace wt2 {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+// ----------------------------------------------------------- GLL operators
+
+static void legendre(int n, double x, double* pn, double* pnm1)
+{
+    double p0 = 1.0, p1 = x;
+    if (n == 0) { *pn = 1.0; *pnm1 = 0.0; return; }
+    for (int m = 2; m <= n; ++m) {
+        const double p2 = ((2.0 * m - 1.0) * x * p1 - (m - 1.0) * p0) / m;
+        p0 = p1;
+        p1 = p2;
+    }
+    *pn = p1;
+    *pnm1 = p0;
+}
+
+// GLL nodes on [0,1] (Newton on the Lobatto polynomial), weights summing to 1,
+// differentiation matrix by the barycentric formula with negative-sum diagonal.
+static int gll(int N, double* xi, double* w, double* G)
+{
+    if (N < 2 || N > 16) return -1;
+    const int n = N - 1;
+    double x[16];
+    for (int k = 0; k < N; ++k) {
+        double xk = -std::cos(M_PI * k / n);
+        for (int it = 0; it < 100; ++it) {
+            double pn, pnm1;
+            legendre(n, xk, &pn, &pnm1);
+            const double dx = (xk * pn - pnm1) / ((n + 1.0) * pn);
+            xk -= dx;
+            if (std::fabs(dx) < 1e-16) break;
+        }
+        x[k] = xk;
+    }
+    x[0] = -1.0;
+    x[n] = 1.0;
+    for (int k = 0; k <= n / 2; ++k) {
+        const double s = 0.5 * (x[k] - x[n - k]);
+        x[k] = s;
+        x[n - k] = -s;
+    }
+    if (N % 2 == 1) x[n / 2] = 0.0;
+    for (int k = 0; k < N; ++k) {
+        double pn, pnm1;
+        legendre(n, x[k], &pn, &pnm1);
+        w[k] = 1.0 / (n * (n + 1.0) * pn * pn);
+        xi[k] = 0.5 * (x[k] + 1.0);
+    }
+    double lam[16];
+    for (int j = 0; j < N; ++j) {
+        double p = 1.0;
+        for (int k = 0; k < N; ++k)
+            if (k != j) p *= (xi[j] - xi[k]);
+        lam[j] = 1.0 / p;
+    }
+    for (int i = 0; i < N; ++i) {
+        double diag = 0.0;
+        for (int j = 0; j < N; ++j) {
+            if (i == j) continue;
+            const double v = (lam[j] / lam[i]) / (xi[i] - xi[j]);
+            G[i * N + j] = v;
+            diag -= v;
+        }
+        G[i * N + i] = diag;
+    }
+    return 0;
+}
+
+// 1D SAT-coupled derivative on a line of M elements (host, for mu).
+static void apply_D_1d(int N, int M, bool periodic, double h, const double* G, const double* w,
+                       const double* x, double* y)
+{
+    for (int e = 0; e < M; ++e) {
+        const double* xe = x + e * N;
+        double* ye = y + e * N;
+        for (int i = 0; i < N; ++i) {
+            double s = 0.0;
+            for (int p = 0; p < N; ++p) s += G[i * N + p] * xe[p];
+            ye[i] = s / h;
+        }
+        const bool hl = e > 0 || periodic, hr = e < M - 1 || periodic;
+        if (hl) {
+            const double xl = x[((e - 1 + M) % M) * N + N - 1];
+            ye[0] += -1.0 / (2.0 * w[0] * h) * (xl - xe[0]);
+        }
+        if (hr) {
+            const double xr = x[((e + 1) % M) * N];
+            ye[N - 1] += 1.0 / (2.0 * w[N - 1] * h) * (xr - xe[N - 1]);
+        }
+    }
+}
+
+// Largest eigenvalue of sym(H L) for the 1D SIPG operator (homogeneous
+// Dirichlet or periodic ends), in unit-element units, by bisection on the
+// inertia of sym(H L) - s I (LDL^T without pivoting on a symmetric matrix is
+// the Sylvester count).  Returns the bracketed max eigenvalue.
+static double sipg_max_sym_eig(int N, int M, bool periodic, double tau, const double* G,
+                               const double* w)
+{
+    const int L = M * N;
+    std::vector<double> A(size_t(L) * L, 0.0), x(L), y(L);
+    auto seg = [&](const double* u, const double* ul, const double* ur, double* out) {
+        double g[16];
+        for (int i = 0; i < N; ++i) {
+            double s = 0.0;
+            for (int p = 0; p < N; ++p) s += G[i * N + p] * u[p];
+            g[i] = s;
+        }
+        double duL, dgL, aL, duR, dgR, aR;
+        if (ul) {
+            double gn = 0.0;
+            for (int p = 0; p < N; ++p) gn += G[(N - 1) * N + p] * ul[p];
+            duL = u[0] - ul[N - 1]; dgL = g[0] - gn; aL = 0.5;
+        } else { duL = u[0]; dgL = 0.0; aL = 1.0; }
+        if (ur) {
+            double gn = 0.0;
+            for (int p = 0; p < N; ++p) gn += G[p] * ur[p];
+            duR = u[N - 1] - ur[0]; dgR = g[N - 1] - gn; aR = 0.5;
+        } else { duR = u[N - 1]; dgR = 0.0; aR = 1.0; }
+        for (int i = 0; i < N; ++i) {
+            double s = 0.0;
+            for (int p = 0; p < N; ++p) s += G[i * N + p] * g[p];
+            s += (aL * G[i] * (-duL) + aR * G[(N - 1) * N + i] * duR) / w[i];
+            if (i == 0) s += (0.5 * dgL - tau * duL) / w[0];
+            if (i == N - 1) s += (-0.5 * dgR - tau * duR) / w[N - 1];
+            out[i] = s;
+        }
+    };
+    for (int col = 0; col < L; ++col) {
+        std::fill(x.begin(), x.end(), 0.0);
+        x[col] = 1.0;
+        for (int e = 0; e < M; ++e) {
+            const double* ul = e > 0 ? &x[(e - 1) * N] : (periodic ? &x[(M - 1) * N] : nullptr);
+            const double* ur = e < M - 1 ? &x[(e + 1) * N] : (periodic ? &x[0] : nullptr);
+            seg(&x[e * N], ul, ur, &y[e * N]);
+        }
+        for (int row = 0; row < L; ++row) A[size_t(row) * L + col] = y[row] * w[row % N];
+    }
+    for (int r = 0; r < L; ++r)
+        for (int c = r + 1; c < L; ++c) {
+            const double s = 0.5 * (A[size_t(r) * L + c] + A[size_t(c) * L + r]);
+            A[size_t(r) * L + c] = A[size_t(c) * L + r] = s;
+        }
+    double bound = 0.0;
+    for (int r = 0; r < L; ++r) {
+        double s = 0.0;
+        for (int c = 0; c < L; ++c) s += std::fabs(A[size_t(r) * L + c]);
+        bound = std::max(bound, s);
+    }
+    // number of eigenvalues > s via LDL^T of (A - s I)
+    std::vector<double> B(size_t(L) * L);
+    auto count_above = [&](double s) {
+        B = A;
+        for (int r = 0; r < L; ++r) B[size_t(r) * L + r] -= s;
+        int pos = 0;
+        for (int k = 0; k < L; ++k) {
+            double d = B[size_t(k) * L + k];
+            if (d == 0.0) d = 1e-300;
+            if (d > 0) ++pos;
+            for (int i = k + 1; i < L; ++i) {
+                const double f = B[size_t(i) * L + k] / d;
+                if (f == 0.0) continue;
+                for (int j = k + 1; j <= i; ++j) B[size_t(i) * L + j] -= f * B[size_t(j) * L + k];
+            }
+        }
+        return pos;
+    };
+    double lo = -bound - 1.0, hi = bound + 1.0;
+    for (int it = 0; it < 60; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (count_above(mid) > 0) lo = mid; else hi = mid;
+        if (hi - lo < 1e-9 * (1.0 + std::fabs(hi))) break;
+    }
+    return hi;
+}
+
+}  // namespace wt2
+
+using namespace wt2;
+
+
+struct wt2_ctx {
+    wt2_mesh_desc d;
+    int N, Ne;
+    int64_t npts;
+    double xi[16], w[16], G[256];
+    double h[3];
+    double mu, inv_mu5, kos, max_sym_eig, h_min;
+    int curved;
+    int nb[3];
+    int device;
+    hipStream_t own_stream, stream;
+    // workspace: each block holds two fields of npts elements (FP64, or FP32
+    // in a mixed context, where a block is also one FP64 field of scratch)
+    void* Sb;       // stage input
+    void* Pb;       // partial sums
+    void* Ab;       // RK accumulator (FP64) / increment (FP32)
+    double* Y[2];   // device staging for host-pointer calls (lazy)
+    double* O[2];   // device staging for host-pointer outputs (lazy)
+    double* Lu;     // energy scratch
+    double* red;    // reduction partials + result
+    double* Edev;   // energy history (device)
+    int Ecap;
+    CurvedData cd;
+    int64_t bytes;
+    // captured RK4 step
+    hipGraphExec_t gexec;
+    bool gvalid;
+    double g_dt;
+    const void* g_u; const void* g_v;
+};
+
+#define HIPCHK(x)                                                                     \
+    do {                                                                              \
+        hipError_t _e = (x);                                                          \
+        if (_e != hipSuccess) {                                                       \
+            set_error(std::string("HIP error: ") + hipGetErrorString(_e) + " at " #x); \
+            return WT2_ERR_HIP;                                                       \
+        }                                                                             \
+    } while (0)
+
+static int alloc_dev(wt2_ctx* c, void** p, size_t bytes)
+{
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        set_error("device allocation of " + std::to_string(bytes) + " bytes failed");
+        return WT2_ERR_OOM;
+    }
+    c->bytes += int64_t(bytes);
+    return WT2_OK;
+}
+
+static bool is_mixed(const wt2_ctx* c) { return c->d.precision == WT2_MIXED; }
+static size_t elem_size(const wt2_ctx* c) { return is_mixed(c) ? sizeof(float) : sizeof(double); }
+static void* fld(const wt2_ctx* c, void* blk, int f)
+{
+    return static_cast<char*>(blk) + size_t(f) * size_t(c->npts) * elem_size(c);
+}
+
+static void fill_axis_ops(const wt2_ctx* c, int axis, const double bv[6], AxisOps& o)
+{
+    const int N = c->N;
+    const double h = c->h[axis];
+    for (int i = 0; i < N * N; ++i) {
+        o.G[i] = c->G[i];
+        o.Gk[i] = c->kos * c->G[i] / h;
+    }
+    for (int i = 0; i < N; ++i) o.winv[i] = 1.0 / c->w[i];
+    o.koL = -c->kos / (2.0 * c->w[0] * h);
+    o.koR = c->kos / (2.0 * c->w[N - 1] * h);
+    o.tau = c->d.tau;
+    o.ih2 = 1.0 / (h * h);
+    o.gL = bv[2 * axis];
+    o.gR = bv[2 * axis + 1];
+}
+
+// Default element planes per tile: the smallest divisor nb of N that gives a
+// workgroup of at least `want` wavefronts.
+static int default_nb(int N, int Ma, int want)
+{
+    const int lpw = 64 / Ma;
+    for (int nb = 1; nb <= N; ++nb) {
+        if (N % nb) continue;
+        const int waves = (N * nb + lpw - 1) / lpw;
+        if (waves >= want) return nb;
+    }
+    return N;
+}
+
+static PassDesc base_pass(const wt2_ctx* c, int axis, const double bv[6])
+{
+    PassDesc p;
+    std::memset(&p, 0, sizeof(p));
+    p.N = c->N;
+    p.axis = axis;
+    p.periodic = c->d.periodic[axis];
+    for (int a = 0; a < 3; ++a) p.M[a] = c->d.M[a];
+    p.nb = c->nb[axis];
+    fill_axis_ops(c, axis, bv, p.ops);
+    return p;
+}
+
+static int pass_rc(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return WT2_OK;
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return WT2_ERR_HIP;
+}
+
+// ------------------------------------------------------------ sequencing
+
+// Drop-in Laplacian out = L u (flat) or wgt (Ltilde u + A v) (curved).
+// fp64 forces FP64 arithmetic in a mixed context (energy).
+static int run_lap(wt2_ctx* c, const double* u, const double* v, double* out, const double bv[6],
+                   bool fp64, bool with_adv)
+{
+    const bool mixed = is_mixed(c) && !fp64;
+    if (c->curved) {
+        CurvedCall cc;
+        std::memset(&cc, 0, sizeof(cc));
+        cc.u = u; cc.v = with_adv ? v : nullptr; cc.out = out;
+        cc.mixed = mixed; cc.weighted = 1;
+        for (int f = 0; f < 6; ++f) cc.bv[f] = bv[f];
+        return pass_rc(curved_operator(c->cd, cc, c->stream), "curved operator");
+    }
+    for (int axis = 0; axis < 3; ++axis) {
+        PassDesc p = base_pass(c, axis, bv);
+        p.variant = mixed ? VAR_MIX_IN64 : VAR_F64;
+        p.lap = 1;
+        p.Su = u;
+        p.P = c->Pb;   // FP64 scratch in either precision (one block = one FP64 field)
+        p.epi = axis == 0 ? EPI_FIRST : (axis == 1 ? EPI_MID : EPI_OUT_LAP);
+        p.Ou = out;
+        int rc = pass_rc(launch_flat_pass(p, c->stream), "flat lap pass");
+        if (rc) return rc;
+    }
+    return WT2_OK;
+}
+
+// One full RHS evaluation of u' = u_t, u_t' = lap(u, u_t) + KO(u_t).
+// stage 0: write (du, dv) to (Ou, Ov); stage 1..4: classical RK4 stage on the
+// context's registers with state Y (stage 1 reads Y, later stages S).
+static int run_stage(wt2_ctx* c, int stage, const double* u, const double* v, double* Ou,
+                     double* Ov, double* Yu, double* Yv, double dt)
+{
+    const bool mixed = is_mixed(c);
+    const bool ko = c->d.eps_ko > 0 && c->kos > 0;
+    const double* bv = c->d.bdry_values;
+    const void* Su = stage <= 1 ? static_cast<const void*>(stage == 0 ? u : Yu) : fld(c, c->Sb, 0);
+    const void* Sv = stage <= 1 ? static_cast<const void*>(stage == 0 ? v : Yv) : fld(c, c->Sb, 1);
+    const int variant = !mixed ? VAR_F64 : (stage <= 1 ? VAR_MIX_IN64 : VAR_MIX_IN32);
+    const int final_epi = stage == 0 ? EPI_OUT_RHS : EPI_RK1 + (stage - 1);
+    void* P = c->Pb;
+    bool have_p = false;
+    if (c->curved) {
+        CurvedCall cc;
+        std::memset(&cc, 0, sizeof(cc));
+        cc.u = Su; cc.v = Sv; cc.in32 = (variant == VAR_MIX_IN32);
+        cc.Pv = P;
+        cc.mixed = mixed; cc.weighted = 1;
+        for (int f = 0; f < 6; ++f) cc.bv[f] = bv[f];
+        int rc = pass_rc(curved_operator(c->cd, cc, c->stream), "curved operator");
+        if (rc) return rc;
+        have_p = true;
+    }
+    for (int axis = 0; axis < 3; ++axis) {
+        const bool last = axis == 2;
+        const bool lap = !c->curved;
+        if (!last && !lap && !ko) continue;   // nothing to add along this axis
+        PassDesc p = base_pass(c, axis, bv);
+        p.variant = variant;
+        p.lap = lap;
+        p.ko = ko;
+        p.needv = ko || last;
+        p.keepv = last;
+        p.Su = Su; p.Sv = Sv;
+        p.P = P;
+        p.epi = last ? final_epi : (have_p ? EPI_MID : EPI_FIRST);
+        p.So_u = fld(c, c->Sb, 0); p.So_v = fld(c, c->Sb, 1);
+        p.Yu = Yu; p.Yv = Yv;
+        p.Au = fld(c, c->Ab, 0); p.Av = fld(c, c->Ab, 1);
+        p.Ou = Ou; p.Ov = Ov;
+        p.dt = dt;
+        int rc = pass_rc(launch_flat_pass(p, c->stream), "flat pass");
+        if (rc) return rc;
+        have_p = true;
+    }
+    return WT2_OK;
+}
+
+static int run_energy(wt2_ctx* c, const double* u, const double* v, double* Edst)
+{
+    static const double zero6[6] = {0, 0, 0, 0, 0, 0};
+    if (c->curved) {
+        CurvedCall cc;
+        std::memset(&cc, 0, sizeof(cc));
+        cc.u = u; cc.v = nullptr; cc.out = c->Lu;
+        cc.mixed = false; cc.weighted = 0;
+        int rc = pass_rc(curved_operator(c->cd, cc, c->stream), "curved operator (energy)");
+        if (rc) return rc;
+    } else {
+        int rc = run_lap(c, u, nullptr, c->Lu, zero6, true, false);
+        if (rc) return rc;
+    }
+    EnergyDesc e;
+    std::memset(&e, 0, sizeof(e));
+    e.N = c->N; e.Ne = c->Ne;
+    e.u = u; e.v = v; e.Lu = c->Lu;
+    e.wu = c->curved ? c->cd.wu : nullptr;
+    e.wv = c->curved ? c->cd.wv : nullptr;
+    e.hvol = c->h[0] * c->h[1] * c->h[2];
+    for (int i = 0; i < c->N; ++i) e.w[i] = c->w[i];
+    e.partial = c->red;
+    e.out = Edst;
+    return pass_rc(launch_energy(e, c->stream), "energy");
+}
+
+namespace {
+int stage_in(wt2_ctx* c, int nf, const double* const* src, double** dst, bool host)
+{
+    const size_t bytes = size_t(c->npts) * sizeof(double);
+    for (int f = 0; f < nf; ++f) {
+        if (!src[f]) { dst[f] = nullptr; continue; }
+        if (!host) { dst[f] = const_cast<double*>(src[f]); continue; }
+        if (!c->Y[f]) {
+            int rc = alloc_dev(c, reinterpret_cast<void**>(&c->Y[f]), bytes);
+            if (rc) return rc;
+        }
+        HIPCHK(hipMemcpyAsync(c->Y[f], src[f], bytes, hipMemcpyHostToDevice, c->stream));
+        dst[f] = c->Y[f];
+    }
+    return WT2_OK;
+}
+
+int out_buf(wt2_ctx* c, int nf, double* const* user, double** dst, bool host)
+{
+    const size_t bytes = size_t(c->npts) * sizeof(double);
+    for (int f = 0; f < nf; ++f) {
+        if (!host) { dst[f] = user[f]; continue; }
+        if (!c->O[f]) {
+            int rc = alloc_dev(c, reinterpret_cast<void**>(&c->O[f]), bytes);
+            if (rc) return rc;
+        }
+        dst[f] = c->O[f];
+    }
+    return WT2_OK;
+}
+
+int copy_out(wt2_ctx* c, int nf, double* const* user, double* const* dev)
+{
+    const size_t bytes = size_t(c->npts) * sizeof(double);
+    for (int f = 0; f < nf; ++f)
+        HIPCHK(hipMemcpyAsync(user[f], dev[f], bytes, hipMemcpyDeviceToHost, c->stream));
+    return WT2_OK;
+}
+
+int sync(wt2_ctx* c)
+{
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return WT2_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char* wt2_last_error(void) { return g_err.c_str(); }
+const char* wt2_version(void) { return "wt2 1.0 (gfx950)"; }
+
+int wt2_gll(int32_t N, double* xi, double* w, double* G)
+{
+    if (!xi || !w || !G) { set_error("wt2_gll: null pointer"); return WT2_ERR_ARG; }
+    if (N < NMIN || N > NMAX) { set_error("wt2_gll: N out of range 3..10"); return WT2_ERR_SHAPE; }
+    gll(N, xi, w, G);
+    return WT2_OK;
+}
+
+int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
+{
+    if (!desc || !out) { set_error("wt2_ctx_create: null pointer"); return WT2_ERR_ARG; }
+    *out = nullptr;
+    if (desc->N < NMIN || desc->N > NMAX) {
+        set_error("wt2_ctx_create: N must be in 3..10");
+        return WT2_ERR_SHAPE;
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (desc->M[a] < 1) { set_error("wt2_ctx_create: M per axis must be >= 1"); return WT2_ERR_SHAPE; }
+        if (desc->M[a] > MMAX_AXIS) {
+            set_error("wt2_ctx_create: more than 64 elements along one axis is not supported");
+            return WT2_ERR_UNSUPPORTED;
+        }
+        if (!(desc->x1[a] > desc->x0[a])) { set_error("wt2_ctx_create: x1 <= x0"); return WT2_ERR_ARG; }
+        if (desc->periodic[a] && desc->M[a] < 2) {
+            set_error("wt2_ctx_create: periodic axis needs M >= 2");
+            return WT2_ERR_SHAPE;
+        }
+    }
+    if (desc->precision != WT2_FP64 && desc->precision != WT2_MIXED) {
+        set_error("wt2_ctx_create: precision must be WT2_FP64 or WT2_MIXED");
+        return WT2_ERR_ARG;
+    }
+    if (!(desc->tau > 0) || !(desc->eps_ko >= 0)) {
+        set_error("wt2_ctx_create: tau must be > 0 and eps_ko >= 0");
+        return WT2_ERR_ARG;
+    }
+    wt2_ctx* c = new wt2_ctx();
+    c->d = *desc;
+    c->N = desc->N;
+    c->Ne = desc->M[0] * desc->M[1] * desc->M[2];
+    c->npts = int64_t(c->N) * c->N * c->N * c->Ne;
+    gll(c->N, c->xi, c->w, c->G);
+    for (int a = 0; a < 3; ++a) c->h[a] = (desc->x1[a] - desc->x0[a]) / desc->M[a];
+    c->curved = desc->warp_A != 0.0;
+    const int N = c->N;
+
+    // Coercivity of the 1D SIPG operator: the flat operator is a Kronecker sum
+    // of these, so its H-symmetric part is NSD iff each axis's is.
+    c->max_sym_eig = -1e300;
+    for (int a = 0; a < 3; ++a) {
+        const int Mt = std::min(desc->M[a], 8);
+        const double ev = sipg_max_sym_eig(N, Mt, desc->periodic[a] != 0, desc->tau, c->G, c->w);
+        c->max_sym_eig = std::max(c->max_sym_eig, ev);
+    }
+    if (c->max_sym_eig > 1e-9 * desc->tau && !(desc->flags & WT2_ALLOW_NONCOERCIVE)) {
+        char buf[320];
+        std::snprintf(buf, sizeof(buf),
+                      "wt2_ctx_create: tau=%g is below SIPG coercivity for N=%d (max eig of sym(HL) = %g > 0); "
+                      "use tau >= 1.5*(N-1)^2 or pass WT2_ALLOW_NONCOERCIVE",
+                      desc->tau, N, c->max_sym_eig);
+        set_error(buf);
+        delete c;
+        return WT2_ERR_TAU;
+    }
+
+    // mu: 30 power iterations on D^2 along axis 1 from the alternating start
+    // vector (src/wave3d_curved.jl:158-178).  D_1 acts line by line and every
+    // x-line of that start vector is +-p, p(i,ex) = (-1)^(i+ex) (0-based:
+    // +1 when i+ex is even), so after the first normalisation the iteration
+    // is the 1D one on p.
+    {
+        const int M = desc->M[0], L = M * N;
+        std::vector<double> v(L), wv(L);
+        for (int ex = 0; ex < M; ++ex)
+            for (int i = 0; i < N; ++i) v[ex * N + i] = ((i + ex) % 2 == 0) ? 1.0 : -1.0;
+        double mu2 = 0.0;
+        for (int it = 0; it < 30; ++it) {
+            apply_D_1d(N, M, desc->periodic[0] != 0, c->h[0], c->G, c->w, v.data(), wv.data());
+            apply_D_1d(N, M, desc->periodic[0] != 0, c->h[0], c->G, c->w, wv.data(), v.data());
+            double s = 0.0;
+            for (int q = 0; q < L; ++q) s += v[q] * v[q];
+            mu2 = std::sqrt(s);
+            if (!(mu2 > 0)) break;
+            for (int q = 0; q < L; ++q) v[q] /= mu2;
+        }
+        c->mu = std::sqrt(mu2);
+        c->inv_mu5 = c->mu > 0 ? std::pow(1.0 / c->mu, 5) : 0.0;
+        const double koT = desc->eps_ko * c->inv_mu5;
+        c->kos = koT > 0 ? std::pow(koT, 1.0 / 6.0) : 0.0;
+    }
+    {
+        double gap = 1e300;
+        for (int i = 1; i < N; ++i) gap = std::min(gap, c->xi[i] - c->xi[i - 1]);
+        c->h_min = gap * std::min(c->h[0], std::min(c->h[1], c->h[2]));
+    }
+    for (int a = 0; a < 3; ++a) {
+        c->nb[a] = default_nb(N, desc->M[a], a == 2 ? 8 : 4);
+        const char* env = std::getenv("WT2_TILE_NB");
+        if (env) {
+            int t[3];
+            if (std::sscanf(env, "%d,%d,%d", &t[0], &t[1], &t[2]) == 3 && t[a] >= 1 && N % t[a] == 0)
+                c->nb[a] = t[a];
+        }
+    }
+
+    if (hipGetDevice(&c->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("wt2_ctx_create: no HIP device / stream");
+        delete c;
+        return WT2_ERR_HIP;
+    }
+    c->stream = c->own_stream;
+    const size_t n = size_t(c->npts);
+    const size_t blk = 2 * n * elem_size(c);
+    int rc = alloc_dev(c, &c->Sb, blk);
+    if (rc == WT2_OK) rc = alloc_dev(c, &c->Pb, blk);
+    if (rc == WT2_OK) rc = alloc_dev(c, &c->Ab, blk);
+    if (rc == WT2_OK) rc = alloc_dev(c, reinterpret_cast<void**>(&c->Lu), n * sizeof(double));
+    if (rc == WT2_OK) rc = alloc_dev(c, reinterpret_cast<void**>(&c->red), (RED_BLOCKS + 8) * sizeof(double));
+    if (rc == WT2_OK && c->curved)
+        rc = curved_init(c->cd, *desc, c->xi, c->w, c->G, nullptr, c->stream, &c->bytes);
+    if (rc != WT2_OK) {
+        wt2_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return WT2_OK;
+}
+
+void wt2_ctx_destroy(wt2_ctx* c)
+{
+    if (!c) return;
+    if (c->gvalid) hipGraphExecDestroy(c->gexec);
+    hipFree(c->Sb); hipFree(c->Pb); hipFree(c->Ab);
+    for (int f = 0; f < 2; ++f) { hipFree(c->Y[f]); hipFree(c->O[f]); }
+    hipFree(c->Lu); hipFree(c->red); hipFree(c->Edev);
+    curved_free(c->cd);
+    if (c->own_stream) hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+int wt2_ctx_info(const wt2_ctx* c, wt2_info* info)
+{
+    if (!c || !info) { set_error("wt2_ctx_info: null pointer"); return WT2_ERR_ARG; }
+    info->npoints = c->npts;
+    info->Ne = c->Ne;
+    info->curved = c->curved;
+    info->mu = c->mu;
+    info->h_min = c->curved ? c->cd.h_min : c->h_min;
+    info->tau = c->d.tau;
+    info->max_sym_eig = c->max_sym_eig;
+    info->device_bytes = c->bytes;
+    return WT2_OK;
+}
+
+int wt2_set_stream(wt2_ctx* c, void* stream)
+{
+    if (!c) { set_error("wt2_set_stream: null ctx"); return WT2_ERR_ARG; }
+    c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+    if (c->gvalid) { hipGraphExecDestroy(c->gexec); c->gvalid = false; }
+    return WT2_OK;
+}
+
+int wt2_coords(const wt2_ctx* c, double* xyz)
+{
+    if (!c || !xyz) { set_error("wt2_coords: null pointer"); return WT2_ERR_ARG; }
+    mesh_coords(c->d, c->xi, xyz);
+    return WT2_OK;
+}
+
+int wt2_ctx_set_background(wt2_ctx* c, const double* bg10)
+{
+    if (!c || !bg10) { set_error("wt2_ctx_set_background: null pointer"); return WT2_ERR_ARG; }
+    if (c->gvalid) { hipGraphExecDestroy(c->gexec); c->gvalid = false; }
+    int rc;
+    if (!c->curved) {
+        rc = curved_init(c->cd, c->d, c->xi, c->w, c->G, bg10, c->stream, &c->bytes);
+        if (rc == WT2_OK) c->curved = 1;
+    } else {
+        rc = curved_set_background(c->cd, c->d, c->xi, c->w, c->G, bg10, c->stream);
+    }
+    return rc;
+}
+
+int wt2_rhs3d(wt2_ctx* c, double* udd, const double* u, const double* ud,
+              const double bdry_values[6], int32_t flags)
+{
+    if (!c || !udd || !u || !bdry_values) { set_error("wt2_rhs3d: null pointer"); return WT2_ERR_ARG; }
+    const bool host = flags & WT2_HOST_PTRS;
+    const double* src[2] = {u, ud};
+    double* din[2];
+    double* dout[1];
+    double* user[1] = {udd};
+    int rc = stage_in(c, 2, src, din, host);
+    if (!rc) rc = out_buf(c, 1, user, dout, host);
+    if (!rc) rc = run_lap(c, din[0], din[1], dout[0], bdry_values, false, ud != nullptr);
+    if (!rc && host) rc = copy_out(c, 1, user, dout);
+    if (!rc) rc = sync(c);
+    return rc;
+}
+
+int wt2_rhs_full(wt2_ctx* c, double* du, double* dud, const double* u, const double* ud,
+                 int32_t flags)
+{
+    if (!c || !du || !dud || !u || !ud) { set_error("wt2_rhs_full: null pointer"); return WT2_ERR_ARG; }
+    const bool host = flags & WT2_HOST_PTRS;
+    const double* src[2] = {u, ud};
+    double* din[2];
+    double* dout[2];
+    double* user[2] = {du, dud};
+    int rc = stage_in(c, 2, src, din, host);
+    if (!rc) rc = out_buf(c, 2, user, dout, host);
+    if (!rc) rc = run_stage(c, 0, din[0], din[1], dout[0], dout[1], nullptr, nullptr, 0.0);
+    if (!rc && host) rc = copy_out(c, 2, user, dout);
+    if (!rc) rc = sync(c);
+    return rc;
+}
+
+static int rk4_step_launches(wt2_ctx* c, double* Yu, double* Yv, double dt)
+{
+    for (int s = 1; s <= 4; ++s) {
+        int rc = run_stage(c, s, nullptr, nullptr, nullptr, nullptr, Yu, Yv, dt);
+        if (rc) return rc;
+    }
+    return WT2_OK;
+}
+
+int wt2_rk4_steps(wt2_ctx* c, double* u, double* ud, double dt, int32_t nsteps,
+                  int32_t energy_every, double* energy_out, int32_t flags)
+{
+    if (!c || !u || !ud) { set_error("wt2_rk4_steps: null pointer"); return WT2_ERR_ARG; }
+    if (nsteps < 0 || !(dt > 0) || !std::isfinite(dt)) { set_error("wt2_rk4_steps: bad dt / nsteps"); return WT2_ERR_ARG; }
+    const bool host = flags & WT2_HOST_PTRS;
+    const bool want_e = energy_out && energy_every > 0;
+    const double* src[2] = {u, ud};
+    double* Y[2];
+    int rc = stage_in(c, 2, src, Y, host);
+    if (rc) return rc;
+    const int ne = want_e ? 1 + nsteps / energy_every : 0;
+    if (want_e && ne > c->Ecap) {
+        hipFree(c->Edev);
+        c->Edev = nullptr;
+        rc = alloc_dev(c, reinterpret_cast<void**>(&c->Edev), size_t(ne) * sizeof(double));
+        if (rc) return rc;
+        c->Ecap = ne;
+    }
+    if (want_e) {
+        rc = run_energy(c, Y[0], Y[1], c->Edev);
+        if (rc) return rc;
+    }
+    const bool use_graph = !(flags & WT2_NO_GRAPH);
+    for (int st = 0; st < nsteps; ++st) {
+        if (use_graph) {
+            if (!(c->gvalid && c->g_dt == dt && c->g_u == Y[0] && c->g_v == Y[1])) {
+                if (c->gvalid) { hipGraphExecDestroy(c->gexec); c->gvalid = false; }
+                // first step of a new configuration runs eagerly (sets kernel
+                // attributes), then the step is captured for the rest
+                rc = rk4_step_launches(c, Y[0], Y[1], dt);
+                if (rc) return rc;
+                hipGraph_t g;
+                HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                rc = rk4_step_launches(c, Y[0], Y[1], dt);
+                hipError_t ce = hipStreamEndCapture(c->stream, &g);
+                if (rc) return rc;
+                if (ce != hipSuccess) return pass_rc(ce, "graph capture");
+                hipError_t ie = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                if (ie != hipSuccess) return pass_rc(ie, "graph instantiate");
+                c->gvalid = true;
+                c->g_dt = dt; c->g_u = Y[0]; c->g_v = Y[1];
+            } else {
+                HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+            }
+        } else {
+            rc = rk4_step_launches(c, Y[0], Y[1], dt);
+            if (rc) return rc;
+        }
+        if (want_e && (st + 1) % energy_every == 0) {
+            rc = run_energy(c, Y[0], Y[1], c->Edev + (st + 1) / energy_every);
+            if (rc) return rc;
+        }
+    }
+    if (host) {
+        double* user[2] = {u, ud};
+        rc = copy_out(c, 2, user, Y);
+        if (rc) return rc;
+    }
+    if (want_e) HIPCHK(hipMemcpyAsync(energy_out, c->Edev, size_t(ne) * sizeof(double),
+                                      hipMemcpyDeviceToHost, c->stream));
+    rc = sync(c);
+    if (rc) return rc;
+    if (want_e) {
+        for (int q = 0; q < ne; ++q)
+            if (!std::isfinite(energy_out[q])) {
+                set_error("wt2_rk4_steps: non-finite energy (the evolution blew up)");
+                return WT2_ERR_NONFINITE;
+            }
+    }
+    return WT2_OK;
+}
+
+int wt2_energy(wt2_ctx* c, const double* u, const double* ud, double* E, int32_t flags)
+{
+    if (!c || !u || !ud || !E) { set_error("wt2_energy: null pointer"); return WT2_ERR_ARG; }
+    const bool host = flags & WT2_HOST_PTRS;
+    const double* src[2] = {u, ud};
+    double* din[2];
+    int rc = stage_in(c, 2, src, din, host);
+    if (!rc) rc = run_energy(c, din[0], din[1], c->red + RED_BLOCKS);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(E, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+}  // extern "C"

@@ -1,0 +1,52 @@
+// wt2_curved.hpp -- curved (warped mesh and/or ADM background) operator of
+// libwt2: static FP64 geometry/background coefficients and the element
+// kernels that apply the variable-coefficient SIPG operator plus the u_t
+// advection terms.  KO on curved meshes reuses the flat pencil passes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/wt2.h"
+
+namespace wt2 {
+
+struct CurvedData {
+    int N = 0;
+    int M[3] = {0, 0, 0};
+    int periodic[3] = {0, 0, 0};
+    int64_t Ne = 0, npts = 0;
+    double tau = 0.0;
+    double w[16] = {0};
+    double G[256] = {0};
+    double h_min = 0.0;
+    double* coef64 = nullptr;   // 13 per node, SoA: coef64[q*npts + n]
+    float* coef32 = nullptr;    // same, FP32 copy (mixed precision)
+    double* wu = nullptr;       // energy weight of u*Lu: H3 (per node)
+    double* wv = nullptr;       // energy weight of v^2: H3 / wgt
+    double* fbuf = nullptr;     // face flux buffer: 6 N^2 per element
+};
+
+struct CurvedCall {
+    const void* u;      // stage u (double, or float when in32)
+    const void* v;      // stage u_t; nullptr: no advection term
+    int in32;           // stage input is FP32
+    int mixed;          // FP32 arithmetic
+    int weighted;       // 1: multiply by wgt (RHS); 0: unweighted Ltilde u (energy)
+    double bv[6];       // Dirichlet values per tag
+    double* out;        // FP64 output (drop-in / energy), or nullptr
+    void* Pv;           // partial sum of the u_t equation (FP64 or FP32), written when out == nullptr
+};
+
+int curved_init(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const double* w,
+                const double* G, const double* bg10, hipStream_t s, int64_t* bytes);
+int curved_set_background(CurvedData& cd, const wt2_mesh_desc& d, const double* xi,
+                          const double* w, const double* G, const double* bg10, hipStream_t s);
+void curved_free(CurvedData& cd);
+hipError_t curved_operator(const CurvedData& cd, const CurvedCall& cc, hipStream_t s);
+
+// Node coordinates (3,N,N,N,Ne) of the (possibly warped) uniform hex mesh.
+void mesh_coords(const wt2_mesh_desc& d, const double* xi, double* xyz);
+
+}  // namespace wt2

@@ -1,0 +1,449 @@
+// wt2_flat.hip -- pencil-pass kernels for the affine (flat) SBP-SAT operator.
+//
+// On an axis-aligned affine mesh the SIPG Laplacian is a Kronecker sum of 1D
+// operators (each face term involves only its normal axis; the tangential H
+// cancels, README.md:57-81) and the Kreiss-Oliger term eps mu^-5 D_d^6 is a 1D
+// operator per axis too (apply_D! along d, src/wave3d_curved.jl:182-192; here
+// applied to u_t, with outer faces closed against a zero ghost value).  So
+// one RHS evaluation is three passes, one per axis; each workgroup owns whole
+// element lines along its axis (every element of the mesh along that axis), so
+// the six SAT-coupled D applications of KO -- whose reach is six elements --
+// and the SIPG neighbour traces are all inside the workgroup.
+//
+// One pass:
+//   1. coalesced copy of the tile (element chunks of contiguous memory) from
+//      HBM into LDS;
+//   2. one work-item per (line, element): its N-node segments go to registers;
+//      the whole 1D operator runs in registers; the only neighbour data (face
+//      values of the adjacent elements on the same line) come from wavefront
+//      shuffles, because a line's elements sit in adjacent lanes;
+//   3. the axis contribution c = L_axis u + KO_axis u_t goes back to LDS and a
+//      coalesced epilogue adds it to the partial sum of the u_t equation, or
+//      (last axis) finishes the RHS (u_t' = P + c, u' = u_t) and applies the
+//      RK4 stage update in the same pass.
+#include "wt2_internal.hpp"
+
+#include <type_traits>
+
+namespace wt2 {
+
+template <int N, typename TC>
+struct AxisConst {
+    TC G[N * N];
+    TC Gk[N * N];
+    TC winv[N];
+    TC koL, koR, tau, ih2, gL, gR;
+};
+
+template <int N, typename TC>
+struct KArgs {
+    AxisConst<N, TC> k;
+    const void* Su; const void* Sv;
+    void* P;
+    void* So_u; void* So_v;
+    double* Yu; double* Yv;
+    void* Au; void* Av;
+    double* Ou; double* Ov;
+    double dt;
+    int M0, M1, M2;
+    int axis, periodic, nb, epi;
+    int lap, ko, needv, keepv;
+    int lpw, CS, ES;
+};
+
+template <typename T>
+__device__ __forceinline__ T shfl64(T v, int src)
+{
+    return __shfl(v, src, 64);
+}
+
+// x <- (kos D)^6 x on this lane's element segment; D = (1/h) G plus the
+// centred-flux SAT (apply_D!, METHODS.md:256-264); an outer face takes the same
+// SAT against a zero ghost value, which keeps H D skew on a bounded line.
+template <int N, typename TC>
+__device__ __forceinline__ void ko_chain(TC (&x)[N], const AxisConst<N, TC>& k, int srcL,
+                                         int srcR, bool hasL, bool hasR)
+{
+#pragma unroll
+    for (int rep = 0; rep < 6; ++rep) {
+        TC xl = shfl64(x[N - 1], srcL);
+        TC xr = shfl64(x[0], srcR);
+        if (!hasL) xl = TC(0);
+        if (!hasR) xr = TC(0);
+        TC y[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            TC s = TC(0);
+#pragma unroll
+            for (int p = 0; p < N; ++p) s = fma(k.Gk[i * N + p], x[p], s);
+            y[i] = s;
+        }
+        y[0] = fma(k.koL, xl - x[0], y[0]);
+        y[N - 1] = fma(k.koR, xr - x[N - 1], y[N - 1]);
+#pragma unroll
+        for (int i = 0; i < N; ++i) x[i] = y[i];
+    }
+}
+
+// out += 1D SIPG Laplacian of u along this axis (README.md:57-81):
+//   (1/h^2){ G G u + H^-1 [ alpha G^T B du - 1/2 B d(Gu) - tau |B| du ] }.
+template <int N, typename TC>
+__device__ __forceinline__ void sipg_add(const TC (&u)[N], TC (&out)[N],
+                                         const AxisConst<N, TC>& k, int srcL, int srcR,
+                                         bool hasL, bool hasR)
+{
+    TC g[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        TC s = TC(0);
+#pragma unroll
+        for (int p = 0; p < N; ++p) s = fma(k.G[i * N + p], u[p], s);
+        g[i] = s;
+    }
+    TC uL = shfl64(u[N - 1], srcL);
+    TC gnL = shfl64(g[N - 1], srcL);
+    TC uR = shfl64(u[0], srcR);
+    TC gnR = shfl64(g[0], srcR);
+    TC duL, dgL, aL, duR, dgR, aR;
+    if (hasL) { duL = u[0] - uL; dgL = g[0] - gnL; aL = TC(0.5); }
+    else      { duL = u[0] - k.gL; dgL = TC(0); aL = TC(1); }
+    if (hasR) { duR = u[N - 1] - uR; dgR = g[N - 1] - gnR; aR = TC(0.5); }
+    else      { duR = u[N - 1] - k.gR; dgR = TC(0); aR = TC(1); }
+    const TC tL = -aL * duL, tR = aR * duR;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        TC s = TC(0);
+#pragma unroll
+        for (int p = 0; p < N; ++p) s = fma(k.G[i * N + p], g[p], s);
+        TC f = k.G[i] * tL + k.G[(N - 1) * N + i] * tR;
+        if (i == 0) f += TC(0.5) * dgL - k.tau * duL;
+        if (i == N - 1) f += TC(-0.5) * dgR - k.tau * duR;
+        s = fma(k.winv[i], f, s);
+        out[i] = fma(s, k.ih2, out[i]);
+    }
+}
+
+template <int N, typename TC, typename TS, typename TP>
+__global__ void __launch_bounds__(1024) flat_pass_kernel(KArgs<N, TC> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    TS* lds = reinterpret_cast<TS*>(smem_raw);
+    constexpr int N3 = N * N * N;
+    constexpr bool MIXED = std::is_same<TP, float>::value;
+    const int axis = a.axis;
+    const int Mx = a.M0, My = a.M1, Mz = a.M2;
+    const int nb = a.nb, nbk = N / nb, LT = N * nb;
+
+    // ---- tile decode: two fixed element coordinates and a line block
+    int t = blockIdx.x;
+    const int blk = t % nbk;
+    t /= nbk;
+    int ebase, estride, Ma;
+    if (axis == 0) { const int ey = t % My, ez = t / My; ebase = Mx * (ey + My * ez); estride = 1; Ma = Mx; }
+    else if (axis == 1) { const int ex = t % Mx, ez = t / Mx; ebase = ex + Mx * My * ez; estride = Mx; Ma = My; }
+    else { const int ex = t % Mx, ey = t / Mx; ebase = ex + Mx * ey; estride = Mx * My; Ma = Mz; }
+    const int nch = axis < 2 ? Ma : Ma * N;
+    const int clen = axis < 2 ? N * N * nb : N * nb;
+    const int fstride = axis < 2 ? Ma * a.CS : Ma * a.ES;
+    const int coff = axis < 2 ? blk * nb * N * N : blk * nb * N;
+
+    // LDS slots: u (SIPG input), v (KO input / u' of the final pass), c (result)
+    const bool lap = a.lap != 0, needv = a.needv != 0, keepv = a.keepv != 0;
+    const int su = 0;
+    const int sv = lap ? 1 : 0;
+    const int sc = lap ? 0 : (keepv ? 1 : 0);
+
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    // copy-loop mapping: (chunk, offset) pairs; chunks of clen contiguous values
+    int o, c0, cstep, ostep;
+    if (clen <= nthr) { o = tid % clen; c0 = tid / clen; cstep = nthr / clen; ostep = clen; }
+    else { o = tid; c0 = 0; cstep = 1; ostep = nthr; }
+
+    auto goff = [&](int c, int oo) -> int64_t {
+        if (axis < 2) return int64_t(ebase + c * estride) * N3 + coff + oo;
+        const int m = c / N, kk = c - m * N;
+        return int64_t(ebase + m * estride) * N3 + kk * N * N + coff + oo;
+    };
+    auto loff = [&](int c, int oo) -> int {
+        if (axis < 2) return c * a.CS + oo;
+        const int m = c / N, kk = c - m * N;
+        return m * a.ES + kk * a.CS + oo;
+    };
+
+    // ---- 1. coalesced tile load
+    {
+        const TS* gu = static_cast<const TS*>(a.Su);
+        const TS* gv = static_cast<const TS*>(a.Sv);
+        if (c0 < cstep) {
+            for (int c = c0; c < nch; c += cstep) {
+                for (int oo = o; oo < clen; oo += ostep) {
+                    const int64_t g = goff(c, oo);
+                    const int l = loff(c, oo);
+                    if (lap) lds[su * fstride + l] = gu[g];
+                    if (needv) lds[sv * fstride + l] = gv[g];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- 2. per-lane segment: lane = element * lpw + line-in-wave
+    const int wave = tid >> 6, lane = tid & 63;
+    const int lpw = a.lpw;
+    const int m = lane / lpw, lw = lane - m * lpw;
+    const int line = wave * lpw + lw;
+    const bool active = (m < Ma) && (line < LT);
+    const bool per = a.periodic != 0;
+    const bool hasL = active && (m > 0 || per);
+    const bool hasR = active && (m < Ma - 1 || per);
+    const int srcL = hasL ? ((m > 0 ? m - 1 : Ma - 1) * lpw + lw) : lane;
+    const int srcR = hasR ? ((m < Ma - 1 ? m + 1 : 0) * lpw + lw) : lane;
+    int SE, SR, lo;
+    if (axis == 0) { SE = a.CS; SR = 1; lo = line * N; }
+    else if (axis == 1) { SE = a.CS; SR = N; lo = (line / N) * N * N + (line % N); }
+    else { SE = a.ES; SR = a.CS; lo = line; }
+    const int base = active ? (m * SE + lo) : 0;
+
+    TC cv[N];
+#pragma unroll
+    for (int r = 0; r < N; ++r) cv[r] = TC(0);
+    if (a.ko) {
+        TC x[N];
+#pragma unroll
+        for (int r = 0; r < N; ++r) x[r] = TC(lds[sv * fstride + base + r * SR]);
+        ko_chain<N, TC>(x, a.k, srcL, srcR, hasL, hasR);
+#pragma unroll
+        for (int r = 0; r < N; ++r) cv[r] = x[r];
+    }
+    if (lap) {
+        TC u[N];
+#pragma unroll
+        for (int r = 0; r < N; ++r) u[r] = TC(lds[su * fstride + base + r * SR]);
+        sipg_add<N, TC>(u, cv, a.k, srcL, srcR, hasL, hasR);
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int r = 0; r < N; ++r) lds[sc * fstride + base + r * SR] = TS(cv[r]);
+    }
+    __syncthreads();
+
+    // ---- 3. coalesced epilogue
+    if (c0 >= cstep) return;
+    TP* P = static_cast<TP*>(a.P);
+    const double dt = a.dt;
+    const int epi = a.epi;
+    for (int c = c0; c < nch; c += cstep)
+    for (int oo = o; oo < clen; oo += ostep) {
+        const int64_t g = goff(c, oo);
+        const int l = loff(c, oo);
+        const TC x = TC(lds[sc * fstride + l]);
+        if (epi == EPI_FIRST) { P[g] = TP(x); continue; }
+        if (epi == EPI_MID) { P[g] = TP(TC(P[g]) + x); continue; }
+        if (epi == EPI_OUT_LAP) { a.Ou[g] = double(TC(P[g]) + x); continue; }
+        // u' = u_t (the stage's own u_t, exactly as held), u_t' = P + c
+        const double ku = double(lds[sv * fstride + l]);
+        const double kv = double(TC(P[g]) + x);
+        if (epi == EPI_OUT_RHS) { a.Ou[g] = ku; a.Ov[g] = kv; continue; }
+        TP* sou = static_cast<TP*>(a.So_u);
+        TP* sov = static_cast<TP*>(a.So_v);
+        if (MIXED) {
+            float* du = static_cast<float*>(a.Au);
+            float* dv = static_cast<float*>(a.Av);
+            if (epi == EPI_RK1) {
+                sou[g] = TP(a.Yu[g] + 0.5 * dt * ku);
+                sov[g] = TP(a.Yv[g] + 0.5 * dt * kv);
+                du[g] = float(dt / 6.0 * ku);
+                dv[g] = float(dt / 6.0 * kv);
+            } else if (epi == EPI_RK2) {
+                sou[g] = TP(a.Yu[g] + 0.5 * dt * ku);
+                sov[g] = TP(a.Yv[g] + 0.5 * dt * kv);
+                du[g] = float(double(du[g]) + dt / 3.0 * ku);
+                dv[g] = float(double(dv[g]) + dt / 3.0 * kv);
+            } else if (epi == EPI_RK3) {
+                sou[g] = TP(a.Yu[g] + dt * ku);
+                sov[g] = TP(a.Yv[g] + dt * kv);
+                du[g] = float(double(du[g]) + dt / 3.0 * ku);
+                dv[g] = float(double(dv[g]) + dt / 3.0 * kv);
+            } else {
+                a.Yu[g] = a.Yu[g] + (double(du[g]) + dt / 6.0 * ku);
+                a.Yv[g] = a.Yv[g] + (double(dv[g]) + dt / 6.0 * kv);
+            }
+        } else {
+            double* Au = static_cast<double*>(a.Au);
+            double* Av = static_cast<double*>(a.Av);
+            if (epi == EPI_RK1) {
+                const double yu = a.Yu[g], yv = a.Yv[g];
+                sou[g] = TP(yu + 0.5 * dt * ku);
+                sov[g] = TP(yv + 0.5 * dt * kv);
+                Au[g] = yu + dt / 6.0 * ku;
+                Av[g] = yv + dt / 6.0 * kv;
+            } else if (epi == EPI_RK2) {
+                sou[g] = TP(a.Yu[g] + 0.5 * dt * ku);
+                sov[g] = TP(a.Yv[g] + 0.5 * dt * kv);
+                Au[g] += dt / 3.0 * ku;
+                Av[g] += dt / 3.0 * kv;
+            } else if (epi == EPI_RK3) {
+                sou[g] = TP(a.Yu[g] + dt * ku);
+                sov[g] = TP(a.Yv[g] + dt * kv);
+                Au[g] += dt / 3.0 * ku;
+                Av[g] += dt / 3.0 * kv;
+            } else {
+                a.Yu[g] = Au[g] + dt / 6.0 * ku;
+                a.Yv[g] = Av[g] + dt / 6.0 * kv;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ launch
+
+static int pick_lpw(int Ma) { return 64 / Ma; }
+
+template <int N, typename TC, typename TS, typename TP>
+static hipError_t launch_t(const PassDesc& p, hipStream_t s)
+{
+    KArgs<N, TC> a;
+    for (int i = 0; i < N * N; ++i) {
+        a.k.G[i] = TC(p.ops.G[i]);
+        a.k.Gk[i] = TC(p.ops.Gk[i]);
+    }
+    for (int i = 0; i < N; ++i) a.k.winv[i] = TC(p.ops.winv[i]);
+    a.k.koL = TC(p.ops.koL); a.k.koR = TC(p.ops.koR);
+    a.k.tau = TC(p.ops.tau); a.k.ih2 = TC(p.ops.ih2);
+    a.k.gL = TC(p.ops.gL); a.k.gR = TC(p.ops.gR);
+    a.Su = p.Su; a.Sv = p.Sv; a.P = p.P;
+    a.So_u = p.So_u; a.So_v = p.So_v; a.Yu = p.Yu; a.Yv = p.Yv;
+    a.Au = p.Au; a.Av = p.Av; a.Ou = p.Ou; a.Ov = p.Ov;
+    a.dt = p.dt;
+    a.M0 = p.M[0]; a.M1 = p.M[1]; a.M2 = p.M[2];
+    a.axis = p.axis; a.periodic = p.periodic; a.nb = p.nb; a.epi = p.epi;
+    a.lap = p.lap; a.ko = p.ko; a.needv = p.needv; a.keepv = p.keepv;
+    const int Ma = p.M[p.axis];
+    a.lpw = pick_lpw(Ma);
+    const int LT = N * p.nb;
+    const int waves = (LT + a.lpw - 1) / a.lpw;
+    const int nthr = waves * 64;
+    const int clen = p.axis < 2 ? N * N * p.nb : N * p.nb;
+    if (nthr > 1024) return hipErrorInvalidConfiguration;
+    int fstride;
+    if (p.axis < 2) {
+        a.CS = clen + 1;
+        a.ES = 0;
+        fstride = Ma * a.CS;
+    } else {
+        a.CS = clen;
+        int es = N * clen;
+        const int want = a.lpw % 16;
+        int pad = ((want - es % 16) + 16) % 16;
+        if (pad == 0 && want == 0) pad = 1;
+        a.ES = es + pad;
+        fstride = Ma * a.ES;
+    }
+    const int nslots = (p.lap && p.needv) || (!p.lap && p.keepv) ? 2 : 1;
+    const size_t shmem = size_t(fstride) * nslots * sizeof(TS);
+    if (shmem > 160 * 1024) return hipErrorInvalidConfiguration;
+    const int nbk = N / p.nb;
+    int tiles;
+    if (p.axis == 0) tiles = p.M[1] * p.M[2] * nbk;
+    else if (p.axis == 1) tiles = p.M[0] * p.M[2] * nbk;
+    else tiles = p.M[0] * p.M[1] * nbk;
+    auto kern = flat_pass_kernel<N, TC, TS, TP>;
+    // raise the dynamic-LDS limit once per instantiation (outside any capture:
+    // the first launch of a configuration is always eager)
+    static size_t lds_limit = 64 * 1024;
+    if (shmem > lds_limit) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        lds_limit = 160 * 1024;
+    }
+    hipLaunchKernelGGL(kern, dim3(tiles), dim3(nthr), shmem, s, a);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_n(const PassDesc& p, hipStream_t s)
+{
+    switch (p.variant) {
+    case VAR_F64: return launch_t<N, double, double, double>(p, s);
+    case VAR_MIX_IN64: return launch_t<N, float, double, float>(p, s);
+    case VAR_MIX_IN32: return launch_t<N, float, float, float>(p, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_flat_pass(const PassDesc& p, hipStream_t s)
+{
+    if (p.nb < 1 || p.N % p.nb != 0) return hipErrorInvalidValue;
+    if (p.M[p.axis] > MMAX_AXIS) return hipErrorInvalidValue;
+    switch (p.N) {
+    case 3: return launch_n<3>(p, s);
+    case 4: return launch_n<4>(p, s);
+    case 5: return launch_n<5>(p, s);
+    case 6: return launch_n<6>(p, s);
+    case 7: return launch_n<7>(p, s);
+    case 8: return launch_n<8>(p, s);
+    case 9: return launch_n<9>(p, s);
+    case 10: return launch_n<10>(p, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ------------------------------------------------------------------ energy
+
+// E = 1/2 sum_n [ wv_n v_n^2 - wu_n u_n Lu_n ]; flat weights are
+// w_i w_j w_k hx hy hz (Hphys).  Fixed grid and fixed per-thread order: the
+// result is bitwise reproducible.
+__global__ void __launch_bounds__(RED_THREADS) energy_partial_kernel(EnergyDesc e)
+{
+    __shared__ double sh[RED_THREADS];
+    const int64_t n = int64_t(e.N) * e.N * e.N * e.Ne;
+    double acc = 0.0;
+    for (int64_t q = int64_t(blockIdx.x) * RED_THREADS + threadIdx.x; q < n;
+         q += int64_t(RED_BLOCKS) * RED_THREADS) {
+        double wu, wv;
+        if (e.wu) {
+            wu = e.wu[q];
+            wv = e.wv[q];
+        } else {
+            const int i = int(q % e.N), j = int((q / e.N) % e.N), k = int((q / (e.N * e.N)) % e.N);
+            wu = e.w[i] * e.w[j] * e.w[k] * e.hvol;
+            wv = wu;
+        }
+        acc += 0.5 * (wv * e.v[q] * e.v[q] - wu * e.u[q] * e.Lu[q]);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
+        if (int(threadIdx.x) < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) e.partial[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(RED_THREADS) energy_final_kernel(const double* partial, double* out)
+{
+    __shared__ double sh[RED_THREADS];
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < RED_BLOCKS; q += RED_THREADS) acc += partial[q];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
+        if (int(threadIdx.x) < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+hipError_t launch_energy(const EnergyDesc& e, hipStream_t s)
+{
+    hipLaunchKernelGGL(energy_partial_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, e);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(energy_final_kernel, dim3(1), dim3(RED_THREADS), 0, s, e.partial, e.out);
+    return hipGetLastError();
+}
+
+}  // namespace wt2

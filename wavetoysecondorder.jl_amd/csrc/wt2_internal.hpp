@@ -1,0 +1,89 @@
+// wt2_internal.hpp -- shared declarations of libwt2 (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/wt2.h"
+
+namespace wt2 {
+
+constexpr int NMIN = 3;
+constexpr int NMAX = 10;
+constexpr int MMAX_AXIS = 64;   // one element line per axis fits one wavefront
+
+// Epilogue of a pencil pass: what is done with the axis contribution c.
+enum Epi : int {
+    EPI_FIRST = 0,    // P  = c
+    EPI_MID = 1,      // P += c
+    EPI_OUT_RHS = 2,  // (du, dv) = (u_t, P + c)
+    EPI_OUT_LAP = 3,  // out = P + c
+    EPI_RK1 = 4,      // classical RK4 stage updates with k = (u_t, P + c)
+    EPI_RK2 = 5,
+    EPI_RK3 = 6,
+    EPI_RK4 = 7,
+};
+
+// Precision variant of a pass kernel.
+enum Variant : int {
+    VAR_F64 = 0,      // compute, stage input, partial: double
+    VAR_MIX_IN64 = 1, // compute float, stage input double, partial float
+    VAR_MIX_IN32 = 2, // compute float, stage input float, partial float
+};
+
+// Per-axis 1D operator constants in FP64 (converted to the compute type at
+// launch).
+struct AxisOps {
+    double G[NMAX * NMAX];   // unit-element GLL differentiation matrix
+    double Gk[NMAX * NMAX];  // KO-scaled derivative: kos * G / h
+    double winv[NMAX];       // 1 / w
+    double koL, koR;         // KO interface SAT: -kos/(2 w_0 h), +kos/(2 w_{N-1} h)
+    double tau, ih2;         // penalty, 1/h^2
+    double gL, gR;           // Dirichlet values at the low / high outer face
+};
+
+// Everything one pencil-pass launch needs (type-erased pointers).
+struct PassDesc {
+    int N, variant;
+    int axis, periodic;
+    int M[3];
+    int nb;             // lines-block size: element planes per tile (KT / JT)
+    int epi;
+    int lap;            // add the SIPG Laplacian of u along this axis
+    int ko;             // add eps mu^-5 D^6 u_t along this axis
+    int needv;          // load u_t (KO input or kept for the final epilogue)
+    int keepv;          // final epilogue: u' = u_t needs u_t after the compute
+    const void* Su; const void* Sv;   // stage input (variant input type)
+    void* P;                          // partial sum of the u_t equation (variant partial type)
+    void* So_u; void* So_v;           // next stage input (partial type)
+    double* Yu; double* Yv;           // RK state
+    void* Au; void* Av;               // RK accumulator (FP64) / increment (FP32, mixed)
+    double* Ou; double* Ov;           // outputs
+    double dt;
+    AxisOps ops;
+};
+
+// Launch one pencil pass on stream s.  Returns hipError_t.
+hipError_t launch_flat_pass(const PassDesc& p, hipStream_t s);
+
+// Deterministic FP64 energy reduction: E = 1/2 sum H (v^2 - u Lu) over a flat
+// or curved mesh.  partial must hold red_blocks doubles; result written to
+// out (device).
+constexpr int RED_BLOCKS = 512;
+constexpr int RED_THREADS = 256;
+struct EnergyDesc {
+    int N; int64_t Ne;
+    const double* u; const double* v; const double* Lu;
+    const double* wu;   // per-node weight of u*Lu (nullptr: flat H3*hx*hy*hz)
+    const double* wv;   // per-node weight of v^2 (nullptr: flat)
+    double hvol;        // hx*hy*hz for the flat weights
+    double w[NMAX];
+    double* partial; double* out;
+};
+hipError_t launch_energy(const EnergyDesc& e, hipStream_t s);
+
+void set_error(const std::string& msg);
+
+}  // namespace wt2

@@ -1,0 +1,424 @@
+"""Host-side mirror of the reference's hot-path API over libwt2.so.
+
+Names follow eschnett/WaveToySecondOrder.jl (Julia's `f!` becomes `f`; the
+first argument is still the one that is overwritten):
+
+    Params3d                    src/wave.jl:67-95
+    make_element/make_operators HexSBPSAT (used at src/wave.jl:603)
+    make_uniform_hex            HexMeshes (src/evolve.jl:915)
+    make_warped_uniform_hex     HexMeshes (test/dirichlet_warped_tests.jl:23,40)
+    make_geometry/make_workspace/to_device   src/evolve.jl:935-936
+    rhs_wave3d                  rhs_wave3d! (src/wave.jl:573-611)
+    wave3d_rhs_full             (u, u_t) analogue of wave3d_curved_rhs! (src/wave3d_curved.jl:205-258)
+    AnalyticBackground3D, sample_background3d, make_coef3d   src/wave3d_curved.jl:36-124
+    discrete_energy             test/kernels3d_tests.jl:32-47
+    recommended_dt_rk4          dt = cfl * h_min (src/evolve.jl:1207-1213)
+
+Field arrays are either torch CUDA tensors (used in place on the device) or
+numpy float64 arrays (copied in and out); both hold the Julia column-major
+(N,N,N,Ne) layout, i.e. a C-contiguous (Ne,N,N,N) block, flattened or not.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import WT2_FP64, WT2_MIXED, WT2_HOST_PTRS, WT2_ALLOW_NONCOERCIVE, WT2Error, check, lib
+
+try:  # torch is plumbing for device memory; numpy host arrays work without it
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
+
+
+# ----------------------------------------------------------------- params
+
+class Params3d:
+    """Params3d(A, k, omega, tau, bdry_values, sommerfeld_R=inf) (src/wave.jl:67-95)."""
+
+    def __init__(self, A=1.0, k=(0.0, 0.0, 0.0), omega=0.0, tau=None, bdry_values=(0.0,) * 6,
+                 sommerfeld_R=math.inf):
+        self.A = float(A)
+        self.k = tuple(float(x) for x in k)
+        self.omega = float(omega)
+        self.tau = None if tau is None else float(tau)
+        self.bdry_values = tuple(float(x) for x in bdry_values)
+        if len(self.bdry_values) != 6:
+            raise ValueError("bdry_values must have 6 entries (one per boundary tag 1..6)")
+        self.sommerfeld_R = float(sommerfeld_R)
+
+
+def tau_effective(tau, N):
+    """Penalty actually used: max(tau, 1.5 (N-1)^2), the reference driver's
+    scaling for cubical meshes (src/evolve.jl:964-968), which keeps SIPG coercive."""
+    t = 1.5 * (N - 1) ** 2
+    return t if tau is None else max(float(tau), t)
+
+
+# ---------------------------------------------------------------- element
+
+class Element:
+    def __init__(self, N):
+        xi = np.zeros(N); w = np.zeros(N); G = np.zeros((N, N))
+        check(lib().wt2_gll(N, xi.ctypes.data, w.ctypes.data, G.ctypes.data))
+        self.N = N
+        self.xs = xi
+        self.w = w
+        self.G = G
+        self.h = float(np.min(np.diff(xi)))   # smallest node gap on the unit element
+
+
+class SBPOps:
+    def __init__(self, elem):
+        self.N = elem.N
+        self.G = elem.G
+        self.H = np.diag(elem.w)
+
+
+def make_element(N):
+    return Element(N)
+
+
+def make_operators(elem):
+    return SBPOps(elem)
+
+
+# ------------------------------------------------------------------- mesh
+
+class Mesh:
+    def __init__(self, M, x0, x1, periodic, warp_A=0.0):
+        self.M = tuple(int(m) for m in M)
+        self.x0 = tuple(float(v) for v in x0)
+        self.x1 = tuple(float(v) for v in x1)
+        self.periodic = tuple(bool(p) for p in periodic)
+        self.warp_A = float(warp_A)
+        self.Ne = self.M[0] * self.M[1] * self.M[2]
+
+
+def _triple(v):
+    if isinstance(v, (tuple, list)):
+        return tuple(v)
+    return (v, v, v)
+
+
+def make_uniform_hex(M, x0=0.0, x1=1.0, periodic=False):
+    """M^3 (or Mx,My,Mz) affine hex elements on [x0,x1]^3, x-fastest element order,
+    outer face f tagged f (1..6 = -x,+x,-y,+y,-z,+z)."""
+    return Mesh(_triple(M), _triple(x0), _triple(x1), _triple(periodic))
+
+
+def make_warped_uniform_hex(Mx, My, Mz, x0, x1, A, periodic=False):
+    """Warped cube: x = x0 + L (xi + A sin(2 pi xi1) sin(2 pi xi2) sin(2 pi xi3))."""
+    return Mesh((Mx, My, Mz), _triple(x0), _triple(x1), _triple(periodic), warp_A=A)
+
+
+# -------------------------------------------------------- device workspace
+
+class Workspace:
+    """A libwt2 context: mesh + operators + geometry + device workspace + stream
+    (the reference's geom/ops/work bundle after to_device)."""
+
+    def __init__(self, mesh, elem, tau=None, eps_ko=0.0, bdry_values=(0.0,) * 6,
+                 precision="fp64", allow_noncoercive=False, bg=None):
+        d = _lib.MeshDesc()
+        d.N = elem.N
+        for a in range(3):
+            d.M[a] = mesh.M[a]
+            d.x0[a] = mesh.x0[a]
+            d.x1[a] = mesh.x1[a]
+            d.periodic[a] = int(mesh.periodic[a])
+        d.warp_A = mesh.warp_A
+        d.tau = tau_effective(None, elem.N) if tau is None else float(tau)
+        d.eps_ko = float(eps_ko)
+        for f in range(6):
+            d.bdry_values[f] = float(bdry_values[f])
+        d.precision = {"fp64": WT2_FP64, "mixed": WT2_MIXED}[precision]
+        d.flags = WT2_ALLOW_NONCOERCIVE if allow_noncoercive else 0
+        self.desc = d
+        self.mesh = mesh
+        self.elem = elem
+        self.N = elem.N
+        self.precision = precision
+        self.ctx = ctypes.c_void_p()
+        check(lib().wt2_ctx_create(ctypes.byref(d), ctypes.byref(self.ctx)))
+        self.npoints = elem.N ** 3 * mesh.Ne
+        if bg is not None:
+            self.set_background(bg)
+
+    def close(self):
+        if getattr(self, "ctx", None) and self.ctx.value:
+            lib().wt2_ctx_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        inf = _lib.Info()
+        check(lib().wt2_ctx_info(self.ctx, ctypes.byref(inf)))
+        return {k: getattr(inf, k) for k, _ in inf._fields_}
+
+    @property
+    def mu(self):
+        return self.info()["mu"]
+
+    @property
+    def h_min(self):
+        return self.info()["h_min"]
+
+    @property
+    def tau(self):
+        return self.desc.tau
+
+    def set_background(self, bg10):
+        bg = np.ascontiguousarray(bg10, dtype=np.float64).reshape(-1)
+        if bg.size != 10 * self.npoints:
+            raise ValueError("background needs 10 values per node")
+        check(lib().wt2_ctx_set_background(self.ctx, bg.ctypes.data))
+
+    def coords(self):
+        x = np.zeros((self.npoints, 3))
+        check(lib().wt2_coords(self.ctx, x.ctypes.data))
+        return x
+
+    # -- pointer plumbing
+    def _ptr(self, a, writable=False):
+        if a is None:
+            return None, False
+        if torch is not None and isinstance(a, torch.Tensor):
+            if a.dtype != torch.float64 or not a.is_contiguous() or a.numel() != self.npoints:
+                raise ValueError("field tensors must be contiguous float64 with N^3*Ne entries")
+            if not a.is_cuda:
+                raise ValueError("torch field tensors must live on the GPU")
+            return a.data_ptr(), False
+        if isinstance(a, np.ndarray):
+            if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"] or a.size != self.npoints:
+                raise ValueError("field arrays must be C-contiguous float64 with N^3*Ne entries")
+            if writable and not a.flags["WRITEABLE"]:
+                raise ValueError("output array is read-only")
+            return a.ctypes.data, True
+        raise TypeError("fields must be torch CUDA tensors or numpy arrays")
+
+    def _ptrs(self, *arrs, writable=()):
+        ptrs, hosts = [], set()
+        for i, a in enumerate(arrs):
+            p, h = self._ptr(a, i in writable)
+            ptrs.append(p)
+            if a is not None:
+                hosts.add(h)
+        if len(hosts) > 1:
+            raise ValueError("mix of host and device arrays in one call")
+        return ptrs, (WT2_HOST_PTRS if hosts == {True} else 0)
+
+    def _sync_torch(self):
+        if torch is not None and torch.cuda.is_available() and torch.cuda.is_initialized():
+            torch.cuda.current_stream().synchronize()
+
+    def rhs(self, udd, u, ud, bdry_values):
+        (pa, pu, pud), fl = self._ptrs(udd, u, ud, writable=(0,))
+        bv = (ctypes.c_double * 6)(*bdry_values)
+        self._sync_torch()
+        check(lib().wt2_rhs3d(self.ctx, pa, pu, pud, ctypes.cast(bv, ctypes.c_void_p), fl))
+        return udd
+
+    def rhs_full(self, du, dud, u, ud):
+        (p1, p2, p3, p4), fl = self._ptrs(du, dud, u, ud, writable=(0, 1))
+        self._sync_torch()
+        check(lib().wt2_rhs_full(self.ctx, p1, p2, p3, p4, fl))
+        return du, dud
+
+    def rk4_steps(self, u, ud, dt, nsteps, energy_every=0, graph=True):
+        (pu, pud), fl = self._ptrs(u, ud, writable=(0, 1))
+        ne = 1 + nsteps // energy_every if energy_every > 0 else 0
+        E = np.zeros(max(ne, 1))
+        flags = fl | (0 if graph else _lib.WT2_NO_GRAPH)
+        self._sync_torch()
+        check(lib().wt2_rk4_steps(self.ctx, pu, pud, float(dt), int(nsteps), int(energy_every),
+                                  E.ctypes.data if ne else None, flags))
+        return E[:ne] if ne else None
+
+    def energy(self, u, ud):
+        (pu, pud), fl = self._ptrs(u, ud)
+        E = ctypes.c_double()
+        self._sync_torch()
+        check(lib().wt2_energy(self.ctx, pu, pud, ctypes.byref(E), fl))
+        return E.value
+
+
+class Geometry:
+    def __init__(self, mesh, elem):
+        self.mesh = mesh
+        self.elem = elem
+        self.N = elem.N
+        self.Ne = mesh.Ne
+        ws = None
+        self._coords = None
+
+    @property
+    def coords(self):
+        """(Ne, N, N, N, 3)-shaped view of the node coordinates (host)."""
+        if self._coords is None:
+            self._coords = host_coords(self.mesh, self.elem)
+        return self._coords
+
+    @property
+    def Hphys(self):
+        """H3 * |J| per node (host); only the affine case is closed form here."""
+        if self.mesh.warp_A != 0.0:
+            raise NotImplementedError("Hphys on warped meshes comes from the device context")
+        w = self.elem.w
+        hvol = np.prod([(b - a) / m for a, b, m in zip(self.mesh.x0, self.mesh.x1, self.mesh.M)])
+        H3 = np.einsum("k,j,i->kji", w, w, w)
+        return np.broadcast_to(H3 * hvol, (self.Ne,) + H3.shape).reshape(-1).copy()
+
+
+def host_coords(mesh, elem):
+    """Node coordinates, flat (npoints, 3), computed on the host like wt2_coords."""
+    N = elem.N
+    M = mesh.M
+    ez, ey, ex = np.meshgrid(np.arange(M[2]), np.arange(M[1]), np.arange(M[0]), indexing="ij")
+    ex, ey, ez = ex.reshape(-1), ey.reshape(-1), ez.reshape(-1)
+    xs = elem.xs
+    k, j, i = np.meshgrid(xs, xs, xs, indexing="ij")
+    xi0 = (ex[:, None, None, None] + i[None]) / M[0]
+    xi1 = (ey[:, None, None, None] + j[None]) / M[1]
+    xi2 = (ez[:, None, None, None] + k[None]) / M[2]
+    s = 0.0
+    if mesh.warp_A != 0.0:
+        tp = 2.0 * np.pi
+        s = mesh.warp_A * np.sin(tp * xi0) * np.sin(tp * xi1) * np.sin(tp * xi2)
+    out = np.empty(xi0.shape + (3,))
+    for a, xi in enumerate((xi0, xi1, xi2)):
+        out[..., a] = mesh.x0[a] + (mesh.x1[a] - mesh.x0[a]) * (xi + s)
+    return out.reshape(-1, 3)
+
+
+def make_geometry(mesh, elem):
+    return Geometry(mesh, elem)
+
+
+def make_workspace(geom, tau=None, eps_ko=0.0, bdry_values=(0.0,) * 6, precision="fp64", bg=None,
+                   allow_noncoercive=False):
+    return Workspace(geom.mesh, geom.elem, tau=tau, eps_ko=eps_ko, bdry_values=bdry_values,
+                     precision=precision, bg=bg, allow_noncoercive=allow_noncoercive)
+
+
+to_device = make_workspace
+
+
+# ------------------------------------------------------------- RHS / energy
+
+def rhs_wave3d(udd, u, ud, params, geom=None, ops=None, work=None):
+    """rhs_wave3d!(udd, u, ud, params; geom, ops, work) (src/wave.jl:573-611):
+    udd = L_SIPG u with Dirichlet values params.bdry_values[tag-1].  work is a
+    Workspace (its tau is the context's; params.tau must agree).  Tag-7
+    Sommerfeld faces do not exist on these meshes."""
+    if work is None:
+        raise ValueError("rhs_wave3d needs work= (a Workspace from make_workspace)")
+    if params.tau is not None and abs(params.tau - work.tau) > 1e-12 * work.tau:
+        raise ValueError(f"params.tau={params.tau} differs from the workspace tau={work.tau}")
+    return work.rhs(udd, u, ud, params.bdry_values)
+
+
+def wave3d_rhs_full(du, dud, u, ud, work):
+    """(du, dud) = (ud + KO(u), L u + KO(ud)) with KO = eps mu^-5 sum_d D_d^6."""
+    return work.rhs_full(du, dud, u, ud)
+
+
+def discrete_energy(u, ud, work):
+    """E = 1/2 <ud,ud>_H + 1/2 <u, -L u>_H with zero boundary data
+    (test/kernels3d_tests.jl:32-47), FP64 on the device."""
+    return work.energy(u, ud)
+
+
+# ------------------------------------------------------------- background
+
+class AnalyticBackground3D:
+    """Closures alpha_fn(t,x,y,z), beta_fn -> (b1,b2,b3), gamma_fn -> (g11,g12,g13,g22,g23,g33)
+    (src/wave3d_curved.jl:45-57); vectorised over numpy arrays."""
+
+    def __init__(self, alpha_fn, beta_fn, gamma_fn):
+        self.alpha_fn = alpha_fn
+        self.beta_fn = beta_fn
+        self.gamma_fn = gamma_fn
+
+
+def sample_background3d(bg, t, xg, yg, zg):
+    """Per-node (alpha, beta1..3, gamma11..33) as an (n, 10) array, the input of
+    Workspace.set_background (sample_background3d!, src/wave3d_curved.jl:106-115)."""
+    n = np.size(xg)
+    out = np.empty((n, 10))
+    out[:, 0] = np.broadcast_to(bg.alpha_fn(t, xg, yg, zg), (n,))
+    b = bg.beta_fn(t, xg, yg, zg)
+    for q in range(3):
+        out[:, 1 + q] = np.broadcast_to(b[q], (n,))
+    g = bg.gamma_fn(t, xg, yg, zg)
+    for q in range(6):
+        out[:, 4 + q] = np.broadcast_to(g[q], (n,))
+    return out
+
+
+def make_coef3d(geom):
+    return np.zeros((geom.N ** 3 * geom.Ne, 10))
+
+
+def adm_bump_background(amp_lapse=0.2, amp_shift=0.05, amp_metric=0.2, width2=0.1, centre=0.5):
+    """The curved-background configuration: with r = |x - centre|,
+    alpha = 1 - 0.2 exp(-r^2/0.1), beta^i = 0.05 exp(-r^2/0.1)(x_i - centre),
+    gamma_ij = (1 + 0.2 exp(-r^2/0.1)) delta_ij."""
+    def env(x, y, z):
+        return np.exp(-((x - centre) ** 2 + (y - centre) ** 2 + (z - centre) ** 2) / width2)
+
+    def alpha(t, x, y, z):
+        return 1.0 - amp_lapse * env(x, y, z)
+
+    def beta(t, x, y, z):
+        e = amp_shift * env(x, y, z)
+        return (e * (x - centre), e * (y - centre), e * (z - centre))
+
+    def gamma(t, x, y, z):
+        g = 1.0 + amp_metric * env(x, y, z)
+        z0 = np.zeros_like(g)
+        return (g, z0, z0, g, z0, g)
+
+    return AnalyticBackground3D(alpha, beta, gamma)
+
+
+# -------------------------------------------------------------------- ICs
+
+def eigenmode_cartesian(coords, t, A=1.0, kx=math.pi, ky=math.pi, kz=math.pi, omega=None,
+                        x0=0.0, x1=1.0):
+    """eigenmode_cartesian! (src/wave.jl:163-185): u, u_t of
+    A sin(kx X) sin(ky Y) sin(kz Z) cos(omega t), X = (x - x0)/(x1 - x0)."""
+    L = x1 - x0
+    if omega is None:
+        omega = math.sqrt(kx * kx + ky * ky + kz * kz) / L
+    X = (coords[:, 0] - x0) / L
+    Y = (coords[:, 1] - x0) / L
+    Z = (coords[:, 2] - x0) / L
+    s = A * np.sin(kx * X) * np.sin(ky * Y) * np.sin(kz * Z)
+    return s * math.cos(omega * t), -omega * s * math.sin(omega * t)
+
+
+def gaussian_pulse(coords, centre, width=None, denom=None, amplitude=1.0):
+    """Gaussian pulse u = A exp(-|x-c|^2 / denom), u_t = 0.  denom defaults to
+    2 width^2 (src/evolve.jl:1260-1263)."""
+    if denom is None:
+        denom = 2.0 * width * width
+    c = np.asarray(centre, dtype=np.float64)
+    r2 = np.sum((coords - c[None, :]) ** 2, axis=1)
+    return amplitude * np.exp(-r2 / denom), np.zeros(len(coords))
+
+
+def recommended_dt_rk4(work, cfl=0.2):
+    """dt = cfl * h_min (h_min per src/evolve.jl:60-69), capped by the KO limit
+    1.4/(eps mu) (src/evolve.jl:1211-1213)."""
+    dt = cfl * work.h_min
+    eps = work.desc.eps_ko
+    if eps > 0:
+        dt = min(dt, 1.4 / (eps * work.mu))
+    return dt
