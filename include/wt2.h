@@ -110,7 +110,10 @@ int wt2_rhs3d(wt2_ctx* ctx, double* udd, const double* u, const double* ud,
 
 /* Full first-order right-hand side with KO, the (u, u_t) analogue of
  * wave3d_curved_rhs! (src/wave3d_curved.jl:205-258):
- *   du = ud + eps mu^-5 sum_d D_d^6 u,  dud = rhs_wave3d!(u, ud) + eps mu^-5 sum_d D_d^6 ud. */
+ *   du = ud,  dud = rhs_wave3d!(u, ud) + eps mu^-5 sum_d D_d^6 ud.
+ * D_d is apply_D!'s SAT-coupled derivative with the outer faces closed against
+ * a zero ghost value (H D skew, so the term only removes energy); KO acts on
+ * u_t only because on u it would not commute with the SIPG operator. */
 int wt2_rhs_full(wt2_ctx* ctx, double* du, double* dud, const double* u,
                  const double* ud, int32_t flags);
 
@@ -131,6 +134,15 @@ int wt2_energy(wt2_ctx* ctx, const double* u, const double* ud, double* E, int32
 /* Use an external HIP stream (hipStream_t passed as void*); NULL restores the
  * context's own stream. */
 int wt2_set_stream(wt2_ctx* ctx, void* stream);
+
+/* Diagnostics (no counterpart in the reference; used by bench.py to time one
+ * kernel in isolation with events on the context stream): enqueue, without
+ * synchronising, the single pencil pass of RK4 stage `stage` (1..4) along
+ * `axis` (0..2) on state (u, ud) (device pointers) -- or, for a curved
+ * context and axis == -1, the curved element operator of that stage.  The
+ * context registers must have been initialised by a wt2_rk4_steps call with
+ * the same state pointers. */
+int wt2_launch_pass(wt2_ctx* ctx, int32_t axis, int32_t stage, double* u, double* ud, double dt);
 
 const char* wt2_last_error(void);
 const char* wt2_version(void);

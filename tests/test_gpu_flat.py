@@ -1,0 +1,218 @@
+"""GPU parity of the flat (affine) path through the C ABI against the CPU oracle.
+
+FP64 bar: max |gpu - oracle| <= 1e-12 * max |oracle| (the RHS tolerance the
+north star states); mixed precision (FP32 RHS bulk, FP64 state): relative
+RHS error <= 2e-5 and RK4 H-norm error <= 1e-5.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import wt_oracle as wo
+import wavetoy_hip as wh
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _pair(N, M, x1=1.0, periodic=False, eps_ko=0.0, bv=(0.0,) * 6, precision="fp64", tau=None):
+    Mt = M if isinstance(M, tuple) else (M, M, M)
+    x1t = x1 if isinstance(x1, tuple) else (x1, x1, x1)
+    pt = periodic if isinstance(periodic, tuple) else (periodic,) * 3
+    mesh = wh.make_uniform_hex(Mt, 0.0, x1t, pt)
+    elem = wh.make_element(N)
+    tau = wh.tau_effective(None, N) if tau is None else tau
+    ws = wh.Workspace(mesh, elem, tau=tau, eps_ko=eps_ko, bdry_values=bv, precision=precision)
+    o = wo.Oracle(N, Mt, x1=x1t, periodic=pt, tau=tau, eps_ko=eps_ko, bdry_values=bv)
+    return ws, o
+
+
+CASES = [
+    # N, M, x1, periodic, bv
+    (3, 2, 1.0, False, (0.0,) * 6),
+    (4, (3, 2, 5), (1.0, 0.7, 1.3), False, (0.1, -0.2, 0.3, 0.4, -0.5, 0.6)),
+    (5, 3, 1.0, (True, False, True), (0.0, 0.0, 0.2, -0.1, 0.0, 0.0)),
+    (6, 4, 1.0, False, (0.0,) * 6),
+    (7, (2, 3, 3), 1.0, True, (0.0,) * 6),
+    (8, 4, 1.0, False, (0.3,) * 6),
+    (9, 2, 1.0, False, (0.0,) * 6),
+    (10, 3, 1.0, False, (0.0,) * 6),
+    (4, (16, 2, 3), 1.0, False, (0.0,) * 6),   # 16 elements on one axis: 4 lines per wave
+    (3, (33, 2, 2), 1.0, False, (0.0,) * 6),   # 33: one line per wave, partial lanes
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"N{c[0]}-M{c[1]}" for c in CASES])
+def test_rhs3d_matches_oracle(case):
+    N, M, x1, per, bv = case
+    _say("rhs3d case", case)
+    ws, o = _pair(N, M, x1, per, bv=bv)
+    rng = np.random.default_rng(1000 + N)
+    u = rng.standard_normal(o.n)
+    ud = rng.standard_normal(o.n)
+    ref = o.lap(u)
+    out = np.zeros(o.n)
+    ws.rhs(out, u, ud, bv)
+    assert relerr(out, ref) <= TOL64
+
+
+@pytest.mark.parametrize("case", CASES[:8], ids=[f"N{c[0]}-M{c[1]}" for c in CASES[:8]])
+def test_rhs_full_with_ko_matches_oracle(case):
+    N, M, x1, per, bv = case
+    _say("rhs_full case", case)
+    ws, o = _pair(N, M, x1, per, eps_ko=0.1, bv=bv)
+    assert abs(ws.mu / o.mu - 1) < 1e-12
+    rng = np.random.default_rng(2000 + N)
+    u = rng.standard_normal(o.n)
+    v = rng.standard_normal(o.n)
+    du_ref, dv_ref = o.rhs(u, v)
+    du = np.zeros(o.n); dv = np.zeros(o.n)
+    ws.rhs_full(du, dv, u, v)
+    assert relerr(du, du_ref) == 0.0          # u' = u_t exactly
+    assert relerr(dv, dv_ref) <= TOL64
+
+
+@pytest.mark.parametrize("N,M", [(4, 3), (8, 2), (6, (2, 3, 4)), (10, 2)])
+def test_rk4_steps_match_oracle(N, M):
+    _say("rk4 case", N, M)
+    ws, o = _pair(N, M, eps_ko=0.1, bv=(0.0,) * 6)
+    x = o.coords()
+    u0 = np.exp(-np.sum((x - 0.45) ** 2, axis=1) / 0.05)
+    v0 = 0.3 * np.sin(2 * math.pi * x[:, 1])
+    dt = 0.2 * o.hmin
+    ns = 7
+    uo, vo, Eo = o.rk4(u0, v0, dt, ns, energy=True)
+    u = u0.copy(); v = v0.copy()
+    E = ws.rk4_steps(u, v, dt, ns, energy_every=1)
+    assert relerr(u, uo) <= TOL64
+    assert relerr(v, vo) <= TOL64
+    assert np.max(np.abs(E - Eo)) <= TOL64 * np.max(np.abs(Eo))
+
+
+def test_device_pointers_graph_and_determinism():
+    import torch
+    _say("device ptr / graph / determinism")
+    ws, o = _pair(8, 4, eps_ko=0.1)
+    x = o.coords()
+    u0 = np.exp(-np.sum((x - 0.5) ** 2, axis=1) / 0.02)
+    v0 = np.zeros(o.n)
+    dt = 0.2 * o.hmin
+    runs = []
+    for graph in (True, True, False):
+        u = torch.from_numpy(u0).cuda()
+        v = torch.from_numpy(v0).cuda()
+        ws.rk4_steps(u, v, dt, 9, graph=graph)
+        runs.append((u.cpu().numpy(), v.cpu().numpy()))
+    uh = u0.copy(); vh = v0.copy()
+    ws.rk4_steps(uh, vh, dt, 9)
+    for r in runs[1:] + [(uh, vh)]:
+        assert np.array_equal(r[0], runs[0][0]) and np.array_equal(r[1], runs[0][1])
+    uo, vo = o.rk4(u0, v0, dt, 9)
+    assert relerr(runs[0][0], uo) <= TOL64
+
+
+def test_energy_matches_oracle():
+    _say("energy")
+    for N, M in ((4, 3), (8, 2)):
+        ws, o = _pair(N, M)
+        rng = np.random.default_rng(5)
+        u = rng.standard_normal(o.n); v = rng.standard_normal(o.n)
+        assert abs(ws.energy(u, v) / o.energy(u, v) - 1) <= TOL64
+
+
+def test_rhs_wave3d_signature_and_tau_mismatch():
+    _say("rhs_wave3d signature")
+    N, M = 4, 2
+    mesh = wh.make_uniform_hex(M, 0.0, 1.0)
+    elem = wh.make_element(N)
+    geom = wh.make_geometry(mesh, elem)
+    ops = wh.make_operators(elem)
+    work = wh.make_workspace(geom, tau=13.5)
+    p = wh.Params3d(A=1.0, k=(math.pi,) * 3, omega=math.sqrt(3) * math.pi, tau=13.5)
+    x = wh.host_coords(mesh, elem)
+    u, ud = wh.eigenmode_cartesian(x, 0.0)
+    udd = np.zeros_like(u)
+    wh.rhs_wave3d(udd, u, ud, p, geom=geom, ops=ops, work=work)
+    assert relerr(udd, wo.Oracle(N, M).lap(u)) <= TOL64
+    with pytest.raises(ValueError):
+        wh.rhs_wave3d(udd, u, ud, wh.Params3d(tau=20.0), geom=geom, ops=ops, work=work)
+
+
+# ------------------------------------------------------------ mixed precision
+
+def test_mixed_rhs_close_to_fp64():
+    _say("mixed rhs")
+    ws, o = _pair(8, 3, eps_ko=0.1, precision="mixed")
+    x = o.coords()
+    u = np.exp(-np.sum((x - 0.5) ** 2, axis=1) / 0.05)
+    v = np.sin(math.pi * x[:, 0]) * np.sin(math.pi * x[:, 1])
+    du_ref, dv_ref = o.rhs(u, v)
+    du = np.zeros(o.n); dv = np.zeros(o.n)
+    ws.rhs_full(du, dv, u, v)
+    assert relerr(dv, dv_ref) <= 2e-5
+    assert relerr(du, du_ref) == 0.0
+
+
+def test_mixed_rk4_hnorm_error():
+    _say("mixed rk4")
+    ws, o = _pair(8, 3, eps_ko=0.1, precision="mixed")
+    x = o.coords()
+    u0 = np.sin(math.pi * x[:, 0]) * np.sin(math.pi * x[:, 1]) * np.sin(math.pi * x[:, 2])
+    v0 = np.zeros(o.n)
+    dt = 0.2 * o.hmin
+    ns = 60
+    uo, vo = o.rk4(u0, v0, dt, ns)
+    u = u0.copy(); v = v0.copy()
+    ws.rk4_steps(u, v, dt, ns)
+    H = o.hphys()
+    err = math.sqrt(np.sum((u - uo) ** 2 * H)) / math.sqrt(np.sum(uo ** 2 * H))
+    assert err <= 1e-5
+
+
+# -------------------------------------------- full-size properties (16.8M points)
+
+def test_full_size_linearity_symmetry_energy():
+    """At the plain FP64 GPU configuration's size (32^3 elements x 8^3 nodes) the
+    oracle is too slow to run whole, so check what is size independent: the
+    RHS is linear, H L is symmetric, and the energy with KO on does not grow."""
+    import torch
+    _say("full size 32^3 x 8^3")
+    N, M = 8, 32
+    mesh = wh.make_uniform_hex(M, 0.0, 1.0)
+    elem = wh.make_element(N)
+    ws = wh.Workspace(mesh, elem, tau=73.5, eps_ko=0.1)
+    n = ws.npoints
+    g = torch.Generator(device="cuda").manual_seed(11)
+    a = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    b = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    zero = torch.zeros(n, dtype=torch.float64, device="cuda")
+    La = torch.empty_like(a); Lb = torch.empty_like(a); Lab = torch.empty_like(a)
+    ws.rhs(La, a, zero, (0.0,) * 6)
+    ws.rhs(Lb, b, zero, (0.0,) * 6)
+    ws.rhs(Lab, 2.0 * a - 3.0 * b, zero, (0.0,) * 6)
+    lin = (Lab - (2.0 * La - 3.0 * Lb)).abs().max().item() / Lab.abs().max().item()
+    assert lin < 1e-13
+    H = torch.from_numpy(wh.Geometry(mesh, elem).Hphys).cuda()
+    s1 = torch.sum(H * a * Lb).item()
+    s2 = torch.sum(H * b * La).item()
+    assert abs(s1 - s2) <= 1e-10 * math.sqrt(torch.sum(H * a * La).abs().item() * torch.sum(H * b * Lb).abs().item())
+    # energy non-increasing over a few steps of a resolved pulse
+    x = torch.from_numpy(wh.host_coords(mesh, elem)).cuda()
+    c = torch.tensor([0.42, 0.55, 0.48], dtype=torch.float64, device="cuda")
+    u = torch.exp(-((x - c) ** 2).sum(dim=1) / 0.01).contiguous()
+    v = torch.zeros_like(u)
+    dt = 0.2 * ws.h_min
+    E = ws.rk4_steps(u, v, dt, 6, energy_every=1)
+    assert np.all(np.diff(E) <= 1e-13 * E[0])

@@ -338,7 +338,7 @@ static int run_lap(wt2_ctx* c, const double* u, const double* v, double* out, co
 // stage 0: write (du, dv) to (Ou, Ov); stage 1..4: classical RK4 stage on the
 // context's registers with state Y (stage 1 reads Y, later stages S).
 static int run_stage(wt2_ctx* c, int stage, const double* u, const double* v, double* Ou,
-                     double* Ov, double* Yu, double* Yv, double dt)
+                     double* Ov, double* Yu, double* Yv, double dt, int only = -2)
 {
     const bool mixed = is_mixed(c);
     const bool ko = c->d.eps_ko > 0 && c->kos > 0;
@@ -349,7 +349,7 @@ static int run_stage(wt2_ctx* c, int stage, const double* u, const double* v, do
     const int final_epi = stage == 0 ? EPI_OUT_RHS : EPI_RK1 + (stage - 1);
     void* P = c->Pb;
     bool have_p = false;
-    if (c->curved) {
+    if (c->curved && (only == -2 || only == -1)) {
         CurvedCall cc;
         std::memset(&cc, 0, sizeof(cc));
         cc.u = Su; cc.v = Sv; cc.in32 = (variant == VAR_MIX_IN32);
@@ -358,12 +358,16 @@ static int run_stage(wt2_ctx* c, int stage, const double* u, const double* v, do
         for (int f = 0; f < 6; ++f) cc.bv[f] = bv[f];
         int rc = pass_rc(curved_operator(c->cd, cc, c->stream), "curved operator");
         if (rc) return rc;
-        have_p = true;
     }
+    if (c->curved) have_p = true;
     for (int axis = 0; axis < 3; ++axis) {
         const bool last = axis == 2;
         const bool lap = !c->curved;
         if (!last && !lap && !ko) continue;   // nothing to add along this axis
+        if (only != -2 && only != axis) {
+            have_p = true;
+            continue;
+        }
         PassDesc p = base_pass(c, axis, bv);
         p.variant = variant;
         p.lap = lap;
@@ -403,9 +407,8 @@ static int run_energy(wt2_ctx* c, const double* u, const double* v, double* Edst
     std::memset(&e, 0, sizeof(e));
     e.N = c->N; e.Ne = c->Ne;
     e.u = u; e.v = v; e.Lu = c->Lu;
-    e.wu = c->curved ? c->cd.wu : nullptr;
     e.wv = c->curved ? c->cd.wv : nullptr;
-    e.hvol = c->h[0] * c->h[1] * c->h[2];
+    e.hvol = c->curved ? 1.0 : c->h[0] * c->h[1] * c->h[2];
     for (int i = 0; i < c->N; ++i) e.w[i] = c->w[i];
     e.partial = c->red;
     e.out = Edst;
@@ -762,6 +765,14 @@ int wt2_rk4_steps(wt2_ctx* c, double* u, double* ud, double dt, int32_t nsteps,
             }
     }
     return WT2_OK;
+}
+
+int wt2_launch_pass(wt2_ctx* c, int32_t axis, int32_t stage, double* u, double* ud, double dt)
+{
+    if (!c || !u || !ud) { set_error("wt2_launch_pass: null pointer"); return WT2_ERR_ARG; }
+    if (axis < -1 || axis > 2 || stage < 1 || stage > 4) { set_error("wt2_launch_pass: bad axis / stage"); return WT2_ERR_ARG; }
+    if (axis == -1 && !c->curved) { set_error("wt2_launch_pass: axis -1 needs a curved context"); return WT2_ERR_ARG; }
+    return run_stage(c, stage, nullptr, nullptr, nullptr, nullptr, u, ud, dt, axis);
 }
 
 int wt2_energy(wt2_ctx* c, const double* u, const double* ud, double* E, int32_t flags)

@@ -1,10 +1,31 @@
-// wt2_curved.hip -- curved operator (placeholder until the element kernels land).
+// wt2_curved.hip -- curved (warped mesh and/or static ADM background) operator.
+//
+// Second-order form of the reference's conservative ADM system
+// (src/wave3d_curved.jl:1-12): for a static background
+//   u_tt = wgt [ d_a (K^ab d_b u) + c^a d_a u_t + d_a (c^a u_t) ]
+// in unit-element coordinates, with per-node FP64 coefficients precomputed once
+// (the reference resamples them every stage, src/evolve.jl:1289):
+//   K^ab = |J| Jinv_ai K^ij Jinv_bj,  K^ij = sqrt(g)(alpha g^ij - b^i b^j / alpha),
+//   c^a  = |J| Jinv_ai sqrt(g) b^i / alpha,   wgt = alpha / (sqrt(g) |J|),
+// and the SIPG interface treatment of README.md:57-81 generalised to K:
+//   vol   sum_a G_a (F^a + c^a v),   F^a = K^ab G_b u,   plus c^a G_a v
+//   face  (1/w_r)[ -1/2 n dF^a - tau kappa du - n c^a (v - v_nbr) ]
+//         + alpha n sum_b (H_b^-1 G_b^T H_b)(K^ab du)   (symmetrising term)
+// kappa = K^aa + sum_{b!=a} |K^ab| averaged over the two sides.  Two kernels
+// per evaluation: one writes the normal flux F^a of every element face node
+// into a compact face buffer (6 N^2 per element), the second does everything
+// else for one element per workgroup with the element staged in LDS and the
+// neighbours' face data read from the face buffer and the field arrays.
 #include <cmath>
+#include <type_traits>
+#include <vector>
 
 #include "wt2_curved.hpp"
 #include "wt2_internal.hpp"
 
 namespace wt2 {
+
+// ------------------------------------------------------------ host set-up
 
 void mesh_coords(const wt2_mesh_desc& d, const double* xi_n, double* xyz)
 {
@@ -30,18 +51,168 @@ void mesh_coords(const wt2_mesh_desc& d, const double* xi_n, double* xyz)
     }
 }
 
-int curved_init(CurvedData&, const wt2_mesh_desc&, const double*, const double*, const double*,
-                const double*, hipStream_t, int64_t*)
+static int upload(CurvedData& cd, const std::vector<double>& coef, const std::vector<double>& wv,
+                  hipStream_t s)
 {
-    set_error("curved meshes / backgrounds are not built yet");
-    return WT2_ERR_UNSUPPORTED;
+    const size_t n = size_t(cd.npts);
+    std::vector<float> c32(coef.size());
+    for (size_t q = 0; q < coef.size(); ++q) c32[q] = float(coef[q]);
+    if (hipMemcpyAsync(cd.coef64, coef.data(), 13 * n * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(cd.coef32, c32.data(), 13 * n * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(cd.wv, wv.data(), n * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        set_error("curved set-up: host-to-device copy failed");
+        return WT2_ERR_HIP;
+    }
+    return WT2_OK;
 }
 
-int curved_set_background(CurvedData&, const wt2_mesh_desc&, const double*, const double*,
-                          const double*, const double*, hipStream_t)
+// Per-node coefficients (SoA, 13 x npts) and the energy weight H3/wgt, from the
+// analytic warp Jacobian and the background (nullptr = Minkowski).
+static void compute_coefs(const wt2_mesh_desc& d, const double* xi_n, const double* w,
+                          const double* bg10, std::vector<double>& coef, std::vector<double>& wv,
+                          double* h_min)
 {
-    set_error("curved meshes / backgrounds are not built yet");
-    return WT2_ERR_UNSUPPORTED;
+    const int N = d.N;
+    const int* M = d.M;
+    const int64_t Ne = int64_t(M[0]) * M[1] * M[2];
+    const int64_t npts = int64_t(N) * N * N * Ne;
+    coef.assign(size_t(13 * npts), 0.0);
+    wv.assign(size_t(npts), 0.0);
+    double L[3];
+    for (int a = 0; a < 3; ++a) L[a] = d.x1[a] - d.x0[a];
+    const double tp = 2.0 * M_PI;
+    double hmin = 1e300;
+#pragma omp parallel for reduction(min : hmin) schedule(static)
+    for (int64_t e = 0; e < Ne; ++e) {
+        const int ec[3] = {int(e % M[0]), int((e / M[0]) % M[1]), int(e / (int64_t(M[0]) * M[1]))};
+        double xl[16][16][16][3];
+        for (int k = 0; k < N; ++k)
+            for (int j = 0; j < N; ++j)
+                for (int i = 0; i < N; ++i) {
+                    const int ii[3] = {i, j, k};
+                    double xi[3];
+                    for (int a = 0; a < 3; ++a) xi[a] = (ec[a] + xi_n[ii[a]]) / M[a];
+                    const double s1 = std::sin(tp * xi[0]), s2 = std::sin(tp * xi[1]), s3 = std::sin(tp * xi[2]);
+                    const double c1 = std::cos(tp * xi[0]), c2 = std::cos(tp * xi[1]), c3 = std::cos(tp * xi[2]);
+                    const double A = d.warp_A;
+                    const double sv = A * s1 * s2 * s3;
+                    const double ds[3] = {A * tp * c1 * s2 * s3, A * tp * s1 * c2 * s3, A * tp * s1 * s2 * c3};
+                    for (int a = 0; a < 3; ++a) xl[k][j][i][a] = d.x0[a] + L[a] * (xi[a] + sv);
+                    double J[3][3];
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) J[a][b] = L[a] * ((a == b ? 1.0 : 0.0) + ds[b]) / M[b];
+                    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+                    double Ji[3][3];
+                    Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+                    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+                    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+                    Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+                    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+                    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+                    Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+                    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+                    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+                    const double aJ = std::fabs(det);
+                    const int64_t n = i + int64_t(N) * (j + int64_t(N) * (k + int64_t(N) * e));
+                    double al = 1.0, be[3] = {0, 0, 0}, g[6] = {1, 0, 0, 1, 0, 1};
+                    if (bg10) {
+                        const double* b = bg10 + 10 * n;
+                        al = b[0];
+                        for (int q = 0; q < 3; ++q) be[q] = b[1 + q];
+                        for (int q = 0; q < 6; ++q) g[q] = b[4 + q];
+                    }
+                    // sqrt(det gamma) and the cofactor inverse (src/wave3d_curved.jl:85-94)
+                    const double g11 = g[0], g12 = g[1], g13 = g[2], g22 = g[3], g23 = g[4], g33 = g[5];
+                    const double dg = g11 * (g22 * g33 - g23 * g23) - g12 * (g12 * g33 - g23 * g13) +
+                                      g13 * (g12 * g23 - g22 * g13);
+                    const double sg = std::sqrt(dg);
+                    double gu[3][3];
+                    gu[0][0] = (g22 * g33 - g23 * g23) / dg;
+                    gu[0][1] = gu[1][0] = (g13 * g23 - g12 * g33) / dg;
+                    gu[0][2] = gu[2][0] = (g12 * g23 - g13 * g22) / dg;
+                    gu[1][1] = (g11 * g33 - g13 * g13) / dg;
+                    gu[1][2] = gu[2][1] = (g12 * g13 - g11 * g23) / dg;
+                    gu[2][2] = (g11 * g22 - g12 * g12) / dg;
+                    double K[3][3], cv[3];
+                    for (int p = 0; p < 3; ++p) {
+                        cv[p] = sg * be[p] / al;
+                        for (int q = 0; q < 3; ++q) K[p][q] = sg * (al * gu[p][q] - be[p] * be[q] / al);
+                    }
+                    double Kh[3][3], ch[3];
+                    for (int a = 0; a < 3; ++a) {
+                        ch[a] = 0.0;
+                        for (int p = 0; p < 3; ++p) ch[a] += aJ * Ji[a][p] * cv[p];
+                        for (int b = 0; b < 3; ++b) {
+                            double acc = 0.0;
+                            for (int p = 0; p < 3; ++p)
+                                for (int q = 0; q < 3; ++q) acc += Ji[a][p] * K[p][q] * Ji[b][q];
+                            Kh[a][b] = aJ * acc;
+                        }
+                    }
+                    const double vals[13] = {Kh[0][0], Kh[0][1], Kh[0][2], Kh[1][1], Kh[1][2], Kh[2][2],
+                                             ch[0], ch[1], ch[2], al / (sg * aJ),
+                                             Kh[0][0] + std::fabs(Kh[0][1]) + std::fabs(Kh[0][2]),
+                                             Kh[1][1] + std::fabs(Kh[0][1]) + std::fabs(Kh[1][2]),
+                                             Kh[2][2] + std::fabs(Kh[0][2]) + std::fabs(Kh[1][2])};
+                    for (int q = 0; q < 13; ++q) coef[size_t(q * npts + n)] = vals[q];
+                    wv[size_t(n)] = w[i] * w[j] * w[k] / vals[9];
+                }
+        // smallest spacing between reference-axis-adjacent nodes (src/evolve.jl:60-69)
+        for (int k = 0; k < N; ++k)
+            for (int j = 0; j < N; ++j)
+                for (int i = 0; i < N; ++i) {
+                    const int ii[3] = {i, j, k};
+                    for (int a = 0; a < 3; ++a) {
+                        if (ii[a] == 0) continue;
+                        int jj[3] = {i, j, k};
+                        jj[a] -= 1;
+                        const double* p = xl[k][j][i];
+                        const double* q = xl[jj[2]][jj[1]][jj[0]];
+                        const double dd = std::sqrt((p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) +
+                                                    (p[2] - q[2]) * (p[2] - q[2]));
+                        hmin = std::min(hmin, dd);
+                    }
+                }
+    }
+    *h_min = hmin;
+}
+
+int curved_init(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const double* w,
+                const double* G, const double* bg10, hipStream_t s, int64_t* bytes)
+{
+    cd = CurvedData();
+    cd.N = d.N;
+    for (int a = 0; a < 3; ++a) { cd.M[a] = d.M[a]; cd.periodic[a] = d.periodic[a]; }
+    cd.Ne = int64_t(d.M[0]) * d.M[1] * d.M[2];
+    cd.npts = cd.Ne * d.N * d.N * d.N;
+    cd.tau = d.tau;
+    for (int i = 0; i < d.N; ++i) cd.w[i] = w[i];
+    for (int i = 0; i < d.N * d.N; ++i) cd.G[i] = G[i];
+    const size_t n = size_t(cd.npts);
+    const size_t fb = size_t(cd.Ne) * 6 * d.N * d.N;
+    if (hipMalloc(&cd.coef64, 13 * n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&cd.coef32, 13 * n * sizeof(float)) != hipSuccess ||
+        hipMalloc(&cd.wv, n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&cd.fbuf, fb * sizeof(double)) != hipSuccess) {
+        curved_free(cd);
+        set_error("curved set-up: device allocation failed");
+        return WT2_ERR_OOM;
+    }
+    *bytes += int64_t(13 * n * (sizeof(double) + sizeof(float)) + n * sizeof(double) + fb * sizeof(double));
+    std::vector<double> coef, wv;
+    compute_coefs(d, xi, w, bg10, coef, wv, &cd.h_min);
+    return upload(cd, coef, wv, s);
+}
+
+int curved_set_background(CurvedData& cd, const wt2_mesh_desc& d, const double* xi,
+                          const double* w, const double*, const double* bg10, hipStream_t s)
+{
+    std::vector<double> coef, wv;
+    compute_coefs(d, xi, w, bg10, coef, wv, &cd.h_min);
+    return upload(cd, coef, wv, s);
 }
 
 void curved_free(CurvedData& cd)
@@ -50,9 +221,289 @@ void curved_free(CurvedData& cd)
     cd = CurvedData();
 }
 
-hipError_t curved_operator(const CurvedData&, const CurvedCall&, hipStream_t)
+// ---------------------------------------------------------------- kernels
+
+template <int N, typename TC>
+struct CConst {
+    TC G[N * N];
+    TC w[N];
+    TC winv[N];
+    TC tau;
+    TC bv[6];
+};
+
+template <int N, typename TC>
+struct CArgs {
+    CConst<N, TC> k;
+    const void* u; const void* v;   // stage fields (TS)
+    const TC* coef;                 // 13 x npts SoA
+    TC* fbuf;                       // 6 N^2 per element
+    void* out;                      // output (TO): wgt (...) or unweighted
+    int64_t npts;
+    int M0, M1, M2, per0, per1, per2;
+    int adv, weighted;
+};
+
+template <int N>
+__device__ __forceinline__ int fidx(int a, int r, int p, int q)
 {
-    return hipErrorNotSupported;
+    if (a == 0) return r + N * (p + N * q);
+    if (a == 1) return p + N * (r + N * q);
+    return p + N * (q + N * r);
+}
+
+// neighbour element across face (a, side), -1 at a boundary
+__device__ __forceinline__ int64_t nbr_elem(int64_t e, int a, int side, int M0, int M1, int M2,
+                                            int p0, int p1, int p2)
+{
+    int ec[3] = {int(e % M0), int((e / M0) % M1), int(e / (int64_t(M0) * M1))};
+    const int Ms[3] = {M0, M1, M2};
+    const int ps[3] = {p0, p1, p2};
+    int t = ec[a] + (side ? 1 : -1);
+    if (t < 0 || t >= Ms[a]) {
+        if (!ps[a]) return -1;
+        t = (t + Ms[a]) % Ms[a];
+    }
+    ec[a] = t;
+    return ec[0] + int64_t(M0) * (ec[1] + int64_t(M1) * ec[2]);
+}
+
+// symmetric K^ab stored as (11, 12, 13, 22, 23, 33)
+__device__ __forceinline__ int kidx(int a, int b)
+{
+    if (a == b) return a == 0 ? 0 : (a == 1 ? 3 : 5);
+    const int s = a + b;
+    return s == 1 ? 1 : (s == 2 ? 2 : 4);
+}
+
+template <int N, typename TC>
+__device__ __forceinline__ TC kab(const TC* coef, int64_t npts, int64_t n, int a, int b)
+{
+    return coef[kidx(a, b) * npts + n];
+}
+
+// reference gradient of the element field in LDS at node (i,j,k)
+template <int N, typename TC>
+__device__ __forceinline__ void grad_lds(const TC* f, const CConst<N, TC>& k, int i, int j, int l, TC g[3])
+{
+    TC s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int p = 0; p < N; ++p) {
+        s0 = fma(k.G[i * N + p], f[p + N * (j + N * l)], s0);
+        s1 = fma(k.G[j * N + p], f[i + N * (p + N * l)], s1);
+        s2 = fma(k.G[l * N + p], f[i + N * (j + N * p)], s2);
+    }
+    g[0] = s0; g[1] = s1; g[2] = s2;
+}
+
+// Kernel A: normal flux F^a = K^ab G_b u at every face node of every element.
+template <int N, typename TC, typename TS>
+__global__ void __launch_bounds__(256) curved_faceflux_kernel(CArgs<N, TC> a)
+{
+    __shared__ TC su[N * N * N];
+    const int64_t e = blockIdx.x;
+    const TS* u = static_cast<const TS*>(a.u);
+    for (int q = threadIdx.x; q < N * N * N; q += blockDim.x) su[q] = TC(u[e * N * N * N + q]);
+    __syncthreads();
+    for (int q = threadIdx.x; q < 6 * N * N; q += blockDim.x) {
+        const int f = q / (N * N), pq = q - f * N * N;
+        const int p = pq % N, qq = pq / N;
+        const int ax = f >> 1, r = (f & 1) ? N - 1 : 0;
+        const int n = fidx<N>(ax, r, p, qq);
+        const int i = n % N, j = (n / N) % N, l = n / (N * N);
+        TC g[3];
+        grad_lds<N, TC>(su, a.k, i, j, l, g);
+        const int64_t gn = e * N * N * N + n;
+        TC F = kab<N, TC>(a.coef, a.npts, gn, ax, 0) * g[0];
+        F = fma(kab<N, TC>(a.coef, a.npts, gn, ax, 1), g[1], F);
+        F = fma(kab<N, TC>(a.coef, a.npts, gn, ax, 2), g[2], F);
+        a.fbuf[e * 6 * N * N + q] = F;
+    }
+}
+
+// Kernel B: the whole curved operator for one element per workgroup.
+template <int N, typename TC, typename TS, typename TO>
+__global__ void __launch_bounds__(1024) curved_volume_kernel(CArgs<N, TC> a)
+{
+    constexpr int N3 = N * N * N, N2 = N * N;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    TC* su = reinterpret_cast<TC*>(smem_raw);   // N3
+    TC* sv = su + N3;                           // N3
+    TC* W = sv + N3;                            // 3 N3: F^a + c^a v
+    TC* S = W + 3 * N3;                         // 3 x 6 N2: alpha n K^ab du per face node
+    TC* Pt = S + 18 * N2;                       // 6 N2: point terms of the face SAT
+    const int64_t e = blockIdx.x;
+    const int64_t eb = e * N3;
+    const TS* u = static_cast<const TS*>(a.u);
+    const TS* v = static_cast<const TS*>(a.v);
+    const int tid = threadIdx.x;
+    for (int q = tid; q < N3; q += blockDim.x) {
+        su[q] = TC(u[eb + q]);
+        sv[q] = a.adv ? TC(v[eb + q]) : TC(0);
+    }
+    __syncthreads();
+    // fluxes at every node
+    TC cgv = TC(0);   // c^a G_a v at this thread's node
+    for (int q = tid; q < N3; q += blockDim.x) {
+        const int i = q % N, j = (q / N) % N, l = q / N2;
+        TC g[3];
+        grad_lds<N, TC>(su, a.k, i, j, l, g);
+        const int64_t gn = eb + q;
+        for (int ax = 0; ax < 3; ++ax) {
+            TC F = kab<N, TC>(a.coef, a.npts, gn, ax, 0) * g[0];
+            F = fma(kab<N, TC>(a.coef, a.npts, gn, ax, 1), g[1], F);
+            F = fma(kab<N, TC>(a.coef, a.npts, gn, ax, 2), g[2], F);
+            if (a.adv) F = fma(a.coef[(6 + ax) * a.npts + gn], sv[q], F);
+            W[ax * N3 + q] = F;
+        }
+        if (a.adv) {
+            TC gv[3];
+            grad_lds<N, TC>(sv, a.k, i, j, l, gv);
+            cgv = a.coef[6 * a.npts + gn] * gv[0] + a.coef[7 * a.npts + gn] * gv[1] +
+                  a.coef[8 * a.npts + gn] * gv[2];
+        }
+    }
+    // face-node data (jumps against the neighbour or the Dirichlet value)
+    for (int q = tid; q < 6 * N2; q += blockDim.x) {
+        const int f = q / N2, pq = q - f * N2;
+        const int p = pq % N, qq = pq / N;
+        const int ax = f >> 1, side = f & 1;
+        const int r = side ? N - 1 : 0, rn = side ? 0 : N - 1;
+        const TC nrm = side ? TC(1) : TC(-1);
+        const int n = fidx<N>(ax, r, p, qq);
+        const int64_t gn = eb + n;
+        const int64_t en = nbr_elem(e, ax, side, a.M0, a.M1, a.M2, a.per0, a.per1, a.per2);
+        TC du, dF = TC(0), kap = a.coef[(10 + ax) * a.npts + gn], dv = TC(0), alpha;
+        const int i = n % N, j = (n / N) % N, l = n / N2;
+        if (en >= 0) {
+            const int nn = fidx<N>(ax, rn, p, qq);
+            const int64_t gnn = en * N3 + nn;
+            du = su[n] - TC(u[gnn]);
+            TC g[3];
+            grad_lds<N, TC>(su, a.k, i, j, l, g);
+            TC Fs = kab<N, TC>(a.coef, a.npts, gn, ax, 0) * g[0];
+            Fs = fma(kab<N, TC>(a.coef, a.npts, gn, ax, 1), g[1], Fs);
+            Fs = fma(kab<N, TC>(a.coef, a.npts, gn, ax, 2), g[2], Fs);
+            const int fn = (ax << 1) | (1 - side);
+            dF = Fs - a.fbuf[en * 6 * N2 + fn * N2 + pq];
+            kap = TC(0.5) * (kap + a.coef[(10 + ax) * a.npts + gnn]);
+            if (a.adv) dv = sv[n] - TC(v[gnn]);
+            alpha = TC(0.5);
+        } else {
+            du = su[n] - a.k.bv[f];
+            if (a.adv) dv = sv[n];
+            alpha = TC(1);
+        }
+        for (int b = 0; b < 3; ++b) S[(b * 6 + f) * N2 + pq] = alpha * nrm * kab<N, TC>(a.coef, a.npts, gn, ax, b) * du;
+        TC pt = TC(-0.5) * nrm * dF - a.k.tau * kap * du;
+        if (a.adv) pt -= nrm * a.coef[(6 + ax) * a.npts + gn] * dv;
+        Pt[q] = pt * a.k.winv[r];
+    }
+    __syncthreads();
+    TO* out = static_cast<TO*>(a.out);
+    for (int q = tid; q < N3; q += blockDim.x) {
+        const int i = q % N, j = (q / N) % N, l = q / N2;
+        const int idx[3] = {i, j, l};
+        TC s = cgv;
+        // volume: sum_a G_a W^a
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            s = fma(a.k.G[i * N + p], W[0 * N3 + p + N * (j + N * l)], s);
+            s = fma(a.k.G[j * N + p], W[1 * N3 + i + N * (p + N * l)], s);
+            s = fma(a.k.G[l * N + p], W[2 * N3 + i + N * (j + N * p)], s);
+        }
+        for (int ax = 0; ax < 3; ++ax) {
+            // tangential indices (p, q) of this node for faces normal to ax
+            const int tp = ax == 0 ? j : i;
+            const int tq = ax == 2 ? j : l;
+            const int m = idx[ax];
+            const int tb0 = ax == 0 ? 1 : 0;   // axis of tangential index p
+            const int tb1 = ax == 2 ? 1 : 2;   // axis of tangential index q
+            for (int side = 0; side < 2; ++side) {
+                const int f = (ax << 1) | side;
+                const int r = side ? N - 1 : 0;
+                // b = a symmetrising term along the normal line
+                s = fma(a.k.G[r * N + m] * a.k.winv[m], S[(ax * 6 + f) * N2 + tp + N * tq], s);
+                if (m != r) continue;
+                // point terms at the face node
+                s += Pt[f * N2 + tp + N * tq];
+                // b != a symmetrising terms along the face
+                TC t0 = TC(0), t1 = TC(0);
+#pragma unroll
+                for (int ll = 0; ll < N; ++ll) {
+                    t0 = fma(a.k.G[ll * N + tp] * a.k.w[ll], S[(tb0 * 6 + f) * N2 + ll + N * tq], t0);
+                    t1 = fma(a.k.G[ll * N + tq] * a.k.w[ll], S[(tb1 * 6 + f) * N2 + tp + N * ll], t1);
+                }
+                s = fma(a.k.winv[r], fma(a.k.winv[tp], t0, a.k.winv[tq] * t1), s);
+            }
+        }
+        if (a.weighted) s *= a.coef[9 * a.npts + eb + q];
+        out[eb + q] = TO(s);
+    }
+}
+
+template <int N, typename TC, typename TS, typename TO>
+static hipError_t launch_c(const CurvedData& cd, const CurvedCall& cc, hipStream_t st)
+{
+    CArgs<N, TC> a;
+    for (int i = 0; i < N * N; ++i) a.k.G[i] = TC(cd.G[i]);
+    for (int i = 0; i < N; ++i) { a.k.w[i] = TC(cd.w[i]); a.k.winv[i] = TC(1.0 / cd.w[i]); }
+    a.k.tau = TC(cd.tau);
+    for (int f = 0; f < 6; ++f) a.k.bv[f] = TC(cc.bv[f]);
+    a.u = cc.u; a.v = cc.v;
+    a.coef = reinterpret_cast<const TC*>(std::is_same<TC, float>::value ? static_cast<const void*>(cd.coef32)
+                                                                          : static_cast<const void*>(cd.coef64));
+    a.fbuf = reinterpret_cast<TC*>(cd.fbuf);
+    a.out = cc.out ? static_cast<void*>(cc.out) : cc.Pv;
+    a.npts = cd.npts;
+    a.M0 = cd.M[0]; a.M1 = cd.M[1]; a.M2 = cd.M[2];
+    a.per0 = cd.periodic[0]; a.per1 = cd.periodic[1]; a.per2 = cd.periodic[2];
+    a.adv = cc.v != nullptr;
+    a.weighted = cc.weighted;
+    hipLaunchKernelGGL((curved_faceflux_kernel<N, TC, TS>), dim3(unsigned(cd.Ne)), dim3(256), 0, st, a);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    constexpr int N3 = N * N * N;
+    const int nthr = ((N3 + 63) / 64) * 64;
+    const size_t shmem = (5 * N3 + 24 * N * N) * sizeof(TC);
+    auto kern = curved_volume_kernel<N, TC, TS, TO>;
+    static bool raised = false;
+    if (shmem > 64 * 1024 && !raised) {
+        err = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (err != hipSuccess) return err;
+        raised = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(unsigned(cd.Ne)), dim3(nthr), shmem, st, a);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_cn(const CurvedData& cd, const CurvedCall& cc, hipStream_t st)
+{
+    if (!cc.mixed) {
+        if (cc.out) return launch_c<N, double, double, double>(cd, cc, st);
+        return launch_c<N, double, double, double>(cd, cc, st);
+    }
+    if (cc.out) return launch_c<N, float, double, double>(cd, cc, st);
+    if (cc.in32) return launch_c<N, float, float, float>(cd, cc, st);
+    return launch_c<N, float, double, float>(cd, cc, st);
+}
+
+hipError_t curved_operator(const CurvedData& cd, const CurvedCall& cc, hipStream_t s)
+{
+    if (!cd.coef64) return hipErrorNotInitialized;
+    switch (cd.N) {
+    case 3: return launch_cn<3>(cd, cc, s);
+    case 4: return launch_cn<4>(cd, cc, s);
+    case 5: return launch_cn<5>(cd, cc, s);
+    case 6: return launch_cn<6>(cd, cc, s);
+    case 7: return launch_cn<7>(cd, cc, s);
+    case 8: return launch_cn<8>(cd, cc, s);
+    case 9: return launch_cn<9>(cd, cc, s);
+    case 10: return launch_cn<10>(cd, cc, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace wt2

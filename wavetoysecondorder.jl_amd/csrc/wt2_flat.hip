@@ -45,10 +45,18 @@ struct KArgs {
     void* Au; void* Av;
     double* Ou; double* Ov;
     double dt;
-    int M0, M1, M2;
-    int axis, periodic, nb, epi;
+    // per-axis geometry, precomputed on the host (the kernel has no branch on
+    // the axis; see launch_t for the meaning of each field)
+    int Ma;            // elements along the line
+    int T0, slo, shi;  // tile -> (t % T0) * slo + (t / T0) * shi = first element
+    int estride;       // element-index stride along the line
+    int nbk;           // line blocks per element column
+    int CPE, CO, clen; // chunks per element, chunk offset per block, chunk length
+    int ES, CS;        // LDS strides: per element, per chunk within an element
+    int SR, LA, LB;    // segment: node stride, line -> (line/N)*LA + (line%N)*LB
+    int LT, lpw;       // lines per tile, lines per wavefront
+    int periodic, epi;
     int lap, ko, needv, keepv;
-    int lpw, CS, ES;
 };
 
 template <typename T>
@@ -130,22 +138,20 @@ __global__ void __launch_bounds__(1024) flat_pass_kernel(KArgs<N, TC> a)
     TS* lds = reinterpret_cast<TS*>(smem_raw);
     constexpr int N3 = N * N * N;
     constexpr bool MIXED = std::is_same<TP, float>::value;
-    const int axis = a.axis;
-    const int Mx = a.M0, My = a.M1, Mz = a.M2;
-    const int nb = a.nb, nbk = N / nb, LT = N * nb;
 
     // ---- tile decode: two fixed element coordinates and a line block
     int t = blockIdx.x;
-    const int blk = t % nbk;
-    t /= nbk;
-    int ebase, estride, Ma;
-    if (axis == 0) { const int ey = t % My, ez = t / My; ebase = Mx * (ey + My * ez); estride = 1; Ma = Mx; }
-    else if (axis == 1) { const int ex = t % Mx, ez = t / Mx; ebase = ex + Mx * My * ez; estride = Mx; Ma = My; }
-    else { const int ex = t % Mx, ey = t / Mx; ebase = ex + Mx * ey; estride = Mx * My; Ma = Mz; }
-    const int nch = axis < 2 ? Ma : Ma * N;
-    const int clen = axis < 2 ? N * N * nb : N * nb;
-    const int fstride = axis < 2 ? Ma * a.CS : Ma * a.ES;
-    const int coff = axis < 2 ? blk * nb * N * N : blk * nb * N;
+    const int blk = t % a.nbk;
+    t /= a.nbk;
+    const int ebase = (t % a.T0) * a.slo + (t / a.T0) * a.shi;
+    const int estride = a.estride;
+    const int Ma = a.Ma;
+    const int CPE = a.CPE;
+    const int nch = Ma * CPE;
+    const int clen = a.clen;
+    const int fstride = Ma * a.ES;
+    const int coff = blk * a.CO;
+    const int LT = a.LT;
 
     // LDS slots: u (SIPG input), v (KO input / u' of the final pass), c (result)
     const bool lap = a.lap != 0, needv = a.needv != 0, keepv = a.keepv != 0;
@@ -160,14 +166,13 @@ __global__ void __launch_bounds__(1024) flat_pass_kernel(KArgs<N, TC> a)
     if (clen <= nthr) { o = tid % clen; c0 = tid / clen; cstep = nthr / clen; ostep = clen; }
     else { o = tid; c0 = 0; cstep = 1; ostep = nthr; }
 
+    // chunk c = (element m along the line, layer kk < CPE) of clen contiguous values
     auto goff = [&](int c, int oo) -> int64_t {
-        if (axis < 2) return int64_t(ebase + c * estride) * N3 + coff + oo;
-        const int m = c / N, kk = c - m * N;
+        const int m = c / CPE, kk = c - m * CPE;
         return int64_t(ebase + m * estride) * N3 + kk * N * N + coff + oo;
     };
     auto loff = [&](int c, int oo) -> int {
-        if (axis < 2) return c * a.CS + oo;
-        const int m = c / N, kk = c - m * N;
+        const int m = c / CPE, kk = c - m * CPE;
         return m * a.ES + kk * a.CS + oo;
     };
 
@@ -199,11 +204,8 @@ __global__ void __launch_bounds__(1024) flat_pass_kernel(KArgs<N, TC> a)
     const bool hasR = active && (m < Ma - 1 || per);
     const int srcL = hasL ? ((m > 0 ? m - 1 : Ma - 1) * lpw + lw) : lane;
     const int srcR = hasR ? ((m < Ma - 1 ? m + 1 : 0) * lpw + lw) : lane;
-    int SE, SR, lo;
-    if (axis == 0) { SE = a.CS; SR = 1; lo = line * N; }
-    else if (axis == 1) { SE = a.CS; SR = N; lo = (line / N) * N * N + (line % N); }
-    else { SE = a.ES; SR = a.CS; lo = line; }
-    const int base = active ? (m * SE + lo) : 0;
+    const int SR = a.SR;
+    const int base = active ? (m * a.ES + (line / N) * a.LA + (line % N) * a.LB) : 0;
 
     TC cv[N];
 #pragma unroll
@@ -317,38 +319,53 @@ static hipError_t launch_t(const PassDesc& p, hipStream_t s)
     a.So_u = p.So_u; a.So_v = p.So_v; a.Yu = p.Yu; a.Yv = p.Yv;
     a.Au = p.Au; a.Av = p.Av; a.Ou = p.Ou; a.Ov = p.Ov;
     a.dt = p.dt;
-    a.M0 = p.M[0]; a.M1 = p.M[1]; a.M2 = p.M[2];
-    a.axis = p.axis; a.periodic = p.periodic; a.nb = p.nb; a.epi = p.epi;
+    a.periodic = p.periodic; a.epi = p.epi;
     a.lap = p.lap; a.ko = p.ko; a.needv = p.needv; a.keepv = p.keepv;
-    const int Ma = p.M[p.axis];
-    a.lpw = pick_lpw(Ma);
-    const int LT = N * p.nb;
-    const int waves = (LT + a.lpw - 1) / a.lpw;
-    const int nthr = waves * 64;
-    const int clen = p.axis < 2 ? N * N * p.nb : N * p.nb;
-    if (nthr > 1024) return hipErrorInvalidConfiguration;
-    int fstride;
-    if (p.axis < 2) {
-        a.CS = clen + 1;
-        a.ES = 0;
-        fstride = Ma * a.CS;
-    } else {
-        a.CS = clen;
-        int es = N * clen;
+    const int Mx = p.M[0], My = p.M[1], Mz = p.M[2];
+    const int nb = p.nb, nbk = N / nb;
+    int tiles;
+    // Tiles own whole element lines along the axis.  Memory chunks: for x and y
+    // a tile's share of one element is contiguous (nb planes of N^2 values);
+    // for z it is N layers of nb*N values.  LDS keeps the same chunk order with
+    // padded strides; a lane's segment is N values at stride SR.
+    if (p.axis == 0) {          // fixed (ey, ez), lines (j, k in block)
+        a.Ma = Mx; a.T0 = My; a.slo = Mx; a.shi = Mx * My; a.estride = 1;
+        a.CPE = 1; a.CO = nb * N * N; a.clen = nb * N * N;
+        a.CS = a.clen; a.ES = a.clen + 1;
+        a.SR = 1; a.LA = N * N; a.LB = N;
+        tiles = My * Mz * nbk;
+    } else if (p.axis == 1) {   // fixed (ex, ez), lines (i, k in block)
+        a.Ma = My; a.T0 = Mx; a.slo = 1; a.shi = Mx * My; a.estride = Mx;
+        a.CPE = 1; a.CO = nb * N * N; a.clen = nb * N * N;
+        a.CS = a.clen; a.ES = a.clen + 1;
+        a.SR = N; a.LA = N * N; a.LB = 1;
+        tiles = Mx * Mz * nbk;
+    } else {                    // fixed (ex, ey), lines (i, j in block)
+        a.Ma = Mz; a.T0 = Mx; a.slo = 1; a.shi = Mx; a.estride = Mx * My;
+        a.CPE = N; a.CO = nb * N; a.clen = nb * N;
+        a.CS = a.clen;
+        a.SR = a.CS; a.LA = N; a.LB = 1;
+        tiles = Mx * My * nbk;
+    }
+    a.nbk = nbk;
+    a.LT = N * nb;
+    a.lpw = pick_lpw(a.Ma);
+    if (p.axis == 2) {
+        // pad the per-element stride so that lanes (element, line) of one
+        // wavefront fall on distinct LDS banks
+        const int es = N * a.clen;
         const int want = a.lpw % 16;
         int pad = ((want - es % 16) + 16) % 16;
         if (pad == 0 && want == 0) pad = 1;
         a.ES = es + pad;
-        fstride = Ma * a.ES;
     }
+    const int waves = (a.LT + a.lpw - 1) / a.lpw;
+    const int nthr = waves * 64;
+    if (nthr > 1024) return hipErrorInvalidConfiguration;
+    const int fstride = a.Ma * a.ES;
     const int nslots = (p.lap && p.needv) || (!p.lap && p.keepv) ? 2 : 1;
     const size_t shmem = size_t(fstride) * nslots * sizeof(TS);
     if (shmem > 160 * 1024) return hipErrorInvalidConfiguration;
-    const int nbk = N / p.nb;
-    int tiles;
-    if (p.axis == 0) tiles = p.M[1] * p.M[2] * nbk;
-    else if (p.axis == 1) tiles = p.M[0] * p.M[2] * nbk;
-    else tiles = p.M[0] * p.M[1] * nbk;
     auto kern = flat_pass_kernel<N, TC, TS, TP>;
     // raise the dynamic-LDS limit once per instantiation (outside any capture:
     // the first launch of a configuration is always eager)
@@ -393,8 +410,9 @@ hipError_t launch_flat_pass(const PassDesc& p, hipStream_t s)
 
 // ------------------------------------------------------------------ energy
 
-// E = 1/2 sum_n [ wv_n v_n^2 - wu_n u_n Lu_n ]; flat weights are
-// w_i w_j w_k hx hy hz (Hphys).  Fixed grid and fixed per-thread order: the
+// E = 1/2 sum_n [ wv_n v_n^2 - wu_n u_n Lu_n ] with wu_n = w_i w_j w_k hvol
+// (Hphys = H3 hx hy hz on flat meshes, H3 for the unweighted curved operator)
+// and wv_n = wu_n, or the per-node array wv (H3 sqrt(g)|J|/alpha).  Fixed grid and fixed per-thread order: the
 // result is bitwise reproducible.
 __global__ void __launch_bounds__(RED_THREADS) energy_partial_kernel(EnergyDesc e)
 {
@@ -403,15 +421,9 @@ __global__ void __launch_bounds__(RED_THREADS) energy_partial_kernel(EnergyDesc 
     double acc = 0.0;
     for (int64_t q = int64_t(blockIdx.x) * RED_THREADS + threadIdx.x; q < n;
          q += int64_t(RED_BLOCKS) * RED_THREADS) {
-        double wu, wv;
-        if (e.wu) {
-            wu = e.wu[q];
-            wv = e.wv[q];
-        } else {
-            const int i = int(q % e.N), j = int((q / e.N) % e.N), k = int((q / (e.N * e.N)) % e.N);
-            wu = e.w[i] * e.w[j] * e.w[k] * e.hvol;
-            wv = wu;
-        }
+        const int i = int(q % e.N), j = int((q / e.N) % e.N), k = int((q / (e.N * e.N)) % e.N);
+        const double wu = e.w[i] * e.w[j] * e.w[k] * e.hvol;
+        const double wv = e.wv ? e.wv[q] : wu;
         acc += 0.5 * (wv * e.v[q] * e.v[q] - wu * e.u[q] * e.Lu[q]);
     }
     sh[threadIdx.x] = acc;

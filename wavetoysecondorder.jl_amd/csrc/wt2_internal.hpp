@@ -76,9 +76,8 @@ constexpr int RED_THREADS = 256;
 struct EnergyDesc {
     int N; int64_t Ne;
     const double* u; const double* v; const double* Lu;
-    const double* wu;   // per-node weight of u*Lu (nullptr: flat H3*hx*hy*hz)
-    const double* wv;   // per-node weight of v^2 (nullptr: flat)
-    double hvol;        // hx*hy*hz for the flat weights
+    const double* wv;   // per-node weight of v^2 (nullptr: same as the u weight)
+    double hvol;        // u weight is w_i w_j w_k hvol
     double w[NMAX];
     double* partial; double* out;
 };

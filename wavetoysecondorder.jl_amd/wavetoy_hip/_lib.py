@@ -30,7 +30,7 @@ WT2_NO_GRAPH = 4
 EXPORTS = (
     "wt2_gll", "wt2_ctx_create", "wt2_ctx_destroy", "wt2_ctx_info", "wt2_ctx_set_background",
     "wt2_coords", "wt2_rhs3d", "wt2_rhs_full", "wt2_rk4_steps", "wt2_energy", "wt2_set_stream",
-    "wt2_last_error", "wt2_version",
+    "wt2_launch_pass", "wt2_last_error", "wt2_version",
 )
 
 
@@ -96,6 +96,7 @@ def lib():
                                     dp, ctypes.c_int32]
         L.wt2_energy.argtypes = [vp, dp, dp, dp, ctypes.c_int32]
         L.wt2_set_stream.argtypes = [vp, vp]
+        L.wt2_launch_pass.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, dp, dp, ctypes.c_double]
         L.wt2_last_error.restype = ctypes.c_char_p
         L.wt2_version.restype = ctypes.c_char_p
         _lib = L

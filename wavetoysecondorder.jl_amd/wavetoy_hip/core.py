@@ -241,6 +241,19 @@ class Workspace:
                                   E.ctypes.data if ne else None, flags))
         return E[:ne] if ne else None
 
+    def set_stream(self, stream_handle):
+        """Run this context's kernels on an external HIP stream (int handle, e.g.
+        torch.cuda.current_stream().cuda_stream); None restores its own."""
+        check(lib().wt2_set_stream(self.ctx, stream_handle))
+
+    def launch_pass(self, axis, stage, u, ud, dt):
+        """Diagnostics: enqueue one pencil pass (axis 0..2) or the curved
+        operator (axis -1) of RK4 stage `stage` without synchronising."""
+        (pu, pud), fl = self._ptrs(u, ud)
+        if fl:
+            raise ValueError("launch_pass needs device tensors")
+        check(lib().wt2_launch_pass(self.ctx, int(axis), int(stage), pu, pud, float(dt)))
+
     def energy(self, u, ud):
         (pu, pud), fl = self._ptrs(u, ud)
         E = ctypes.c_double()
