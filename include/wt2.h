@@ -12,8 +12,8 @@
  * e = ex + Mx*(ey + My*ez).  Faces 1..6 are (-x,+x,-y,+y,-z,+z); an outer face
  * f carries boundary tag f and Dirichlet value bdry_values[f-1].
  *
- * Pointers are device pointers unless WT2_HOST_PTRS is passed in flags, in
- * which case they are host arrays copied in and out around the call.
+ * Pointers are device pointers (16-byte aligned) unless WT2_HOST_PTRS is passed
+ * in flags, in which case they are host arrays copied in and out around the call.
  * Contexts are not thread-safe; separate contexts are independent.  Calls are
  * synchronous with respect to the host (they end with a stream sync).
  */

@@ -205,6 +205,7 @@ struct wt2_ctx {
     double mu, inv_mu5, kos, max_sym_eig, h_min;
     int curved;
     int nb[3];
+    int direct[3];  // per axis: LDS-free pass kernel
     int device;
     hipStream_t own_stream, stream;
     // workspace: each block holds two fields of npts elements (FP64, or FP32
@@ -217,6 +218,8 @@ struct wt2_ctx {
     double* Lu;     // energy scratch
     double* red;    // reduction partials + result
     double* Edev;   // energy history (device)
+    void* op64;     // operator table (FP64)
+    void* op32;     // operator table (FP32)
     int Ecap;
     CurvedData cd;
     int64_t bytes;
@@ -258,11 +261,9 @@ static void fill_axis_ops(const wt2_ctx* c, int axis, const double bv[6], AxisOp
 {
     const int N = c->N;
     const double h = c->h[axis];
-    for (int i = 0; i < N * N; ++i) {
-        o.G[i] = c->G[i];
-        o.Gk[i] = c->kos * c->G[i] / h;
-    }
-    for (int i = 0; i < N; ++i) o.winv[i] = 1.0 / c->w[i];
+    o.kh = c->kos / h;
+    o.winv0 = 1.0 / c->w[0];
+    o.winvN = 1.0 / c->w[N - 1];
     o.koL = -c->kos / (2.0 * c->w[0] * h);
     o.koR = c->kos / (2.0 * c->w[N - 1] * h);
     o.tau = c->d.tau;
@@ -276,12 +277,15 @@ static void fill_axis_ops(const wt2_ctx* c, int axis, const double bv[6], AxisOp
 static int default_nb(int N, int Ma, int want)
 {
     const int lpw = 64 / Ma;
+    int best = 1;
     for (int nb = 1; nb <= N; ++nb) {
         if (N % nb) continue;
         const int waves = (N * nb + lpw - 1) / lpw;
+        if (waves > 8) break;       // at most 512 threads per workgroup
+        best = nb;
         if (waves >= want) return nb;
     }
-    return N;
+    return best;
 }
 
 static PassDesc base_pass(const wt2_ctx* c, int axis, const double bv[6])
@@ -293,6 +297,9 @@ static PassDesc base_pass(const wt2_ctx* c, int axis, const double bv[6])
     p.periodic = c->d.periodic[axis];
     for (int a = 0; a < 3; ++a) p.M[a] = c->d.M[a];
     p.nb = c->nb[axis];
+    p.direct = c->direct[axis];
+    p.op64 = c->op64;
+    p.op32 = c->op32;
     fill_axis_ops(c, axis, bv, p.ops);
     return p;
 }
@@ -320,13 +327,15 @@ static int run_lap(wt2_ctx* c, const double* u, const double* v, double* out, co
         for (int f = 0; f < 6; ++f) cc.bv[f] = bv[f];
         return pass_rc(curved_operator(c->cd, cc, c->stream), "curved operator");
     }
-    for (int axis = 0; axis < 3; ++axis) {
+    // passes z, y, x: the x pass (LDS-free) finishes the sum
+    for (int q = 0; q < 3; ++q) {
+        const int axis = 2 - q;
         PassDesc p = base_pass(c, axis, bv);
         p.variant = mixed ? VAR_MIX_IN64 : VAR_F64;
         p.lap = 1;
         p.Su = u;
         p.P = c->Pb;   // FP64 scratch in either precision (one block = one FP64 field)
-        p.epi = axis == 0 ? EPI_FIRST : (axis == 1 ? EPI_MID : EPI_OUT_LAP);
+        p.epi = q == 0 ? EPI_FIRST : (q == 1 ? EPI_MID : EPI_OUT_LAP);
         p.Ou = out;
         int rc = pass_rc(launch_flat_pass(p, c->stream), "flat lap pass");
         if (rc) return rc;
@@ -360,8 +369,11 @@ static int run_stage(wt2_ctx* c, int stage, const double* u, const double* v, do
         if (rc) return rc;
     }
     if (c->curved) have_p = true;
-    for (int axis = 0; axis < 3; ++axis) {
-        const bool last = axis == 2;
+    // passes z, y, x: the x pass (LDS-free, contiguous segments) carries the
+    // final epilogue (RHS output or RK stage update)
+    for (int q = 0; q < 3; ++q) {
+        const int axis = 2 - q;
+        const bool last = q == 2;
         const bool lap = !c->curved;
         if (!last && !lap && !ko) continue;   // nothing to add along this axis
         if (only != -2 && only != axis) {
@@ -416,12 +428,18 @@ static int run_energy(wt2_ctx* c, const double* u, const double* v, double* Edst
 }
 
 namespace {
+bool misaligned(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) != 0; }
+
 int stage_in(wt2_ctx* c, int nf, const double* const* src, double** dst, bool host)
 {
     const size_t bytes = size_t(c->npts) * sizeof(double);
     for (int f = 0; f < nf; ++f) {
         if (!src[f]) { dst[f] = nullptr; continue; }
-        if (!host) { dst[f] = const_cast<double*>(src[f]); continue; }
+        if (!host) {
+            if (misaligned(src[f])) { set_error("device field pointers must be 16-byte aligned"); return WT2_ERR_ARG; }
+            dst[f] = const_cast<double*>(src[f]);
+            continue;
+        }
         if (!c->Y[f]) {
             int rc = alloc_dev(c, reinterpret_cast<void**>(&c->Y[f]), bytes);
             if (rc) return rc;
@@ -436,7 +454,11 @@ int out_buf(wt2_ctx* c, int nf, double* const* user, double** dst, bool host)
 {
     const size_t bytes = size_t(c->npts) * sizeof(double);
     for (int f = 0; f < nf; ++f) {
-        if (!host) { dst[f] = user[f]; continue; }
+        if (!host) {
+            if (misaligned(user[f])) { set_error("device field pointers must be 16-byte aligned"); return WT2_ERR_ARG; }
+            dst[f] = user[f];
+            continue;
+        }
         if (!c->O[f]) {
             int rc = alloc_dev(c, reinterpret_cast<void**>(&c->O[f]), bytes);
             if (rc) return rc;
@@ -486,6 +508,10 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
         if (desc->M[a] < 1) { set_error("wt2_ctx_create: M per axis must be >= 1"); return WT2_ERR_SHAPE; }
         if (desc->M[a] > MMAX_AXIS) {
             set_error("wt2_ctx_create: more than 64 elements along one axis is not supported");
+            return WT2_ERR_UNSUPPORTED;
+        }
+        if (desc->N >= 9 && desc->M[a] > 32) {
+            set_error("wt2_ctx_create: N >= 9 supports at most 32 elements along one axis");
             return WT2_ERR_UNSUPPORTED;
         }
         if (!(desc->x1[a] > desc->x0[a])) { set_error("wt2_ctx_create: x1 <= x0"); return WT2_ERR_ARG; }
@@ -561,8 +587,17 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
         for (int i = 1; i < N; ++i) gap = std::min(gap, c->xi[i] - c->xi[i - 1]);
         c->h_min = gap * std::min(c->h[0], std::min(c->h[1], c->h[2]));
     }
+    // Pass kernels and tile sizes (measured on the 32^3 x 8^3 workload): the x
+    // pass is always LDS-free; in FP64 the y and z passes are LDS-free too
+    // (strided loads merged in L2, 8-wavefront tiles), in mixed precision they
+    // stage through LDS (z tiles of 8 wavefronts, y of 4).
+    const bool mixed_ctx = desc->precision == WT2_MIXED;
+    const char* env_direct = std::getenv("WT2_DIRECT");
     for (int a = 0; a < 3; ++a) {
-        c->nb[a] = default_nb(N, desc->M[a], a == 2 ? 8 : 4);
+        c->direct[a] = a == 0 ? 1 : (mixed_ctx ? 0 : 1);
+        if (env_direct && a > 0) c->direct[a] = std::atoi(env_direct) != 0;
+        const int want = c->direct[a] ? (a == 0 ? 4 : 8) : (a == 2 ? 8 : 4);
+        c->nb[a] = default_nb(N, desc->M[a], want);
         const char* env = std::getenv("WT2_TILE_NB");
         if (env) {
             int t[3];
@@ -585,6 +620,20 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
     if (rc == WT2_OK) rc = alloc_dev(c, &c->Ab, blk);
     if (rc == WT2_OK) rc = alloc_dev(c, reinterpret_cast<void**>(&c->Lu), n * sizeof(double));
     if (rc == WT2_OK) rc = alloc_dev(c, reinterpret_cast<void**>(&c->red), (RED_BLOCKS + 8) * sizeof(double));
+    if (rc == WT2_OK) {
+        const int ts = op_table_size(N);
+        std::vector<double> tab(ts);
+        std::vector<float> tab32(ts);
+        build_op_table(N, c->G, c->w, tab.data());
+        for (int q = 0; q < ts; ++q) tab32[q] = float(tab[q]);
+        rc = alloc_dev(c, &c->op64, ts * sizeof(double));
+        if (rc == WT2_OK) rc = alloc_dev(c, &c->op32, ts * sizeof(float));
+        if (rc == WT2_OK && (hipMemcpy(c->op64, tab.data(), ts * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+                             hipMemcpy(c->op32, tab32.data(), ts * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)) {
+            set_error("wt2_ctx_create: operator table upload failed");
+            rc = WT2_ERR_HIP;
+        }
+    }
     if (rc == WT2_OK && c->curved)
         rc = curved_init(c->cd, *desc, c->xi, c->w, c->G, nullptr, c->stream, &c->bytes);
     if (rc != WT2_OK) {
@@ -602,6 +651,7 @@ void wt2_ctx_destroy(wt2_ctx* c)
     hipFree(c->Sb); hipFree(c->Pb); hipFree(c->Ab);
     for (int f = 0; f < 2; ++f) { hipFree(c->Y[f]); hipFree(c->O[f]); }
     hipFree(c->Lu); hipFree(c->red); hipFree(c->Edev);
+    hipFree(c->op64); hipFree(c->op32);
     curved_free(c->cd);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
     delete c;

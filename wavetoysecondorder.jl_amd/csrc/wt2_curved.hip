@@ -460,6 +460,7 @@ static hipError_t launch_c(const CurvedData& cd, const CurvedCall& cc, hipStream
     a.per0 = cd.periodic[0]; a.per1 = cd.periodic[1]; a.per2 = cd.periodic[2];
     a.adv = cc.v != nullptr;
     a.weighted = cc.weighted;
+    (void)hipGetLastError();   // launch errors below are this launch's own
     hipLaunchKernelGGL((curved_faceflux_kernel<N, TC, TS>), dim3(unsigned(cd.Ne)), dim3(256), 0, st, a);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
@@ -470,7 +471,7 @@ static hipError_t launch_c(const CurvedData& cd, const CurvedCall& cc, hipStream
     static bool raised = false;
     if (shmem > 64 * 1024 && !raised) {
         err = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
         if (err != hipSuccess) return err;
         raised = true;
     }

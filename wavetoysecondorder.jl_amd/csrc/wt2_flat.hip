@@ -23,21 +23,43 @@
 //      RK4 stage update in the same pass.
 #include "wt2_internal.hpp"
 
+#include <cstdlib>
 #include <type_traits>
 
 namespace wt2 {
 
+// Per-axis scalars (kernel arguments).  The differentiation matrix itself is
+// not a kernel argument: it lives in a small device table that each workgroup
+// copies to LDS (as kernel arguments the matrices took more SGPRs than exist
+// and were spilled).
 template <int N, typename TC>
 struct AxisConst {
-    TC G[N * N];
-    TC Gk[N * N];
-    TC winv[N];
+    TC kh;                   // KO scale kos / h
     TC koL, koR, tau, ih2, gL, gR;
+    TC winv0, winvN;         // 1 / w at the two face nodes
+};
+
+// Operator table layout (TC values), built by build_op_table on the host:
+//   [0, E*H)          Ae[i][p] (i < H, p < E)  even part
+//   [E*H, 2*E*H)      Ao[i][p] (i < E, p < H)  odd part
+//   [2EH, 2EH+N)      G[0][i] / w_i
+//   [2EH+N, 2EH+2N)   G[N-1][i] / w_i
+// with H = floor(N/2), E = ceil(N/2).
+template <int N>
+struct OpLayout {
+    static constexpr int H = N / 2;
+    static constexpr int E = N - N / 2;
+    static constexpr int AE = 0;
+    static constexpr int AO = E * H;
+    static constexpr int C0 = 2 * E * H;
+    static constexpr int CN = 2 * E * H + N;
+    static constexpr int SIZE = 2 * E * H + 2 * N;
 };
 
 template <int N, typename TC>
 struct KArgs {
     AxisConst<N, TC> k;
+    const TC* op;      // operator table (OpLayout<N>)
     const void* Su; const void* Sv;
     void* P;
     void* So_u; void* So_v;
@@ -55,6 +77,7 @@ struct KArgs {
     int ES, CS;        // LDS strides: per element, per chunk within an element
     int SR, LA, LB;    // segment: node stride, line -> (line/N)*LA + (line%N)*LB
     int LT, lpw;       // lines per tile, lines per wavefront
+    int GS;            // node stride of a segment in memory (LDS-free kernel)
     int periodic, epi;
     int lap, ko, needv, keepv;
 };
@@ -65,49 +88,73 @@ __device__ __forceinline__ T shfl64(T v, int src)
     return __shfl(v, src, 64);
 }
 
+// y = G x on one element segment by the even/odd split of the GLL
+// differentiation matrix (G[N-1-i][p] = -G[i][N-1-p]): with e_p, o_p the
+// symmetric / antisymmetric halves of x, y_i = a_i + b_i and
+// y_{N-1-i} = b_i - a_i, a = Ae e, b = Ao o -- half the multiply-adds of G x.
+template <int N, typename TC>
+__device__ __forceinline__ void apply_G(const TC* __restrict__ op, const TC (&x)[N], TC (&y)[N])
+{
+    using L = OpLayout<N>;
+    constexpr int H = L::H, E = L::E;
+    TC e[E], o[H];
+#pragma unroll
+    for (int p = 0; p < H; ++p) {
+        e[p] = TC(0.5) * (x[p] + x[N - 1 - p]);
+        o[p] = TC(0.5) * (x[p] - x[N - 1 - p]);
+    }
+    if (E > H) e[E - 1] = x[H];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        TC b = TC(0);
+#pragma unroll
+        for (int p = 0; p < H; ++p) b = fma(op[L::AO + i * H + p], o[p], b);
+        if (i < H) {
+            TC av = TC(0);
+#pragma unroll
+            for (int p = 0; p < E; ++p) av = fma(op[L::AE + i * E + p], e[p], av);
+            y[i] = b + av;
+            y[N - 1 - i] = b - av;
+        } else {
+            y[i] = b;   // middle node of an odd N
+        }
+    }
+}
+
 // x <- (kos D)^6 x on this lane's element segment; D = (1/h) G plus the
 // centred-flux SAT (apply_D!, METHODS.md:256-264); an outer face takes the same
 // SAT against a zero ghost value, which keeps H D skew on a bounded line.
 template <int N, typename TC>
-__device__ __forceinline__ void ko_chain(TC (&x)[N], const AxisConst<N, TC>& k, int srcL,
-                                         int srcR, bool hasL, bool hasR)
+__device__ __forceinline__ void ko_chain(TC (&x)[N], const TC* __restrict__ op,
+                                         const AxisConst<N, TC>& k, int srcL, int srcR,
+                                         bool hasL, bool hasR)
 {
-#pragma unroll
+#pragma unroll 1
     for (int rep = 0; rep < 6; ++rep) {
         TC xl = shfl64(x[N - 1], srcL);
         TC xr = shfl64(x[0], srcR);
         if (!hasL) xl = TC(0);
         if (!hasR) xr = TC(0);
         TC y[N];
+        apply_G<N, TC>(op, x, y);
+        const TC x0 = x[0], xN = x[N - 1];
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-            TC s = TC(0);
-#pragma unroll
-            for (int p = 0; p < N; ++p) s = fma(k.Gk[i * N + p], x[p], s);
-            y[i] = s;
-        }
-        y[0] = fma(k.koL, xl - x[0], y[0]);
-        y[N - 1] = fma(k.koR, xr - x[N - 1], y[N - 1]);
-#pragma unroll
-        for (int i = 0; i < N; ++i) x[i] = y[i];
+        for (int i = 0; i < N; ++i) x[i] = k.kh * y[i];
+        x[0] = fma(k.koL, xl - x0, x[0]);
+        x[N - 1] = fma(k.koR, xr - xN, x[N - 1]);
     }
 }
 
 // out += 1D SIPG Laplacian of u along this axis (README.md:57-81):
 //   (1/h^2){ G G u + H^-1 [ alpha G^T B du - 1/2 B d(Gu) - tau |B| du ] }.
 template <int N, typename TC>
-__device__ __forceinline__ void sipg_add(const TC (&u)[N], TC (&out)[N],
+__device__ __forceinline__ void sipg_add(const TC (&u)[N], TC (&out)[N], const TC* __restrict__ op,
                                          const AxisConst<N, TC>& k, int srcL, int srcR,
                                          bool hasL, bool hasR)
 {
+    using L = OpLayout<N>;
     TC g[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        TC s = TC(0);
-#pragma unroll
-        for (int p = 0; p < N; ++p) s = fma(k.G[i * N + p], u[p], s);
-        g[i] = s;
-    }
+    apply_G<N, TC>(op, u, g);
     TC uL = shfl64(u[N - 1], srcL);
     TC gnL = shfl64(g[N - 1], srcL);
     TC uR = shfl64(u[0], srcR);
@@ -118,21 +165,20 @@ __device__ __forceinline__ void sipg_add(const TC (&u)[N], TC (&out)[N],
     if (hasR) { duR = u[N - 1] - uR; dgR = g[N - 1] - gnR; aR = TC(0.5); }
     else      { duR = u[N - 1] - k.gR; dgR = TC(0); aR = TC(1); }
     const TC tL = -aL * duL, tR = aR * duR;
+    TC gg[N];
+    apply_G<N, TC>(op, g, gg);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        TC s = TC(0);
-#pragma unroll
-        for (int p = 0; p < N; ++p) s = fma(k.G[i * N + p], g[p], s);
-        TC f = k.G[i] * tL + k.G[(N - 1) * N + i] * tR;
-        if (i == 0) f += TC(0.5) * dgL - k.tau * duL;
-        if (i == N - 1) f += TC(-0.5) * dgR - k.tau * duR;
-        s = fma(k.winv[i], f, s);
+        TC s = fma(op[L::C0 + i], tL, gg[i]);
+        s = fma(op[L::CN + i], tR, s);
+        if (i == 0) s = fma(k.winv0, TC(0.5) * dgL - k.tau * duL, s);
+        if (i == N - 1) s = fma(k.winvN, TC(-0.5) * dgR - k.tau * duR, s);
         out[i] = fma(s, k.ih2, out[i]);
     }
 }
 
 template <int N, typename TC, typename TS, typename TP>
-__global__ void __launch_bounds__(1024) flat_pass_kernel(KArgs<N, TC> a)
+__global__ void __launch_bounds__(512) flat_pass_kernel(KArgs<N, TC> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     TS* lds = reinterpret_cast<TS*>(smem_raw);
@@ -161,6 +207,8 @@ __global__ void __launch_bounds__(1024) flat_pass_kernel(KArgs<N, TC> a)
 
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
+    __shared__ TC op[OpLayout<N>::SIZE];
+    for (int q = tid; q < OpLayout<N>::SIZE; q += nthr) op[q] = a.op[q];
     // copy-loop mapping: (chunk, offset) pairs; chunks of clen contiguous values
     int o, c0, cstep, ostep;
     if (clen <= nthr) { o = tid % clen; c0 = tid / clen; cstep = nthr / clen; ostep = clen; }
@@ -214,7 +262,7 @@ __global__ void __launch_bounds__(1024) flat_pass_kernel(KArgs<N, TC> a)
         TC x[N];
 #pragma unroll
         for (int r = 0; r < N; ++r) x[r] = TC(lds[sv * fstride + base + r * SR]);
-        ko_chain<N, TC>(x, a.k, srcL, srcR, hasL, hasR);
+        ko_chain<N, TC>(x, op, a.k, srcL, srcR, hasL, hasR);
 #pragma unroll
         for (int r = 0; r < N; ++r) cv[r] = x[r];
     }
@@ -222,7 +270,7 @@ __global__ void __launch_bounds__(1024) flat_pass_kernel(KArgs<N, TC> a)
         TC u[N];
 #pragma unroll
         for (int r = 0; r < N; ++r) u[r] = TC(lds[su * fstride + base + r * SR]);
-        sipg_add<N, TC>(u, cv, a.k, srcL, srcR, hasL, hasR);
+        sipg_add<N, TC>(u, cv, op, a.k, srcL, srcR, hasL, hasR);
     }
     __syncthreads();
     if (active) {
@@ -299,7 +347,236 @@ __global__ void __launch_bounds__(1024) flat_pass_kernel(KArgs<N, TC> a)
     }
 }
 
+// Segment load / store: N values at stride gs (gs == 1: widest aligned vector).
+template <int N, typename TS, typename TC>
+__device__ __forceinline__ void load_seg_s(const TS* __restrict__ p, int gs, TC (&x)[N]);
+template <int N, typename TD, typename TC>
+__device__ __forceinline__ void store_seg_s(TD* __restrict__ p, int gs, const TC (&x)[N]);
+
+template <int N, typename TS, typename TC>
+__device__ __forceinline__ void load_seg(const TS* __restrict__ p, TC (&x)[N])
+{
+    if constexpr (std::is_same<TS, double>::value && N % 2 == 0) {
+        const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+        for (int r = 0; r < N / 2; ++r) {
+            const double2 t = q[r];
+            x[2 * r] = TC(t.x);
+            x[2 * r + 1] = TC(t.y);
+        }
+    } else if constexpr (std::is_same<TS, float>::value && N % 4 == 0) {
+        const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+        for (int r = 0; r < N / 4; ++r) {
+            const float4 t = q[r];
+            x[4 * r] = TC(t.x); x[4 * r + 1] = TC(t.y); x[4 * r + 2] = TC(t.z); x[4 * r + 3] = TC(t.w);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < N; ++r) x[r] = TC(p[r]);
+    }
+}
+
+template <int N, typename TD, typename TC>
+__device__ __forceinline__ void store_seg(TD* __restrict__ p, const TC (&x)[N])
+{
+    if constexpr (std::is_same<TD, double>::value && N % 2 == 0) {
+        double2* q = reinterpret_cast<double2*>(p);
+#pragma unroll
+        for (int r = 0; r < N / 2; ++r) q[r] = make_double2(double(x[2 * r]), double(x[2 * r + 1]));
+    } else if constexpr (std::is_same<TD, float>::value && N % 4 == 0) {
+        float4* q = reinterpret_cast<float4*>(p);
+#pragma unroll
+        for (int r = 0; r < N / 4; ++r)
+            q[r] = make_float4(float(x[4 * r]), float(x[4 * r + 1]), float(x[4 * r + 2]), float(x[4 * r + 3]));
+    } else {
+#pragma unroll
+        for (int r = 0; r < N; ++r) p[r] = TD(x[r]);
+    }
+}
+
+template <int N, typename TS, typename TC>
+__device__ __forceinline__ void load_seg_s(const TS* __restrict__ p, int gs, TC (&x)[N])
+{
+    if (gs == 1) { load_seg<N, TS, TC>(p, x); return; }
+#pragma unroll
+    for (int r = 0; r < N; ++r) x[r] = TC(p[r * gs]);
+}
+
+template <int N, typename TD, typename TC>
+__device__ __forceinline__ void store_seg_s(TD* __restrict__ p, int gs, const TC (&x)[N])
+{
+    if (gs == 1) { store_seg<N, TD, TC>(p, x); return; }
+#pragma unroll
+    for (int r = 0; r < N; ++r) p[r * gs] = TD(x[r]);
+}
+
+// The x pass: along x a lane's element segment is N contiguous values in
+// memory, so it is loaded and stored straight from / to HBM with vector
+// accesses -- no LDS staging and no workgroup barrier -- and the whole RK4
+// epilogue runs on register-resident segments.  It is the last pass of an
+// RHS evaluation (passes run z, y, x).
+template <int N, typename TC, typename TS, typename TP>
+__global__ void __launch_bounds__(512) flat_xpass_kernel(KArgs<N, TC> a)
+{
+    constexpr int N3 = N * N * N;
+    constexpr bool MIXED = std::is_same<TP, float>::value;
+    __shared__ TC op[OpLayout<N>::SIZE];
+    for (int q = threadIdx.x; q < OpLayout<N>::SIZE; q += blockDim.x) op[q] = a.op[q];
+    __syncthreads();
+
+    int t = blockIdx.x;
+    const int blk = t % a.nbk;
+    t /= a.nbk;
+    const int ebase = (t % a.T0) * a.slo + (t / a.T0) * a.shi;
+    const int Ma = a.Ma;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int lpw = a.lpw;
+    const int m = lane / lpw, lw = lane - m * lpw;
+    const int line = wave * lpw + lw;               // (j, k in block) = (line % N, line / N)
+    const bool active = (m < Ma) && (line < a.LT);
+    const bool per = a.periodic != 0;
+    const bool hasL = active && (m > 0 || per);
+    const bool hasR = active && (m < Ma - 1 || per);
+    const int srcL = hasL ? ((m > 0 ? m - 1 : Ma - 1) * lpw + lw) : lane;
+    const int srcR = hasR ? ((m < Ma - 1 ? m + 1 : 0) * lpw + lw) : lane;
+    const int mm = active ? m : 0;
+    const int ll = active ? line : 0;
+    const int64_t g0 = int64_t(ebase + mm * a.estride) * N3 + int64_t(blk) * a.CO +
+                       int64_t(ll / N) * a.LA + int64_t(ll % N) * a.LB;
+    const int gs = a.GS;
+
+    TC u[N], cv[N];
+    TS v[N];   // the stage's u_t exactly as held: it is also u' of the RHS
+#pragma unroll
+    for (int r = 0; r < N; ++r) { u[r] = TC(0); v[r] = TS(0); cv[r] = TC(0); }
+    if (a.lap) load_seg_s<N, TS, TC>(static_cast<const TS*>(a.Su) + g0, gs, u);
+    if (a.needv) load_seg_s<N, TS, TS>(static_cast<const TS*>(a.Sv) + g0, gs, v);
+    if (a.ko) {
+        TC x[N];
+#pragma unroll
+        for (int r = 0; r < N; ++r) x[r] = TC(v[r]);
+        ko_chain<N, TC>(x, op, a.k, srcL, srcR, hasL, hasR);
+#pragma unroll
+        for (int r = 0; r < N; ++r) cv[r] = x[r];
+    }
+    if (a.lap) sipg_add<N, TC>(u, cv, op, a.k, srcL, srcR, hasL, hasR);
+    if (!active) return;
+
+    TP* P = static_cast<TP*>(a.P);
+    const int epi = a.epi;
+    if (epi == EPI_FIRST) { store_seg_s<N, TP, TC>(P + g0, gs, cv); return; }
+    TC pv[N];
+    load_seg_s<N, TP, TC>(P + g0, gs, pv);
+#pragma unroll
+    for (int r = 0; r < N; ++r) cv[r] += pv[r];          // u_t' = P + c
+    if (epi == EPI_MID) { store_seg_s<N, TP, TC>(P + g0, gs, cv); return; }
+    if (epi == EPI_OUT_LAP) { store_seg_s<N, double, TC>(a.Ou + g0, gs, cv); return; }
+    if (epi == EPI_OUT_RHS) {
+        store_seg_s<N, double, TS>(a.Ou + g0, gs, v);            // u' = u_t
+        store_seg_s<N, double, TC>(a.Ov + g0, gs, cv);
+        return;
+    }
+    // classical RK4 stage update, k = (u_t, P + c)
+    const double dt = a.dt;
+    double yu[N], yv[N];
+    if (epi != EPI_RK4 || MIXED) {
+        load_seg_s<N, double, double>(a.Yu + g0, gs, yu);
+        load_seg_s<N, double, double>(a.Yv + g0, gs, yv);
+    }
+    if (MIXED) {
+        float* du = static_cast<float*>(a.Au) + g0;
+        float* dv = static_cast<float*>(a.Av) + g0;
+        double su[N], sv[N], au[N], av[N];
+        if (epi == EPI_RK1 || epi == EPI_RK2 || epi == EPI_RK3) {
+            const double c = epi == EPI_RK3 ? dt : 0.5 * dt;
+            const double wgt = epi == EPI_RK1 ? dt / 6.0 : dt / 3.0;
+            double du0[N], dv0[N];
+            if (epi == EPI_RK1) {
+#pragma unroll
+                for (int r = 0; r < N; ++r) { du0[r] = 0.0; dv0[r] = 0.0; }
+            } else {
+                load_seg_s<N, float, double>(du, gs, du0);
+                load_seg_s<N, float, double>(dv, gs, dv0);
+            }
+#pragma unroll
+            for (int r = 0; r < N; ++r) {
+                const double ku = double(v[r]), kv = double(cv[r]);
+                su[r] = yu[r] + c * ku;
+                sv[r] = yv[r] + c * kv;
+                au[r] = du0[r] + wgt * ku;
+                av[r] = dv0[r] + wgt * kv;
+            }
+            store_seg_s<N, float, double>(static_cast<float*>(a.So_u) + g0, gs, su);
+            store_seg_s<N, float, double>(static_cast<float*>(a.So_v) + g0, gs, sv);
+            store_seg_s<N, float, double>(du, gs, au);
+            store_seg_s<N, float, double>(dv, gs, av);
+        } else {
+            double du0[N], dv0[N];
+            load_seg_s<N, float, double>(du, gs, du0);
+            load_seg_s<N, float, double>(dv, gs, dv0);
+#pragma unroll
+            for (int r = 0; r < N; ++r) {
+                yu[r] = yu[r] + (du0[r] + dt / 6.0 * double(v[r]));
+                yv[r] = yv[r] + (dv0[r] + dt / 6.0 * double(cv[r]));
+            }
+            store_seg_s<N, double, double>(a.Yu + g0, gs, yu);
+            store_seg_s<N, double, double>(a.Yv + g0, gs, yv);
+        }
+    } else {
+        double* Au = static_cast<double*>(a.Au) + g0;
+        double* Av = static_cast<double*>(a.Av) + g0;
+        double au[N], av[N];
+        if (epi == EPI_RK4) {
+            load_seg_s<N, double, double>(Au, gs, au);
+            load_seg_s<N, double, double>(Av, gs, av);
+#pragma unroll
+            for (int r = 0; r < N; ++r) {
+                yu[r] = au[r] + dt / 6.0 * double(v[r]);
+                yv[r] = av[r] + dt / 6.0 * double(cv[r]);
+            }
+            store_seg_s<N, double, double>(a.Yu + g0, gs, yu);
+            store_seg_s<N, double, double>(a.Yv + g0, gs, yv);
+        } else {
+            const double c = epi == EPI_RK3 ? dt : 0.5 * dt;
+            double su[N], sv[N];
+            if (epi == EPI_RK1) {
+#pragma unroll
+                for (int r = 0; r < N; ++r) { au[r] = yu[r] + dt / 6.0 * double(v[r]); av[r] = yv[r] + dt / 6.0 * double(cv[r]); }
+            } else {
+                load_seg_s<N, double, double>(Au, gs, au);
+                load_seg_s<N, double, double>(Av, gs, av);
+#pragma unroll
+                for (int r = 0; r < N; ++r) { au[r] += dt / 3.0 * double(v[r]); av[r] += dt / 3.0 * double(cv[r]); }
+            }
+#pragma unroll
+            for (int r = 0; r < N; ++r) { su[r] = yu[r] + c * double(v[r]); sv[r] = yv[r] + c * double(cv[r]); }
+            store_seg_s<N, double, double>(static_cast<double*>(a.So_u) + g0, gs, su);
+            store_seg_s<N, double, double>(static_cast<double*>(a.So_v) + g0, gs, sv);
+            store_seg_s<N, double, double>(Au, gs, au);
+            store_seg_s<N, double, double>(Av, gs, av);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ launch
+
+int op_table_size(int N) { return 2 * (N - N / 2) * (N / 2) + 2 * N; }
+
+void build_op_table(int N, const double* G, const double* w, double* tab)
+{
+    const int H = N / 2, E = N - N / 2;
+    for (int i = 0; i < H; ++i)
+        for (int p = 0; p < E; ++p)
+            tab[i * E + p] = (p < H) ? G[i * N + p] + G[i * N + (N - 1 - p)] : G[i * N + p];
+    for (int i = 0; i < E; ++i)
+        for (int p = 0; p < H; ++p) tab[E * H + i * H + p] = G[i * N + p] - G[i * N + (N - 1 - p)];
+    for (int i = 0; i < N; ++i) {
+        tab[2 * E * H + i] = G[i] / w[i];
+        tab[2 * E * H + N + i] = G[(N - 1) * N + i] / w[i];
+    }
+}
 
 static int pick_lpw(int Ma) { return 64 / Ma; }
 
@@ -307,14 +584,12 @@ template <int N, typename TC, typename TS, typename TP>
 static hipError_t launch_t(const PassDesc& p, hipStream_t s)
 {
     KArgs<N, TC> a;
-    for (int i = 0; i < N * N; ++i) {
-        a.k.G[i] = TC(p.ops.G[i]);
-        a.k.Gk[i] = TC(p.ops.Gk[i]);
-    }
-    for (int i = 0; i < N; ++i) a.k.winv[i] = TC(p.ops.winv[i]);
+    a.op = static_cast<const TC*>(std::is_same<TC, float>::value ? p.op32 : p.op64);
+    a.k.kh = TC(p.ops.kh);
     a.k.koL = TC(p.ops.koL); a.k.koR = TC(p.ops.koR);
     a.k.tau = TC(p.ops.tau); a.k.ih2 = TC(p.ops.ih2);
     a.k.gL = TC(p.ops.gL); a.k.gR = TC(p.ops.gR);
+    a.k.winv0 = TC(p.ops.winv0); a.k.winvN = TC(p.ops.winvN);
     a.Su = p.Su; a.Sv = p.Sv; a.P = p.P;
     a.So_u = p.So_u; a.So_v = p.So_v; a.Yu = p.Yu; a.Yv = p.Yv;
     a.Au = p.Au; a.Av = p.Av; a.Ou = p.Ou; a.Ov = p.Ov;
@@ -361,21 +636,32 @@ static hipError_t launch_t(const PassDesc& p, hipStream_t s)
     }
     const int waves = (a.LT + a.lpw - 1) / a.lpw;
     const int nthr = waves * 64;
-    if (nthr > 1024) return hipErrorInvalidConfiguration;
+    if (nthr > 512) return hipErrorInvalidConfiguration;
     const int fstride = a.Ma * a.ES;
     const int nslots = (p.lap && p.needv) || (!p.lap && p.keepv) ? 2 : 1;
     const size_t shmem = size_t(fstride) * nslots * sizeof(TS);
-    if (shmem > 160 * 1024) return hipErrorInvalidConfiguration;
+    const size_t static_lds = sizeof(TC) * OpLayout<N>::SIZE;
+    if (shmem + static_lds > 160 * 1024) return hipErrorInvalidConfiguration;
+    a.GS = p.axis == 0 ? 1 : (p.axis == 1 ? N : N * N);
+    if (p.axis == 0 || p.direct) {
+        // LDS-free pass
+        auto xk = flat_xpass_kernel<N, TC, TS, TP>;
+        (void)hipGetLastError();
+        hipLaunchKernelGGL(xk, dim3(tiles), dim3(nthr), 0, s, a);
+        return hipGetLastError();
+    }
     auto kern = flat_pass_kernel<N, TC, TS, TP>;
     // raise the dynamic-LDS limit once per instantiation (outside any capture:
     // the first launch of a configuration is always eager)
-    static size_t lds_limit = 64 * 1024;
+    static size_t lds_limit = 64 * 1024 - static_lds;
     if (shmem > lds_limit) {
+        const size_t want = 160 * 1024 - static_lds;
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
         if (e != hipSuccess) return e;
-        lds_limit = 160 * 1024;
+        lds_limit = want;
     }
+    (void)hipGetLastError();   // launch errors below are this launch's own
     hipLaunchKernelGGL(kern, dim3(tiles), dim3(nthr), shmem, s, a);
     return hipGetLastError();
 }

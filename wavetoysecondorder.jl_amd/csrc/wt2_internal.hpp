@@ -33,16 +33,19 @@ enum Variant : int {
     VAR_MIX_IN32 = 2, // compute float, stage input float, partial float
 };
 
-// Per-axis 1D operator constants in FP64 (converted to the compute type at
-// launch).
+// Per-axis 1D operator scalars in FP64 (converted to the compute type at
+// launch); the matrix itself is the device table PassDesc::op64 / op32.
 struct AxisOps {
-    double G[NMAX * NMAX];   // unit-element GLL differentiation matrix
-    double Gk[NMAX * NMAX];  // KO-scaled derivative: kos * G / h
-    double winv[NMAX];       // 1 / w
+    double kh;               // KO scale kos / h
     double koL, koR;         // KO interface SAT: -kos/(2 w_0 h), +kos/(2 w_{N-1} h)
     double tau, ih2;         // penalty, 1/h^2
     double gL, gR;           // Dirichlet values at the low / high outer face
+    double winv0, winvN;     // 1/w_0, 1/w_{N-1}
 };
+
+// Even/odd operator table of the GLL differentiation matrix (see wt2_flat.hip).
+int op_table_size(int N);
+void build_op_table(int N, const double* G, const double* w, double* tab);
 
 // Everything one pencil-pass launch needs (type-erased pointers).
 struct PassDesc {
@@ -50,6 +53,7 @@ struct PassDesc {
     int axis, periodic;
     int M[3];
     int nb;             // lines-block size: element planes per tile (KT / JT)
+    int direct;         // 1: LDS-free kernel (strided segment loads through L2)
     int epi;
     int lap;            // add the SIPG Laplacian of u along this axis
     int ko;             // add eps mu^-5 D^6 u_t along this axis
@@ -63,6 +67,7 @@ struct PassDesc {
     double* Ou; double* Ov;           // outputs
     double dt;
     AxisOps ops;
+    const void* op64; const void* op32;   // device operator tables
 };
 
 // Launch one pencil pass on stream s.  Returns hipError_t.
