@@ -1,0 +1,89 @@
+# WaveToyHIP.jl -- ccall binding of libwt2.so (include/wt2.h) for
+# WaveToySecondOrder.jl callers.  Not exercised in this repository (no Julia
+# toolchain here); it is the binding a maintainer adds next to
+# src/wave.jl:573-611 so that `rhs_wave3d!` and the RK4 step dispatch to the
+# MI355X kernels when the arrays live on an AMDGPU (ROCArray) device.
+module WaveToyHIP
+
+const libwt2 = get(ENV, "WT2_LIB", joinpath(@__DIR__, "..", "wavetoy_hip", "libwt2.so"))
+
+const WT2_HOST_PTRS = Cint(1)
+
+struct MeshDesc
+    N::Int32
+    M::NTuple{3,Int32}
+    x0::NTuple{3,Float64}
+    x1::NTuple{3,Float64}
+    periodic::NTuple{3,Int32}
+    warp_A::Float64
+    tau::Float64
+    eps_ko::Float64
+    bdry_values::NTuple{6,Float64}
+    precision::Int32      # 0 = FP64, 1 = mixed
+    flags::Int32
+end
+
+mutable struct Context
+    ptr::Ptr{Cvoid}
+    npoints::Int
+end
+
+function check(rc)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:wt2_last_error, libwt2), Cstring, ()))
+    error("libwt2 error $rc: $msg")
+end
+
+"""
+    Context(N, M; x0=0.0, x1=1.0, periodic=false, warp_A=0.0, τ, ε_KO=0.0,
+            bdry_values=ntuple(_->0.0, 6), precision=:fp64)
+
+The mesh + operators + geometry + workspace bundle (make_uniform_hex +
+make_geometry + make_workspace + to_device of src/evolve.jl:910-936).
+"""
+function Context(N::Integer, M::Integer; x0 = 0.0, x1 = 1.0, periodic = false,
+                 warp_A = 0.0, τ, ε_KO = 0.0, bdry_values = ntuple(_ -> 0.0, 6),
+                 precision = :fp64)
+    d = MeshDesc(Int32(N), ntuple(_ -> Int32(M), 3), ntuple(_ -> Float64(x0), 3),
+                 ntuple(_ -> Float64(x1), 3), ntuple(_ -> Int32(periodic), 3),
+                 Float64(warp_A), Float64(τ), Float64(ε_KO), NTuple{6,Float64}(bdry_values),
+                 Int32(precision === :mixed ? 1 : 0), Int32(0))
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:wt2_ctx_create, libwt2), Cint, (Ref{MeshDesc}, Ref{Ptr{Cvoid}}), d, out))
+    ctx = Context(out[], N^3 * M^3)
+    finalizer(c -> ccall((:wt2_ctx_destroy, libwt2), Cvoid, (Ptr{Cvoid},), c.ptr), ctx)
+    return ctx
+end
+
+"""
+    rhs_wave3d!(ü, u, u̇, bdry_values; ctx) -> ü
+
+Same arguments and result as WaveToySecondOrder.rhs_wave3d! (src/wave.jl:583-611);
+host arrays are copied through, device pointers (ROCArray) are used in place.
+"""
+function rhs_wave3d!(ü::Array{Float64,4}, u::Array{Float64,4}, u̇::Array{Float64,4},
+                     bdry_values::NTuple{6,Float64}; ctx::Context)
+    @assert size(ü) == size(u) == size(u̇) && length(u) == ctx.npoints
+    bv = Ref(bdry_values)
+    check(ccall((:wt2_rhs3d, libwt2), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{NTuple{6,Float64}}, Cint),
+                ctx.ptr, ü, u, u̇, bv, WT2_HOST_PTRS))
+    return ü
+end
+
+"""
+    rk4_steps!(u, u̇, dt, nsteps; ctx, energy_every=0) -> energies or nothing
+
+nsteps classical RK4 steps in place (the step! loop of src/evolve.jl:1028-1031).
+"""
+function rk4_steps!(u::Array{Float64,4}, u̇::Array{Float64,4}, dt::Float64, nsteps::Integer;
+                    ctx::Context, energy_every::Integer = 0)
+    ne = energy_every > 0 ? 1 + nsteps ÷ energy_every : 0
+    E = zeros(max(ne, 1))
+    check(ccall((:wt2_rk4_steps, libwt2), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cint, Cint, Ptr{Float64}, Cint),
+                ctx.ptr, u, u̇, dt, nsteps, energy_every, ne > 0 ? E : C_NULL, WT2_HOST_PTRS))
+    return ne > 0 ? E[1:ne] : nothing
+end
+
+end # module
