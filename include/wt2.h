@@ -135,6 +135,29 @@ int wt2_energy(wt2_ctx* ctx, const double* u, const double* ud, double* E, int32
  * context's own stream. */
 int wt2_set_stream(wt2_ctx* ctx, void* stream);
 
+/* ---- batches: many independent evolutions advanced together ----
+ * No counterpart in the reference (it evolves one mesh at a time); this is the
+ * batched form of wt2_rk4_steps.  Each instance is a context plus its
+ * device-resident state (u, ud: device pointers owned by the caller) with its
+ * own dt and number of steps; every RK4 stage runs as one launch per group of
+ * instances sharing N, precision and pass kind, over a packed work list, and
+ * the step is replayed from a hipGraph while the set of unfinished instances
+ * is unchanged.  Adding a context moves its kernels onto the batch's stream
+ * until it is removed; a context belongs to at most one batch. */
+typedef struct wt2_batch wt2_batch;
+int wt2_batch_create(wt2_batch** out);
+void wt2_batch_destroy(wt2_batch* b);
+/* energy_every > 0 records the FP64 energy at the start and every energy_every
+ * steps of this instance. */
+int wt2_batch_add(wt2_batch* b, wt2_ctx* ctx, double* u, double* ud, double dt, int64_t nsteps,
+                  int32_t energy_every, int32_t* slot);
+int wt2_batch_remove(wt2_batch* b, int32_t slot);
+/* Advance every unfinished instance by up to max_steps steps (each stops at its
+ * own nsteps); *finished (nullable) counts the instances that finished. */
+int wt2_batch_step(wt2_batch* b, int32_t max_steps, int32_t* finished);
+int wt2_batch_slot(const wt2_batch* b, int32_t slot, int64_t* steps_done, double* energies,
+                   int32_t cap, int32_t* n_energies);
+
 /* Diagnostics (no counterpart in the reference; used by bench.py to time one
  * kernel in isolation with events on the context stream): enqueue, without
  * synchronising, the single pencil pass of RK4 stage `stage` (1..4) along

@@ -1,8 +1,11 @@
 // wt2_capi.hip -- the extern "C" entry points of libwt2.so (declared in
 // include/wt2.h): context set-up (operators, mu, coercivity check, device
 // workspace) and the host-side sequencing of the pencil passes.
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <map>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -343,11 +346,20 @@ static int run_lap(wt2_ctx* c, const double* u, const double* v, double* out, co
     return WT2_OK;
 }
 
-// One full RHS evaluation of u' = u_t, u_t' = lap(u, u_t) + KO(u_t).
-// stage 0: write (du, dv) to (Ou, Ov); stage 1..4: classical RK4 stage on the
+// The launches of one full RHS evaluation of u' = u_t,
+// u_t' = lap(u, u_t) + KO(u_t): an optional curved operator (phase 0) and the
+// pencil passes z, y, x (phases 1..3; the x pass carries the final epilogue).
+// stage 0: output (du, dv) to (Ou, Ov); stage 1..4: classical RK4 stage on the
 // context's registers with state Y (stage 1 reads Y, later stages S).
-static int run_stage(wt2_ctx* c, int stage, const double* u, const double* v, double* Ou,
-                     double* Ov, double* Yu, double* Yv, double dt, int only = -2)
+struct StageItems {
+    bool has_curved = false;
+    CurvedCall cc;
+    bool has_pass[3] = {false, false, false};
+    PassDesc pass[3];
+};
+
+static void stage_items(wt2_ctx* c, int stage, const double* u, const double* v, double* Ou, double* Ov,
+                        double* Yu, double* Yv, double dt, StageItems& it)
 {
     const bool mixed = is_mixed(c);
     const bool ko = c->d.eps_ko > 0 && c->kos > 0;
@@ -358,28 +370,22 @@ static int run_stage(wt2_ctx* c, int stage, const double* u, const double* v, do
     const int final_epi = stage == 0 ? EPI_OUT_RHS : EPI_RK1 + (stage - 1);
     void* P = c->Pb;
     bool have_p = false;
-    if (c->curved && (only == -2 || only == -1)) {
-        CurvedCall cc;
+    it = StageItems();
+    if (c->curved) {
+        CurvedCall& cc = it.cc;
         std::memset(&cc, 0, sizeof(cc));
         cc.u = Su; cc.v = Sv; cc.in32 = (variant == VAR_MIX_IN32);
         cc.Pv = P;
         cc.mixed = mixed; cc.weighted = 1;
         for (int f = 0; f < 6; ++f) cc.bv[f] = bv[f];
-        int rc = pass_rc(curved_operator(c->cd, cc, c->stream), "curved operator");
-        if (rc) return rc;
+        it.has_curved = true;
+        have_p = true;
     }
-    if (c->curved) have_p = true;
-    // passes z, y, x: the x pass (LDS-free, contiguous segments) carries the
-    // final epilogue (RHS output or RK stage update)
     for (int q = 0; q < 3; ++q) {
         const int axis = 2 - q;
         const bool last = q == 2;
         const bool lap = !c->curved;
         if (!last && !lap && !ko) continue;   // nothing to add along this axis
-        if (only != -2 && only != axis) {
-            have_p = true;
-            continue;
-        }
         PassDesc p = base_pass(c, axis, bv);
         p.variant = variant;
         p.lap = lap;
@@ -394,9 +400,27 @@ static int run_stage(wt2_ctx* c, int stage, const double* u, const double* v, do
         p.Au = fld(c, c->Ab, 0); p.Av = fld(c, c->Ab, 1);
         p.Ou = Ou; p.Ov = Ov;
         p.dt = dt;
-        int rc = pass_rc(launch_flat_pass(p, c->stream), "flat pass");
-        if (rc) return rc;
+        it.pass[q] = p;
+        it.has_pass[q] = true;
         have_p = true;
+    }
+}
+
+// only: -2 = everything, -1 = the curved operator alone, 0..2 = that axis's pass alone
+static int run_stage(wt2_ctx* c, int stage, const double* u, const double* v, double* Ou,
+                     double* Ov, double* Yu, double* Yv, double dt, int only = -2)
+{
+    StageItems it;
+    stage_items(c, stage, u, v, Ou, Ov, Yu, Yv, dt, it);
+    if (it.has_curved && (only == -2 || only == -1)) {
+        int rc = pass_rc(curved_operator(c->cd, it.cc, c->stream), "curved operator");
+        if (rc) return rc;
+    }
+    for (int q = 0; q < 3; ++q) {
+        if (!it.has_pass[q]) continue;
+        if (only != -2 && only != 2 - q) continue;
+        int rc = pass_rc(launch_flat_pass(it.pass[q], c->stream), "flat pass");
+        if (rc) return rc;
     }
     return WT2_OK;
 }
@@ -836,6 +860,279 @@ int wt2_energy(wt2_ctx* c, const double* u, const double* ud, double* E, int32_t
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(E, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     return sync(c);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+//
+// A batch advances many independent instances (contexts with their own mesh,
+// N, precision, dt and step count, and device-resident state) together: every
+// RK4 stage is one launch per group of instances that share a launch shape
+// (phase, N, precision, pass-kernel kind), over a packed work list of all
+// their tiles or elements.  The launches of one step are captured as a
+// hipGraph and replayed; the plans and the graph are rebuilt only when the set
+// of unfinished instances changes.
+
+namespace {
+
+struct BatchSlot {
+    wt2_ctx* ctx = nullptr;
+    double* u = nullptr;
+    double* v = nullptr;
+    double dt = 0.0;
+    int64_t nsteps = 0, done = 0;
+    int32_t every = 0;
+    std::vector<double> energy;
+};
+
+struct StagePlans {
+    std::vector<CurvedBatchPlan> curved;     // phase 0
+    std::vector<FlatBatchPlan> flat[3];      // phases 1..3 (z, y, x)
+};
+
+}  // namespace
+
+struct wt2_batch {
+    std::vector<BatchSlot> slots;
+    hipStream_t stream = nullptr;
+    bool dirty = true;
+    StagePlans plans[4];
+    hipGraphExec_t gexec = nullptr;
+    bool gvalid = false;
+    double* Ed = nullptr;
+};
+
+static void batch_free_plans(wt2_batch* b)
+{
+    for (auto& sp : b->plans) {
+        for (auto& p : sp.curved) curved_batch_free(p);
+        sp.curved.clear();
+        for (auto& v : sp.flat) {
+            for (auto& p : v) flat_batch_free(p);
+            v.clear();
+        }
+    }
+    if (b->gvalid) { (void)hipGraphExecDestroy(b->gexec); b->gvalid = false; }
+}
+
+static bool batch_active(const BatchSlot& s) { return s.ctx && s.done < s.nsteps; }
+
+// Group key of a pass: everything a single launch must share.
+static std::string pass_key(const PassDesc& p)
+{
+    char buf[96];
+    std::snprintf(buf, sizeof(buf), "%d/%d/%d/%d/%d/%d/%d/%d", p.N, p.variant, p.axis,
+                  (p.axis == 0 || p.direct) ? 1 : 0, p.epi, p.lap, p.ko, p.needv);
+    return buf;
+}
+
+static int batch_build(wt2_batch* b)
+{
+    batch_free_plans(b);
+    for (int stage = 1; stage <= 4; ++stage) {
+        StagePlans& sp = b->plans[stage - 1];
+        std::map<std::string, std::vector<PassDesc>> groups[3];
+        std::map<std::string, std::pair<std::vector<const CurvedData*>, std::vector<CurvedCall>>> cgroups;
+        for (auto& s : b->slots) {
+            if (!batch_active(s)) continue;
+            StageItems it;
+            stage_items(s.ctx, stage, nullptr, nullptr, nullptr, nullptr, s.u, s.v, s.dt, it);
+            if (it.has_curved) {
+                char key[64];
+                std::snprintf(key, sizeof(key), "%d/%d/%d", s.ctx->N, it.cc.mixed, it.cc.in32);
+                auto& g = cgroups[key];
+                g.first.push_back(&s.ctx->cd);
+                g.second.push_back(it.cc);
+            }
+            for (int q = 0; q < 3; ++q)
+                if (it.has_pass[q]) groups[q][pass_key(it.pass[q])].push_back(it.pass[q]);
+        }
+        for (auto& kv : cgroups) {
+            CurvedBatchPlan plan;
+            hipError_t e = curved_batch_build(kv.second.first.data(), kv.second.second.data(),
+                                              int(kv.second.first.size()), plan);
+            if (e != hipSuccess) { curved_batch_free(plan); return pass_rc(e, "batch build (curved)"); }
+            sp.curved.push_back(plan);
+        }
+        for (int q = 0; q < 3; ++q)
+            for (auto& kv : groups[q]) {
+                FlatBatchPlan plan;
+                hipError_t e = flat_batch_build(kv.second.data(), int(kv.second.size()), plan);
+                if (e != hipSuccess) { flat_batch_free(plan); return pass_rc(e, "batch build (pass)"); }
+                sp.flat[q].push_back(plan);
+            }
+    }
+    b->dirty = false;
+    return WT2_OK;
+}
+
+static int batch_launch_step(wt2_batch* b)
+{
+    for (int stage = 0; stage < 4; ++stage) {
+        const StagePlans& sp = b->plans[stage];
+        for (const auto& p : sp.curved) {
+            int rc = pass_rc(curved_batch_launch(p, b->stream), "batch curved operator");
+            if (rc) return rc;
+        }
+        for (int q = 0; q < 3; ++q)
+            for (const auto& p : sp.flat[q]) {
+                int rc = pass_rc(flat_batch_launch(p, b->stream), "batch pass");
+                if (rc) return rc;
+            }
+    }
+    return WT2_OK;
+}
+
+extern "C" {
+
+int wt2_batch_create(wt2_batch** out)
+{
+    if (!out) { set_error("wt2_batch_create: null pointer"); return WT2_ERR_ARG; }
+    wt2_batch* b = new wt2_batch();
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&b->Ed, sizeof(double) * (RED_BLOCKS + 8)) != hipSuccess) {
+        set_error("wt2_batch_create: HIP stream / allocation failed");
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+        return WT2_ERR_HIP;
+    }
+    *out = b;
+    return WT2_OK;
+}
+
+void wt2_batch_destroy(wt2_batch* b)
+{
+    if (!b) return;
+    batch_free_plans(b);
+    for (auto& s : b->slots)
+        if (s.ctx) s.ctx->stream = s.ctx->own_stream;
+    (void)hipFree(b->Ed);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int wt2_batch_add(wt2_batch* b, wt2_ctx* ctx, double* u, double* ud, double dt, int64_t nsteps,
+                  int32_t energy_every, int32_t* slot)
+{
+    if (!b || !ctx || !u || !ud || !slot) { set_error("wt2_batch_add: null pointer"); return WT2_ERR_ARG; }
+    if (!(dt > 0) || nsteps < 0 || energy_every < 0) { set_error("wt2_batch_add: bad dt / nsteps / energy_every"); return WT2_ERR_ARG; }
+    if (misaligned(u) || misaligned(ud)) { set_error("wt2_batch_add: state pointers must be 16-byte aligned"); return WT2_ERR_ARG; }
+    for (auto& s : b->slots)
+        if (s.ctx == ctx) { set_error("wt2_batch_add: context already in the batch"); return WT2_ERR_STATE; }
+    if (ctx->gvalid) { (void)hipGraphExecDestroy(ctx->gexec); ctx->gvalid = false; }
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->stream = b->stream;
+    BatchSlot s;
+    s.ctx = ctx; s.u = u; s.v = ud; s.dt = dt; s.nsteps = nsteps; s.done = 0; s.every = energy_every;
+    if (energy_every > 0) {
+        double E;
+        int rc = run_energy(ctx, u, ud, b->Ed);
+        if (!rc) rc = hipMemcpyAsync(&E, b->Ed, sizeof(double), hipMemcpyDeviceToHost, b->stream) == hipSuccess ? 0 : WT2_ERR_HIP;
+        if (!rc) rc = hipStreamSynchronize(b->stream) == hipSuccess ? 0 : WT2_ERR_HIP;
+        if (rc) { ctx->stream = ctx->own_stream; return rc; }
+        s.energy.push_back(E);
+    }
+    int id = -1;
+    for (size_t q = 0; q < b->slots.size(); ++q)
+        if (!b->slots[q].ctx) { id = int(q); break; }
+    if (id < 0) { id = int(b->slots.size()); b->slots.push_back(s); }
+    else b->slots[id] = s;
+    *slot = id;
+    if (nsteps > 0) b->dirty = true;
+    return WT2_OK;
+}
+
+int wt2_batch_remove(wt2_batch* b, int32_t slot)
+{
+    if (!b || slot < 0 || slot >= int32_t(b->slots.size()) || !b->slots[slot].ctx) {
+        set_error("wt2_batch_remove: no such slot");
+        return WT2_ERR_ARG;
+    }
+    (void)hipStreamSynchronize(b->stream);
+    BatchSlot& s = b->slots[slot];
+    if (batch_active(s)) b->dirty = true;
+    s.ctx->stream = s.ctx->own_stream;
+    s = BatchSlot();
+    return WT2_OK;
+}
+
+int wt2_batch_step(wt2_batch* b, int32_t max_steps, int32_t* finished)
+{
+    if (!b || max_steps < 0) { set_error("wt2_batch_step: bad arguments"); return WT2_ERR_ARG; }
+    int32_t nfin = 0;
+    int32_t left = max_steps;
+    while (left > 0) {
+        int64_t k = left;
+        bool any = false;
+        for (auto& s : b->slots) {
+            if (!batch_active(s)) continue;
+            any = true;
+            k = std::min<int64_t>(k, s.nsteps - s.done);
+            if (s.every > 0) k = std::min<int64_t>(k, s.every - s.done % s.every);
+        }
+        if (!any) break;
+        if (b->dirty) {
+            int rc = batch_build(b);
+            if (rc) return rc;
+        }
+        int64_t r = k;
+        if (!b->gvalid) {
+            // one eager step (sets kernel attributes), then capture the step
+            int rc = batch_launch_step(b);
+            if (rc) return rc;
+            --r;
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            rc = batch_launch_step(b);
+            hipError_t ce = hipStreamEndCapture(b->stream, &g);
+            if (rc) return rc;
+            if (ce != hipSuccess) return pass_rc(ce, "batch graph capture");
+            hipError_t ie = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) return pass_rc(ie, "batch graph instantiate");
+            b->gvalid = true;
+        }
+        for (int64_t q = 0; q < r; ++q) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+        left -= int32_t(k);
+        for (auto& s : b->slots) {
+            if (!batch_active(s)) continue;
+            s.done += k;
+            if (s.every > 0 && s.done % s.every == 0) {
+                double E;
+                int rc = run_energy(s.ctx, s.u, s.v, b->Ed);
+                if (rc) return rc;
+                HIPCHK(hipMemcpyAsync(&E, b->Ed, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+                HIPCHK(hipStreamSynchronize(b->stream));
+                if (!std::isfinite(E)) { set_error("wt2_batch_step: non-finite energy in a batch slot"); return WT2_ERR_NONFINITE; }
+                s.energy.push_back(E);
+            }
+            if (s.done >= s.nsteps) {
+                ++nfin;
+                b->dirty = true;
+                if (b->gvalid) { (void)hipGraphExecDestroy(b->gexec); b->gvalid = false; }
+            }
+        }
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (finished) *finished = nfin;
+    return WT2_OK;
+}
+
+int wt2_batch_slot(const wt2_batch* b, int32_t slot, int64_t* steps_done, double* energies, int32_t cap,
+                   int32_t* n_energies)
+{
+    if (!b || slot < 0 || slot >= int32_t(b->slots.size()) || !b->slots[slot].ctx) {
+        set_error("wt2_batch_slot: no such slot");
+        return WT2_ERR_ARG;
+    }
+    const BatchSlot& s = b->slots[slot];
+    if (steps_done) *steps_done = s.done;
+    const int32_t n = int32_t(s.energy.size());
+    if (n_energies) *n_energies = n;
+    if (energies)
+        for (int32_t q = 0; q < std::min(n, cap); ++q) energies[q] = s.energy[q];
+    return WT2_OK;
 }
 
 }  // extern "C"

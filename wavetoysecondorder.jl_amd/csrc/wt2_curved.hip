@@ -298,10 +298,9 @@ __device__ __forceinline__ void grad_lds(const TC* f, const CConst<N, TC>& k, in
 
 // Kernel A: normal flux F^a = K^ab G_b u at every face node of every element.
 template <int N, typename TC, typename TS>
-__global__ void __launch_bounds__(256) curved_faceflux_kernel(CArgs<N, TC> a)
+__device__ __forceinline__ void faceflux_body(const CArgs<N, TC>& a, const int64_t e)
 {
     __shared__ TC su[N * N * N];
-    const int64_t e = blockIdx.x;
     const TS* u = static_cast<const TS*>(a.u);
     for (int q = threadIdx.x; q < N * N * N; q += blockDim.x) su[q] = TC(u[e * N * N * N + q]);
     __syncthreads();
@@ -323,7 +322,7 @@ __global__ void __launch_bounds__(256) curved_faceflux_kernel(CArgs<N, TC> a)
 
 // Kernel B: the whole curved operator for one element per workgroup.
 template <int N, typename TC, typename TS, typename TO>
-__global__ void __launch_bounds__(1024) curved_volume_kernel(CArgs<N, TC> a)
+__device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t e)
 {
     constexpr int N3 = N * N * N, N2 = N * N;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -332,7 +331,6 @@ __global__ void __launch_bounds__(1024) curved_volume_kernel(CArgs<N, TC> a)
     TC* W = sv + N3;                            // 3 N3: F^a + c^a v
     TC* S = W + 3 * N3;                         // 3 x 6 N2: alpha n K^ab du per face node
     TC* Pt = S + 18 * N2;                       // 6 N2: point terms of the face SAT
-    const int64_t e = blockIdx.x;
     const int64_t eb = e * N3;
     const TS* u = static_cast<const TS*>(a.u);
     const TS* v = static_cast<const TS*>(a.v);
@@ -442,10 +440,37 @@ __global__ void __launch_bounds__(1024) curved_volume_kernel(CArgs<N, TC> a)
     }
 }
 
-template <int N, typename TC, typename TS, typename TO>
-static hipError_t launch_c(const CurvedData& cd, const CurvedCall& cc, hipStream_t st)
+template <int N, typename TC, typename TS>
+__global__ void __launch_bounds__(256) curved_faceflux_kernel(CArgs<N, TC> a)
 {
-    CArgs<N, TC> a;
+    faceflux_body<N, TC, TS>(a, blockIdx.x);
+}
+
+template <int N, typename TC, typename TS, typename TO>
+__global__ void __launch_bounds__(1024) curved_volume_kernel(CArgs<N, TC> a)
+{
+    volume_body<N, TC, TS, TO>(a, blockIdx.x);
+}
+
+template <int N, typename TC, typename TS>
+__global__ void __launch_bounds__(256) curved_faceflux_batch_kernel(const CArgs<N, TC>* __restrict__ tab,
+                                                                    const int2* __restrict__ work)
+{
+    const int2 w = work[blockIdx.x];
+    faceflux_body<N, TC, TS>(tab[w.x], w.y);
+}
+
+template <int N, typename TC, typename TS, typename TO>
+__global__ void __launch_bounds__(1024) curved_volume_batch_kernel(const CArgs<N, TC>* __restrict__ tab,
+                                                                   const int2* __restrict__ work)
+{
+    const int2 w = work[blockIdx.x];
+    volume_body<N, TC, TS, TO>(tab[w.x], w.y);
+}
+
+template <int N, typename TC>
+static void make_cargs(const CurvedData& cd, const CurvedCall& cc, CArgs<N, TC>& a)
+{
     for (int i = 0; i < N * N; ++i) a.k.G[i] = TC(cd.G[i]);
     for (int i = 0; i < N; ++i) { a.k.w[i] = TC(cd.w[i]); a.k.winv[i] = TC(1.0 / cd.w[i]); }
     a.k.tau = TC(cd.tau);
@@ -460,6 +485,13 @@ static hipError_t launch_c(const CurvedData& cd, const CurvedCall& cc, hipStream
     a.per0 = cd.periodic[0]; a.per1 = cd.periodic[1]; a.per2 = cd.periodic[2];
     a.adv = cc.v != nullptr;
     a.weighted = cc.weighted;
+}
+
+template <int N, typename TC, typename TS, typename TO>
+static hipError_t launch_c(const CurvedData& cd, const CurvedCall& cc, hipStream_t st)
+{
+    CArgs<N, TC> a;
+    make_cargs<N, TC>(cd, cc, a);
     (void)hipGetLastError();   // launch errors below are this launch's own
     hipLaunchKernelGGL((curved_faceflux_kernel<N, TC, TS>), dim3(unsigned(cd.Ne)), dim3(256), 0, st, a);
     hipError_t err = hipGetLastError();
@@ -489,6 +521,117 @@ static hipError_t launch_cn(const CurvedData& cd, const CurvedCall& cc, hipStrea
     if (cc.out) return launch_c<N, float, double, double>(cd, cc, st);
     if (cc.in32) return launch_c<N, float, float, float>(cd, cc, st);
     return launch_c<N, float, double, float>(cd, cc, st);
+}
+
+// ------------------------------------------------------------------ batches
+
+template <int N, typename TC, typename TS, typename TO>
+static hipError_t cbatch_build_t(const CurvedData* const* cds, const CurvedCall* ccs, int n,
+                                 CurvedBatchPlan& plan)
+{
+    std::vector<CArgs<N, TC>> tab(n);
+    std::vector<int2> work;
+    for (int i = 0; i < n; ++i) {
+        make_cargs<N, TC>(*cds[i], ccs[i], tab[i]);
+        for (int64_t e = 0; e < cds[i]->Ne; ++e) work.push_back(make_int2(i, int(e)));
+    }
+    const size_t tb = sizeof(CArgs<N, TC>) * size_t(n), wb = sizeof(int2) * work.size();
+    if (tb > plan.tab_cap) {
+        (void)hipFree(plan.tab);
+        if (hipMalloc(&plan.tab, tb) != hipSuccess) { plan.tab = nullptr; plan.tab_cap = 0; return hipErrorOutOfMemory; }
+        plan.tab_cap = tb;
+    }
+    if (wb > plan.work_cap) {
+        (void)hipFree(plan.work);
+        if (hipMalloc(reinterpret_cast<void**>(&plan.work), wb) != hipSuccess) {
+            plan.work = nullptr; plan.work_cap = 0;
+            return hipErrorOutOfMemory;
+        }
+        plan.work_cap = wb;
+    }
+    hipError_t e = hipMemcpy(plan.tab, tab.data(), tb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(plan.work, work.data(), wb, hipMemcpyHostToDevice);
+    plan.N = N;
+    plan.nwork = int(work.size());
+    return e;
+}
+
+template <int N, typename TC, typename TS, typename TO>
+static hipError_t cbatch_launch_t(const CurvedBatchPlan& plan, hipStream_t st)
+{
+    if (plan.nwork == 0) return hipSuccess;
+    const CArgs<N, TC>* tab = static_cast<const CArgs<N, TC>*>(plan.tab);
+    (void)hipGetLastError();
+    hipLaunchKernelGGL((curved_faceflux_batch_kernel<N, TC, TS>), dim3(plan.nwork), dim3(256), 0, st, tab, plan.work);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    constexpr int N3 = N * N * N;
+    const int nthr = ((N3 + 63) / 64) * 64;
+    const size_t shmem = (5 * N3 + 24 * N * N) * sizeof(TC);
+    auto kern = curved_volume_batch_kernel<N, TC, TS, TO>;
+    static bool raised = false;
+    if (shmem > 64 * 1024 && !raised) {
+        err = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
+        if (err != hipSuccess) return err;
+        raised = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(plan.nwork), dim3(nthr), shmem, st, tab, plan.work);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t cbatch_n(const CurvedData* const* cds, const CurvedCall* ccs, int n, CurvedBatchPlan* plan,
+                           const CurvedBatchPlan* lp, hipStream_t st)
+{
+    const int kind = plan ? (ccs[0].mixed ? (ccs[0].in32 ? 2 : 1) : 0) : lp->kind;
+    if (plan) plan->kind = kind;
+    switch (kind) {
+    case 0: return plan ? cbatch_build_t<N, double, double, double>(cds, ccs, n, *plan)
+                        : cbatch_launch_t<N, double, double, double>(*lp, st);
+    case 1: return plan ? cbatch_build_t<N, float, double, float>(cds, ccs, n, *plan)
+                        : cbatch_launch_t<N, float, double, float>(*lp, st);
+    case 2: return plan ? cbatch_build_t<N, float, float, float>(cds, ccs, n, *plan)
+                        : cbatch_launch_t<N, float, float, float>(*lp, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+static hipError_t cbatch_dispatch(int N, const CurvedData* const* cds, const CurvedCall* ccs, int n,
+                                  CurvedBatchPlan* plan, const CurvedBatchPlan* lp, hipStream_t st)
+{
+    switch (N) {
+    case 3: return cbatch_n<3>(cds, ccs, n, plan, lp, st);
+    case 4: return cbatch_n<4>(cds, ccs, n, plan, lp, st);
+    case 5: return cbatch_n<5>(cds, ccs, n, plan, lp, st);
+    case 6: return cbatch_n<6>(cds, ccs, n, plan, lp, st);
+    case 7: return cbatch_n<7>(cds, ccs, n, plan, lp, st);
+    case 8: return cbatch_n<8>(cds, ccs, n, plan, lp, st);
+    case 9: return cbatch_n<9>(cds, ccs, n, plan, lp, st);
+    case 10: return cbatch_n<10>(cds, ccs, n, plan, lp, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t curved_batch_build(const CurvedData* const* cds, const CurvedCall* ccs, int n, CurvedBatchPlan& plan)
+{
+    if (n <= 0) { plan.nwork = 0; return hipSuccess; }
+    for (int i = 0; i < n; ++i)
+        if (cds[i]->N != cds[0]->N || ccs[i].mixed != ccs[0].mixed || ccs[i].in32 != ccs[0].in32 || ccs[i].out)
+            return hipErrorInvalidValue;
+    return cbatch_dispatch(cds[0]->N, cds, ccs, n, &plan, nullptr, nullptr);
+}
+
+hipError_t curved_batch_launch(const CurvedBatchPlan& plan, hipStream_t s)
+{
+    return cbatch_dispatch(plan.N, nullptr, nullptr, 0, nullptr, &plan, s);
+}
+
+void curved_batch_free(CurvedBatchPlan& plan)
+{
+    (void)hipFree(plan.tab);
+    (void)hipFree(plan.work);
+    plan = CurvedBatchPlan();
 }
 
 hipError_t curved_operator(const CurvedData& cd, const CurvedCall& cc, hipStream_t s)

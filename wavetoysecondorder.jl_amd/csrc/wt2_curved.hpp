@@ -39,6 +39,18 @@ struct CurvedCall {
     void* Pv;           // partial sum of the u_t equation (FP64 or FP32), written when out == nullptr
 };
 
+// Batched curved operator: one (face-flux, volume) launch pair over every
+// element of every instance in the group (same N and precision).
+struct CurvedBatchPlan {
+    int N = 0, kind = 0, nwork = 0;
+    void* tab = nullptr;
+    int2* work = nullptr;
+    size_t tab_cap = 0, work_cap = 0;
+};
+hipError_t curved_batch_build(const CurvedData* const* cds, const CurvedCall* ccs, int n, CurvedBatchPlan& plan);
+hipError_t curved_batch_launch(const CurvedBatchPlan& plan, hipStream_t s);
+void curved_batch_free(CurvedBatchPlan& plan);
+
 int curved_init(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const double* w,
                 const double* G, const double* bg10, hipStream_t s, int64_t* bytes);
 int curved_set_background(CurvedData& cd, const wt2_mesh_desc& d, const double* xi,

@@ -23,8 +23,10 @@
 //      RK4 stage update in the same pass.
 #include "wt2_internal.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 #include <type_traits>
+#include <vector>
 
 namespace wt2 {
 
@@ -178,7 +180,7 @@ __device__ __forceinline__ void sipg_add(const TC (&u)[N], TC (&out)[N], const T
 }
 
 template <int N, typename TC, typename TS, typename TP>
-__global__ void __launch_bounds__(512) flat_pass_kernel(KArgs<N, TC> a)
+__device__ __forceinline__ void flat_pass_body(const KArgs<N, TC>& a, const int tile)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     TS* lds = reinterpret_cast<TS*>(smem_raw);
@@ -186,7 +188,7 @@ __global__ void __launch_bounds__(512) flat_pass_kernel(KArgs<N, TC> a)
     constexpr bool MIXED = std::is_same<TP, float>::value;
 
     // ---- tile decode: two fixed element coordinates and a line block
-    int t = blockIdx.x;
+    int t = tile;
     const int blk = t % a.nbk;
     t /= a.nbk;
     const int ebase = (t % a.T0) * a.slo + (t / a.T0) * a.shi;
@@ -417,7 +419,7 @@ __device__ __forceinline__ void store_seg_s(TD* __restrict__ p, int gs, const TC
 // epilogue runs on register-resident segments.  It is the last pass of an
 // RHS evaluation (passes run z, y, x).
 template <int N, typename TC, typename TS, typename TP>
-__global__ void __launch_bounds__(512) flat_xpass_kernel(KArgs<N, TC> a)
+__device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int tile)
 {
     constexpr int N3 = N * N * N;
     constexpr bool MIXED = std::is_same<TP, float>::value;
@@ -425,7 +427,7 @@ __global__ void __launch_bounds__(512) flat_xpass_kernel(KArgs<N, TC> a)
     for (int q = threadIdx.x; q < OpLayout<N>::SIZE; q += blockDim.x) op[q] = a.op[q];
     __syncthreads();
 
-    int t = blockIdx.x;
+    int t = tile;
     const int blk = t % a.nbk;
     t /= a.nbk;
     const int ebase = (t % a.T0) * a.slo + (t / a.T0) * a.shi;
@@ -560,6 +562,37 @@ __global__ void __launch_bounds__(512) flat_xpass_kernel(KArgs<N, TC> a)
     }
 }
 
+// Kernel entry points: one context (descriptor in the kernel arguments) or a
+// batch (descriptor table indexed through a packed (instance, tile) work list,
+// one launch covering every instance of the same N and precision).
+template <int N, typename TC, typename TS, typename TP>
+__global__ void __launch_bounds__(512) flat_pass_kernel(KArgs<N, TC> a)
+{
+    flat_pass_body<N, TC, TS, TP>(a, blockIdx.x);
+}
+
+template <int N, typename TC, typename TS, typename TP>
+__global__ void __launch_bounds__(512) flat_xpass_kernel(KArgs<N, TC> a)
+{
+    flat_xpass_body<N, TC, TS, TP>(a, blockIdx.x);
+}
+
+template <int N, typename TC, typename TS, typename TP>
+__global__ void __launch_bounds__(512) flat_pass_batch_kernel(const KArgs<N, TC>* __restrict__ tab,
+                                                              const int2* __restrict__ work)
+{
+    const int2 w = work[blockIdx.x];
+    flat_pass_body<N, TC, TS, TP>(tab[w.x], w.y);
+}
+
+template <int N, typename TC, typename TS, typename TP>
+__global__ void __launch_bounds__(512) flat_xpass_batch_kernel(const KArgs<N, TC>* __restrict__ tab,
+                                                               const int2* __restrict__ work)
+{
+    const int2 w = work[blockIdx.x];
+    flat_xpass_body<N, TC, TS, TP>(tab[w.x], w.y);
+}
+
 // ------------------------------------------------------------------ launch
 
 int op_table_size(int N) { return 2 * (N - N / 2) * (N / 2) + 2 * N; }
@@ -581,9 +614,9 @@ void build_op_table(int N, const double* G, const double* w, double* tab)
 static int pick_lpw(int Ma) { return 64 / Ma; }
 
 template <int N, typename TC, typename TS, typename TP>
-static hipError_t launch_t(const PassDesc& p, hipStream_t s)
+static hipError_t make_kargs(const PassDesc& p, KArgs<N, TC>& a, int* tiles_out, int* nthr_out,
+                             size_t* shmem_out)
 {
-    KArgs<N, TC> a;
     a.op = static_cast<const TC*>(std::is_same<TC, float>::value ? p.op32 : p.op64);
     a.k.kh = TC(p.ops.kh);
     a.k.koL = TC(p.ops.koL); a.k.koR = TC(p.ops.koR);
@@ -643,6 +676,21 @@ static hipError_t launch_t(const PassDesc& p, hipStream_t s)
     const size_t static_lds = sizeof(TC) * OpLayout<N>::SIZE;
     if (shmem + static_lds > 160 * 1024) return hipErrorInvalidConfiguration;
     a.GS = p.axis == 0 ? 1 : (p.axis == 1 ? N : N * N);
+    *tiles_out = tiles;
+    *nthr_out = nthr;
+    *shmem_out = (p.axis == 0 || p.direct) ? 0 : shmem;
+    return hipSuccess;
+}
+
+template <int N, typename TC, typename TS, typename TP>
+static hipError_t launch_t(const PassDesc& p, hipStream_t s)
+{
+    KArgs<N, TC> a;
+    int tiles, nthr;
+    size_t shmem;
+    hipError_t e0 = make_kargs<N, TC, TS, TP>(p, a, &tiles, &nthr, &shmem);
+    if (e0 != hipSuccess) return e0;
+    const size_t static_lds = sizeof(TC) * OpLayout<N>::SIZE;
     if (p.axis == 0 || p.direct) {
         // LDS-free pass
         auto xk = flat_xpass_kernel<N, TC, TS, TP>;
@@ -692,6 +740,136 @@ hipError_t launch_flat_pass(const PassDesc& p, hipStream_t s)
     case 10: return launch_n<10>(p, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+// ------------------------------------------------------------------ batches
+
+template <int N, typename TC, typename TS, typename TP>
+static hipError_t batch_build_t(const PassDesc* d, int n, FlatBatchPlan& plan)
+{
+    std::vector<KArgs<N, TC>> tab(n);
+    std::vector<int2> work;
+    int nthr = 64;
+    size_t shmem = 0;
+    for (int i = 0; i < n; ++i) {
+        int tiles, nt;
+        size_t sh;
+        hipError_t e = make_kargs<N, TC, TS, TP>(d[i], tab[i], &tiles, &nt, &sh);
+        if (e != hipSuccess) return e;
+        nthr = std::max(nthr, nt);
+        shmem = std::max(shmem, sh);
+        for (int t = 0; t < tiles; ++t) work.push_back(make_int2(i, t));
+    }
+    const size_t tb = sizeof(KArgs<N, TC>) * size_t(n);
+    const size_t wb = sizeof(int2) * work.size();
+    if (tb > plan.tab_cap) {
+        (void)hipFree(plan.tab);
+        if (hipMalloc(&plan.tab, tb) != hipSuccess) { plan.tab = nullptr; plan.tab_cap = 0; return hipErrorOutOfMemory; }
+        plan.tab_cap = tb;
+    }
+    if (wb > plan.work_cap) {
+        (void)hipFree(plan.work);
+        if (hipMalloc(reinterpret_cast<void**>(&plan.work), wb) != hipSuccess) {
+            plan.work = nullptr; plan.work_cap = 0;
+            return hipErrorOutOfMemory;
+        }
+        plan.work_cap = wb;
+    }
+    hipError_t e = hipMemcpy(plan.tab, tab.data(), tb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(plan.work, work.data(), wb, hipMemcpyHostToDevice);
+    plan.N = N;
+    plan.variant = d[0].variant;
+    plan.direct = d[0].axis == 0 || d[0].direct;
+    plan.ntiles = int(work.size());
+    plan.nthr = nthr;
+    plan.shmem = plan.direct ? 0 : shmem;
+    return e;
+}
+
+template <int N, typename TC, typename TS, typename TP>
+static hipError_t batch_launch_t(const FlatBatchPlan& plan, hipStream_t s)
+{
+    if (plan.ntiles == 0) return hipSuccess;
+    const KArgs<N, TC>* tab = static_cast<const KArgs<N, TC>*>(plan.tab);
+    if (plan.direct) {
+        (void)hipGetLastError();
+        hipLaunchKernelGGL((flat_xpass_batch_kernel<N, TC, TS, TP>), dim3(plan.ntiles), dim3(plan.nthr), 0, s,
+                           tab, plan.work);
+        return hipGetLastError();
+    }
+    auto kern = flat_pass_batch_kernel<N, TC, TS, TP>;
+    const size_t static_lds = sizeof(TC) * OpLayout<N>::SIZE;
+    static size_t lds_limit = 64 * 1024 - static_lds;
+    if (plan.shmem > lds_limit) {
+        const size_t want = 160 * 1024 - static_lds;
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
+        if (e != hipSuccess) return e;
+        lds_limit = want;
+    }
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(kern, dim3(plan.ntiles), dim3(plan.nthr), plan.shmem, s, tab, plan.work);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t batch_n(const PassDesc* d, int n, FlatBatchPlan* plan, const FlatBatchPlan* lp,
+                          hipStream_t s)
+{
+    const int variant = plan ? d[0].variant : lp->variant;
+    switch (variant) {
+    case VAR_F64:
+        return plan ? batch_build_t<N, double, double, double>(d, n, *plan)
+                    : batch_launch_t<N, double, double, double>(*lp, s);
+    case VAR_MIX_IN64:
+        return plan ? batch_build_t<N, float, double, float>(d, n, *plan)
+                    : batch_launch_t<N, float, double, float>(*lp, s);
+    case VAR_MIX_IN32:
+        return plan ? batch_build_t<N, float, float, float>(d, n, *plan)
+                    : batch_launch_t<N, float, float, float>(*lp, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+static hipError_t batch_dispatch(int N, const PassDesc* d, int n, FlatBatchPlan* plan,
+                                 const FlatBatchPlan* lp, hipStream_t s)
+{
+    switch (N) {
+    case 3: return batch_n<3>(d, n, plan, lp, s);
+    case 4: return batch_n<4>(d, n, plan, lp, s);
+    case 5: return batch_n<5>(d, n, plan, lp, s);
+    case 6: return batch_n<6>(d, n, plan, lp, s);
+    case 7: return batch_n<7>(d, n, plan, lp, s);
+    case 8: return batch_n<8>(d, n, plan, lp, s);
+    case 9: return batch_n<9>(d, n, plan, lp, s);
+    case 10: return batch_n<10>(d, n, plan, lp, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t flat_batch_build(const PassDesc* descs, int n, FlatBatchPlan& plan)
+{
+    if (n <= 0) { plan.ntiles = 0; return hipSuccess; }
+    for (int i = 0; i < n; ++i) {
+        const PassDesc& p = descs[i];
+        if (p.N != descs[0].N || p.variant != descs[0].variant || p.axis != descs[0].axis ||
+            p.direct != descs[0].direct || p.epi != descs[0].epi)
+            return hipErrorInvalidValue;
+        if (p.nb < 1 || p.N % p.nb != 0 || p.M[p.axis] > MMAX_AXIS) return hipErrorInvalidValue;
+    }
+    return batch_dispatch(descs[0].N, descs, n, &plan, nullptr, nullptr);
+}
+
+hipError_t flat_batch_launch(const FlatBatchPlan& plan, hipStream_t s)
+{
+    return batch_dispatch(plan.N, nullptr, 0, nullptr, &plan, s);
+}
+
+void flat_batch_free(FlatBatchPlan& plan)
+{
+    (void)hipFree(plan.tab);
+    (void)hipFree(plan.work);
+    plan = FlatBatchPlan();
 }
 
 // ------------------------------------------------------------------ energy

@@ -73,6 +73,20 @@ struct PassDesc {
 // Launch one pencil pass on stream s.  Returns hipError_t.
 hipError_t launch_flat_pass(const PassDesc& p, hipStream_t s);
 
+// A batch of pencil passes: one launch over every instance of a group (same N,
+// precision variant, axis, kernel kind and epilogue), driven by a device table
+// of per-instance descriptors and a packed (instance, tile) work list.
+struct FlatBatchPlan {
+    int N = 0, variant = 0, direct = 0, ntiles = 0, nthr = 0;
+    size_t shmem = 0;
+    void* tab = nullptr;     // device descriptors
+    int2* work = nullptr;    // device work list
+    size_t tab_cap = 0, work_cap = 0;
+};
+hipError_t flat_batch_build(const PassDesc* descs, int n, FlatBatchPlan& plan);
+hipError_t flat_batch_launch(const FlatBatchPlan& plan, hipStream_t s);
+void flat_batch_free(FlatBatchPlan& plan);
+
 // Deterministic FP64 energy reduction: E = 1/2 sum H (v^2 - u Lu) over a flat
 // or curved mesh.  partial must hold red_blocks doubles; result written to
 // out (device).

@@ -435,3 +435,54 @@ def recommended_dt_rk4(work, cfl=0.2):
     if eps > 0:
         dt = min(dt, 1.4 / (eps * work.mu))
     return dt
+
+
+# ------------------------------------------------------------------ batches
+
+class Batch:
+    """Many independent evolutions advanced together (wt2_batch_*): each slot is
+    a Workspace with device-resident state and its own dt and step count."""
+
+    def __init__(self):
+        self.ptr = ctypes.c_void_p()
+        check(lib().wt2_batch_create(ctypes.byref(self.ptr)))
+        self.slots = {}
+
+    def close(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            lib().wt2_batch_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, work, u, ud, dt, nsteps, energy_every=0):
+        if torch is None or not (isinstance(u, torch.Tensor) and u.is_cuda):
+            raise TypeError("batch state must be torch CUDA tensors (device-resident)")
+        (pu, pud), fl = work._ptrs(u, ud, writable=(0, 1))
+        slot = ctypes.c_int32()
+        work._sync_torch()
+        check(lib().wt2_batch_add(self.ptr, work.ctx, pu, pud, float(dt), int(nsteps), int(energy_every),
+                                  ctypes.byref(slot)))
+        self.slots[slot.value] = (work, u, ud, int(nsteps), int(energy_every))
+        return slot.value
+
+    def remove(self, slot):
+        check(lib().wt2_batch_remove(self.ptr, int(slot)))
+        self.slots.pop(slot, None)
+
+    def step(self, max_steps):
+        fin = ctypes.c_int32()
+        check(lib().wt2_batch_step(self.ptr, int(max_steps), ctypes.byref(fin)))
+        return fin.value
+
+    def slot_info(self, slot):
+        done = ctypes.c_int64()
+        n = ctypes.c_int32()
+        check(lib().wt2_batch_slot(self.ptr, int(slot), ctypes.byref(done), None, 0, ctypes.byref(n)))
+        E = np.zeros(max(n.value, 1))
+        check(lib().wt2_batch_slot(self.ptr, int(slot), None, E.ctypes.data, n.value, None))
+        return done.value, E[:n.value]
