@@ -158,6 +158,15 @@ int wt2_batch_step(wt2_batch* b, int32_t max_steps, int32_t* finished);
 int wt2_batch_slot(const wt2_batch* b, int32_t slot, int64_t* steps_done, double* energies,
                    int32_t cap, int32_t* n_energies);
 
+/* Autotuner (no counterpart in the reference): times every valid pass-kernel
+ * kind and tile size per axis on this context with state (u, ud) (device
+ * pointers, only read) and keeps the fastest; the choice is cached per
+ * (N, M, precision, curved, KO, periodicity) and applied to later contexts of
+ * the same shape.  flags & 1 forces a re-measurement. */
+int wt2_ctx_autotune(wt2_ctx* ctx, double* u, double* ud, double dt, int32_t flags);
+/* Current tile choice: element planes per tile and LDS-free flag per axis. */
+int wt2_ctx_tiles(const wt2_ctx* ctx, int32_t* nb3, int32_t* direct3);
+
 /* Diagnostics (no counterpart in the reference; used by bench.py to time one
  * kernel in isolation with events on the context stream): enqueue, without
  * synchronising, the single pencil pass of RK4 stage `stage` (1..4) along

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -507,6 +508,28 @@ int sync(wt2_ctx* c)
 }
 }  // namespace
 
+namespace {
+struct TuneKey {
+    int v[8];
+    bool operator<(const TuneKey& o) const { return std::lexicographical_compare(v, v + 8, o.v, o.v + 8); }
+};
+struct TuneVal {
+    int nb[3], direct[3];
+    float ms[3];
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+
+TuneKey tune_key(const wt2_ctx* c)
+{
+    TuneKey k;
+    k.v[0] = c->N; k.v[1] = c->d.M[0]; k.v[2] = c->d.M[1]; k.v[3] = c->d.M[2];
+    k.v[4] = c->d.precision; k.v[5] = c->curved; k.v[6] = c->d.eps_ko > 0;
+    k.v[7] = c->d.periodic[0] + 2 * c->d.periodic[1] + 4 * c->d.periodic[2];
+    return k;
+}
+}  // namespace
+
 extern "C" {
 
 const char* wt2_last_error(void) { return g_err.c_str(); }
@@ -630,6 +653,12 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
         }
     }
 
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(tune_key(c));
+        if (it != g_tune.end())
+            for (int a = 0; a < 3; ++a) { c->nb[a] = it->second.nb[a]; c->direct[a] = it->second.direct[a]; }
+    }
     if (hipGetDevice(&c->device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
         set_error("wt2_ctx_create: no HIP device / stream");
@@ -847,6 +876,82 @@ int wt2_launch_pass(wt2_ctx* c, int32_t axis, int32_t stage, double* u, double* 
     if (axis < -1 || axis > 2 || stage < 1 || stage > 4) { set_error("wt2_launch_pass: bad axis / stage"); return WT2_ERR_ARG; }
     if (axis == -1 && !c->curved) { set_error("wt2_launch_pass: axis -1 needs a curved context"); return WT2_ERR_ARG; }
     return run_stage(c, stage, nullptr, nullptr, nullptr, nullptr, u, ud, dt, axis);
+}
+
+}  // extern "C"
+
+
+extern "C" {
+
+// Autotuner (no counterpart in the reference): picks, per axis, the pass kernel
+// (LDS-free or LDS-staged) and the element planes per tile by timing every
+// valid candidate of that axis's stage-2 pass on the context's stream with the
+// given state (device pointers; the state is only read).  Results are cached
+// per (N, M, precision, curved, KO, periodicity) for later contexts.
+int wt2_ctx_autotune(wt2_ctx* c, double* u, double* ud, double dt, int32_t flags)
+{
+    if (!c || !u || !ud || !(dt > 0)) { set_error("wt2_ctx_autotune: bad arguments"); return WT2_ERR_ARG; }
+    if (misaligned(u) || misaligned(ud)) { set_error("wt2_ctx_autotune: device pointers must be 16-byte aligned"); return WT2_ERR_ARG; }
+    const TuneKey key = tune_key(c);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end() && !(flags & 1)) {
+            for (int a = 0; a < 3; ++a) { c->nb[a] = it->second.nb[a]; c->direct[a] = it->second.direct[a]; }
+            if (c->gvalid) { (void)hipGraphExecDestroy(c->gexec); c->gvalid = false; }
+            return WT2_OK;
+        }
+    }
+    if (c->gvalid) { (void)hipGraphExecDestroy(c->gexec); c->gvalid = false; }
+    int rc = run_stage(c, 1, nullptr, nullptr, nullptr, nullptr, u, ud, dt);   // fill S, P, A
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    TuneVal best;
+    const int N = c->N;
+    for (int a = 2; a >= 0; --a) {
+        float tbest = 1e30f;
+        int bnb = c->nb[a], bdir = c->direct[a];
+        const int lpw = 64 / c->d.M[a];
+        for (int dir = 1; dir >= (a == 0 ? 1 : 0); --dir) {
+            for (int nb = 1; nb <= N; ++nb) {
+                if (N % nb || (N * nb + lpw - 1) / lpw > 8) continue;
+                c->direct[a] = dir;
+                c->nb[a] = nb;
+                rc = run_stage(c, 2, nullptr, nullptr, nullptr, nullptr, u, ud, dt, a);
+                if (rc) { set_error(""); continue; }     // configuration not launchable: skip it
+                const int reps = 8;
+                (void)hipEventRecord(e0, c->stream);
+                for (int r = 0; r < reps && !rc; ++r) rc = run_stage(c, 2, nullptr, nullptr, nullptr, nullptr, u, ud, dt, a);
+                (void)hipEventRecord(e1, c->stream);
+                (void)hipEventSynchronize(e1);
+                if (rc) return rc;
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                ms /= reps;
+                if (ms < tbest) { tbest = ms; bnb = nb; bdir = dir; }
+            }
+        }
+        c->nb[a] = bnb;
+        c->direct[a] = bdir;
+        best.nb[a] = bnb; best.direct[a] = bdir; best.ms[a] = tbest;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = best;
+    return WT2_OK;
+}
+
+int wt2_ctx_tiles(const wt2_ctx* c, int32_t* nb3, int32_t* direct3)
+{
+    if (!c) { set_error("wt2_ctx_tiles: null ctx"); return WT2_ERR_ARG; }
+    for (int a = 0; a < 3; ++a) {
+        if (nb3) nb3[a] = c->nb[a];
+        if (direct3) direct3[a] = c->direct[a];
+    }
+    return WT2_OK;
 }
 
 int wt2_energy(wt2_ctx* c, const double* u, const double* ud, double* E, int32_t flags)

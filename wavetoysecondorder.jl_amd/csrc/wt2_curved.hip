@@ -16,7 +16,10 @@
 // into a compact face buffer (6 N^2 per element), the second does everything
 // else for one element per workgroup with the element staged in LDS and the
 // neighbours' face data read from the face buffer and the field arrays.
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -82,9 +85,12 @@ static void compute_coefs(const wt2_mesh_desc& d, const double* xi_n, const doub
     double L[3];
     for (int a = 0; a < 3; ++a) L[a] = d.x1[a] - d.x0[a];
     const double tp = 2.0 * M_PI;
+    unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    if (const char* env = std::getenv("OMP_NUM_THREADS")) nt = std::max(1, std::min(64, std::atoi(env)));
+    std::vector<double> hmins(nt, 1e300);
+    auto work = [&](unsigned tid) {
     double hmin = 1e300;
-#pragma omp parallel for reduction(min : hmin) schedule(static)
-    for (int64_t e = 0; e < Ne; ++e) {
+    for (int64_t e = tid; e < Ne; e += nt) {
         const int ec[3] = {int(e % M[0]), int((e / M[0]) % M[1]), int(e / (int64_t(M[0]) * M[1]))};
         double xl[16][16][16][3];
         for (int k = 0; k < N; ++k)
@@ -177,7 +183,13 @@ static void compute_coefs(const wt2_mesh_desc& d, const double* xi_n, const doub
                     }
                 }
     }
-    *h_min = hmin;
+    hmins[tid] = hmin;
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto& t : th) t.join();
+    *h_min = *std::min_element(hmins.begin(), hmins.end());
 }
 
 int curved_init(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const double* w,

@@ -31,7 +31,7 @@ EXPORTS = (
     "wt2_gll", "wt2_ctx_create", "wt2_ctx_destroy", "wt2_ctx_info", "wt2_ctx_set_background",
     "wt2_coords", "wt2_rhs3d", "wt2_rhs_full", "wt2_rk4_steps", "wt2_energy", "wt2_set_stream",
     "wt2_batch_create", "wt2_batch_destroy", "wt2_batch_add", "wt2_batch_remove", "wt2_batch_step",
-    "wt2_batch_slot", "wt2_launch_pass", "wt2_last_error", "wt2_version",
+    "wt2_batch_slot", "wt2_ctx_autotune", "wt2_ctx_tiles", "wt2_launch_pass", "wt2_last_error", "wt2_version",
 )
 
 
@@ -107,6 +107,8 @@ def lib():
         L.wt2_batch_step.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
         L.wt2_batch_slot.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), dp, ctypes.c_int32,
                                      ctypes.POINTER(ctypes.c_int32)]
+        L.wt2_ctx_autotune.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_int32]
+        L.wt2_ctx_tiles.argtypes = [vp, dp, dp]
         L.wt2_last_error.restype = ctypes.c_char_p
         L.wt2_version.restype = ctypes.c_char_p
         _lib = L

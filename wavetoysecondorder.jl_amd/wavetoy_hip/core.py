@@ -254,6 +254,22 @@ class Workspace:
             raise ValueError("launch_pass needs device tensors")
         check(lib().wt2_launch_pass(self.ctx, int(axis), int(stage), pu, pud, float(dt)))
 
+    def autotune(self, u, ud, dt, force=False):
+        """Pick the fastest pass kernel / tile size per axis on this state (device
+        tensors); cached per shape for later workspaces."""
+        (pu, pud), fl = self._ptrs(u, ud)
+        if fl:
+            raise ValueError("autotune needs device tensors")
+        self._sync_torch()
+        check(lib().wt2_ctx_autotune(self.ctx, pu, pud, float(dt), 1 if force else 0))
+        return self.tiles()
+
+    def tiles(self):
+        nb = (ctypes.c_int32 * 3)()
+        d = (ctypes.c_int32 * 3)()
+        check(lib().wt2_ctx_tiles(self.ctx, ctypes.cast(nb, ctypes.c_void_p), ctypes.cast(d, ctypes.c_void_p)))
+        return {"planes_per_tile": tuple(nb), "lds_free": tuple(bool(x) for x in d)}
+
     def energy(self, u, ud):
         (pu, pud), fl = self._ptrs(u, ud)
         E = ctypes.c_double()
