@@ -88,6 +88,13 @@ int wt2_gll(int32_t N, double* xi, double* w, double* G);
  * (src/wave3d_curved.jl:158-178).  Fails with WT2_ERR_TAU when the 1D SIPG
  * operator is indefinite at desc->tau unless WT2_ALLOW_NONCOERCIVE is set. */
 int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out);
+/* Device bytes that wt2_ctx_create (and, with with_background != 0 on a flat
+ * mesh, wt2_ctx_set_background) will allocate for desc, without allocating:
+ * the figure wt2_info.device_bytes reports afterwards.  Caller-owned state and
+ * the staging buffers of host-pointer calls are not included.  No counterpart
+ * in the reference (it allocates on the host); used for admission by memory
+ * footprint. */
+int wt2_ctx_footprint(const wt2_mesh_desc* desc, int32_t with_background, int64_t* bytes);
 void wt2_ctx_destroy(wt2_ctx* ctx);
 int wt2_ctx_info(const wt2_ctx* ctx, wt2_info* info);
 
@@ -155,6 +162,16 @@ int wt2_batch_remove(wt2_batch* b, int32_t slot);
 /* Advance every unfinished instance by up to max_steps steps (each stops at its
  * own nsteps); *finished (nullable) counts the instances that finished. */
 int wt2_batch_step(wt2_batch* b, int32_t max_steps, int32_t* finished);
+/* As wt2_batch_step, but returns once the steps are queued on the batch's
+ * stream (it still waits where the set of unfinished instances changes and
+ * the work lists are rebuilt).  Step counts advance at queue time; the state
+ * of a finished instance is final after wt2_batch_sync, wt2_batch_remove or a
+ * wt2_batch_slot call that copies energies. */
+int wt2_batch_step_async(wt2_batch* b, int32_t max_steps, int32_t* finished);
+int wt2_batch_sync(wt2_batch* b);
+/* Progress of one slot: steps done, and the energy samples recorded so far
+ * (kept on the device while stepping; copying them waits for the batch's
+ * stream).  Returns WT2_ERR_NONFINITE when a copied sample is not finite. */
 int wt2_batch_slot(const wt2_batch* b, int32_t slot, int64_t* steps_done, double* energies,
                    int32_t cap, int32_t* n_energies);
 

@@ -543,6 +543,38 @@ int wt2_gll(int32_t N, double* xi, double* w, double* G)
     return WT2_OK;
 }
 
+// Curved contexts compute their Minkowski coefficients on first use.
+static int ensure_coefs(const wt2_ctx* cc)
+{
+    wt2_ctx* c = const_cast<wt2_ctx*>(cc);
+    if (!c->curved || !c->cd.pending) return WT2_OK;
+    return curved_set_background(c->cd, c->d, c->xi, c->w, c->G, nullptr, c->stream);
+}
+
+int wt2_ctx_footprint(const wt2_mesh_desc* desc, int32_t with_background, int64_t* bytes)
+{
+    if (!desc || !bytes) { set_error("wt2_ctx_footprint: null pointer"); return WT2_ERR_ARG; }
+    if (desc->N < NMIN || desc->N > NMAX || desc->M[0] < 1 || desc->M[1] < 1 || desc->M[2] < 1 ||
+        (desc->precision != WT2_FP64 && desc->precision != WT2_MIXED)) {
+        set_error("wt2_ctx_footprint: bad N, M or precision");
+        return WT2_ERR_SHAPE;
+    }
+    const int N = desc->N;
+    const int64_t Ne = int64_t(desc->M[0]) * desc->M[1] * desc->M[2];
+    const int64_t n = Ne * N * N * N;
+    const int64_t es = desc->precision == WT2_MIXED ? int64_t(sizeof(float)) : int64_t(sizeof(double));
+    const int64_t ts = op_table_size(N);
+    // the same allocations as wt2_ctx_create: S, P, A register blocks, Lu,
+    // reduction partials, operator tables; plus the curved coefficients
+    int64_t b = 3 * (2 * n * es) + n * int64_t(sizeof(double)) + (RED_BLOCKS + 8) * int64_t(sizeof(double)) +
+                ts * int64_t(sizeof(double) + sizeof(float));
+    if (desc->warp_A != 0.0 || with_background)
+        b += 13 * n * int64_t(sizeof(double) + sizeof(float)) + n * int64_t(sizeof(double)) +
+             Ne * 6 * N * N * int64_t(sizeof(double));
+    *bytes = b;
+    return WT2_OK;
+}
+
 int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
 {
     if (!desc || !out) { set_error("wt2_ctx_create: null pointer"); return WT2_ERR_ARG; }
@@ -713,6 +745,10 @@ void wt2_ctx_destroy(wt2_ctx* c)
 int wt2_ctx_info(const wt2_ctx* c, wt2_info* info)
 {
     if (!c || !info) { set_error("wt2_ctx_info: null pointer"); return WT2_ERR_ARG; }
+    {
+        const int crc = ensure_coefs(c);
+        if (crc) return crc;
+    }
     info->npoints = c->npts;
     info->Ne = c->Ne;
     info->curved = c->curved;
@@ -757,6 +793,10 @@ int wt2_rhs3d(wt2_ctx* c, double* udd, const double* u, const double* ud,
               const double bdry_values[6], int32_t flags)
 {
     if (!c || !udd || !u || !bdry_values) { set_error("wt2_rhs3d: null pointer"); return WT2_ERR_ARG; }
+    {
+        const int crc = ensure_coefs(c);
+        if (crc) return crc;
+    }
     const bool host = flags & WT2_HOST_PTRS;
     const double* src[2] = {u, ud};
     double* din[2];
@@ -774,6 +814,10 @@ int wt2_rhs_full(wt2_ctx* c, double* du, double* dud, const double* u, const dou
                  int32_t flags)
 {
     if (!c || !du || !dud || !u || !ud) { set_error("wt2_rhs_full: null pointer"); return WT2_ERR_ARG; }
+    {
+        const int crc = ensure_coefs(c);
+        if (crc) return crc;
+    }
     const bool host = flags & WT2_HOST_PTRS;
     const double* src[2] = {u, ud};
     double* din[2];
@@ -800,6 +844,10 @@ int wt2_rk4_steps(wt2_ctx* c, double* u, double* ud, double dt, int32_t nsteps,
                   int32_t energy_every, double* energy_out, int32_t flags)
 {
     if (!c || !u || !ud) { set_error("wt2_rk4_steps: null pointer"); return WT2_ERR_ARG; }
+    {
+        const int crc = ensure_coefs(c);
+        if (crc) return crc;
+    }
     if (nsteps < 0 || !(dt > 0) || !std::isfinite(dt)) { set_error("wt2_rk4_steps: bad dt / nsteps"); return WT2_ERR_ARG; }
     const bool host = flags & WT2_HOST_PTRS;
     const bool want_e = energy_out && energy_every > 0;
@@ -874,6 +922,10 @@ int wt2_launch_pass(wt2_ctx* c, int32_t axis, int32_t stage, double* u, double* 
 {
     if (!c || !u || !ud) { set_error("wt2_launch_pass: null pointer"); return WT2_ERR_ARG; }
     if (axis < -1 || axis > 2 || stage < 1 || stage > 4) { set_error("wt2_launch_pass: bad axis / stage"); return WT2_ERR_ARG; }
+    {
+        const int crc = ensure_coefs(c);
+        if (crc) return crc;
+    }
     if (axis == -1 && !c->curved) { set_error("wt2_launch_pass: axis -1 needs a curved context"); return WT2_ERR_ARG; }
     return run_stage(c, stage, nullptr, nullptr, nullptr, nullptr, u, ud, dt, axis);
 }
@@ -892,6 +944,10 @@ int wt2_ctx_autotune(wt2_ctx* c, double* u, double* ud, double dt, int32_t flags
 {
     if (!c || !u || !ud || !(dt > 0)) { set_error("wt2_ctx_autotune: bad arguments"); return WT2_ERR_ARG; }
     if (misaligned(u) || misaligned(ud)) { set_error("wt2_ctx_autotune: device pointers must be 16-byte aligned"); return WT2_ERR_ARG; }
+    {
+        const int crc = ensure_coefs(c);
+        if (crc) return crc;
+    }
     const TuneKey key = tune_key(c);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -957,6 +1013,10 @@ int wt2_ctx_tiles(const wt2_ctx* c, int32_t* nb3, int32_t* direct3)
 int wt2_energy(wt2_ctx* c, const double* u, const double* ud, double* E, int32_t flags)
 {
     if (!c || !u || !ud || !E) { set_error("wt2_energy: null pointer"); return WT2_ERR_ARG; }
+    {
+        const int crc = ensure_coefs(c);
+        if (crc) return crc;
+    }
     const bool host = flags & WT2_HOST_PTRS;
     const double* src[2] = {u, ud};
     double* din[2];
@@ -988,7 +1048,8 @@ struct BatchSlot {
     double dt = 0.0;
     int64_t nsteps = 0, done = 0;
     int32_t every = 0;
-    std::vector<double> energy;
+    double* Edev = nullptr;      // energy samples, written on the device in step order
+    int32_t ecap = 0, ne = 0;
 };
 
 struct StagePlans {
@@ -1010,6 +1071,8 @@ struct wt2_batch {
 
 static void batch_free_plans(wt2_batch* b)
 {
+    // queued replays may still read the plans' work lists and the graph
+    (void)hipStreamSynchronize(b->stream);
     for (auto& sp : b->plans) {
         for (auto& p : sp.curved) curved_batch_free(p);
         sp.curved.clear();
@@ -1110,8 +1173,11 @@ void wt2_batch_destroy(wt2_batch* b)
 {
     if (!b) return;
     batch_free_plans(b);
-    for (auto& s : b->slots)
+    (void)hipStreamSynchronize(b->stream);
+    for (auto& s : b->slots) {
         if (s.ctx) s.ctx->stream = s.ctx->own_stream;
+        if (s.Edev) (void)hipFree(s.Edev);
+    }
     (void)hipFree(b->Ed);
     (void)hipStreamDestroy(b->stream);
     delete b;
@@ -1121,6 +1187,10 @@ int wt2_batch_add(wt2_batch* b, wt2_ctx* ctx, double* u, double* ud, double dt, 
                   int32_t energy_every, int32_t* slot)
 {
     if (!b || !ctx || !u || !ud || !slot) { set_error("wt2_batch_add: null pointer"); return WT2_ERR_ARG; }
+    {
+        const int crc = ensure_coefs(ctx);
+        if (crc) return crc;
+    }
     if (!(dt > 0) || nsteps < 0 || energy_every < 0) { set_error("wt2_batch_add: bad dt / nsteps / energy_every"); return WT2_ERR_ARG; }
     if (misaligned(u) || misaligned(ud)) { set_error("wt2_batch_add: state pointers must be 16-byte aligned"); return WT2_ERR_ARG; }
     for (auto& s : b->slots)
@@ -1131,12 +1201,17 @@ int wt2_batch_add(wt2_batch* b, wt2_ctx* ctx, double* u, double* ud, double dt, 
     BatchSlot s;
     s.ctx = ctx; s.u = u; s.v = ud; s.dt = dt; s.nsteps = nsteps; s.done = 0; s.every = energy_every;
     if (energy_every > 0) {
-        double E;
-        int rc = run_energy(ctx, u, ud, b->Ed);
-        if (!rc) rc = hipMemcpyAsync(&E, b->Ed, sizeof(double), hipMemcpyDeviceToHost, b->stream) == hipSuccess ? 0 : WT2_ERR_HIP;
-        if (!rc) rc = hipStreamSynchronize(b->stream) == hipSuccess ? 0 : WT2_ERR_HIP;
-        if (rc) { ctx->stream = ctx->own_stream; return rc; }
-        s.energy.push_back(E);
+        const int64_t cap = nsteps / energy_every + 2;
+        if (cap > INT32_MAX / 2) { ctx->stream = ctx->own_stream; set_error("wt2_batch_add: too many energy samples"); return WT2_ERR_ARG; }
+        s.ecap = int32_t(cap);
+        if (hipMalloc(&s.Edev, sizeof(double) * size_t(cap)) != hipSuccess) {
+            ctx->stream = ctx->own_stream;
+            set_error("wt2_batch_add: energy buffer allocation failed");
+            return WT2_ERR_OOM;
+        }
+        int rc = run_energy(ctx, u, ud, s.Edev);
+        if (rc) { (void)hipFree(s.Edev); ctx->stream = ctx->own_stream; return rc; }
+        s.ne = 1;
     }
     int id = -1;
     for (size_t q = 0; q < b->slots.size(); ++q)
@@ -1158,11 +1233,27 @@ int wt2_batch_remove(wt2_batch* b, int32_t slot)
     BatchSlot& s = b->slots[slot];
     if (batch_active(s)) b->dirty = true;
     s.ctx->stream = s.ctx->own_stream;
+    if (s.Edev) (void)hipFree(s.Edev);
     s = BatchSlot();
     return WT2_OK;
 }
 
 int wt2_batch_step(wt2_batch* b, int32_t max_steps, int32_t* finished)
+{
+    const int rc = wt2_batch_step_async(b, max_steps, finished);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return WT2_OK;
+}
+
+int wt2_batch_sync(wt2_batch* b)
+{
+    if (!b) { set_error("wt2_batch_sync: null pointer"); return WT2_ERR_ARG; }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return WT2_OK;
+}
+
+int wt2_batch_step_async(wt2_batch* b, int32_t max_steps, int32_t* finished)
 {
     if (!b || max_steps < 0) { set_error("wt2_batch_step: bad arguments"); return WT2_ERR_ARG; }
     int32_t nfin = 0;
@@ -1203,23 +1294,18 @@ int wt2_batch_step(wt2_batch* b, int32_t max_steps, int32_t* finished)
         for (auto& s : b->slots) {
             if (!batch_active(s)) continue;
             s.done += k;
-            if (s.every > 0 && s.done % s.every == 0) {
-                double E;
-                int rc = run_energy(s.ctx, s.u, s.v, b->Ed);
+            if (s.every > 0 && s.done % s.every == 0 && s.ne < s.ecap) {
+                // sampled on the stream in step order; read back by wt2_batch_slot
+                int rc = run_energy(s.ctx, s.u, s.v, s.Edev + s.ne);
                 if (rc) return rc;
-                HIPCHK(hipMemcpyAsync(&E, b->Ed, sizeof(double), hipMemcpyDeviceToHost, b->stream));
-                HIPCHK(hipStreamSynchronize(b->stream));
-                if (!std::isfinite(E)) { set_error("wt2_batch_step: non-finite energy in a batch slot"); return WT2_ERR_NONFINITE; }
-                s.energy.push_back(E);
+                ++s.ne;
             }
             if (s.done >= s.nsteps) {
                 ++nfin;
-                b->dirty = true;
-                if (b->gvalid) { (void)hipGraphExecDestroy(b->gexec); b->gvalid = false; }
+                b->dirty = true;     // the rebuild waits for the queued steps, then drops the graph
             }
         }
     }
-    HIPCHK(hipStreamSynchronize(b->stream));
     if (finished) *finished = nfin;
     return WT2_OK;
 }
@@ -1233,10 +1319,18 @@ int wt2_batch_slot(const wt2_batch* b, int32_t slot, int64_t* steps_done, double
     }
     const BatchSlot& s = b->slots[slot];
     if (steps_done) *steps_done = s.done;
-    const int32_t n = int32_t(s.energy.size());
+    const int32_t n = s.ne;
     if (n_energies) *n_energies = n;
-    if (energies)
-        for (int32_t q = 0; q < std::min(n, cap); ++q) energies[q] = s.energy[q];
+    if (energies && cap > 0 && n > 0) {
+        const int32_t m = std::min(n, cap);
+        HIPCHK(hipMemcpyAsync(energies, s.Edev, sizeof(double) * size_t(m), hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        for (int32_t q = 0; q < m; ++q)
+            if (!std::isfinite(energies[q])) {
+                set_error("wt2_batch_slot: non-finite energy sample (the evolution blew up)");
+                return WT2_ERR_NONFINITE;
+            }
+    }
     return WT2_OK;
 }
 

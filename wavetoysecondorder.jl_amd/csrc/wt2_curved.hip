@@ -214,6 +214,12 @@ int curved_init(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const 
         return WT2_ERR_OOM;
     }
     *bytes += int64_t(13 * n * (sizeof(double) + sizeof(float)) + n * sizeof(double) + fb * sizeof(double));
+    if (!bg10) {
+        // Minkowski coefficients are computed on first use, so that a
+        // background set right after creation is computed only once
+        cd.pending = true;
+        return WT2_OK;
+    }
     std::vector<double> coef, wv;
     compute_coefs(d, xi, w, bg10, coef, wv, &cd.h_min);
     return upload(cd, coef, wv, s);
@@ -224,7 +230,9 @@ int curved_set_background(CurvedData& cd, const wt2_mesh_desc& d, const double* 
 {
     std::vector<double> coef, wv;
     compute_coefs(d, xi, w, bg10, coef, wv, &cd.h_min);
-    return upload(cd, coef, wv, s);
+    const int rc = upload(cd, coef, wv, s);
+    if (rc == WT2_OK) cd.pending = false;
+    return rc;
 }
 
 void curved_free(CurvedData& cd)

@@ -26,6 +26,7 @@ struct CurvedData {
     double* wu = nullptr;       // energy weight of u*Lu: H3 (per node)
     double* wv = nullptr;       // energy weight of v^2: H3 / wgt
     double* fbuf = nullptr;     // face flux buffer: 6 N^2 per element
+    bool pending = false;       // allocated; coefficients not computed yet
 };
 
 struct CurvedCall {

@@ -4,7 +4,7 @@ libwt2.so (include/wt2.h) and a host mirror of the reference's call names."""
 from ._lib import EXPORTS, LIB_PATH, WT2Error, lib  # noqa: F401
 from .core import (  # noqa: F401
     AnalyticBackground3D, Batch, Element, Geometry, Mesh, Params3d, SBPOps, Workspace,
-    adm_bump_background, discrete_energy, eigenmode_cartesian, gaussian_pulse, host_coords,
+    adm_bump_background, ctx_footprint, mesh_desc, discrete_energy, eigenmode_cartesian, gaussian_pulse, host_coords,
     make_coef3d, make_element, make_geometry, make_operators, make_uniform_hex,
     make_warped_uniform_hex, make_workspace, recommended_dt_rk4, rhs_wave3d,
     sample_background3d, tau_effective, to_device, wave3d_rhs_full,

@@ -115,26 +115,41 @@ def make_warped_uniform_hex(Mx, My, Mz, x0, x1, A, periodic=False):
 
 # -------------------------------------------------------- device workspace
 
+def mesh_desc(mesh, elem, tau=None, eps_ko=0.0, bdry_values=(0.0,) * 6, precision="fp64",
+              allow_noncoercive=False):
+    d = _lib.MeshDesc()
+    d.N = elem.N
+    for a in range(3):
+        d.M[a] = mesh.M[a]
+        d.x0[a] = mesh.x0[a]
+        d.x1[a] = mesh.x1[a]
+        d.periodic[a] = int(mesh.periodic[a])
+    d.warp_A = mesh.warp_A
+    d.tau = tau_effective(None, elem.N) if tau is None else float(tau)
+    d.eps_ko = float(eps_ko)
+    for f in range(6):
+        d.bdry_values[f] = float(bdry_values[f])
+    d.precision = {"fp64": WT2_FP64, "mixed": WT2_MIXED}[precision]
+    d.flags = WT2_ALLOW_NONCOERCIVE if allow_noncoercive else 0
+    return d
+
+
+def ctx_footprint(mesh, elem, precision="fp64", with_background=False):
+    """Device bytes a Workspace on this mesh allocates (wt2_ctx_footprint),
+    computed without touching the device."""
+    d = mesh_desc(mesh, elem, precision=precision)
+    b = ctypes.c_int64()
+    check(lib().wt2_ctx_footprint(ctypes.byref(d), int(bool(with_background)), ctypes.byref(b)))
+    return b.value
+
+
 class Workspace:
     """A libwt2 context: mesh + operators + geometry + device workspace + stream
     (the reference's geom/ops/work bundle after to_device)."""
 
     def __init__(self, mesh, elem, tau=None, eps_ko=0.0, bdry_values=(0.0,) * 6,
                  precision="fp64", allow_noncoercive=False, bg=None):
-        d = _lib.MeshDesc()
-        d.N = elem.N
-        for a in range(3):
-            d.M[a] = mesh.M[a]
-            d.x0[a] = mesh.x0[a]
-            d.x1[a] = mesh.x1[a]
-            d.periodic[a] = int(mesh.periodic[a])
-        d.warp_A = mesh.warp_A
-        d.tau = tau_effective(None, elem.N) if tau is None else float(tau)
-        d.eps_ko = float(eps_ko)
-        for f in range(6):
-            d.bdry_values[f] = float(bdry_values[f])
-        d.precision = {"fp64": WT2_FP64, "mixed": WT2_MIXED}[precision]
-        d.flags = WT2_ALLOW_NONCOERCIVE if allow_noncoercive else 0
+        d = mesh_desc(mesh, elem, tau, eps_ko, bdry_values, precision, allow_noncoercive)
         self.desc = d
         self.mesh = mesh
         self.elem = elem
@@ -494,6 +509,22 @@ class Batch:
         fin = ctypes.c_int32()
         check(lib().wt2_batch_step(self.ptr, int(max_steps), ctypes.byref(fin)))
         return fin.value
+
+    def step_async(self, max_steps):
+        """Queue up to max_steps steps of every unfinished instance; returns the
+        number of instances whose last step is now queued."""
+        fin = ctypes.c_int32()
+        check(lib().wt2_batch_step_async(self.ptr, int(max_steps), ctypes.byref(fin)))
+        return fin.value
+
+    def sync(self):
+        check(lib().wt2_batch_sync(self.ptr))
+
+    def steps_done(self, slot):
+        """Steps queued so far for a slot (no device synchronisation)."""
+        done = ctypes.c_int64()
+        check(lib().wt2_batch_slot(self.ptr, int(slot), ctypes.byref(done), None, 0, None))
+        return done.value
 
     def slot_info(self, slot):
         done = ctypes.c_int64()
