@@ -270,6 +270,51 @@ def batch_run(wh, torch, args):
     return out
 
 
+def serving_run(wh, torch, args, rates=None):
+    """The serving workload: requests drawn like the batched workload with
+    T ~ U[0.5,2], Poisson arrivals at each rate, served by the continuous-
+    batching server (mixed precision, 200 GB budget); beside it the same
+    requests one at a time through the plain FP64 single-instance path, whose
+    latencies follow from a FIFO queue on the same arrival times."""
+    from wavetoy_hip import serving as sv
+    rates = rates or [float(r) for r in args.rates.split(",")]
+    reqs = sv.draw_requests(args.requests, None, args.seed)
+    work = sum(r.npoints for r in reqs)
+    srv = sv.Server(budget_bytes=200e9, precision="mixed", energy_every=20)
+    t0 = time.perf_counter()
+    srv.prewarm(reqs, verbose=True)
+    prewarm_s = time.perf_counter() - t0
+    print(f"serving: {len(reqs)} requests, {work} points, tuned in {prewarm_s:.1f} s", file=sys.stderr, flush=True)
+    out = {"config": {"workload": f"{len(reqs)} requests drawn like the batched workload, T~U[0.5,2], "
+                                  "Poisson arrivals; continuous batching, mixed precision, 200 GB budget",
+                      "seed": args.seed, "requests": len(reqs), "tile_tuning_s_before_start": prewarm_s}}
+    fp64_resp, service, work64 = sv.run_one_at_a_time(reqs, "fp64", energy_every=20)
+    out["one_at_a_time_fp64"] = {"point_updates": work64, "busy_s": float(sum(service)),
+                                 "throughput_point_updates_per_s": work64 / sum(service)}
+    print(f"serving: one-at-a-time FP64 {sum(service):.1f} s", file=sys.stderr, flush=True)
+    by_rid = {r.rid: r for r in fp64_resp}
+    out["rates"] = {}
+    for lam in rates:
+        rq = sv.with_arrivals(reqs, lam, args.seed + int(lam))
+        st = srv.run(rq)
+        summ = st.summary()
+        lat_seq = sv.fifo_latencies([r.arrival for r in rq], service)
+        # mixed vs FP64 result of each request: relative H-free L2 error of the final u
+        errs = []
+        for resp in st.responses:
+            ref = by_rid[resp.rid].u
+            errs.append(float(np.linalg.norm(resp.u - ref) / max(np.linalg.norm(ref), 1e-300)))
+        summ["one_at_a_time_fp64_p50_s"] = float(np.percentile(lat_seq, 50))
+        summ["one_at_a_time_fp64_p99_s"] = float(np.percentile(lat_seq, 99))
+        summ["throughput_vs_one_at_a_time_fp64"] = summ["throughput_point_updates_per_s"] / \
+            out["one_at_a_time_fp64"]["throughput_point_updates_per_s"]
+        summ["mixed_vs_fp64_rel_l2_err_max"] = max(errs)
+        out["rates"][str(int(lam))] = summ
+        print(f"serving: rate {lam}: p50 {summ['p50_latency_s']:.3f} s p99 {summ['p99_latency_s']:.3f} s "
+              f"{summ['throughput_point_updates_per_s']:.3e} pt-upd/s", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,10 +323,12 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--config", choices=["plain", "curved", "batch"], default="plain",
+    ap.add_argument("--config", choices=["plain", "curved", "batch", "serving"], default="plain",
                     help="plain: the FP64 flat workload (headline); curved: the warped-mesh "
                          "ADM workload, FP64 vs mixed, error at --t-final")
     ap.add_argument("--instances", type=int, default=512, help="batch config: number of instances")
+    ap.add_argument("--requests", type=int, default=64, help="serving config: requests per rate")
+    ap.add_argument("--rates", default="50,100,200,300,500", help="serving config: arrival rates (1/s)")
     ap.add_argument("--t-final", type=float, default=None,
                     help="curved config: evolve to this time (default: --steps steps)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -309,6 +356,9 @@ def main():
         return
     if args.config == "batch":
         print(json.dumps(batch_run(wh, torch, args)), flush=True)
+        return
+    if args.config == "serving":
+        print(json.dumps(serving_run(wh, torch, args)), flush=True)
         return
 
     mesh = wh.make_uniform_hex(M_ELEMS, 0.0, 1.0)

@@ -216,3 +216,48 @@ def test_full_size_linearity_symmetry_energy():
     dt = 0.2 * ws.h_min
     E = ws.rk4_steps(u, v, dt, 6, energy_every=1)
     assert np.all(np.diff(E) <= 1e-13 * E[0])
+
+
+@pytest.mark.parametrize("N,M", [(10, 9), (10, 12), (8, 16)])
+def test_mixed_large_lds_tiles(N, M):
+    """Mixed-precision z/y passes whose LDS tiles exceed 64 KB (the dynamic-LDS
+    limit is raised per kernel): 4 RK4 steps against the FP64 path."""
+    import torch
+    _say(f"mixed large-LDS N={N} M={M}")
+    mesh = wh.make_uniform_hex(M, 0.0, 1.0)
+    elem = wh.make_element(N)
+    x = wh.host_coords(mesh, elem)
+    u0, v0 = wh.gaussian_pulse(x, (0.45, 0.5, 0.55), width=0.1)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ws = wh.Workspace(mesh, elem, tau=wh.tau_effective(20.0, N), eps_ko=0.1, precision=prec)
+        u = torch.from_numpy(u0).cuda(); v = torch.from_numpy(v0).cuda()
+        ws.rk4_steps(u, v, 0.2 * ws.h_min, 4)
+        res[prec] = u.cpu().numpy()
+        ws.close()
+    H = wh.Geometry(mesh, elem).Hphys
+    d = res["mixed"] - res["fp64"]
+    assert math.sqrt(np.sum(H * d * d) / np.sum(H * res["fp64"] ** 2)) <= 1e-5
+
+
+def test_autotuned_tiles_give_the_same_evolution():
+    import torch
+    _say("autotune")
+    N, M = 6, (5, 3, 4)
+    ws, o = _pair(N, M, eps_ko=0.1)
+    x = o.coords()
+    u0 = np.exp(-np.sum((x - 0.5) ** 2, axis=1) / 0.02)
+    v0 = np.zeros(o.n)
+    dt = 0.2 * o.hmin
+    u = torch.from_numpy(u0).cuda(); v = torch.from_numpy(v0).cuda()
+    ws.rk4_steps(u, v, dt, 5)
+    u_def = u.cpu().numpy()
+    u = torch.from_numpy(u0).cuda(); v = torch.from_numpy(v0).cuda()
+    ws.autotune(u, v, dt, force=True)
+    t = ws.tiles()
+    assert len(t["planes_per_tile"]) == 3 and all(N % nb == 0 for nb in t["planes_per_tile"])
+    assert t["lds_free"][0]                  # the x pass is always LDS-free
+    ws.rk4_steps(u, v, dt, 5)
+    uo, _ = o.rk4(u0, v0, dt, 5)
+    assert relerr(u.cpu().numpy(), uo) <= TOL64
+    assert relerr(u_def, uo) <= TOL64

@@ -25,14 +25,15 @@ def test_footprint_equals_allocated_bytes(prec, curved):
 
 
 def _requests():
-    # small, uneven requests; arrivals 5 ms apart so later ones join a running batch
-    shapes = [(4, 2, False, 0.30), (6, 3, True, 0.15), (4, 3, False, 0.20), (8, 2, True, 0.10),
-              (6, 2, False, 0.25), (4, 4, True, 0.10), (10, 2, False, 0.05), (6, 3, False, 0.12)]
+    # small, uneven requests; half arrive together (and queue behind the
+    # budget), the rest arrive 2 ms apart and join a running batch
+    shapes = [(4, 2, False, 0.60), (6, 3, True, 0.30), (4, 3, False, 0.40), (8, 2, True, 0.20),
+              (6, 2, False, 0.50), (4, 4, True, 0.20), (10, 2, False, 0.10), (6, 3, False, 0.24)]
     rng = np.random.default_rng(11)
     out = []
     for q, (N, M, curved, T) in enumerate(shapes):
         out.append(sv.Request(rid=q, N=N, M=M, curved=curved, centre=tuple(rng.uniform(0.3, 0.7, 3)),
-                              width=float(rng.uniform(0.05, 0.2)), T=T, arrival=0.005 * q))
+                              width=float(rng.uniform(0.05, 0.2)), T=T, arrival=0.0 if q < 4 else 0.002 * q))
     return out
 
 
@@ -57,7 +58,9 @@ def test_served_results_equal_alone(prec):
         assert np.array_equal(resp.u, ua), f"request {r.rid}: u differs from the alone run"
         assert np.array_equal(resp.energy, E), f"request {r.rid}: energy history differs"
         assert resp.latency >= 0 and resp.finished >= resp.admitted >= r.arrival - 1e-9
-        assert np.all(np.isfinite(resp.energy)) and resp.energy[-1] <= resp.energy[0] * (1 + 1e-9)
+        assert np.all(np.isfinite(resp.energy))
+        if not r.curved:    # flat SIPG + KO + Dirichlet: the energy does not grow
+            assert resp.energy[-1] <= resp.energy[0] * (1 + 1e-9)
 
 
 def test_server_rejects_oversized_request():

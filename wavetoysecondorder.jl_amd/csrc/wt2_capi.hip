@@ -420,8 +420,14 @@ static int run_stage(wt2_ctx* c, int stage, const double* u, const double* v, do
     for (int q = 0; q < 3; ++q) {
         if (!it.has_pass[q]) continue;
         if (only != -2 && only != 2 - q) continue;
-        int rc = pass_rc(launch_flat_pass(it.pass[q], c->stream), "flat pass");
-        if (rc) return rc;
+        const PassDesc& pd = it.pass[q];
+        const hipError_t e = launch_flat_pass(pd, c->stream);
+        if (e != hipSuccess) {
+            char what[160];
+            std::snprintf(what, sizeof(what), "flat pass (N=%d M=%dx%dx%d axis=%d stage=%d nb=%d lds=%d variant=%d)",
+                          pd.N, pd.M[0], pd.M[1], pd.M[2], pd.axis, stage, pd.nb, pd.direct ? 0 : 1, int(pd.variant));
+            return pass_rc(e, what);
+        }
     }
     return WT2_OK;
 }

@@ -613,6 +613,15 @@ void build_op_table(int N, const double* G, const double* w, double* tab)
 
 static int pick_lpw(int Ma) { return 64 / Ma; }
 
+// Upper bound of a pass kernel's static LDS (the operator table): the
+// compiler pads the table's size (280 B of FP32 at N = 10 occupy 288 B), and
+// the dynamic-LDS limit requested below must leave room for the padded size.
+template <int N, typename TC>
+static constexpr size_t static_lds_bound()
+{
+    return (sizeof(TC) * OpLayout<N>::SIZE + 255) / 256 * 256 + 256;
+}
+
 template <int N, typename TC, typename TS, typename TP>
 static hipError_t make_kargs(const PassDesc& p, KArgs<N, TC>& a, int* tiles_out, int* nthr_out,
                              size_t* shmem_out)
@@ -673,7 +682,7 @@ static hipError_t make_kargs(const PassDesc& p, KArgs<N, TC>& a, int* tiles_out,
     const int fstride = a.Ma * a.ES;
     const int nslots = (p.lap && p.needv) || (!p.lap && p.keepv) ? 2 : 1;
     const size_t shmem = size_t(fstride) * nslots * sizeof(TS);
-    const size_t static_lds = sizeof(TC) * OpLayout<N>::SIZE;
+    const size_t static_lds = static_lds_bound<N, TC>();
     if (shmem + static_lds > 160 * 1024) return hipErrorInvalidConfiguration;
     a.GS = p.axis == 0 ? 1 : (p.axis == 1 ? N : N * N);
     *tiles_out = tiles;
@@ -690,7 +699,7 @@ static hipError_t launch_t(const PassDesc& p, hipStream_t s)
     size_t shmem;
     hipError_t e0 = make_kargs<N, TC, TS, TP>(p, a, &tiles, &nthr, &shmem);
     if (e0 != hipSuccess) return e0;
-    const size_t static_lds = sizeof(TC) * OpLayout<N>::SIZE;
+    const size_t static_lds = static_lds_bound<N, TC>();
     if (p.axis == 0 || p.direct) {
         // LDS-free pass
         auto xk = flat_xpass_kernel<N, TC, TS, TP>;
@@ -798,7 +807,7 @@ static hipError_t batch_launch_t(const FlatBatchPlan& plan, hipStream_t s)
         return hipGetLastError();
     }
     auto kern = flat_pass_batch_kernel<N, TC, TS, TP>;
-    const size_t static_lds = sizeof(TC) * OpLayout<N>::SIZE;
+    const size_t static_lds = static_lds_bound<N, TC>();
     static size_t lds_limit = 64 * 1024 - static_lds;
     if (plan.shmem > lds_limit) {
         const size_t want = 160 * 1024 - static_lds;

@@ -11,6 +11,7 @@ chunk of steps for all live requests, admits further arrivals while that chunk
 runs on the device, then retires the requests whose last step has completed.
 """
 import math
+import sys
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -219,7 +220,7 @@ class Server:
             self._fp[key] = request_footprint(req, self.precision, self.energy_every)
         return self._fp[key]
 
-    def prewarm(self, requests):
+    def prewarm(self, requests, verbose=False):
         """Tune the pass tiles once per distinct (N, M, flat/warped) shape, as
         a server does when it starts, so that no request pays for it."""
         seen = set()
@@ -228,6 +229,8 @@ class Server:
             if key in seen:
                 continue
             seen.add(key)
+            if verbose:
+                print(f"tuning N={r.N} M={r.M} {'warped' if r.curved else 'flat'}", file=sys.stderr, flush=True)
             ws, u, v, dt, _ = build_instance(r, self.precision)
             ws.autotune(u, v, dt)
             ws.close()
