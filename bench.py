@@ -95,10 +95,20 @@ def run_gpu(wh, torch, prec, u0, v0, steps, warmup, stream):
         b.record(stream)
         torch.cuda.synchronize()
         pass_ms[name] = a.elapsed_time(b) / reps
+    # the drop-in right-hand side (wt2_rhs3d's kernel) alone, same way
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    ws.launch_pass(3, 1, u, v, dt)
+    a.record(stream)
+    for _ in range(reps):
+        ws.launch_pass(3, 1, u, v, dt)
+    b.record(stream)
+    torch.cuda.synchronize()
+    rhs_ms = a.elapsed_time(b) / reps
     ws.set_stream(None)
     E = ws.energy(u, v)
     return dict(ws=ws, u=u.cpu().numpy(), v=v.cpu().numpy(), dt=dt, wall=wall, ev_ms=ev_ms,
-                pass_ms=pass_ms, energy=E, npts=ws.npoints, h_min=ws.h_min, mu=ws.mu, tau=tau)
+                pass_ms=pass_ms, rhs_ms=rhs_ms, energy=E, npts=ws.npoints, h_min=ws.h_min, mu=ws.mu, tau=tau)
 
 
 def cpu_baseline(u0, v0, dt, budget_s=12.0):
@@ -369,12 +379,19 @@ def main():
     r64 = run_gpu(wh, torch, "fp64", u0, v0, args.steps, args.warmup, stream)
     npts, K = r64["npts"], args.steps
     value = npts * K / r64["wall"]
-    # dominant kernel: the flat pencil-pass kernel (every launch of a step);
-    # average launch duration over the timed region from the stream events
+    # dominant kernel: the flat pencil-pass kernel (all 12 launches of a step);
+    # average launch duration over the timed region from the stream events.
+    # Algorithmic bytes: the fused 3-register RK4 minimum, 256 B per point and
+    # step (u, u_t of Y, S, A: 16 field-pair accesses), i.e. 256/12 B per point
+    # per launch; the pass design itself moves step_bytes("fp64") = 512 B.
     nlaunch = 12 * K
     avg_launch_s = r64["ev_ms"] * 1e-3 / nlaunch
-    bytes_per_launch = step_bytes("fp64") * npts / 12
+    bytes_per_launch = FUSED_MIN_BYTES["fp64"] * npts / 12
     achieved = bytes_per_launch / avg_launch_s / 1e9
+    design_GBs = step_bytes("fp64") * npts / 12 / avg_launch_s / 1e9
+    # the drop-in RHS kernel: algorithmic 24 B per point (read u, u_t, write
+    # u_tt); the Dirichlet RHS reads no u_t, so 16 B are compulsory
+    rhs_s = r64["rhs_ms"] * 1e-3
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_flat_pass_fp64_32x8.json")
     if os.path.exists(tfile):
@@ -400,10 +417,15 @@ def main():
                    "seed": args.seed, "pulse_centre": [float(c) for c in x0]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel": "flat_pass_kernel<8,double,double,double> (all 12 pencil passes of a step)",
-                     "algorithmic_bytes_per_point_step": step_bytes("fp64"),
-                     "step_frac_vs_fused_minimum_256B": FUSED_MIN_BYTES["fp64"] * npts * K
-                     / (r64["ev_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS},
+                     "kernel": "flat_xpass_kernel<8,double,double,double> (all 12 pencil passes of a step)",
+                     "algorithmic_bytes_per_point_step": FUSED_MIN_BYTES["fp64"],
+                     "pass_design_bytes_per_point_step": step_bytes("fp64"),
+                     "pass_design_GBs": design_GBs},
+        "rhs_roofline": {"bound": "hbm", "kernel": "flat_lap1_kernel<8,double> (wt2_rhs3d)",
+                         "ms": r64["rhs_ms"], "achieved": 24 * npts / rhs_s / 1e9, "peak": PEAK_HBM_GBS,
+                         "unit": "GB/s", "frac": 24 * npts / rhs_s / 1e9 / PEAK_HBM_GBS,
+                         "algorithmic_bytes_per_point": 24, "compulsory_bytes_per_point": 16,
+                         "compulsory_GBs": 16 * npts / rhs_s / 1e9},
         "pass_ms_fp64": r64["pass_ms"],
         "energy_final": r64["energy"],
     }

@@ -188,9 +188,11 @@ int wt2_ctx_tiles(const wt2_ctx* ctx, int32_t* nb3, int32_t* direct3);
  * kernel in isolation with events on the context stream): enqueue, without
  * synchronising, the single pencil pass of RK4 stage `stage` (1..4) along
  * `axis` (0..2) on state (u, ud) (device pointers) -- or, for a curved
- * context and axis == -1, the curved element operator of that stage.  The
- * context registers must have been initialised by a wt2_rk4_steps call with
- * the same state pointers. */
+ * context and axis == -1, the curved element operator of that stage; with
+ * axis == 3, the drop-in right-hand side of wt2_rhs3d (the context's own
+ * bdry_values) into an internal buffer.  For axes -1..2 the context registers
+ * must have been initialised by a wt2_rk4_steps call with the same state
+ * pointers. */
 int wt2_launch_pass(wt2_ctx* ctx, int32_t axis, int32_t stage, double* u, double* ud, double dt);
 
 const char* wt2_last_error(void);

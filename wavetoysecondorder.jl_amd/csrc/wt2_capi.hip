@@ -331,20 +331,21 @@ static int run_lap(wt2_ctx* c, const double* u, const double* v, double* out, co
         for (int f = 0; f < 6; ++f) cc.bv[f] = bv[f];
         return pass_rc(curved_operator(c->cd, cc, c->stream), "curved operator");
     }
-    // passes z, y, x: the x pass (LDS-free) finishes the sum
-    for (int q = 0; q < 3; ++q) {
-        const int axis = 2 - q;
-        PassDesc p = base_pass(c, axis, bv);
-        p.variant = mixed ? VAR_MIX_IN64 : VAR_F64;
-        p.lap = 1;
-        p.Su = u;
-        p.P = c->Pb;   // FP64 scratch in either precision (one block = one FP64 field)
-        p.epi = q == 0 ? EPI_FIRST : (q == 1 ? EPI_MID : EPI_OUT_LAP);
-        p.Ou = out;
-        int rc = pass_rc(launch_flat_pass(p, c->stream), "flat lap pass");
-        if (rc) return rc;
+    // one launch: every element with its face neighbours' lines
+    Lap1Desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.N = c->N;
+    d.variant = mixed ? VAR_MIX_IN64 : VAR_F64;
+    for (int a = 0; a < 3; ++a) {
+        d.M[a] = c->d.M[a];
+        d.periodic[a] = c->d.periodic[a];
+        fill_axis_ops(c, a, bv, d.ops[a]);
     }
-    return WT2_OK;
+    d.u = u;
+    d.out = out;
+    d.op64 = c->op64;
+    d.op32 = c->op32;
+    return pass_rc(launch_flat_lap1(d, c->stream), "flat Laplacian");
 }
 
 // The launches of one full RHS evaluation of u' = u_t,
@@ -927,12 +928,13 @@ int wt2_rk4_steps(wt2_ctx* c, double* u, double* ud, double dt, int32_t nsteps,
 int wt2_launch_pass(wt2_ctx* c, int32_t axis, int32_t stage, double* u, double* ud, double dt)
 {
     if (!c || !u || !ud) { set_error("wt2_launch_pass: null pointer"); return WT2_ERR_ARG; }
-    if (axis < -1 || axis > 2 || stage < 1 || stage > 4) { set_error("wt2_launch_pass: bad axis / stage"); return WT2_ERR_ARG; }
+    if (axis < -1 || axis > 3 || stage < 1 || stage > 4) { set_error("wt2_launch_pass: bad axis / stage"); return WT2_ERR_ARG; }
     {
         const int crc = ensure_coefs(c);
         if (crc) return crc;
     }
     if (axis == -1 && !c->curved) { set_error("wt2_launch_pass: axis -1 needs a curved context"); return WT2_ERR_ARG; }
+    if (axis == 3) return run_lap(c, u, ud, c->Lu, c->d.bdry_values, false, true);
     return run_stage(c, stage, nullptr, nullptr, nullptr, nullptr, u, ud, dt, axis);
 }
 

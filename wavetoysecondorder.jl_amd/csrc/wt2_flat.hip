@@ -147,20 +147,39 @@ __device__ __forceinline__ void ko_chain(TC (&x)[N], const TC* __restrict__ op,
     }
 }
 
-// out += 1D SIPG Laplacian of u along this axis (README.md:57-81):
-//   (1/h^2){ G G u + H^-1 [ alpha G^T B du - 1/2 B d(Gu) - tau |B| du ] }.
+// The two end values y_0, y_{N-1} of y = G x, with the same arithmetic as
+// apply_G (a neighbour's face derivative).
 template <int N, typename TC>
-__device__ __forceinline__ void sipg_add(const TC (&u)[N], TC (&out)[N], const TC* __restrict__ op,
-                                         const AxisConst<N, TC>& k, int srcL, int srcR,
-                                         bool hasL, bool hasR)
+__device__ __forceinline__ void apply_G_ends(const TC* __restrict__ op, const TC (&x)[N], TC& y0, TC& yN)
 {
     using L = OpLayout<N>;
-    TC g[N];
-    apply_G<N, TC>(op, u, g);
-    TC uL = shfl64(u[N - 1], srcL);
-    TC gnL = shfl64(g[N - 1], srcL);
-    TC uR = shfl64(u[0], srcR);
-    TC gnR = shfl64(g[0], srcR);
+    constexpr int H = L::H, E = L::E;
+    TC e[E], o[H];
+#pragma unroll
+    for (int p = 0; p < H; ++p) {
+        e[p] = TC(0.5) * (x[p] + x[N - 1 - p]);
+        o[p] = TC(0.5) * (x[p] - x[N - 1 - p]);
+    }
+    if (E > H) e[E - 1] = x[H];
+    TC b = TC(0), av = TC(0);
+#pragma unroll
+    for (int p = 0; p < H; ++p) b = fma(op[L::AO + p], o[p], b);
+#pragma unroll
+    for (int p = 0; p < E; ++p) av = fma(op[L::AE + p], e[p], av);
+    y0 = b + av;
+    yN = b - av;
+}
+
+// out += 1D SIPG Laplacian of u along this axis (README.md:57-81):
+//   (1/h^2){ G G u + H^-1 [ alpha G^T B du - 1/2 B d(Gu) - tau |B| du ] },
+// given g = G u and the neighbours' face values (uL, gnL: value and G u at the
+// last node of the low neighbour; uR, gnR: at the first node of the high one).
+template <int N, typename TC>
+__device__ __forceinline__ void sipg_seg(const TC (&u)[N], const TC (&g)[N], TC (&out)[N],
+                                         const TC* __restrict__ op, const AxisConst<N, TC>& k,
+                                         TC uL, TC gnL, TC uR, TC gnR, bool hasL, bool hasR)
+{
+    using L = OpLayout<N>;
     TC duL, dgL, aL, duR, dgR, aR;
     if (hasL) { duL = u[0] - uL; dgL = g[0] - gnL; aL = TC(0.5); }
     else      { duL = u[0] - k.gL; dgL = TC(0); aL = TC(1); }
@@ -177,6 +196,21 @@ __device__ __forceinline__ void sipg_add(const TC (&u)[N], TC (&out)[N], const T
         if (i == N - 1) s = fma(k.winvN, TC(-0.5) * dgR - k.tau * duR, s);
         out[i] = fma(s, k.ih2, out[i]);
     }
+}
+
+// The same with the neighbour values taken from the adjacent lanes of a line.
+template <int N, typename TC>
+__device__ __forceinline__ void sipg_add(const TC (&u)[N], TC (&out)[N], const TC* __restrict__ op,
+                                         const AxisConst<N, TC>& k, int srcL, int srcR,
+                                         bool hasL, bool hasR)
+{
+    TC g[N];
+    apply_G<N, TC>(op, u, g);
+    const TC uL = shfl64(u[N - 1], srcL);
+    const TC gnL = shfl64(g[N - 1], srcL);
+    const TC uR = shfl64(u[0], srcR);
+    const TC gnR = shfl64(g[0], srcR);
+    sipg_seg<N, TC>(u, g, out, op, k, uL, gnL, uR, gnR, hasL, hasR);
 }
 
 template <int N, typename TC, typename TS, typename TP>
@@ -747,6 +781,244 @@ hipError_t launch_flat_pass(const PassDesc& p, hipStream_t s)
     case 8: return launch_n<8>(p, s);
     case 9: return launch_n<9>(p, s);
     case 10: return launch_n<10>(p, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ------------------------------------------------- single-pass Laplacian
+
+// Lines of one element: N^2 per axis, one work-item each.  A workgroup holds
+// EW elements (about 256 work-items).  LDS per element: the element's u and
+// the running sum, both with x-lines padded to N+1 so that the N^2 lanes
+// reading x-lines fall on distinct banks.
+template <int N>
+struct Lap1Shape {
+    static constexpr int L = N * N;
+    static constexpr int EW = (256 / L) > 0 ? 256 / L : 1;
+    static constexpr int NT = EW * L;
+    static constexpr int RS = N + 1;              // padded x-line stride
+    static constexpr int ES = RS * N * N;         // per-element LDS stride
+};
+
+template <int N, typename TC>
+struct Lap1Args {
+    AxisConst<N, TC> k[3];
+    const TC* op;
+    const double* u;
+    double* out;
+    int M0, M1, M2;
+    int per0, per1, per2;
+    int Ne, nwg;
+    int Ty;            // rows per y-block of the processing order
+    int xcd;           // 1: contiguous ranges of the order per XCD
+};
+
+// Processing order of the elements: y-blocks of Ty rows; inside a block x
+// fastest, then y, then z.  A z-neighbour is then Mx*Ty elements away in the
+// order (not Mx*My), so it is still in L2 when it is read.
+template <int N, typename TC>
+__device__ __forceinline__ int lap1_elem(const Lap1Args<N, TC>& a, int t)
+{
+    const int blk = a.M0 * a.Ty * a.M2;
+    const int b = t / blk, r = t - b * blk;
+    const int ty = min(a.Ty, a.M1 - b * a.Ty);
+    const int z = r / (a.M0 * ty), rem = r - z * (a.M0 * ty);
+    const int y = b * a.Ty + rem / a.M0, x = rem % a.M0;
+    return x + a.M0 * (y + a.M1 * z);
+}
+
+// LDS index of node (i, j, k) of an element slot
+template <int N>
+__device__ __forceinline__ int l1idx(int i, int j, int k)
+{
+    return i + Lap1Shape<N>::RS * (j + N * k);
+}
+
+// N values of a line from global memory: node m at base + m * stride
+template <int N, int STRIDE, typename TC>
+__device__ __forceinline__ void lap1_load(const double* __restrict__ p, TC (&x)[N])
+{
+    if constexpr (STRIDE == 1 && N % 2 == 0) {
+#pragma unroll
+        for (int m = 0; m < N; m += 2) {
+            const double2 t = *reinterpret_cast<const double2*>(p + m);
+            x[m] = TC(t.x);
+            x[m + 1] = TC(t.y);
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < N; ++m) x[m] = TC(p[m * STRIDE]);
+    }
+}
+
+// One axis of the single-pass Laplacian for this work-item's line (la, lb)
+// of element e (coordinate c along the axis): c_AX from the element's own line
+// (LDS) and the two neighbours' lines (global memory), added to R.
+template <int N, typename TC, int AX>
+__device__ __forceinline__ void lap1_axis(const Lap1Args<N, TC>& a, const TC* __restrict__ op, const TC* Ue,
+                                          TC* Re, int e, int c, int la, int lb)
+{
+    constexpr int RS = Lap1Shape<N>::RS;
+    constexpr int N3 = N * N * N;
+    constexpr int LS = AX == 0 ? 1 : (AX == 1 ? RS : RS * N);      // LDS stride along the line
+    constexpr int GS = AX == 0 ? 1 : (AX == 1 ? N : N * N);        // memory stride along the line
+    const int lbase = AX == 0 ? RS * (la + N * lb) : (AX == 1 ? la + RS * N * lb : la + RS * lb);
+    const int gbase = AX == 0 ? N * (la + N * lb) : (AX == 1 ? la + N * N * lb : la + N * lb);
+    const int Ma = AX == 0 ? a.M0 : (AX == 1 ? a.M1 : a.M2);
+    const int estr = AX == 0 ? 1 : (AX == 1 ? a.M0 : a.M0 * a.M1);
+    const bool per = (AX == 0 ? a.per0 : (AX == 1 ? a.per1 : a.per2)) != 0;
+    const bool hasL = c > 0 || per, hasR = c < Ma - 1 || per;
+    TC xl[N], xr[N], u[N];
+    if (hasL) lap1_load<N, GS, TC>(a.u + int64_t(e + (c > 0 ? -1 : Ma - 1) * estr) * N3 + gbase, xl);
+    if (hasR) lap1_load<N, GS, TC>(a.u + int64_t(e + (c < Ma - 1 ? 1 : 1 - Ma) * estr) * N3 + gbase, xr);
+#pragma unroll
+    for (int m = 0; m < N; ++m) u[m] = Ue[lbase + m * LS];
+    TC uL = TC(0), gnL = TC(0), uR = TC(0), gnR = TC(0);
+    if (hasL) {
+        TC y0;
+        apply_G_ends<N, TC>(op, xl, y0, gnL);
+        uL = xl[N - 1];
+    }
+    if (hasR) {
+        TC yN;
+        apply_G_ends<N, TC>(op, xr, gnR, yN);
+        uR = xr[0];
+    }
+    TC g[N], cv[N];
+    apply_G<N, TC>(op, u, g);
+#pragma unroll
+    for (int m = 0; m < N; ++m) cv[m] = TC(0);
+    sipg_seg<N, TC>(u, g, cv, op, a.k[AX], uL, gnL, uR, gnR, hasL, hasR);
+    if (AX == 2) {
+#pragma unroll
+        for (int m = 0; m < N; ++m) Re[lbase + m * LS] = cv[m];
+    } else {
+#pragma unroll
+        for (int m = 0; m < N; ++m) Re[lbase + m * LS] = cv[m] + Re[lbase + m * LS];
+    }
+}
+
+template <int N, typename TC>
+__global__ void __launch_bounds__(256) flat_lap1_kernel(Lap1Args<N, TC> a)
+{
+    using S = Lap1Shape<N>;
+    constexpr int N3 = N * N * N;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    TC* U = reinterpret_cast<TC*>(smem_raw);     // EW x ES
+    TC* R = U + S::EW * S::ES;                   // EW x ES
+    __shared__ TC op[OpLayout<N>::SIZE];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < OpLayout<N>::SIZE; q += blockDim.x) op[q] = a.op[q];
+
+    // workgroups of one XCD (dispatch round-robin over 8) take a contiguous
+    // range of element blocks, so that face neighbours share that XCD's L2
+    int wg = blockIdx.x;
+    const int full = (a.nwg / 8) * 8;
+    if (a.xcd && wg < full) wg = (wg % 8) * (a.nwg / 8) + wg / 8;
+    const int t0 = wg * S::EW;
+    const int nel = min(S::EW, a.Ne - t0);
+    __shared__ int elist[S::EW];
+    if (tid < nel) elist[tid] = lap1_elem(a, t0 + tid);
+    __syncthreads();
+
+    // stage the elements' u into LDS (each element contiguous in memory)
+    for (int q = tid; q < nel * N3; q += blockDim.x) {
+        const int el = q / N3, n = q - el * N3;
+        const int i = n % N, j = (n / N) % N, k = n / (N * N);
+        U[el * S::ES + l1idx<N>(i, j, k)] = TC(a.u[int64_t(elist[el]) * N3 + n]);
+    }
+    __syncthreads();
+
+    const int slot = tid / S::L, line = tid - slot * S::L;
+    const bool act = slot < S::EW && slot < nel;
+    const int e = act ? elist[slot] : 0;
+    const int la = line % N, lb = line / N;
+    int ec[3] = {0, 0, 0};
+    if (act) { ec[0] = e % a.M0; ec[1] = (e / a.M0) % a.M1; ec[2] = e / (a.M0 * a.M1); }
+    TC* Ue = U + (act ? slot : 0) * S::ES;
+    TC* Re = R + (act ? slot : 0) * S::ES;
+
+    // axes z, y, x: the running sum R = c_z, R += c_y, R += c_x (the order of
+    // the pencil passes, so both paths round alike)
+    if (act) lap1_axis<N, TC, 2>(a, op, Ue, Re, e, ec[2], la, lb);
+    __syncthreads();
+    if (act) lap1_axis<N, TC, 1>(a, op, Ue, Re, e, ec[1], la, lb);
+    __syncthreads();
+    if (act) lap1_axis<N, TC, 0>(a, op, Ue, Re, e, ec[0], la, lb);
+    __syncthreads();
+    // coalesced write-out
+    for (int q = tid; q < nel * N3; q += blockDim.x) {
+        const int el = q / N3, n = q - el * N3;
+        const int i = n % N, j = (n / N) % N, k = n / (N * N);
+        a.out[int64_t(elist[el]) * N3 + n] = double(R[el * S::ES + l1idx<N>(i, j, k)]);
+    }
+}
+
+template <int N, typename TC>
+static hipError_t launch_lap1_t(const Lap1Desc& d, hipStream_t s)
+{
+    using S = Lap1Shape<N>;
+    Lap1Args<N, TC> a;
+    for (int ax = 0; ax < 3; ++ax) {
+        const AxisOps& o = d.ops[ax];
+        a.k[ax].kh = TC(o.kh); a.k[ax].koL = TC(o.koL); a.k[ax].koR = TC(o.koR);
+        a.k[ax].tau = TC(o.tau); a.k[ax].ih2 = TC(o.ih2);
+        a.k[ax].gL = TC(o.gL); a.k[ax].gR = TC(o.gR);
+        a.k[ax].winv0 = TC(o.winv0); a.k[ax].winvN = TC(o.winvN);
+    }
+    a.op = static_cast<const TC*>(std::is_same<TC, float>::value ? d.op32 : d.op64);
+    a.u = d.u; a.out = d.out;
+    a.M0 = d.M[0]; a.M1 = d.M[1]; a.M2 = d.M[2];
+    a.per0 = d.periodic[0]; a.per1 = d.periodic[1]; a.per2 = d.periodic[2];
+    const int64_t Ne = int64_t(d.M[0]) * d.M[1] * d.M[2];
+    if (Ne > INT32_MAX / 2) return hipErrorInvalidValue;
+    a.Ne = int(Ne);
+    a.nwg = int((Ne + S::EW - 1) / S::EW);
+    // rows per y-block: keep Mx*Ty elements of u and of the output (the reuse
+    // distance of a z-neighbour) within about 2 MB of an XCD's 4 MB L2
+    const int64_t per_row = int64_t(d.M[0]) * N * N * N * 2 * sizeof(double);
+    a.Ty = int(std::max<int64_t>(1, std::min<int64_t>(d.M[1], (int64_t(2) << 20) / per_row)));
+    a.xcd = 1;
+    // tuning overrides: WT2_LAP1_TY (rows per y-block, 0 = all), WT2_LAP1_XCD (0/1)
+    if (const char* ev = std::getenv("WT2_LAP1_TY")) {
+        const int t = std::atoi(ev);
+        a.Ty = t <= 0 ? d.M[1] : std::min(t, d.M[1]);
+    }
+    if (const char* ev = std::getenv("WT2_LAP1_XCD")) a.xcd = std::atoi(ev) != 0;
+    const size_t shmem = size_t(2) * S::EW * S::ES * sizeof(TC);
+    auto kern = flat_lap1_kernel<N, TC>;
+    static bool raised = false;
+    if (shmem + static_lds_bound<N, TC>() > 64 * 1024 && !raised) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           int(160 * 1024 - static_lds_bound<N, TC>()));
+        if (e != hipSuccess) return e;
+        raised = true;
+    }
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(kern, dim3(a.nwg), dim3(S::NT), shmem, s, a);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_lap1_n(const Lap1Desc& d, hipStream_t s)
+{
+    if (d.variant == VAR_F64) return launch_lap1_t<N, double>(d, s);
+    if (d.variant == VAR_MIX_IN64) return launch_lap1_t<N, float>(d, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_flat_lap1(const Lap1Desc& d, hipStream_t s)
+{
+    switch (d.N) {
+    case 3: return launch_lap1_n<3>(d, s);
+    case 4: return launch_lap1_n<4>(d, s);
+    case 5: return launch_lap1_n<5>(d, s);
+    case 6: return launch_lap1_n<6>(d, s);
+    case 7: return launch_lap1_n<7>(d, s);
+    case 8: return launch_lap1_n<8>(d, s);
+    case 9: return launch_lap1_n<9>(d, s);
+    case 10: return launch_lap1_n<10>(d, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -73,6 +73,19 @@ struct PassDesc {
 // Launch one pencil pass on stream s.  Returns hipError_t.
 hipError_t launch_flat_pass(const PassDesc& p, hipStream_t s);
 
+// The flat SIPG Laplacian out = L u in one launch (no KO term): a workgroup
+// per block of elements, the element staged in LDS, the face neighbours'
+// lines read through the caches.  variant: VAR_F64 or VAR_MIX_IN64.
+struct Lap1Desc {
+    int N, variant;
+    int M[3], periodic[3];
+    AxisOps ops[3];
+    const double* u;
+    double* out;
+    const void* op64; const void* op32;
+};
+hipError_t launch_flat_lap1(const Lap1Desc& d, hipStream_t s);
+
 // A batch of pencil passes: one launch over every instance of a group (same N,
 // precision variant, axis, kernel kind and epilogue), driven by a device table
 // of per-instance descriptors and a packed (instance, tile) work list.
