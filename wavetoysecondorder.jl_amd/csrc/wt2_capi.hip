@@ -728,6 +728,7 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
     }
     if (rc == WT2_OK && c->curved)
         rc = curved_init(c->cd, *desc, c->xi, c->w, c->G, nullptr, c->stream, &c->bytes);
+    if (rc == WT2_OK && c->curved) { c->cd.op64 = c->op64; c->cd.op32 = c->op32; }
     if (rc != WT2_OK) {
         wt2_ctx_destroy(c);
         return rc;
@@ -789,6 +790,7 @@ int wt2_ctx_set_background(wt2_ctx* c, const double* bg10)
     int rc;
     if (!c->curved) {
         rc = curved_init(c->cd, c->d, c->xi, c->w, c->G, bg10, c->stream, &c->bytes);
+        if (rc == WT2_OK) { c->cd.op64 = c->op64; c->cd.op32 = c->op32; }
         if (rc == WT2_OK) c->curved = 1;
     } else {
         rc = curved_set_background(c->cd, c->d, c->xi, c->w, c->G, bg10, c->stream);

@@ -25,6 +25,7 @@
 
 #include "wt2_curved.hpp"
 #include "wt2_internal.hpp"
+#include "wt2_ops.hpp"
 
 namespace wt2 {
 
@@ -258,6 +259,7 @@ struct CArgs {
     const void* u; const void* v;   // stage fields (TS)
     const TC* coef;                 // 13 x npts SoA
     TC* fbuf;                       // 6 N^2 per element
+    const TC* op;                   // even/odd operator table (OpLayout<N>)
     void* out;                      // output (TO): wgt (...) or unweighted
     int64_t npts;
     int M0, M1, M2, per0, per1, per2;
@@ -321,8 +323,10 @@ template <int N, typename TC, typename TS>
 __device__ __forceinline__ void faceflux_body(const CArgs<N, TC>& a, const int64_t e)
 {
     __shared__ TC su[N * N * N];
+    __shared__ CConst<N, TC> kl;     // G indexed per lane: from LDS, not kernel arguments
     const TS* u = static_cast<const TS*>(a.u);
     for (int q = threadIdx.x; q < N * N * N; q += blockDim.x) su[q] = TC(u[e * N * N * N + q]);
+    for (int q = threadIdx.x; q < N * N; q += blockDim.x) kl.G[q] = a.k.G[q];
     __syncthreads();
     for (int q = threadIdx.x; q < 6 * N * N; q += blockDim.x) {
         const int f = q / (N * N), pq = q - f * N * N;
@@ -331,7 +335,7 @@ __device__ __forceinline__ void faceflux_body(const CArgs<N, TC>& a, const int64
         const int n = fidx<N>(ax, r, p, qq);
         const int i = n % N, j = (n / N) % N, l = n / (N * N);
         TC g[3];
-        grad_lds<N, TC>(su, a.k, i, j, l, g);
+        grad_lds<N, TC>(su, kl, i, j, l, g);
         const int64_t gn = e * N * N * N + n;
         TC F = kab<N, TC>(a.coef, a.npts, gn, ax, 0) * g[0];
         F = fma(kab<N, TC>(a.coef, a.npts, gn, ax, 1), g[1], F);
@@ -340,123 +344,220 @@ __device__ __forceinline__ void faceflux_body(const CArgs<N, TC>& a, const int64
     }
 }
 
-// Kernel B: the whole curved operator for one element per workgroup.
+// Kernel B: everything else, for a block of EW elements per workgroup.  A
+// work-item owns one z-line (i, j, all k) of an element:
+//   P1  u, u_t of the elements -> LDS (A, B);
+//   P2  gradients: d/dz by the even/odd operator on the line in registers,
+//       d/dx and d/dy from LDS with the work-item's rows G[i], G[j] in
+//       registers; fluxes W^a = K^ab g_b + c^a u_t and c^a d_a u_t per node;
+//       face terms per face node (face-parallel over the element's 6 N^2
+//       face nodes) into LDS: S^b = alpha n K^ab du and the point term;
+//   P3  W^0, W^1 -> LDS (over A, B); tangential lifts of S at the face nodes
+//       (face-parallel, added to the point term);
+//   P4  sum_a G_a W^a (x, y from LDS, z in registers) + c^a G_a u_t + the
+//       normal lifts of S^a along each face-normal line + the face terms,
+//       times wgt, to the output (coalesced over the work-items of a k).
+template <int N>
+struct VolShape {
+    static constexpr int L = N * N;                      // work-items per element
+    static constexpr int EW = (128 / L) > 0 ? 128 / L : 1;
+    static constexpr int NT = EW * L;
+    static constexpr int N3 = N * N * N;
+    static constexpr int F2 = 6 * N * N;                 // face nodes per element
+    static constexpr int PER = 2 * N3 + 24 * N * N;      // LDS values per element
+};
+
+template <int N, typename TC>
+struct VolTables {
+    TC G[N * N];          // G[i][p]
+    TC GW[N * N];         // G[l][t] * w[l] (tangential lifts)
+    TC op[OpLayout<N>::SIZE];
+    TC w[N], winv[N];
+};
+
 template <int N, typename TC, typename TS, typename TO>
-__device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t e)
+__device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t e0, const int nel)
 {
-    constexpr int N3 = N * N * N, N2 = N * N;
+    using S = VolShape<N>;
+    constexpr int N3 = S::N3, N2 = N * N;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    TC* su = reinterpret_cast<TC*>(smem_raw);   // N3
-    TC* sv = su + N3;                           // N3
-    TC* W = sv + N3;                            // 3 N3: F^a + c^a v
-    TC* S = W + 3 * N3;                         // 3 x 6 N2: alpha n K^ab du per face node
-    TC* Pt = S + 18 * N2;                       // 6 N2: point terms of the face SAT
-    const int64_t eb = e * N3;
+    __shared__ VolTables<N, TC> T;
+    const int tid = threadIdx.x;
+    for (int q = tid; q < N2; q += blockDim.x) {
+        T.G[q] = a.k.G[q];
+        T.GW[q] = a.k.G[q] * a.k.w[q / N];
+    }
+    for (int q = tid; q < OpLayout<N>::SIZE; q += blockDim.x) T.op[q] = a.op[q];
+    for (int q = tid; q < N; q += blockDim.x) { T.w[q] = a.k.w[q]; T.winv[q] = a.k.winv[q]; }
+
     const TS* u = static_cast<const TS*>(a.u);
     const TS* v = static_cast<const TS*>(a.v);
-    const int tid = threadIdx.x;
-    for (int q = tid; q < N3; q += blockDim.x) {
-        su[q] = TC(u[eb + q]);
-        sv[q] = a.adv ? TC(v[eb + q]) : TC(0);
+    const bool adv = a.adv != 0;
+    TC* lds = reinterpret_cast<TC*>(smem_raw);
+    // P1
+    for (int q = tid; q < nel * N3; q += blockDim.x) {
+        const int el = q / N3, n = q - el * N3;
+        const int64_t g = e0 * N3 + q;
+        lds[el * S::PER + n] = TC(u[g]);
+        lds[el * S::PER + N3 + n] = adv ? TC(v[g]) : TC(0);
     }
     __syncthreads();
-    // fluxes at every node
-    TC cgv = TC(0);   // c^a G_a v at this thread's node
-    for (int q = tid; q < N3; q += blockDim.x) {
-        const int i = q % N, j = (q / N) % N, l = q / N2;
-        TC g[3];
-        grad_lds<N, TC>(su, a.k, i, j, l, g);
-        const int64_t gn = eb + q;
-        for (int ax = 0; ax < 3; ++ax) {
-            TC F = kab<N, TC>(a.coef, a.npts, gn, ax, 0) * g[0];
-            F = fma(kab<N, TC>(a.coef, a.npts, gn, ax, 1), g[1], F);
-            F = fma(kab<N, TC>(a.coef, a.npts, gn, ax, 2), g[2], F);
-            if (a.adv) F = fma(a.coef[(6 + ax) * a.npts + gn], sv[q], F);
-            W[ax * N3 + q] = F;
-        }
-        if (a.adv) {
-            TC gv[3];
-            grad_lds<N, TC>(sv, a.k, i, j, l, gv);
-            cgv = a.coef[6 * a.npts + gn] * gv[0] + a.coef[7 * a.npts + gn] * gv[1] +
-                  a.coef[8 * a.npts + gn] * gv[2];
-        }
-    }
-    // face-node data (jumps against the neighbour or the Dirichlet value)
-    for (int q = tid; q < 6 * N2; q += blockDim.x) {
-        const int f = q / N2, pq = q - f * N2;
-        const int p = pq % N, qq = pq / N;
-        const int ax = f >> 1, side = f & 1;
-        const int r = side ? N - 1 : 0, rn = side ? 0 : N - 1;
-        const TC nrm = side ? TC(1) : TC(-1);
-        const int n = fidx<N>(ax, r, p, qq);
-        const int64_t gn = eb + n;
-        const int64_t en = nbr_elem(e, ax, side, a.M0, a.M1, a.M2, a.per0, a.per1, a.per2);
-        TC du, dF = TC(0), kap = a.coef[(10 + ax) * a.npts + gn], dv = TC(0), alpha;
-        const int i = n % N, j = (n / N) % N, l = n / N2;
-        if (en >= 0) {
-            const int nn = fidx<N>(ax, rn, p, qq);
-            const int64_t gnn = en * N3 + nn;
-            du = su[n] - TC(u[gnn]);
-            TC g[3];
-            grad_lds<N, TC>(su, a.k, i, j, l, g);
-            TC Fs = kab<N, TC>(a.coef, a.npts, gn, ax, 0) * g[0];
-            Fs = fma(kab<N, TC>(a.coef, a.npts, gn, ax, 1), g[1], Fs);
-            Fs = fma(kab<N, TC>(a.coef, a.npts, gn, ax, 2), g[2], Fs);
-            const int fn = (ax << 1) | (1 - side);
-            dF = Fs - a.fbuf[en * 6 * N2 + fn * N2 + pq];
-            kap = TC(0.5) * (kap + a.coef[(10 + ax) * a.npts + gnn]);
-            if (a.adv) dv = sv[n] - TC(v[gnn]);
-            alpha = TC(0.5);
-        } else {
-            du = su[n] - a.k.bv[f];
-            if (a.adv) dv = sv[n];
-            alpha = TC(1);
-        }
-        for (int b = 0; b < 3; ++b) S[(b * 6 + f) * N2 + pq] = alpha * nrm * kab<N, TC>(a.coef, a.npts, gn, ax, b) * du;
-        TC pt = TC(-0.5) * nrm * dF - a.k.tau * kap * du;
-        if (a.adv) pt -= nrm * a.coef[(6 + ax) * a.npts + gn] * dv;
-        Pt[q] = pt * a.k.winv[r];
-    }
-    __syncthreads();
-    TO* out = static_cast<TO*>(a.out);
-    for (int q = tid; q < N3; q += blockDim.x) {
-        const int i = q % N, j = (q / N) % N, l = q / N2;
-        const int idx[3] = {i, j, l};
-        TC s = cgv;
-        // volume: sum_a G_a W^a
+
+    const int slot = tid / S::L, lane = tid - slot * S::L;
+    const bool act = slot < nel;
+    const int sl = act ? slot : 0;
+    const int64_t e = e0 + sl;
+    const int64_t eb = e * N3;
+    TC* A = lds + sl * S::PER;          // u, then W^0
+    TC* B = A + N3;                     // u_t, then W^1
+    TC* Sf = B + N3;                    // 3 x 6 N^2: S^b per face node
+    TC* Pt = Sf + 18 * N2;              // 6 N^2: point (+ tangential) term per face node
+    const int i = lane % N, j = lane / N;
+    const int64_t np = a.npts;
+
+    TC W0[N], W1[N], W2[N], cgv[N];
+    if (act) {
+        // ---- P2: gradients and fluxes on this z-line
+        TC Gi[N], Gj[N];
 #pragma unroll
-        for (int p = 0; p < N; ++p) {
-            s = fma(a.k.G[i * N + p], W[0 * N3 + p + N * (j + N * l)], s);
-            s = fma(a.k.G[j * N + p], W[1 * N3 + i + N * (p + N * l)], s);
-            s = fma(a.k.G[l * N + p], W[2 * N3 + i + N * (j + N * p)], s);
-        }
-        for (int ax = 0; ax < 3; ++ax) {
-            // tangential indices (p, q) of this node for faces normal to ax
-            const int tp = ax == 0 ? j : i;
-            const int tq = ax == 2 ? j : l;
-            const int m = idx[ax];
-            const int tb0 = ax == 0 ? 1 : 0;   // axis of tangential index p
-            const int tb1 = ax == 2 ? 1 : 2;   // axis of tangential index q
-            for (int side = 0; side < 2; ++side) {
-                const int f = (ax << 1) | side;
-                const int r = side ? N - 1 : 0;
-                // b = a symmetrising term along the normal line
-                s = fma(a.k.G[r * N + m] * a.k.winv[m], S[(ax * 6 + f) * N2 + tp + N * tq], s);
-                if (m != r) continue;
-                // point terms at the face node
-                s += Pt[f * N2 + tp + N * tq];
-                // b != a symmetrising terms along the face
-                TC t0 = TC(0), t1 = TC(0);
+        for (int p = 0; p < N; ++p) { Gi[p] = T.G[i * N + p]; Gj[p] = T.G[j * N + p]; }
+        TC g2[N], gv2[N];
+        {
+            TC x[N];
 #pragma unroll
-                for (int ll = 0; ll < N; ++ll) {
-                    t0 = fma(a.k.G[ll * N + tp] * a.k.w[ll], S[(tb0 * 6 + f) * N2 + ll + N * tq], t0);
-                    t1 = fma(a.k.G[ll * N + tq] * a.k.w[ll], S[(tb1 * 6 + f) * N2 + tp + N * ll], t1);
-                }
-                s = fma(a.k.winv[r], fma(a.k.winv[tp], t0, a.k.winv[tq] * t1), s);
+            for (int k = 0; k < N; ++k) x[k] = A[i + N * j + N2 * k];
+            apply_G<N, TC>(T.op, x, g2);
+#pragma unroll
+            for (int k = 0; k < N; ++k) x[k] = B[i + N * j + N2 * k];
+            if (adv) apply_G<N, TC>(T.op, x, gv2);
+            else {
+#pragma unroll
+                for (int k = 0; k < N; ++k) gv2[k] = TC(0);
             }
         }
-        if (a.weighted) s *= a.coef[9 * a.npts + eb + q];
-        out[eb + q] = TO(s);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const int n = i + N * j + N2 * k;
+            TC g0 = TC(0), g1 = TC(0);
+#pragma unroll
+            for (int p = 0; p < N; ++p) {
+                g0 = fma(Gi[p], A[p + N * j + N2 * k], g0);
+                g1 = fma(Gj[p], A[i + N * p + N2 * k], g1);
+            }
+            const int64_t gn = eb + n;
+            const TC K00 = a.coef[0 * np + gn], K01 = a.coef[1 * np + gn], K02 = a.coef[2 * np + gn];
+            const TC K11 = a.coef[3 * np + gn], K12 = a.coef[4 * np + gn], K22 = a.coef[5 * np + gn];
+            TC F0 = K00 * g0, F1 = K01 * g0, F2 = K02 * g0;
+            F0 = fma(K01, g1, F0); F1 = fma(K11, g1, F1); F2 = fma(K12, g1, F2);
+            F0 = fma(K02, g2[k], F0); F1 = fma(K12, g2[k], F1); F2 = fma(K22, g2[k], F2);
+            TC cg = TC(0);
+            if (adv) {
+                const TC c0 = a.coef[6 * np + gn], c1 = a.coef[7 * np + gn], c2 = a.coef[8 * np + gn];
+                const TC vn = B[n];
+                F0 = fma(c0, vn, F0); F1 = fma(c1, vn, F1); F2 = fma(c2, vn, F2);
+                TC gv0 = TC(0), gv1 = TC(0);
+#pragma unroll
+                for (int p = 0; p < N; ++p) {
+                    gv0 = fma(Gi[p], B[p + N * j + N2 * k], gv0);
+                    gv1 = fma(Gj[p], B[i + N * p + N2 * k], gv1);
+                }
+                cg = c0 * gv0 + c1 * gv1 + c2 * gv2[k];
+            }
+            W0[k] = F0; W1[k] = F1; W2[k] = F2; cgv[k] = cg;
+        }
+        // ---- P2 (face-parallel): face-node terms of this element
+        for (int q = lane; q < S::F2; q += S::L) {
+            const int f = q / N2, pq = q - f * N2;
+            const int p = pq % N, qq = pq / N;
+            const int ax = f >> 1, side = f & 1;
+            const int r = side ? N - 1 : 0, rn = side ? 0 : N - 1;
+            const TC nrm = side ? TC(1) : TC(-1);
+            const int n = fidx<N>(ax, r, p, qq);
+            const int64_t gn = eb + n;
+            const int64_t en = nbr_elem(e, ax, side, a.M0, a.M1, a.M2, a.per0, a.per1, a.per2);
+            TC du, dF = TC(0), kap = a.coef[(10 + ax) * np + gn], dv = TC(0), alpha;
+            if (en >= 0) {
+                const int64_t gnn = en * N3 + fidx<N>(ax, rn, p, qq);
+                du = A[n] - TC(u[gnn]);
+                const int fn = (ax << 1) | (1 - side);
+                dF = a.fbuf[e * S::F2 + q] - a.fbuf[en * S::F2 + fn * N2 + pq];
+                kap = TC(0.5) * (kap + a.coef[(10 + ax) * np + gnn]);
+                if (adv) dv = B[n] - TC(v[gnn]);
+                alpha = TC(0.5);
+            } else {
+                du = A[n] - a.k.bv[f];
+                if (adv) dv = B[n];
+                alpha = TC(1);
+            }
+            const TC an = alpha * nrm;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = an * kab<N, TC>(a.coef, np, gn, ax, b) * du;
+            TC pt = TC(-0.5) * nrm * dF - a.k.tau * kap * du;
+            if (adv) pt -= nrm * a.coef[(6 + ax) * np + gn] * dv;
+            Pt[q] = pt * T.winv[r];
+        }
+    }
+    __syncthreads();
+    if (act) {
+        // ---- P3: fluxes to LDS; tangential lifts of S at the face nodes
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            A[i + N * j + N2 * k] = W0[k];
+            B[i + N * j + N2 * k] = W1[k];
+        }
+        for (int q = lane; q < S::F2; q += S::L) {
+            const int f = q / N2, pq = q - f * N2;
+            const int tp = pq % N, tq = pq / N;
+            const int ax = f >> 1, r = (f & 1) ? N - 1 : 0;
+            const int tb0 = ax == 0 ? 1 : 0, tb1 = ax == 2 ? 1 : 2;
+            TC t0 = TC(0), t1 = TC(0);
+#pragma unroll
+            for (int l = 0; l < N; ++l) {
+                t0 = fma(T.GW[l * N + tp], Sf[(tb0 * 6 + f) * N2 + l + N * tq], t0);
+                t1 = fma(T.GW[l * N + tq], Sf[(tb1 * 6 + f) * N2 + tp + N * l], t1);
+            }
+            Pt[q] = fma(T.winv[r], fma(T.winv[tp], t0, T.winv[tq] * t1), Pt[q]);
+        }
+    }
+    __syncthreads();
+    if (!act) return;
+    // ---- P4: divergence, lifts, face terms, output
+    TC Gi[N], Gj[N];
+#pragma unroll
+    for (int p = 0; p < N; ++p) { Gi[p] = T.G[i * N + p]; Gj[p] = T.G[j * N + p]; }
+    TC d2[N];
+    apply_G<N, TC>(T.op, W2, d2);
+    // normal-line lift coefficients G[r][m] / w[m] of this work-item's i and j
+    const TC cx0 = T.G[i] * T.winv[i], cxN = T.G[(N - 1) * N + i] * T.winv[i];
+    const TC cy0 = T.G[j] * T.winv[j], cyN = T.G[(N - 1) * N + j] * T.winv[j];
+    const TC sz0 = Sf[(2 * 6 + 4) * N2 + i + N * j], szN = Sf[(2 * 6 + 5) * N2 + i + N * j];
+    TO* out = static_cast<TO*>(a.out);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        TC s = cgv[k] + d2[k];
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            s = fma(Gi[p], A[p + N * j + N2 * k], s);
+            s = fma(Gj[p], B[i + N * p + N2 * k], s);
+        }
+        // lifts along the normal lines: x faces (0, 1) at (j, k), y faces (2, 3) at (i, k),
+        // z faces (4, 5) at (i, j)
+        s = fma(cx0, Sf[(0 * 6 + 0) * N2 + j + N * k], s);
+        s = fma(cxN, Sf[(0 * 6 + 1) * N2 + j + N * k], s);
+        s = fma(cy0, Sf[(1 * 6 + 2) * N2 + i + N * k], s);
+        s = fma(cyN, Sf[(1 * 6 + 3) * N2 + i + N * k], s);
+        s = fma(T.G[k] * T.winv[k], sz0, s);
+        s = fma(T.G[(N - 1) * N + k] * T.winv[k], szN, s);
+        // point and tangential terms at the face nodes
+        if (i == 0) s += Pt[0 * N2 + j + N * k];
+        if (i == N - 1) s += Pt[1 * N2 + j + N * k];
+        if (j == 0) s += Pt[2 * N2 + i + N * k];
+        if (j == N - 1) s += Pt[3 * N2 + i + N * k];
+        if (k == 0) s += Pt[4 * N2 + i + N * j];
+        if (k == N - 1) s += Pt[5 * N2 + i + N * j];
+        const int64_t gn = eb + i + N * j + N2 * k;
+        if (a.weighted) s *= a.coef[9 * np + gn];
+        out[gn] = TO(s);
     }
 }
 
@@ -467,9 +568,10 @@ __global__ void __launch_bounds__(256) curved_faceflux_kernel(CArgs<N, TC> a)
 }
 
 template <int N, typename TC, typename TS, typename TO>
-__global__ void __launch_bounds__(1024) curved_volume_kernel(CArgs<N, TC> a)
+__global__ void __launch_bounds__(128) curved_volume_kernel(CArgs<N, TC> a, int64_t Ne)
 {
-    volume_body<N, TC, TS, TO>(a, blockIdx.x);
+    const int64_t e0 = int64_t(blockIdx.x) * VolShape<N>::EW;
+    volume_body<N, TC, TS, TO>(a, e0, int(min<int64_t>(VolShape<N>::EW, Ne - e0)));
 }
 
 template <int N, typename TC, typename TS>
@@ -481,11 +583,14 @@ __global__ void __launch_bounds__(256) curved_faceflux_batch_kernel(const CArgs<
 }
 
 template <int N, typename TC, typename TS, typename TO>
-__global__ void __launch_bounds__(1024) curved_volume_batch_kernel(const CArgs<N, TC>* __restrict__ tab,
-                                                                   const int2* __restrict__ work)
+__global__ void __launch_bounds__(128) curved_volume_batch_kernel(const CArgs<N, TC>* __restrict__ tab,
+                                                                  const int2* __restrict__ work)
 {
+    // work: (instance, first element); the block ends at the instance's last element
     const int2 w = work[blockIdx.x];
-    volume_body<N, TC, TS, TO>(tab[w.x], w.y);
+    const CArgs<N, TC>& a = tab[w.x];
+    const int64_t Ne = a.npts / (N * N * N);
+    volume_body<N, TC, TS, TO>(a, w.y, int(min<int64_t>(VolShape<N>::EW, Ne - w.y)));
 }
 
 template <int N, typename TC>
@@ -499,6 +604,7 @@ static void make_cargs(const CurvedData& cd, const CurvedCall& cc, CArgs<N, TC>&
     a.coef = reinterpret_cast<const TC*>(std::is_same<TC, float>::value ? static_cast<const void*>(cd.coef32)
                                                                           : static_cast<const void*>(cd.coef64));
     a.fbuf = reinterpret_cast<TC*>(cd.fbuf);
+    a.op = static_cast<const TC*>(std::is_same<TC, float>::value ? cd.op32 : cd.op64);
     a.out = cc.out ? static_cast<void*>(cc.out) : cc.Pv;
     a.npts = cd.npts;
     a.M0 = cd.M[0]; a.M1 = cd.M[1]; a.M2 = cd.M[2];
@@ -516,18 +622,18 @@ static hipError_t launch_c(const CurvedData& cd, const CurvedCall& cc, hipStream
     hipLaunchKernelGGL((curved_faceflux_kernel<N, TC, TS>), dim3(unsigned(cd.Ne)), dim3(256), 0, st, a);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
-    constexpr int N3 = N * N * N;
-    const int nthr = ((N3 + 63) / 64) * 64;
-    const size_t shmem = (5 * N3 + 24 * N * N) * sizeof(TC);
+    using VS = VolShape<N>;
+    const size_t shmem = size_t(VS::EW) * VS::PER * sizeof(TC);
     auto kern = curved_volume_kernel<N, TC, TS, TO>;
     static bool raised = false;
-    if (shmem > 64 * 1024 && !raised) {
+    if (shmem + sizeof(VolTables<N, TC>) + 256 > 64 * 1024 && !raised) {
         err = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
         if (err != hipSuccess) return err;
         raised = true;
     }
-    hipLaunchKernelGGL(kern, dim3(unsigned(cd.Ne)), dim3(nthr), shmem, st, a);
+    const int64_t nblk = (cd.Ne + VS::EW - 1) / VS::EW;
+    hipLaunchKernelGGL(kern, dim3(unsigned(nblk)), dim3(VS::NT), shmem, st, a, int64_t(cd.Ne));
     return hipGetLastError();
 }
 
@@ -550,10 +656,20 @@ static hipError_t cbatch_build_t(const CurvedData* const* cds, const CurvedCall*
                                  CurvedBatchPlan& plan)
 {
     std::vector<CArgs<N, TC>> tab(n);
-    std::vector<int2> work;
+    std::vector<int2> work, work2;
     for (int i = 0; i < n; ++i) {
         make_cargs<N, TC>(*cds[i], ccs[i], tab[i]);
         for (int64_t e = 0; e < cds[i]->Ne; ++e) work.push_back(make_int2(i, int(e)));
+        for (int64_t e = 0; e < cds[i]->Ne; e += VolShape<N>::EW) work2.push_back(make_int2(i, int(e)));
+    }
+    const size_t wb2 = sizeof(int2) * work2.size();
+    if (wb2 > plan.work2_cap) {
+        (void)hipFree(plan.work2);
+        if (hipMalloc(reinterpret_cast<void**>(&plan.work2), wb2) != hipSuccess) {
+            plan.work2 = nullptr; plan.work2_cap = 0;
+            return hipErrorOutOfMemory;
+        }
+        plan.work2_cap = wb2;
     }
     const size_t tb = sizeof(CArgs<N, TC>) * size_t(n), wb = sizeof(int2) * work.size();
     if (tb > plan.tab_cap) {
@@ -571,8 +687,10 @@ static hipError_t cbatch_build_t(const CurvedData* const* cds, const CurvedCall*
     }
     hipError_t e = hipMemcpy(plan.tab, tab.data(), tb, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(plan.work, work.data(), wb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(plan.work2, work2.data(), wb2, hipMemcpyHostToDevice);
     plan.N = N;
     plan.nwork = int(work.size());
+    plan.nwork2 = int(work2.size());
     return e;
 }
 
@@ -585,18 +703,17 @@ static hipError_t cbatch_launch_t(const CurvedBatchPlan& plan, hipStream_t st)
     hipLaunchKernelGGL((curved_faceflux_batch_kernel<N, TC, TS>), dim3(plan.nwork), dim3(256), 0, st, tab, plan.work);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
-    constexpr int N3 = N * N * N;
-    const int nthr = ((N3 + 63) / 64) * 64;
-    const size_t shmem = (5 * N3 + 24 * N * N) * sizeof(TC);
+    using VS = VolShape<N>;
+    const size_t shmem = size_t(VS::EW) * VS::PER * sizeof(TC);
     auto kern = curved_volume_batch_kernel<N, TC, TS, TO>;
     static bool raised = false;
-    if (shmem > 64 * 1024 && !raised) {
+    if (shmem + sizeof(VolTables<N, TC>) + 256 > 64 * 1024 && !raised) {
         err = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
         if (err != hipSuccess) return err;
         raised = true;
     }
-    hipLaunchKernelGGL(kern, dim3(plan.nwork), dim3(nthr), shmem, st, tab, plan.work);
+    hipLaunchKernelGGL(kern, dim3(plan.nwork2), dim3(VS::NT), shmem, st, tab, plan.work2);
     return hipGetLastError();
 }
 
@@ -651,6 +768,7 @@ void curved_batch_free(CurvedBatchPlan& plan)
 {
     (void)hipFree(plan.tab);
     (void)hipFree(plan.work);
+    (void)hipFree(plan.work2);
     plan = CurvedBatchPlan();
 }
 

@@ -27,6 +27,8 @@ struct CurvedData {
     double* wv = nullptr;       // energy weight of v^2: H3 / wgt
     double* fbuf = nullptr;     // face flux buffer: 6 N^2 per element
     bool pending = false;       // allocated; coefficients not computed yet
+    const void* op64 = nullptr; // even/odd operator tables (owned by the context)
+    const void* op32 = nullptr;
 };
 
 struct CurvedCall {
@@ -43,10 +45,11 @@ struct CurvedCall {
 // Batched curved operator: one (face-flux, volume) launch pair over every
 // element of every instance in the group (same N and precision).
 struct CurvedBatchPlan {
-    int N = 0, kind = 0, nwork = 0;
+    int N = 0, kind = 0, nwork = 0, nwork2 = 0;
     void* tab = nullptr;
-    int2* work = nullptr;
-    size_t tab_cap = 0, work_cap = 0;
+    int2* work = nullptr;       // (instance, element): face-flux kernel
+    int2* work2 = nullptr;      // (instance, first element of a block): element kernel
+    size_t tab_cap = 0, work_cap = 0, work2_cap = 0;
 };
 hipError_t curved_batch_build(const CurvedData* const* cds, const CurvedCall* ccs, int n, CurvedBatchPlan& plan);
 hipError_t curved_batch_launch(const CurvedBatchPlan& plan, hipStream_t s);
