@@ -322,7 +322,14 @@ def serving_run(wh, torch, args, rates=None):
         summ["one_at_a_time_fp64_p99_s"] = float(np.percentile(lat_seq, 99))
         summ["throughput_vs_one_at_a_time_fp64"] = summ["throughput_point_updates_per_s"] / \
             out["one_at_a_time_fp64"]["throughput_point_updates_per_s"]
-        summ["mixed_vs_fp64_rel_l2_err_max"] = max(errs)
+        errs = np.asarray(errs)
+        w = int(np.argmax(errs))
+        wr = reqs[st.responses[w].rid]
+        summ["mixed_vs_fp64_worst_request"] = {"N": wr.N, "M": wr.M, "warped": wr.curved, "T": wr.T,
+                                               "width": wr.width, "final_u_l2": float(np.linalg.norm(by_rid[wr.rid].u))}
+        summ["mixed_vs_fp64_rel_l2_err_max"] = float(np.max(errs))
+        summ["mixed_vs_fp64_rel_l2_err_median"] = float(np.median(errs))
+        summ["mixed_vs_fp64_nonfinite"] = int(np.sum(~np.isfinite(errs)))
         out["rates"][str(int(lam))] = summ
         print(f"serving: rate {lam}: p50 {summ['p50_latency_s']:.3f} s p99 {summ['p99_latency_s']:.3f} s "
               f"{summ['throughput_point_updates_per_s']:.3e} pt-upd/s", file=sys.stderr, flush=True)

@@ -183,6 +183,11 @@ int wt2_batch_slot(const wt2_batch* b, int32_t slot, int64_t* steps_done, double
 int wt2_ctx_autotune(wt2_ctx* ctx, double* u, double* ud, double dt, int32_t flags);
 /* Current tile choice: element planes per tile and LDS-free flag per axis. */
 int wt2_ctx_tiles(const wt2_ctx* ctx, int32_t* nb3, int32_t* direct3);
+/* Set the pass kernel (direct3[a] != 0: LDS-free) and planes per tile nb3[a]
+ * of each axis, e.g. a table the autotuner produced earlier.  nb3[a] must
+ * divide N and give at most 8 wavefronts per workgroup.  No counterpart in
+ * the reference. */
+int wt2_ctx_set_tiles(wt2_ctx* ctx, const int32_t* nb3, const int32_t* direct3);
 
 /* Diagnostics (no counterpart in the reference; used by bench.py to time one
  * kernel in isolation with events on the context stream): enqueue, without

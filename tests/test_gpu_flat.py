@@ -256,8 +256,36 @@ def test_autotuned_tiles_give_the_same_evolution():
     ws.autotune(u, v, dt, force=True)
     t = ws.tiles()
     assert len(t["planes_per_tile"]) == 3 and all(N % nb == 0 for nb in t["planes_per_tile"])
-    assert t["lds_free"][0]                  # the x pass is always LDS-free
     ws.rk4_steps(u, v, dt, 5)
     uo, _ = o.rk4(u0, v0, dt, 5)
     assert relerr(u.cpu().numpy(), uo) <= TOL64
     assert relerr(u_def, uo) <= TOL64
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_every_tile_choice_gives_the_same_evolution(prec):
+    """Both pass kernels (LDS-free and LDS-staged) on every axis, including the
+    x pass that carries the RK4 update, with two tile sizes: 3 RK4 steps with
+    KO against the oracle (FP64 <= 1e-12; mixed H-norm <= 1e-5)."""
+    import itertools
+    import torch
+    _say(f"tile choices {prec}")
+    N, M = 6, (4, 3, 5)
+    ws, o = _pair(N, M, eps_ko=0.1, precision=prec)
+    x = o.coords()
+    u0 = np.exp(-np.sum((x - 0.45) ** 2, axis=1) / 0.03)
+    v0 = np.sin(math.pi * x[:, 1]) * 0.1
+    dt = 0.2 * o.hmin
+    uo, vo = o.rk4(u0, v0, dt, 3)
+    H = o.hphys()
+    for lds in itertools.product([0, 1], repeat=3):
+        for nb in ((1, 1, 1), (3, 2, 6)):
+            ws.set_tiles(nb, tuple(1 - q for q in lds))
+            u = torch.from_numpy(u0).cuda(); v = torch.from_numpy(v0).cuda()
+            ws.rk4_steps(u, v, dt, 3)
+            uh = u.cpu().numpy()
+            if prec == "fp64":
+                assert relerr(uh, uo) <= TOL64, (lds, nb)
+            else:
+                d = uh - uo
+                assert math.sqrt(np.sum(H * d * d) / np.sum(H * uo * uo)) <= 1e-5, (lds, nb)

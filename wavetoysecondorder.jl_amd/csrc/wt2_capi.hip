@@ -980,7 +980,7 @@ int wt2_ctx_autotune(wt2_ctx* c, double* u, double* ud, double dt, int32_t flags
         float tbest = 1e30f;
         int bnb = c->nb[a], bdir = c->direct[a];
         const int lpw = 64 / c->d.M[a];
-        for (int dir = 1; dir >= (a == 0 ? 1 : 0); --dir) {
+        for (int dir = 1; dir >= 0; --dir) {
             for (int nb = 1; nb <= N; ++nb) {
                 if (N % nb || (N * nb + lpw - 1) / lpw > 8) continue;
                 c->direct[a] = dir;
@@ -1007,6 +1007,22 @@ int wt2_ctx_autotune(wt2_ctx* c, double* u, double* ud, double dt, int32_t flags
     (void)hipEventDestroy(e1);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[key] = best;
+    return WT2_OK;
+}
+
+int wt2_ctx_set_tiles(wt2_ctx* c, const int32_t* nb3, const int32_t* direct3)
+{
+    if (!c || !nb3 || !direct3) { set_error("wt2_ctx_set_tiles: null pointer"); return WT2_ERR_ARG; }
+    for (int a = 0; a < 3; ++a) {
+        const int nb = nb3[a];
+        const int lpw = 64 / c->d.M[a];
+        if (nb < 1 || c->N % nb != 0 || (c->N * nb + lpw - 1) / lpw > 8) {
+            set_error("wt2_ctx_set_tiles: planes per tile must divide N and give at most 8 wavefronts");
+            return WT2_ERR_ARG;
+        }
+    }
+    for (int a = 0; a < 3; ++a) { c->nb[a] = nb3[a]; c->direct[a] = direct3[a] != 0; }
+    if (c->gvalid) { (void)hipGraphExecDestroy(c->gexec); c->gvalid = false; }
     return WT2_OK;
 }
 
@@ -1101,7 +1117,7 @@ static std::string pass_key(const PassDesc& p)
 {
     char buf[96];
     std::snprintf(buf, sizeof(buf), "%d/%d/%d/%d/%d/%d/%d/%d", p.N, p.variant, p.axis,
-                  (p.axis == 0 || p.direct) ? 1 : 0, p.epi, p.lap, p.ko, p.needv);
+                  p.direct ? 1 : 0, p.epi, p.lap, p.ko, p.needv);
     return buf;
 }
 

@@ -567,8 +567,14 @@ __global__ void __launch_bounds__(256) curved_faceflux_kernel(CArgs<N, TC> a)
     faceflux_body<N, TC, TS>(a, blockIdx.x);
 }
 
+// FP32 variants are held to 128 VGPRs (4 wavefronts per SIMD); FP64 ones get
+// what they need (2 per SIMD at N = 10).
+template <typename TC>
+struct VolWaves { static constexpr int value = std::is_same<TC, float>::value ? 4 : 1; };
+
 template <int N, typename TC, typename TS, typename TO>
-__global__ void __launch_bounds__(128) curved_volume_kernel(CArgs<N, TC> a, int64_t Ne)
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(VolWaves<TC>::value)))
+curved_volume_kernel(CArgs<N, TC> a, int64_t Ne)
 {
     const int64_t e0 = int64_t(blockIdx.x) * VolShape<N>::EW;
     volume_body<N, TC, TS, TO>(a, e0, int(min<int64_t>(VolShape<N>::EW, Ne - e0)));
@@ -583,8 +589,8 @@ __global__ void __launch_bounds__(256) curved_faceflux_batch_kernel(const CArgs<
 }
 
 template <int N, typename TC, typename TS, typename TO>
-__global__ void __launch_bounds__(128) curved_volume_batch_kernel(const CArgs<N, TC>* __restrict__ tab,
-                                                                  const int2* __restrict__ work)
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(VolWaves<TC>::value)))
+curved_volume_batch_kernel(const CArgs<N, TC>* __restrict__ tab, const int2* __restrict__ work)
 {
     // work: (instance, first element); the block ends at the instance's last element
     const int2 w = work[blockIdx.x];

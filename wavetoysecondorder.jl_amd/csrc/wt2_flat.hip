@@ -666,7 +666,7 @@ static hipError_t make_kargs(const PassDesc& p, KArgs<N, TC>& a, int* tiles_out,
     a.GS = p.axis == 0 ? 1 : (p.axis == 1 ? N : N * N);
     *tiles_out = tiles;
     *nthr_out = nthr;
-    *shmem_out = (p.axis == 0 || p.direct) ? 0 : shmem;
+    *shmem_out = p.direct ? 0 : shmem;
     return hipSuccess;
 }
 
@@ -679,7 +679,7 @@ static hipError_t launch_t(const PassDesc& p, hipStream_t s)
     hipError_t e0 = make_kargs<N, TC, TS, TP>(p, a, &tiles, &nthr, &shmem);
     if (e0 != hipSuccess) return e0;
     const size_t static_lds = static_lds_bound<N, TC>();
-    if (p.axis == 0 || p.direct) {
+    if (p.direct) {
         // LDS-free pass
         auto xk = flat_xpass_kernel<N, TC, TS, TP>;
         (void)hipGetLastError();
@@ -1005,7 +1005,7 @@ static hipError_t batch_build_t(const PassDesc* d, int n, FlatBatchPlan& plan)
     if (e == hipSuccess) e = hipMemcpy(plan.work, work.data(), wb, hipMemcpyHostToDevice);
     plan.N = N;
     plan.variant = d[0].variant;
-    plan.direct = d[0].axis == 0 || d[0].direct;
+    plan.direct = d[0].direct;
     plan.ntiles = int(work.size());
     plan.nthr = nthr;
     plan.shmem = plan.direct ? 0 : shmem;

@@ -32,7 +32,7 @@ EXPORTS = (
     "wt2_coords", "wt2_rhs3d", "wt2_rhs_full", "wt2_rk4_steps", "wt2_energy", "wt2_set_stream",
     "wt2_batch_create", "wt2_batch_destroy", "wt2_batch_add", "wt2_batch_remove", "wt2_batch_step",
     "wt2_batch_step_async", "wt2_batch_sync",
-    "wt2_batch_slot", "wt2_ctx_autotune", "wt2_ctx_tiles", "wt2_launch_pass", "wt2_last_error", "wt2_version",
+    "wt2_batch_slot", "wt2_ctx_autotune", "wt2_ctx_tiles", "wt2_ctx_set_tiles", "wt2_launch_pass", "wt2_last_error", "wt2_version",
 )
 
 
@@ -113,6 +113,7 @@ def lib():
                                      ctypes.POINTER(ctypes.c_int32)]
         L.wt2_ctx_autotune.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_int32]
         L.wt2_ctx_tiles.argtypes = [vp, dp, dp]
+        L.wt2_ctx_set_tiles.argtypes = [vp, dp, dp]
         L.wt2_last_error.restype = ctypes.c_char_p
         L.wt2_version.restype = ctypes.c_char_p
         _lib = L

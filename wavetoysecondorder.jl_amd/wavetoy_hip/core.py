@@ -285,6 +285,12 @@ class Workspace:
         check(lib().wt2_ctx_tiles(self.ctx, ctypes.cast(nb, ctypes.c_void_p), ctypes.cast(d, ctypes.c_void_p)))
         return {"planes_per_tile": tuple(nb), "lds_free": tuple(bool(x) for x in d)}
 
+    def set_tiles(self, planes_per_tile, lds_free):
+        """Choose the pass kernel (lds_free[a]) and planes per tile of each axis."""
+        nb = (ctypes.c_int32 * 3)(*[int(x) for x in planes_per_tile])
+        d = (ctypes.c_int32 * 3)(*[int(bool(x)) for x in lds_free])
+        check(lib().wt2_ctx_set_tiles(self.ctx, ctypes.cast(nb, ctypes.c_void_p), ctypes.cast(d, ctypes.c_void_p)))
+
     def energy(self, u, ud):
         (pu, pud), fl = self._ptrs(u, ud)
         E = ctypes.c_double()
