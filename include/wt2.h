@@ -182,12 +182,15 @@ int wt2_batch_slot(const wt2_batch* b, int32_t slot, int64_t* steps_done, double
  * the same shape.  flags & 1 forces a re-measurement. */
 int wt2_ctx_autotune(wt2_ctx* ctx, double* u, double* ud, double dt, int32_t flags);
 /* Current tile choice: element planes per tile and LDS-free flag per axis. */
-int wt2_ctx_tiles(const wt2_ctx* ctx, int32_t* nb3, int32_t* direct3);
-/* Set the pass kernel (direct3[a] != 0: LDS-free) and planes per tile nb3[a]
- * of each axis, e.g. a table the autotuner produced earlier.  nb3[a] must
- * divide N and give at most 8 wavefronts per workgroup.  No counterpart in
- * the reference. */
-int wt2_ctx_set_tiles(wt2_ctx* ctx, const int32_t* nb3, const int32_t* direct3);
+int wt2_ctx_tiles(const wt2_ctx* ctx, int32_t* nb3, int32_t* kind3);
+/* Set the pass kernel kind and planes per tile of each axis, e.g. a table
+ * the autotuner produced earlier.  kind3[a]: 0 = LDS-staged tile, 1 = LDS-free
+ * with the line's elements in the lanes of one wavefront (shuffles), 2 =
+ * LDS-free with adjacent lines in adjacent lanes and the neighbour exchange
+ * through LDS.  nb3[a] must divide N and the tile must fit one workgroup.
+ * wt2_ctx_tiles reports the same two arrays.  No counterpart in the
+ * reference. */
+int wt2_ctx_set_tiles(wt2_ctx* ctx, const int32_t* nb3, const int32_t* kind3);
 
 /* Diagnostics (no counterpart in the reference; used by bench.py to time one
  * kernel in isolation with events on the context stream): enqueue, without

@@ -264,9 +264,10 @@ def test_autotuned_tiles_give_the_same_evolution():
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_every_tile_choice_gives_the_same_evolution(prec):
-    """Both pass kernels (LDS-free and LDS-staged) on every axis, including the
-    x pass that carries the RK4 update, with two tile sizes: 3 RK4 steps with
-    KO against the oracle (FP64 <= 1e-12; mixed H-norm <= 1e-5)."""
+    """Every pass kernel kind (LDS-staged, LDS-free with shuffles, LDS-free with
+    the exchange through LDS) on every axis, including the x pass that carries
+    the RK4 update, with two tile sizes: 3 RK4 steps with KO against the oracle
+    (FP64 <= 1e-12; mixed H-norm <= 1e-5)."""
     import itertools
     import torch
     _say(f"tile choices {prec}")
@@ -278,9 +279,10 @@ def test_every_tile_choice_gives_the_same_evolution(prec):
     dt = 0.2 * o.hmin
     uo, vo = o.rk4(u0, v0, dt, 3)
     H = o.hphys()
-    for lds in itertools.product([0, 1], repeat=3):
+    for kinds in itertools.product(["lds", "shfl", "xchg"], repeat=3):
         for nb in ((1, 1, 1), (3, 2, 6)):
-            ws.set_tiles(nb, tuple(1 - q for q in lds))
+            lds = kinds
+            ws.set_tiles(nb, kinds)
             u = torch.from_numpy(u0).cuda(); v = torch.from_numpy(v0).cuda()
             ws.rk4_steps(u, v, dt, 3)
             uh = u.cpu().numpy()

@@ -276,6 +276,17 @@ static void fill_axis_ops(const wt2_ctx* c, int axis, const double bv[6], AxisOp
     o.gR = bv[2 * axis + 1];
 }
 
+// A pass-kernel choice for axis a: kind 0 = LDS-staged, 1 = LDS-free with
+// shuffles, 2 = LDS-free with the neighbour exchange through LDS.
+static bool tiles_ok(const wt2_ctx* c, int a, int nb, int kind)
+{
+    const int N = c->N, Ma = c->d.M[a];
+    if (nb < 1 || N % nb != 0 || kind < 0 || kind > 2) return false;
+    if (kind == 2) return N * nb * Ma <= 512;
+    const int lpw = 64 / Ma;
+    return (N * nb + lpw - 1) / lpw <= (kind == 1 || !is_mixed(c) ? 8 : 16);
+}
+
 // Default element planes per tile: the smallest divisor nb of N that gives a
 // workgroup of at least `want` wavefronts.
 static int default_nb(int N, int Ma, int want)
@@ -684,6 +695,12 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
         if (env_direct && a > 0) c->direct[a] = std::atoi(env_direct) != 0;
         const int want = c->direct[a] ? (a == 0 ? 4 : 8) : (a == 2 ? 8 : 4);
         c->nb[a] = default_nb(N, desc->M[a], want);
+        if (a > 0 && !env_direct) {
+            // y and z: LDS-free with the exchange through LDS (coalesced lanes),
+            // the largest tile that fits 512 work-items
+            for (int nb = N; nb >= 1; --nb)
+                if (N % nb == 0 && N * nb * desc->M[a] <= 512) { c->direct[a] = 2; c->nb[a] = nb; break; }
+        }
         const char* env = std::getenv("WT2_TILE_NB");
         if (env) {
             int t[3];
@@ -980,9 +997,9 @@ int wt2_ctx_autotune(wt2_ctx* c, double* u, double* ud, double dt, int32_t flags
         float tbest = 1e30f;
         int bnb = c->nb[a], bdir = c->direct[a];
         const int lpw = 64 / c->d.M[a];
-        for (int dir = 1; dir >= 0; --dir) {
+        for (int dir = 2; dir >= 0; --dir) {
             for (int nb = 1; nb <= N; ++nb) {
-                if (N % nb || (N * nb + lpw - 1) / lpw > 8) continue;
+                if (!tiles_ok(c, a, nb, dir)) continue;
                 c->direct[a] = dir;
                 c->nb[a] = nb;
                 rc = run_stage(c, 2, nullptr, nullptr, nullptr, nullptr, u, ud, dt, a);
@@ -1016,12 +1033,14 @@ int wt2_ctx_set_tiles(wt2_ctx* c, const int32_t* nb3, const int32_t* direct3)
     for (int a = 0; a < 3; ++a) {
         const int nb = nb3[a];
         const int lpw = 64 / c->d.M[a];
-        if (nb < 1 || c->N % nb != 0 || (c->N * nb + lpw - 1) / lpw > 8) {
-            set_error("wt2_ctx_set_tiles: planes per tile must divide N and give at most 8 wavefronts");
+        if (!tiles_ok(c, a, nb, direct3[a])) {
+            set_error("wt2_ctx_set_tiles: kernel kind must be 0, 1 or 2 and planes per tile must divide N "
+                      "and fit one workgroup (8 wavefronts; 16 for the LDS-staged kernel in mixed precision)");
             return WT2_ERR_ARG;
         }
+        (void)lpw;
     }
-    for (int a = 0; a < 3; ++a) { c->nb[a] = nb3[a]; c->direct[a] = direct3[a] != 0; }
+    for (int a = 0; a < 3; ++a) { c->nb[a] = nb3[a]; c->direct[a] = direct3[a]; }
     if (c->gvalid) { (void)hipGraphExecDestroy(c->gexec); c->gvalid = false; }
     return WT2_OK;
 }
@@ -1117,7 +1136,7 @@ static std::string pass_key(const PassDesc& p)
 {
     char buf[96];
     std::snprintf(buf, sizeof(buf), "%d/%d/%d/%d/%d/%d/%d/%d", p.N, p.variant, p.axis,
-                  p.direct ? 1 : 0, p.epi, p.lap, p.ko, p.needv);
+                  p.direct, p.epi, p.lap, p.ko, p.needv);
     return buf;
 }
 

@@ -68,18 +68,51 @@ struct KArgs {
     int lap, ko, needv, keepv;
 };
 
+// Face values of the neighbouring elements on the same line: the low
+// neighbour's last value and the high neighbour's first value.
+// ShflX: the line's elements sit in lanes of this wavefront (shuffles).
+template <typename TC>
+struct ShflX {
+    int srcL, srcR;
+    __device__ __forceinline__ void operator()(TC last, TC first, TC& fromL, TC& fromR)
+    {
+        fromL = shfl64(last, srcL);
+        fromR = shfl64(first, srcR);
+    }
+};
+
+// LdsX: the line's elements are spread over the workgroup; values go through
+// LDS, one barrier per exchange (two alternating buffers, so a buffer is only
+// rewritten after every work-item has passed the barrier that follows its
+// reads).  Every work-item of the workgroup must make the same calls.
+template <typename TC>
+struct LdsX {
+    TC* buf;           // 2 buffers x (last, first) x n
+    int n, me, idxL, idxR, k;
+    __device__ __forceinline__ void operator()(TC last, TC first, TC& fromL, TC& fromR)
+    {
+        TC* b = buf + (k & 1) * 2 * n;
+        b[me] = last;
+        b[n + me] = first;
+        __syncthreads();
+        fromL = b[idxL];
+        fromR = b[n + idxR];
+        ++k;
+    }
+};
+
 // x <- (kos D)^6 x on this lane's element segment; D = (1/h) G plus the
 // centred-flux SAT (apply_D!, METHODS.md:256-264); an outer face takes the same
 // SAT against a zero ghost value, which keeps H D skew on a bounded line.
-template <int N, typename TC>
+template <int N, typename TC, typename X>
 __device__ __forceinline__ void ko_chain(TC (&x)[N], const TC* __restrict__ op,
-                                         const AxisConst<N, TC>& k, int srcL, int srcR,
+                                         const AxisConst<N, TC>& k, X& xch,
                                          bool hasL, bool hasR)
 {
 #pragma unroll 1
     for (int rep = 0; rep < 6; ++rep) {
-        TC xl = shfl64(x[N - 1], srcL);
-        TC xr = shfl64(x[0], srcR);
+        TC xl, xr;
+        xch(x[N - 1], x[0], xl, xr);
         if (!hasL) xl = TC(0);
         if (!hasR) xr = TC(0);
         TC y[N];
@@ -143,18 +176,17 @@ __device__ __forceinline__ void sipg_seg(const TC (&u)[N], const TC (&g)[N], TC 
     }
 }
 
-// The same with the neighbour values taken from the adjacent lanes of a line.
-template <int N, typename TC>
+// The same with the neighbour values exchanged along the line.
+template <int N, typename TC, typename X>
 __device__ __forceinline__ void sipg_add(const TC (&u)[N], TC (&out)[N], const TC* __restrict__ op,
-                                         const AxisConst<N, TC>& k, int srcL, int srcR,
+                                         const AxisConst<N, TC>& k, X& xch,
                                          bool hasL, bool hasR)
 {
     TC g[N];
     apply_G<N, TC>(op, u, g);
-    const TC uL = shfl64(u[N - 1], srcL);
-    const TC gnL = shfl64(g[N - 1], srcL);
-    const TC uR = shfl64(u[0], srcR);
-    const TC gnR = shfl64(g[0], srcR);
+    TC uL, gnL, uR, gnR;
+    xch(u[N - 1], u[0], uL, uR);
+    xch(g[N - 1], g[0], gnL, gnR);
     sipg_seg<N, TC>(u, g, out, op, k, uL, gnL, uR, gnR, hasL, hasR);
 }
 
@@ -243,7 +275,8 @@ __device__ __forceinline__ void flat_pass_body(const KArgs<N, TC>& a, const int 
         TC x[N];
 #pragma unroll
         for (int r = 0; r < N; ++r) x[r] = TC(lds[sv * fstride + base + r * SR]);
-        ko_chain<N, TC>(x, op, a.k, srcL, srcR, hasL, hasR);
+        ShflX<TC> xs{srcL, srcR};
+        ko_chain<N, TC>(x, op, a.k, xs, hasL, hasR);
 #pragma unroll
         for (int r = 0; r < N; ++r) cv[r] = x[r];
     }
@@ -251,7 +284,8 @@ __device__ __forceinline__ void flat_pass_body(const KArgs<N, TC>& a, const int 
         TC u[N];
 #pragma unroll
         for (int r = 0; r < N; ++r) u[r] = TC(lds[su * fstride + base + r * SR]);
-        sipg_add<N, TC>(u, cv, op, a.k, srcL, srcR, hasL, hasR);
+        ShflX<TC> xs{srcL, srcR};
+        sipg_add<N, TC>(u, cv, op, a.k, xs, hasL, hasR);
     }
     __syncthreads();
     if (active) {
@@ -397,12 +431,18 @@ __device__ __forceinline__ void store_seg_s(TD* __restrict__ p, int gs, const TC
 // accesses -- no LDS staging and no workgroup barrier -- and the whole RK4
 // epilogue runs on register-resident segments.  It is the last pass of an
 // RHS evaluation (passes run z, y, x).
-template <int N, typename TC, typename TS, typename TP>
+// XCH = false: a wavefront holds lpw whole lines, the line's elements in
+// adjacent lanes (neighbour values by shuffles).  XCH = true: work-item t takes
+// line t % LT of element t / LT, so the lanes of a wavefront are adjacent
+// lines of one element (coalesced segment accesses) and neighbour values go
+// through LDS (LT * Ma <= 512).
+template <int N, typename TC, typename TS, typename TP, bool XCH>
 __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int tile)
 {
     constexpr int N3 = N * N * N;
     constexpr bool MIXED = std::is_same<TP, float>::value;
     __shared__ TC op[OpLayout<N>::SIZE];
+    __shared__ TC xbuf[XCH ? 4 * 512 : 1];
     for (int q = threadIdx.x; q < OpLayout<N>::SIZE; q += blockDim.x) op[q] = a.op[q];
     __syncthreads();
 
@@ -412,16 +452,33 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
     const int ebase = (t % a.T0) * a.slo + (t / a.T0) * a.shi;
     const int Ma = a.Ma;
     const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    const int lpw = a.lpw;
-    const int m = lane / lpw, lw = lane - m * lpw;
-    const int line = wave * lpw + lw;               // (j, k in block) = (line % N, line / N)
+    int m, line, lidxL = 0, lidxR = 0;
+    if (XCH) {
+        m = tid / a.LT;
+        line = tid - m * a.LT;
+    } else {
+        const int wave = tid >> 6, lane = tid & 63;
+        m = lane / a.lpw;
+        line = wave * a.lpw + (lane - m * a.lpw);   // (j, k in block) = (line % N, line / N)
+    }
     const bool active = (m < Ma) && (line < a.LT);
     const bool per = a.periodic != 0;
     const bool hasL = active && (m > 0 || per);
     const bool hasR = active && (m < Ma - 1 || per);
-    const int srcL = hasL ? ((m > 0 ? m - 1 : Ma - 1) * lpw + lw) : lane;
-    const int srcR = hasR ? ((m < Ma - 1 ? m + 1 : 0) * lpw + lw) : lane;
+    if (XCH) {
+        lidxL = hasL ? (m > 0 ? m - 1 : Ma - 1) * a.LT + line : tid;
+        lidxR = hasR ? (m < Ma - 1 ? m + 1 : 0) * a.LT + line : tid;
+    } else {
+        const int lane = tid & 63, lw = lane - (lane / a.lpw) * a.lpw;
+        lidxL = hasL ? ((m > 0 ? m - 1 : Ma - 1) * a.lpw + lw) : lane;
+        lidxR = hasR ? ((m < Ma - 1 ? m + 1 : 0) * a.lpw + lw) : lane;
+    }
+    typename std::conditional<XCH, LdsX<TC>, ShflX<TC>>::type xch;
+    if constexpr (XCH) {
+        xch.buf = xbuf; xch.n = 512; xch.me = tid; xch.idxL = lidxL; xch.idxR = lidxR; xch.k = 0;
+    } else {
+        xch.srcL = lidxL; xch.srcR = lidxR;
+    }
     const int mm = active ? m : 0;
     const int ll = active ? line : 0;
     const int64_t g0 = int64_t(ebase + mm * a.estride) * N3 + int64_t(blk) * a.CO +
@@ -438,11 +495,11 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
         TC x[N];
 #pragma unroll
         for (int r = 0; r < N; ++r) x[r] = TC(v[r]);
-        ko_chain<N, TC>(x, op, a.k, srcL, srcR, hasL, hasR);
+        ko_chain<N, TC>(x, op, a.k, xch, hasL, hasR);
 #pragma unroll
         for (int r = 0; r < N; ++r) cv[r] = x[r];
     }
-    if (a.lap) sipg_add<N, TC>(u, cv, op, a.k, srcL, srcR, hasL, hasR);
+    if (a.lap) sipg_add<N, TC>(u, cv, op, a.k, xch, hasL, hasR);
     if (!active) return;
 
     TP* P = static_cast<TP*>(a.P);
@@ -544,32 +601,37 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
 // Kernel entry points: one context (descriptor in the kernel arguments) or a
 // batch (descriptor table indexed through a packed (instance, tile) work list,
 // one launch covering every instance of the same N and precision).
+// Workgroup size limit of the LDS-staged kernel: 16 wavefronts in FP32, 8 in
+// FP64 (whose N = 10 segments would spill under the 128-register cap of 16).
+template <typename TC>
+struct LdsMaxThreads { static constexpr int value = std::is_same<TC, float>::value ? 1024 : 512; };
+
 template <int N, typename TC, typename TS, typename TP>
-__global__ void __launch_bounds__(512) flat_pass_kernel(KArgs<N, TC> a)
+__global__ void __launch_bounds__(LdsMaxThreads<TC>::value) flat_pass_kernel(KArgs<N, TC> a)
 {
     flat_pass_body<N, TC, TS, TP>(a, blockIdx.x);
 }
 
-template <int N, typename TC, typename TS, typename TP>
+template <int N, typename TC, typename TS, typename TP, bool XCH>
 __global__ void __launch_bounds__(512) flat_xpass_kernel(KArgs<N, TC> a)
 {
-    flat_xpass_body<N, TC, TS, TP>(a, blockIdx.x);
+    flat_xpass_body<N, TC, TS, TP, XCH>(a, blockIdx.x);
 }
 
 template <int N, typename TC, typename TS, typename TP>
-__global__ void __launch_bounds__(512) flat_pass_batch_kernel(const KArgs<N, TC>* __restrict__ tab,
+__global__ void __launch_bounds__(LdsMaxThreads<TC>::value) flat_pass_batch_kernel(const KArgs<N, TC>* __restrict__ tab,
                                                               const int2* __restrict__ work)
 {
     const int2 w = work[blockIdx.x];
     flat_pass_body<N, TC, TS, TP>(tab[w.x], w.y);
 }
 
-template <int N, typename TC, typename TS, typename TP>
+template <int N, typename TC, typename TS, typename TP, bool XCH>
 __global__ void __launch_bounds__(512) flat_xpass_batch_kernel(const KArgs<N, TC>* __restrict__ tab,
                                                                const int2* __restrict__ work)
 {
     const int2 w = work[blockIdx.x];
-    flat_xpass_body<N, TC, TS, TP>(tab[w.x], w.y);
+    flat_xpass_body<N, TC, TS, TP, XCH>(tab[w.x], w.y);
 }
 
 // ------------------------------------------------------------------ launch
@@ -655,9 +717,10 @@ static hipError_t make_kargs(const PassDesc& p, KArgs<N, TC>& a, int* tiles_out,
         if (pad == 0 && want == 0) pad = 1;
         a.ES = es + pad;
     }
-    const int waves = (a.LT + a.lpw - 1) / a.lpw;
+    const int waves = p.direct == 2 ? (a.LT * a.Ma + 63) / 64 : (a.LT + a.lpw - 1) / a.lpw;
     const int nthr = waves * 64;
-    if (nthr > 512) return hipErrorInvalidConfiguration;
+    // LDS-free tiles: at most 8 wavefronts (register budget); LDS-staged: 16
+    if (nthr > (p.direct ? 512 : LdsMaxThreads<TC>::value)) return hipErrorInvalidConfiguration;
     const int fstride = a.Ma * a.ES;
     const int nslots = (p.lap && p.needv) || (!p.lap && p.keepv) ? 2 : 1;
     const size_t shmem = size_t(fstride) * nslots * sizeof(TS);
@@ -680,10 +743,12 @@ static hipError_t launch_t(const PassDesc& p, hipStream_t s)
     if (e0 != hipSuccess) return e0;
     const size_t static_lds = static_lds_bound<N, TC>();
     if (p.direct) {
-        // LDS-free pass
-        auto xk = flat_xpass_kernel<N, TC, TS, TP>;
+        // LDS-free pass (neighbour values by shuffles, or through LDS)
         (void)hipGetLastError();
-        hipLaunchKernelGGL(xk, dim3(tiles), dim3(nthr), 0, s, a);
+        if (p.direct == 2)
+            hipLaunchKernelGGL((flat_xpass_kernel<N, TC, TS, TP, true>), dim3(tiles), dim3(nthr), 0, s, a);
+        else
+            hipLaunchKernelGGL((flat_xpass_kernel<N, TC, TS, TP, false>), dim3(tiles), dim3(nthr), 0, s, a);
         return hipGetLastError();
     }
     auto kern = flat_pass_kernel<N, TC, TS, TP>;
@@ -1019,8 +1084,12 @@ static hipError_t batch_launch_t(const FlatBatchPlan& plan, hipStream_t s)
     const KArgs<N, TC>* tab = static_cast<const KArgs<N, TC>*>(plan.tab);
     if (plan.direct) {
         (void)hipGetLastError();
-        hipLaunchKernelGGL((flat_xpass_batch_kernel<N, TC, TS, TP>), dim3(plan.ntiles), dim3(plan.nthr), 0, s,
-                           tab, plan.work);
+        if (plan.direct == 2)
+            hipLaunchKernelGGL((flat_xpass_batch_kernel<N, TC, TS, TP, true>), dim3(plan.ntiles), dim3(plan.nthr),
+                               0, s, tab, plan.work);
+        else
+            hipLaunchKernelGGL((flat_xpass_batch_kernel<N, TC, TS, TP, false>), dim3(plan.ntiles), dim3(plan.nthr),
+                               0, s, tab, plan.work);
         return hipGetLastError();
     }
     auto kern = flat_pass_batch_kernel<N, TC, TS, TP>;

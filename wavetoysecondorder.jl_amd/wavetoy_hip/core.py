@@ -143,6 +143,11 @@ def ctx_footprint(mesh, elem, precision="fp64", with_background=False):
     return b.value
 
 
+# pass kernel kinds: LDS-staged tile; LDS-free with shuffles; LDS-free with the
+# neighbour exchange through LDS
+KERNEL_KINDS = ("lds", "shfl", "xchg")
+
+
 class Workspace:
     """A libwt2 context: mesh + operators + geometry + device workspace + stream
     (the reference's geom/ops/work bundle after to_device)."""
@@ -283,12 +288,13 @@ class Workspace:
         nb = (ctypes.c_int32 * 3)()
         d = (ctypes.c_int32 * 3)()
         check(lib().wt2_ctx_tiles(self.ctx, ctypes.cast(nb, ctypes.c_void_p), ctypes.cast(d, ctypes.c_void_p)))
-        return {"planes_per_tile": tuple(nb), "lds_free": tuple(bool(x) for x in d)}
+        return {"planes_per_tile": tuple(nb), "kernel": tuple(KERNEL_KINDS[x] for x in d)}
 
-    def set_tiles(self, planes_per_tile, lds_free):
-        """Choose the pass kernel (lds_free[a]) and planes per tile of each axis."""
+    def set_tiles(self, planes_per_tile, kernel):
+        """Choose the pass kernel ("lds", "shfl" or "xchg", or 0/1/2) and the
+        element planes per tile of each axis."""
         nb = (ctypes.c_int32 * 3)(*[int(x) for x in planes_per_tile])
-        d = (ctypes.c_int32 * 3)(*[int(bool(x)) for x in lds_free])
+        d = (ctypes.c_int32 * 3)(*[KERNEL_KINDS.index(x) if isinstance(x, str) else int(x) for x in kernel])
         check(lib().wt2_ctx_set_tiles(self.ctx, ctypes.cast(nb, ctypes.c_void_p), ctypes.cast(d, ctypes.c_void_p)))
 
     def energy(self, u, ud):
