@@ -336,6 +336,66 @@ def serving_run(wh, torch, args, rates=None):
     return out
 
 
+KIND_NAMES = ("lds", "shfl", "xchg")
+
+
+def passes_run(wh, torch, args):
+    """Each pencil-pass kernel kind and tile size of each axis, stage 2 of the
+    RK4 step, timed alone (10 launches, HIP events on the pass stream): the
+    plain workload (32^3 x 8^3, FP64 and mixed) and the KO-only passes of the
+    curved workload (24^3 x 10^3).  With --no-sweep: the default tiles only,
+    each pass launched 3 times."""
+    out = {}
+    cases = [(8, 32, "fp64", False), (8, 32, "mixed", False), (10, 24, "fp64", True), (10, 24, "mixed", True)]
+    for N, M, prec, curved in cases:
+        mesh = (wh.make_warped_uniform_hex(M, M, M, 0.0, 1.0, 0.05) if curved
+                else wh.make_uniform_hex(M, 0.0, 1.0))
+        elem = wh.make_element(N)
+        ws = wh.Workspace(mesh, elem, tau=wh.tau_effective(TAU_LITERAL, N), eps_ko=EPS_KO, precision=prec)
+        st = torch.cuda.Stream()
+        ws.set_stream(st.cuda_stream)
+        gen = torch.Generator(device="cuda").manual_seed(args.seed)
+        u = torch.randn(ws.npoints, dtype=torch.float64, device="cuda", generator=gen) * 1e-3
+        v = torch.randn(ws.npoints, dtype=torch.float64, device="cuda", generator=gen) * 1e-3
+        dt = 0.2 * ws.h_min
+        ws.rk4_steps(u, v, dt, 1)           # initialise the stage registers
+        torch.cuda.synchronize()
+        base = ws.tiles()
+        key = f"N{N}_M{M}_{prec}{'_curved_ko' if curved else ''}"
+        res = {"default": base}
+        for axis in (2, 1, 0):
+            if args.no_sweep:
+                for _ in range(3):
+                    ws.launch_pass(axis, 2, u, v, dt)
+                continue
+            times = {}
+            for kind in (2, 1, 0):
+                for nb in range(1, N + 1):
+                    nbs = list(base["planes_per_tile"])
+                    ks = [KIND_NAMES.index(k) for k in base["kernel"]]
+                    nbs[axis] = nb
+                    ks[axis] = kind
+                    try:
+                        ws.set_tiles(nbs, ks)
+                    except wh.WT2Error:
+                        continue                # not a valid tile for this shape
+                    ws.launch_pass(axis, 2, u, v, dt)
+                    a = torch.cuda.Event(enable_timing=True)
+                    b = torch.cuda.Event(enable_timing=True)
+                    a.record(st)
+                    for _ in range(10):
+                        ws.launch_pass(axis, 2, u, v, dt)
+                    b.record(st)
+                    torch.cuda.synchronize()
+                    times[f"{KIND_NAMES[kind]}{nb}"] = round(a.elapsed_time(b) / 10 * 1e3, 1)
+            res["xyz"[axis]] = times
+        torch.cuda.synchronize()
+        ws.close()
+        out[key] = res
+    return {"config": {"workload": "pencil passes of RK4 stage 2, each kernel kind and tile (us per launch)"},
+            "passes": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -345,11 +405,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default tile shapes instead of the autotuner's")
-    ap.add_argument("--config", choices=["plain", "curved", "batch", "serving"], default="plain",
+    ap.add_argument("--config", choices=["plain", "curved", "batch", "serving", "passes"], default="plain",
                     help="plain: the FP64 flat workload (headline); curved: the warped-mesh "
                          "ADM workload, FP64 vs mixed, error at --t-final")
     ap.add_argument("--instances", type=int, default=512, help="batch config: number of instances")
     ap.add_argument("--requests", type=int, default=64, help="serving config: requests per rate")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="passes config: only the default tiles, each pass launched 3 times (for counter runs)")
     ap.add_argument("--rates", default="50,100,200,300,500", help="serving config: arrival rates (1/s)")
     ap.add_argument("--t-final", type=float, default=None,
                     help="curved config: evolve to this time (default: --steps steps)")
@@ -381,6 +443,9 @@ def main():
         return
     if args.config == "serving":
         print(json.dumps(serving_run(wh, torch, args)), flush=True)
+        return
+    if args.config == "passes":
+        print(json.dumps(passes_run(wh, torch, args)), flush=True)
         return
 
     mesh = wh.make_uniform_hex(M_ELEMS, 0.0, 1.0)

@@ -291,3 +291,20 @@ def test_every_tile_choice_gives_the_same_evolution(prec):
             else:
                 d = uh - uo
                 assert math.sqrt(np.sum(H * d * d) / np.sum(H * uo * uo)) <= 1e-5, (lds, nb)
+
+
+@pytest.mark.parametrize("kind", ["shfl", "xchg"])
+def test_exchange_kernels_drop_in_and_rhs_outputs(kind):
+    """The drop-in RHS output and the (u', u_t') output epilogues through the
+    exchange kernels (x pass kind set explicitly), against the oracle."""
+    _say(f"rhs outputs {kind}")
+    N, M = 5, (6, 2, 3)
+    ws, o = _pair(N, M, eps_ko=0.1, periodic=(True, False, False))
+    ws.set_tiles((1, 1, 1), (kind, "xchg", "xchg"))
+    x = o.coords()
+    u = np.cos(2 * math.pi * x[:, 0]) * np.sin(math.pi * x[:, 1]) + x[:, 2] ** 2
+    v = np.sin(2 * math.pi * x[:, 0]) * x[:, 1]
+    du_ref, dv_ref = o.rhs(u, v)
+    du = np.zeros(o.n); dv = np.zeros(o.n)
+    ws.rhs_full(du, dv, u, v)
+    assert relerr(du, du_ref) <= TOL64 and relerr(dv, dv_ref) <= TOL64
