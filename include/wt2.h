@@ -74,6 +74,8 @@ typedef struct wt2_info {
     double  tau;             /* penalty in use                               */
     double  max_sym_eig;     /* max eigenvalue of sym(H L) of the 1D operator */
     int64_t device_bytes;    /* device memory held by the context            */
+    int32_t launches_per_step; /* kernel launches of one RK4 step (8 folded, 12 flat, curved more) */
+    int32_t reserved;
 } wt2_info;
 
 /* GLL nodes xi on [0,1], weights w (sum 1) and differentiation matrix

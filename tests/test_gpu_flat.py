@@ -308,3 +308,41 @@ def test_exchange_kernels_drop_in_and_rhs_outputs(kind):
     du = np.zeros(o.n); dv = np.zeros(o.n)
     ws.rhs_full(du, dv, u, v)
     assert relerr(du, du_ref) <= TOL64 and relerr(dv, dv_ref) <= TOL64
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_folded_schedule(prec, monkeypatch):
+    """The opt-in folded RK4 schedule (WT2_FOLD=1: 8 passes per step, the first
+    pass of each stage computed by the previous stage's RK pass): 6 steps with
+    KO against the oracle, and a batch of it equal to the single run."""
+    import torch
+    _say(f"folded schedule {prec}")
+    monkeypatch.setenv("WT2_FOLD", "1")
+    N, M = 6, (5, 3, 4)
+    ws, o = _pair(N, M, eps_ko=0.1, precision=prec)
+    assert ws.info()["launches_per_step"] == 8
+    x = o.coords()
+    u0 = np.exp(-np.sum((x - 0.5) ** 2, axis=1) / 0.02)
+    v0 = 0.1 * np.sin(math.pi * x[:, 0])
+    dt = 0.2 * o.hmin
+    uo, vo = o.rk4(u0, v0, dt, 6)
+    u = torch.from_numpy(u0).cuda(); v = torch.from_numpy(v0).cuda()
+    E = ws.rk4_steps(u, v, dt, 6, energy_every=2)
+    uh = u.cpu().numpy()
+    if prec == "fp64":
+        assert relerr(uh, uo) <= TOL64 and relerr(v.cpu().numpy(), vo) <= TOL64
+    else:
+        H = o.hphys()
+        d = uh - uo
+        assert math.sqrt(np.sum(H * d * d) / np.sum(H * uo * uo)) <= 1e-5
+    # the same context in a batch: bitwise the single run
+    u2 = torch.from_numpy(u0).cuda(); v2 = torch.from_numpy(v0).cuda()
+    b = wh.Batch()
+    slot = b.add(ws, u2, v2, dt, 6, energy_every=2)
+    for _ in range(4):
+        if b.step(4):
+            break
+    done, E2 = b.slot_info(slot)
+    b.remove(slot)
+    b.close()
+    assert done == 6 and np.array_equal(u2.cpu().numpy(), uh) and np.array_equal(E2, E)

@@ -65,7 +65,7 @@ struct KArgs {
     int LT, lpw;       // lines per tile, lines per wavefront
     int GS;            // node stride of a segment in memory (LDS-free kernel)
     int periodic, epi;
-    int lap, ko, needv, keepv;
+    int lap, ko, needv, keepv, fold;
 };
 
 // Face values of the neighbouring elements on the same line: the low
@@ -500,25 +500,30 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
         for (int r = 0; r < N; ++r) cv[r] = x[r];
     }
     if (a.lap) sipg_add<N, TC>(u, cv, op, a.k, xch, hasL, hasR);
-    if (!active) return;
+    const int epi0 = a.epi;
+    const bool fold = a.fold && epi0 >= EPI_RK1;
+    // a folding pass with the LDS exchange needs every work-item at its barriers
+    if (!active && !(XCH && fold)) return;
 
     TP* P = static_cast<TP*>(a.P);
     const int epi = a.epi;
-    if (epi == EPI_FIRST) { store_seg_s<N, TP, TC>(P + g0, gs, cv); return; }
+    if (epi == EPI_FIRST) { if (active) store_seg_s<N, TP, TC>(P + g0, gs, cv); return; }
     TC pv[N];
     load_seg_s<N, TP, TC>(P + g0, gs, pv);
 #pragma unroll
     for (int r = 0; r < N; ++r) cv[r] += pv[r];          // u_t' = P + c
-    if (epi == EPI_MID) { store_seg_s<N, TP, TC>(P + g0, gs, cv); return; }
-    if (epi == EPI_OUT_LAP) { store_seg_s<N, double, TC>(a.Ou + g0, gs, cv); return; }
+    if (epi == EPI_MID) { if (active) store_seg_s<N, TP, TC>(P + g0, gs, cv); return; }
+    if (epi == EPI_OUT_LAP) { if (active) store_seg_s<N, double, TC>(a.Ou + g0, gs, cv); return; }
     if (epi == EPI_OUT_RHS) {
-        store_seg_s<N, double, TS>(a.Ou + g0, gs, v);            // u' = u_t
-        store_seg_s<N, double, TC>(a.Ov + g0, gs, cv);
+        if (active) store_seg_s<N, double, TS>(a.Ou + g0, gs, v);            // u' = u_t
+        if (active) store_seg_s<N, double, TC>(a.Ov + g0, gs, cv);
         return;
     }
-    // classical RK4 stage update, k = (u_t, P + c)
+    // classical RK4 stage update, k = (u_t, P + c); nu, nv: the next stage's
+    // input (S or, after stage 4, the new Y) for the fold
     const double dt = a.dt;
     double yu[N], yv[N];
+    double nu[N], nv[N];
     if (epi != EPI_RK4 || MIXED) {
         load_seg_s<N, double, double>(a.Yu + g0, gs, yu);
         load_seg_s<N, double, double>(a.Yv + g0, gs, yv);
@@ -546,10 +551,12 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
                 au[r] = du0[r] + wgt * ku;
                 av[r] = dv0[r] + wgt * kv;
             }
-            store_seg_s<N, float, double>(static_cast<float*>(a.So_u) + g0, gs, su);
-            store_seg_s<N, float, double>(static_cast<float*>(a.So_v) + g0, gs, sv);
-            store_seg_s<N, float, double>(du, gs, au);
-            store_seg_s<N, float, double>(dv, gs, av);
+            if (active) store_seg_s<N, float, double>(static_cast<float*>(a.So_u) + g0, gs, su);
+            if (active) store_seg_s<N, float, double>(static_cast<float*>(a.So_v) + g0, gs, sv);
+            if (active) store_seg_s<N, float, double>(du, gs, au);
+            if (active) store_seg_s<N, float, double>(dv, gs, av);
+#pragma unroll
+            for (int r = 0; r < N; ++r) { nu[r] = su[r]; nv[r] = sv[r]; }
         } else {
             double du0[N], dv0[N];
             load_seg_s<N, float, double>(du, gs, du0);
@@ -559,8 +566,10 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
                 yu[r] = yu[r] + (du0[r] + dt / 6.0 * double(v[r]));
                 yv[r] = yv[r] + (dv0[r] + dt / 6.0 * double(cv[r]));
             }
-            store_seg_s<N, double, double>(a.Yu + g0, gs, yu);
-            store_seg_s<N, double, double>(a.Yv + g0, gs, yv);
+            if (active) store_seg_s<N, double, double>(a.Yu + g0, gs, yu);
+            if (active) store_seg_s<N, double, double>(a.Yv + g0, gs, yv);
+#pragma unroll
+            for (int r = 0; r < N; ++r) { nu[r] = yu[r]; nv[r] = yv[r]; }
         }
     } else {
         double* Au = static_cast<double*>(a.Au) + g0;
@@ -574,8 +583,10 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
                 yu[r] = au[r] + dt / 6.0 * double(v[r]);
                 yv[r] = av[r] + dt / 6.0 * double(cv[r]);
             }
-            store_seg_s<N, double, double>(a.Yu + g0, gs, yu);
-            store_seg_s<N, double, double>(a.Yv + g0, gs, yv);
+            if (active) store_seg_s<N, double, double>(a.Yu + g0, gs, yu);
+            if (active) store_seg_s<N, double, double>(a.Yv + g0, gs, yv);
+#pragma unroll
+            for (int r = 0; r < N; ++r) { nu[r] = yu[r]; nv[r] = yv[r]; }
         } else {
             const double c = epi == EPI_RK3 ? dt : 0.5 * dt;
             double su[N], sv[N];
@@ -590,12 +601,30 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
             }
 #pragma unroll
             for (int r = 0; r < N; ++r) { su[r] = yu[r] + c * double(v[r]); sv[r] = yv[r] + c * double(cv[r]); }
-            store_seg_s<N, double, double>(static_cast<double*>(a.So_u) + g0, gs, su);
-            store_seg_s<N, double, double>(static_cast<double*>(a.So_v) + g0, gs, sv);
-            store_seg_s<N, double, double>(Au, gs, au);
-            store_seg_s<N, double, double>(Av, gs, av);
+            if (active) store_seg_s<N, double, double>(static_cast<double*>(a.So_u) + g0, gs, su);
+            if (active) store_seg_s<N, double, double>(static_cast<double*>(a.So_v) + g0, gs, sv);
+            if (active) store_seg_s<N, double, double>(Au, gs, au);
+            if (active) store_seg_s<N, double, double>(Av, gs, av);
+#pragma unroll
+            for (int r = 0; r < N; ++r) { nu[r] = su[r]; nv[r] = sv[r]; }
         }
     }
+    if (!fold) return;
+    // the next stage's contribution along this axis, on its input exactly as
+    // the next stage's passes convert it (TC of the stored value)
+    TC fu[N], fc[N];
+#pragma unroll
+    for (int r = 0; r < N; ++r) { fu[r] = TC(nu[r]); fc[r] = TC(0); }
+    if (a.ko) {
+        TC x[N];
+#pragma unroll
+        for (int r = 0; r < N; ++r) x[r] = TC(nv[r]);
+        ko_chain<N, TC>(x, op, a.k, xch, hasL, hasR);
+#pragma unroll
+        for (int r = 0; r < N; ++r) fc[r] = x[r];
+    }
+    if (a.lap) sipg_add<N, TC>(fu, fc, op, a.k, xch, hasL, hasR);
+    if (active) store_seg_s<N, TP, TC>(P + g0, gs, fc);
 }
 
 // Kernel entry points: one context (descriptor in the kernel arguments) or a
@@ -678,7 +707,8 @@ static hipError_t make_kargs(const PassDesc& p, KArgs<N, TC>& a, int* tiles_out,
     a.Au = p.Au; a.Av = p.Av; a.Ou = p.Ou; a.Ov = p.Ov;
     a.dt = p.dt;
     a.periodic = p.periodic; a.epi = p.epi;
-    a.lap = p.lap; a.ko = p.ko; a.needv = p.needv; a.keepv = p.keepv;
+    a.lap = p.lap; a.ko = p.ko; a.needv = p.needv; a.keepv = p.keepv; a.fold = p.fold;
+    if (p.fold && !p.direct) return hipErrorInvalidConfiguration;     // LDS-free kernels only
     const int Mx = p.M[0], My = p.M[1], Mz = p.M[2];
     const int nb = p.nb, nbk = N / nb;
     int tiles;

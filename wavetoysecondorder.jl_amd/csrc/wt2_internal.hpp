@@ -59,6 +59,7 @@ struct PassDesc {
     int ko;             // add eps mu^-5 D^6 u_t along this axis
     int needv;          // load u_t (KO input or kept for the final epilogue)
     int keepv;          // final epilogue: u' = u_t needs u_t after the compute
+    int fold;           // RK epilogue: also write P = this axis's contribution of the next stage
     const void* Su; const void* Sv;   // stage input (variant input type)
     void* P;                          // partial sum of the u_t equation (variant partial type)
     void* So_u; void* So_v;           // next stage input (partial type)

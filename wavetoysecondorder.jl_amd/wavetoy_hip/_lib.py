@@ -62,6 +62,8 @@ class Info(ctypes.Structure):
         ("tau", ctypes.c_double),
         ("max_sym_eig", ctypes.c_double),
         ("device_bytes", ctypes.c_int64),
+        ("launches_per_step", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
