@@ -31,6 +31,20 @@ def test_plain_cpu_configuration_full_run():
     uo, vo = o.rk4(u0, np.zeros_like(u0), r["dt"], 200)
     assert np.max(np.abs(r["u_final"] - uo)) <= 1e-12 * np.max(np.abs(uo))
     assert np.max(np.abs(r["ud_final"] - vo)) <= 1e-12 * np.max(np.abs(vo))
+    # the sampled line y = z = 1/4 (src/evolve.jl:1008-1012): distinct x, sorted
+    xs = r["xs_line"]
+    assert len(xs) == 4 * 7 + 1 and np.all(np.diff(xs) > 0)
+    assert r["y_target"] == 0.25 and r["z_target"] == 0.25
+    on = (np.abs(x[:, 1] - 0.25) < 1e-12) & (np.abs(x[:, 2] - 0.25) < 1e-12)
+    assert r["us"].shape == (len(xs), 5) and r["uds"].shape == (len(xs), 5)
+    line0 = np.sin(math.pi * xs) * math.sin(math.pi * 0.25) ** 2
+    assert np.allclose(r["us"][:, 0], line0, atol=1e-14)
+    # last sample = the final state on the line
+    xf = x[on]
+    order = np.argsort(xf[:, 0], kind="stable")   # shared face nodes: the lower element's first
+    uf_line = r["u_final"][on][order]
+    _, first = np.unique(np.round(xf[order, 0], 12), return_index=True)
+    assert np.allclose(r["us"][:, -1], uf_line[first], atol=1e-15)
 
 
 def test_convergence_order_against_exact_solution():
