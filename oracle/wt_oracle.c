@@ -185,6 +185,11 @@ wto_ctx* wto_create(const wto_desc* d, const double* bg)
     wto_gll(N, c->xi, c->w, c->G);
     for (int a = 0; a < 3; ++a) c->h[a] = (d->x1[a] - d->x0[a]) / d->M[a];
     c->curved = (d->warp_A != 0.0) || (bg != NULL);
+    /* Sommerfeld faces: affine meshes, non-periodic axes only */
+    for (int f = 0; f < 6; ++f) {
+        if (d->face_kind[f] != 0 && d->face_kind[f] != 7) { free(c); return NULL; }
+        if (d->face_kind[f] == 7 && (c->curved || d->periodic[f / 2])) { free(c); return NULL; }
+    }
     c->coords = (double*)malloc(sizeof(double) * 3 * c->npts);
     c->hphys = (double*)malloc(sizeof(double) * c->npts);
     if (c->curved) c->coef = (double*)malloc(sizeof(double) * 13 * c->npts);
@@ -402,10 +407,28 @@ static void apply_D_line1d(const wto_ctx* c, int M, int periodic, double h,
  * du = u - u_nbr; alpha = 1/2 at interior faces; at an outer Dirichlet face
  * alpha = 1, du = u - g and d(Gu) = 0 (mirror convention).
  * ul / ur: neighbour element segment (N values) or NULL at a boundary face,
- * in which case gl / gr is the Dirichlet value.  Result added into y. */
+ * in which case gl / gr is the Dirichlet value.  somL / somR: that outer face
+ * is a Sommerfeld (tag 7) face instead -- free for the SIPG terms
+ * (src/wave.jl:446-451) -- and carries the Bayliss-Turkel drag of
+ * src/wave.jl:453-518 on its face node:
+ *   y -= wF / Hphys (u_t + d_n u + u / R),   wF / Hphys = 1 / (h w_r)
+ * on an affine mesh (the tangential face weights cancel); v = NULL takes
+ * u_t = 0, which leaves the H-symmetric operator of the energy
+ * 1/2 v'Hv + 1/2 (Gu)'H(Gu) + 1/(2R) sum_faces u^2.  Result added into y. */
+static void sipg_segment_s(const wto_ctx* c, double h, const double* u,
+                           const double* ul, double gl, const double* ur, double gr,
+                           int somL, int somR, const double* v, double* y);
+
 static void sipg_segment(const wto_ctx* c, double h, const double* u,
                          const double* ul, double gl, const double* ur, double gr,
                          double* y)
+{
+    sipg_segment_s(c, h, u, ul, gl, ur, gr, 0, 0, NULL, y);
+}
+
+static void sipg_segment_s(const wto_ctx* c, double h, const double* u,
+                           const double* ul, double gl, const double* ur, double gr,
+                           int somL, int somR, const double* v, double* y)
 {
     int N = c->N;
     const double* G = c->G;
@@ -422,12 +445,14 @@ static void sipg_segment(const wto_ctx* c, double h, const double* u,
         double g_nb = 0.0;
         for (int p = 0; p < N; ++p) g_nb += G[(N - 1) * N + p] * ul[p];
         duL = u[0] - ul[N - 1]; dgL = gu[0] - g_nb; aL = 0.5;
-    } else { duL = u[0] - gl; dgL = 0.0; aL = 1.0; }
+    } else if (somL) { duL = 0.0; dgL = 0.0; aL = 0.0; }
+    else { duL = u[0] - gl; dgL = 0.0; aL = 1.0; }
     if (ur) {
         double g_nb = 0.0;
         for (int p = 0; p < N; ++p) g_nb += G[0 * N + p] * ur[p];
         duR = u[N - 1] - ur[0]; dgR = gu[N - 1] - g_nb; aR = 0.5;
-    } else { duR = u[N - 1] - gr; dgR = 0.0; aR = 1.0; }
+    } else if (somR) { duR = 0.0; dgR = 0.0; aR = 0.0; }
+    else { duR = u[N - 1] - gr; dgR = 0.0; aR = 1.0; }
     double ih2 = 1.0 / (h * h);
     for (int i = 0; i < N; ++i) {
         double s = 0.0;
@@ -437,6 +462,13 @@ static void sipg_segment(const wto_ctx* c, double h, const double* u,
         if (i == 0) s += (0.5 * dgL - tau * duL) / w[0];
         if (i == N - 1) s += (-0.5 * dgR - tau * duR) / w[N - 1];
         y[i] += s * ih2;
+    }
+    {
+        /* outward normal derivative: -(1/h) Gu at the low face, +(1/h) Gu at the
+         * high one; without v the u_t part is zero (the energy's operator) */
+        double iR = c->d.sommerfeld_inv_R;
+        if (!ul && somL) y[0] -= ((v ? v[0] : 0.0) - gu[0] / h + iR * u[0]) / (h * w[0]);
+        if (!ur && somR) y[N - 1] -= ((v ? v[N - 1] : 0.0) + gu[N - 1] / h + iR * u[N - 1]) / (h * w[N - 1]);
     }
 }
 
@@ -451,24 +483,28 @@ static void gather_line(const wto_ctx* c, const double* f, long e, int a, int p,
 /* Flat (affine, Minkowski) Laplacian: Kronecker sum of 1D SIPG operators
  * along the three axes (the face terms of one axis involve only that axis,
  * tangential H cancels). */
-static void lap_flat(const wto_ctx* c, const double* u, const double* bv, double* y)
+/* v (u_t) of the stage, or NULL for u_t = 0 (the energy's operator); it only
+ * enters through the Sommerfeld faces' drag. */
+static void lap_flat(const wto_ctx* c, const double* u, const double* v, const double* bv, double* y)
 {
     int N = c->N;
     #pragma omp parallel for schedule(static)
     for (long e = 0; e < c->Ne; ++e) {
         double* ye = y + e * c->npe;
         for (int n = 0; n < c->npe; ++n) ye[n] = 0.0;
-        double us[WTO_NMAX], ul[WTO_NMAX], ur[WTO_NMAX], acc[WTO_NMAX];
+        double us[WTO_NMAX], ul[WTO_NMAX], ur[WTO_NMAX], vs[WTO_NMAX], acc[WTO_NMAX];
         for (int a = 0; a < 3; ++a) {
             long el = neighbour(c, e, a, 0), er = neighbour(c, e, a, 1);
+            int somL = c->d.face_kind[2 * a] == 7, somR = c->d.face_kind[2 * a + 1] == 7;
             for (int q = 0; q < N; ++q)
             for (int p = 0; p < N; ++p) {
                 gather_line(c, u, e, a, p, q, us);
                 if (el >= 0) gather_line(c, u, el, a, p, q, ul);
                 if (er >= 0) gather_line(c, u, er, a, p, q, ur);
+                if (v) gather_line(c, v, e, a, p, q, vs);
                 for (int r = 0; r < N; ++r) acc[r] = 0.0;
-                sipg_segment(c, c->h[a], us, el >= 0 ? ul : NULL, bv[2 * a],
-                             er >= 0 ? ur : NULL, bv[2 * a + 1], acc);
+                sipg_segment_s(c, c->h[a], us, el >= 0 ? ul : NULL, bv[2 * a],
+                               er >= 0 ? ur : NULL, bv[2 * a + 1], somL, somR, v ? vs : NULL, acc);
                 for (int r = 0; r < N; ++r) ye[fidx(N, a, r, p, q)] += acc[r];
             }
         }
@@ -664,7 +700,7 @@ static const double zero6[6] = {0, 0, 0, 0, 0, 0};
  * (src/wave.jl:567-571). */
 void wto_lap(const wto_ctx* c, const double* u, const double* ud, double* udd)
 {
-    if (!c->curved) { lap_flat(c, u, c->d.bdry_values, udd); return; }
+    if (!c->curved) { lap_flat(c, u, ud, c->d.bdry_values, udd); return; }
     op_curved(c, u, ud, c->d.bdry_values, ud != NULL, udd);
     #pragma omp parallel for schedule(static)
     for (long n = 0; n < c->npts; ++n) udd[n] *= c->coef[13 * n + 9];
@@ -715,7 +751,7 @@ double wto_energy(const wto_ctx* c, const double* u, const double* v)
     double* Lu = (double*)malloc(sizeof(double) * c->npts);
     double E = 0.0;
     if (!c->curved) {
-        lap_flat(c, u, zero6, Lu);
+        lap_flat(c, u, NULL, zero6, Lu);
         #pragma omp parallel for reduction(+ : E) schedule(static)
         for (long n = 0; n < c->npts; ++n)
             E += 0.5 * c->hphys[n] * (v[n] * v[n] - u[n] * Lu[n]);

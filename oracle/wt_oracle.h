@@ -37,6 +37,8 @@ typedef struct wto_desc {
     double tau;            /* SIPG penalty, unit-element units             */
     double eps_ko;         /* Kreiss-Oliger epsilon (0 disables)           */
     double bdry_values[6]; /* Dirichlet value per boundary tag 1..6        */
+    int face_kind[6];      /* outer face f = 1..6: 0 Dirichlet (tag f), 7 Sommerfeld */
+    double sommerfeld_inv_R; /* 1/R of the BGT-1 condition (0: plane-wave BGT-0) */
 } wto_desc;
 
 typedef struct wto_ctx wto_ctx;

@@ -6,6 +6,7 @@ it.  The C source (wt_oracle.c) cites the reference file:line each function
 restates.
 """
 import ctypes
+import math
 import os
 import subprocess
 
@@ -29,6 +30,8 @@ class Desc(ctypes.Structure):
         ("tau", ctypes.c_double),
         ("eps_ko", ctypes.c_double),
         ("bdry_values", ctypes.c_double * 6),
+        ("face_kind", ctypes.c_int * 6),
+        ("sommerfeld_inv_R", ctypes.c_double),
     ]
 
 
@@ -97,7 +100,7 @@ class Oracle:
     fields stored as 1-D numpy arrays of length N^3*Ne."""
 
     def __init__(self, N, M, x0=0.0, x1=1.0, periodic=False, warp_A=0.0, tau=None,
-                 eps_ko=0.0, bdry_values=(0.0,) * 6, bg=None):
+                 eps_ko=0.0, bdry_values=(0.0,) * 6, bg=None, sommerfeld_faces=(), sommerfeld_R=math.inf):
         if isinstance(M, int):
             M = (M, M, M)
         if not isinstance(x0, (tuple, list)):
@@ -115,6 +118,8 @@ class Oracle:
         d.warp_A = warp_A; d.tau = tau; d.eps_ko = eps_ko
         for f in range(6):
             d.bdry_values[f] = bdry_values[f]
+            d.face_kind[f] = 7 if (f + 1) in tuple(sommerfeld_faces) else 0
+        d.sommerfeld_inv_R = 0.0 if math.isinf(sommerfeld_R) else 1.0 / sommerfeld_R
         self.desc = d
         self.N, self.M = N, tuple(M)
         self.Ne = M[0] * M[1] * M[2]

@@ -421,3 +421,112 @@ def test_curved_energy_conserved_without_ko():
         assert np.all(np.diff(E) <= 1e-13 * E[0])
         drift.append(abs(E[-1] / E[0] - 1))
     assert drift[1] < drift[0] / 16
+
+
+# ------------------------------------------------------------------ Sommerfeld faces
+
+def _face_measure(o, f, faces):
+    """Surface integral over the given 1-based faces of the unit box of f(x,y,z),
+    by a tensor Gauss-Legendre rule of its own (exact for the polynomials used)."""
+    g, w = npleg.leggauss(8)
+    g = 0.5 * (g + 1); w = 0.5 * w
+    tot = 0.0
+    for face in faces:
+        a, side = (face - 1) // 2, float((face - 1) % 2)
+        b, c = [d for d in range(3) if d != a]
+        for p, wp in zip(g, w):
+            for q, wq in zip(g, w):
+                x = [0.0, 0.0, 0.0]
+                x[a] = side; x[b] = p; x[c] = q
+                tot += wp * wq * f(*x)
+    return tot
+
+
+def test_sommerfeld_surface_terms_closed_form():
+    """Bayliss-Turkel faces (tag 7, src/wave.jl:453-518): <v, L(0, v)>_H is minus
+    the surface integral of v^2 over the Sommerfeld faces, <1, L(1, 0)>_H is minus
+    their area over R, and with R = inf a constant lies in the operator's kernel
+    (the faces are free for the SIPG terms)."""
+    faces = (1, 2, 4)
+    R = 3.0
+    o = wo.Oracle(5, 2, sommerfeld_faces=faces, sommerfeld_R=R)
+    x = o.coords()
+    H = o.hphys()
+    fv = lambda X, Y, Z: 1.0 + 0.5 * X - Y * Y + 0.25 * X * Z
+    v = fv(x[:, 0], x[:, 1], x[:, 2])
+    Lv = o.lap(np.zeros(o.n), v)
+    assert abs(np.dot(H * v, Lv) + _face_measure(o, lambda *X: fv(*X) ** 2, faces)) < 1e-12
+    one = np.ones(o.n)
+    allf = (1, 2, 3, 4, 5, 6)
+    oR = wo.Oracle(5, 2, sommerfeld_faces=allf, sommerfeld_R=R)
+    assert abs(np.dot(oR.hphys(), oR.lap(one)) + 6 / R) < 1e-12
+    # Dirichlet faces still pull constants down, Sommerfeld ones at R = inf do not
+    oi = wo.Oracle(5, 2, sommerfeld_faces=allf)
+    assert np.max(np.abs(oi.lap(one))) < 1e-10
+    assert np.max(np.abs(oi.lap(one, np.zeros(oi.n)))) < 1e-10
+    od = wo.Oracle(5, 2)
+    assert np.max(np.abs(od.lap(one))) > 1.0
+    # Sommerfeld faces only on bounded axes of flat meshes
+    with pytest.raises(ValueError):
+        wo.Oracle(4, 2, periodic=(True, False, False), sommerfeld_faces=(2,))
+    with pytest.raises(ValueError):
+        wo.Oracle(4, 2, warp_A=0.05, sommerfeld_faces=(1,))
+
+
+def test_sommerfeld_operator_symmetric_nsd():
+    """With u_t = 0 the Sommerfeld faces leave L H-symmetric and negative
+    semidefinite (the operator of the energy 1/2|v|^2 + 1/2|grad u|^2 +
+    1/(2R) |u|^2 on those faces); a wrong sign in the normal derivative or a
+    missing weight breaks the symmetry."""
+    o = wo.Oracle(4, (2, 2, 1), sommerfeld_faces=(1, 4, 5), sommerfeld_R=2.0)
+    A = _dense(o, o.lap, o.n)
+    H = o.hphys()
+    S = H[:, None] * A
+    assert np.max(np.abs(S - S.T)) < 1e-10 * np.max(np.abs(S))
+    ev = np.linalg.eigvalsh(0.5 * (S + S.T))
+    assert ev.max() < 1e-10 * np.abs(ev).max()
+
+
+def test_sommerfeld_energy_rate_and_decay():
+    """dE/dt = <v, L(u, v) - L(u, 0)>_H = -(surface integral of v^2) for any u,
+    and RK4 with Sommerfeld faces and noise data never gains energy."""
+    faces = (2, 3, 6)
+    o = wo.Oracle(4, 2, sommerfeld_faces=faces, sommerfeld_R=1.5, eps_ko=0.1)
+    rng = np.random.default_rng(77)
+    u = rng.standard_normal(o.n)
+    x = o.coords()
+    fv = lambda X, Y, Z: np.cos(X + 2 * Y) * (1 + Z)
+    v = fv(x[:, 0], x[:, 1], x[:, 2])
+    H = o.hphys()
+    rate = np.dot(H * v, o.lap(u, v) - o.lap(u))
+    ref = _face_measure(o, lambda *X: fv(*X) ** 2, faces)
+    assert abs(rate + ref) < 2e-4 * ref      # GLL face quadrature of a non-polynomial
+    amp = math.sqrt(np.finfo(float).eps)
+    u0 = amp * rng.standard_normal(o.n)
+    v0 = amp * rng.standard_normal(o.n)
+    _, _, E = o.rk4(u0, v0, 0.2 * o.hmin, 60, energy=True)
+    assert np.all(np.diff(E) <= 1e-12 * E[0])
+    assert E[-1] < 0.9 * E[0]
+
+
+def test_sommerfeld_absorbs_outgoing_plane_wave():
+    """A right-moving pulse f(x - t) leaves through a Sommerfeld face at x = 1
+    (exact for normal incidence at R = inf) and is reflected back by a Dirichlet
+    face; y and z are periodic.  What stays behind is discretisation error and
+    falls fast with resolution."""
+    def run(faces, M=12):
+        o = wo.Oracle(6, (M, 1, 1), periodic=(False, True, True), sommerfeld_faces=faces)
+        x = o.coords()
+        s = (x[:, 0] - 0.5) / 0.08
+        u0 = np.exp(-s * s)
+        v0 = 2 * s / 0.08 * u0      # u_t = -u_x for f(x - t)
+        dt = 0.25 * o.hmin
+        ns = int(math.ceil(0.9 / dt))
+        _, _, E = o.rk4(u0, v0, 0.9 / ns, ns, energy=True)
+        return E
+    Es = run((1, 2))
+    Ed = run(())
+    assert abs(Ed[-1] / Ed[0] - 1) < 1e-3
+    assert Es[-1] < 1e-6 * Es[0]
+    Ec = run((1, 2), M=6)
+    assert Es[-1] / Es[0] < 1e-3 * Ec[-1] / Ec[0]
