@@ -10,7 +10,8 @@
  * Field layout (every double* / float* field argument): Julia column-major
  * (N,N,N,Ne), i.e. C index i + N*(j + N*(k + N*e)), elements x-fastest
  * e = ex + Mx*(ey + My*ez).  Faces 1..6 are (-x,+x,-y,+y,-z,+z); an outer face
- * f carries boundary tag f and Dirichlet value bdry_values[f-1].
+ * f carries boundary tag f and Dirichlet value bdry_values[f-1], unless
+ * face_kind[f-1] marks it a Sommerfeld face (tag 7, src/wave.jl:23).
  *
  * Pointers are device pointers (16-byte aligned) unless WT2_HOST_PTRS is passed
  * in flags, in which case they are host arrays copied in and out around the call.
@@ -61,6 +62,12 @@ typedef struct wt2_mesh_desc {
     double  bdry_values[6];  /* Dirichlet value per tag 1..6 (RK steps)      */
     int32_t precision;       /* WT2_FP64 or WT2_MIXED                        */
     int32_t flags;           /* WT2_ALLOW_NONCOERCIVE                        */
+    int32_t face_kind[6];    /* per face: 0 = Dirichlet (SIPG, tag f), 7 =
+                                Sommerfeld (free for SIPG plus the
+                                Bayliss-Turkel drag u_t + d_n u + u/R -> 0,
+                                src/wave.jl:444-518); flat meshes, bounded
+                                axes only                                   */
+    double  sommerfeld_inv_R;/* 1/R of that drag; 0 = plane-wave (BGT-0)     */
 } wt2_mesh_desc;
 
 typedef struct wt2_ctx wt2_ctx;
@@ -112,7 +119,8 @@ int wt2_coords(const wt2_ctx* ctx, double* xyz);
 
 /* rhs_wave3d!(udd, u, ud, bdry_values; geom, ops, work, tau)
  * (src/wave.jl:583-611): udd = L_SIPG u with Dirichlet values bdry_values[tag-1]
- * on outer faces.  On a curved/background context udd also carries the u_t
+ * on outer faces, and the Sommerfeld faces' drag, which reads ud at the face
+ * nodes (ud = NULL: taken as zero).  On a curved/background context udd also carries the u_t
  * advection terms of the ADM operator.  No KO (the reference RHS has none). */
 int wt2_rhs3d(wt2_ctx* ctx, double* udd, const double* u, const double* ud,
               const double bdry_values[6], int32_t flags);

@@ -55,6 +55,12 @@ def _desc(**kw):
     d.eps_ko = kw.get("eps", 0.0)
     d.precision = kw.get("precision", 0)
     d.flags = kw.get("flags", 0)
+    for f in range(6):
+        d.face_kind[f] = kw.get("face_kind", (0,) * 6)[f]
+    d.sommerfeld_inv_R = kw.get("inv_R", 0.0)
+    if kw.get("periodic_x"):
+        d.periodic[0] = 1
+    d.warp_A = kw.get("warp_A", 0.0)
     return d
 
 
@@ -74,6 +80,10 @@ def _create(d):
     (dict(eps=-0.1), _lib.WT2_ERR_ARG),
     (dict(N=8, tau=20.0), _lib.WT2_ERR_TAU),   # literal tau = 20 is not coercive at N = 8
     (dict(N=10, tau=20.0), _lib.WT2_ERR_TAU),
+    (dict(face_kind=(0, 3, 0, 0, 0, 0)), _lib.WT2_ERR_ARG),          # only 0 and 7 are face kinds
+    (dict(face_kind=(7, 0, 0, 0, 0, 0), periodic_x=True), _lib.WT2_ERR_ARG),
+    (dict(face_kind=(0, 0, 0, 7, 0, 0), warp_A=0.05), _lib.WT2_ERR_UNSUPPORTED),
+    (dict(face_kind=(7,) * 6, inv_R=-1.0), _lib.WT2_ERR_ARG),
 ])
 def test_ctx_create_rejects(kw, code):
     rc, ctx = _create(_desc(**kw))

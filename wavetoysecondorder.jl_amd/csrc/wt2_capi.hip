@@ -275,6 +275,17 @@ static void fill_axis_ops(const wt2_ctx* c, int axis, const double bv[6], AxisOp
     o.ih2 = 1.0 / (h * h);
     o.gL = bv[2 * axis];
     o.gR = bv[2 * axis + 1];
+    o.hv = h;
+    o.hiR = h * c->d.sommerfeld_inv_R;
+    o.somL = !c->d.periodic[axis] && c->d.face_kind[2 * axis] == SOMMERFELD_TAG;
+    o.somR = !c->d.periodic[axis] && c->d.face_kind[2 * axis + 1] == SOMMERFELD_TAG;
+}
+
+// A lap pass along this axis reads u_t at the axis's Sommerfeld faces.
+static bool som_axis(const wt2_ctx* c, int axis)
+{
+    return !c->d.periodic[axis] &&
+           (c->d.face_kind[2 * axis] == SOMMERFELD_TAG || c->d.face_kind[2 * axis + 1] == SOMMERFELD_TAG);
 }
 
 // A pass-kernel choice for axis a: kind 0 = LDS-staged, 1 = LDS-free with
@@ -354,6 +365,7 @@ static int run_lap(wt2_ctx* c, const double* u, const double* v, double* out, co
         fill_axis_ops(c, a, bv, d.ops[a]);
     }
     d.u = u;
+    d.v = with_adv ? v : nullptr;
     d.out = out;
     d.op64 = c->op64;
     d.op32 = c->op32;
@@ -397,7 +409,7 @@ static void stage_items(wt2_ctx* c, int stage, const double* u, const double* v,
         it = StageItems();
         PassDesc z = base_pass(c, 2, bv);
         z.variant = variant;
-        z.lap = 1; z.ko = ko; z.needv = ko; z.keepv = 0;
+        z.lap = 1; z.ko = ko; z.needv = ko || som_axis(c, 2); z.keepv = 0;
         z.Su = Su; z.Sv = Sv; z.P = c->Pb;
         z.epi = EPI_MID;
         it.pass[0] = z;
@@ -444,7 +456,7 @@ static void stage_items(wt2_ctx* c, int stage, const double* u, const double* v,
         p.variant = variant;
         p.lap = lap;
         p.ko = ko;
-        p.needv = ko || last;
+        p.needv = ko || last || (lap && som_axis(c, axis));
         p.keepv = last;
         p.Su = Su; p.Sv = Sv;
         p.P = P;
@@ -569,8 +581,8 @@ int sync(wt2_ctx* c)
 
 namespace {
 struct TuneKey {
-    int v[8];
-    bool operator<(const TuneKey& o) const { return std::lexicographical_compare(v, v + 8, o.v, o.v + 8); }
+    int v[9];
+    bool operator<(const TuneKey& o) const { return std::lexicographical_compare(v, v + 9, o.v, o.v + 9); }
 };
 struct TuneVal {
     int nb[3], direct[3];
@@ -585,6 +597,7 @@ TuneKey tune_key(const wt2_ctx* c)
     k.v[0] = c->N; k.v[1] = c->d.M[0]; k.v[2] = c->d.M[1]; k.v[3] = c->d.M[2];
     k.v[4] = c->d.precision; k.v[5] = c->curved; k.v[6] = c->d.eps_ko > 0;
     k.v[7] = c->d.periodic[0] + 2 * c->d.periodic[1] + 4 * c->d.periodic[2];
+    k.v[8] = int(som_axis(c, 0)) + 2 * int(som_axis(c, 1)) + 4 * int(som_axis(c, 2));
     return k;
 }
 }  // namespace
@@ -664,6 +677,24 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
     }
     if (!(desc->tau > 0) || !(desc->eps_ko >= 0)) {
         set_error("wt2_ctx_create: tau must be > 0 and eps_ko >= 0");
+        return WT2_ERR_ARG;
+    }
+    for (int f = 0; f < 6; ++f) {
+        if (desc->face_kind[f] != 0 && desc->face_kind[f] != SOMMERFELD_TAG) {
+            set_error("wt2_ctx_create: face_kind must be 0 (Dirichlet) or 7 (Sommerfeld)");
+            return WT2_ERR_ARG;
+        }
+        if (desc->face_kind[f] == SOMMERFELD_TAG && desc->periodic[f / 2]) {
+            set_error("wt2_ctx_create: a Sommerfeld face on a periodic axis");
+            return WT2_ERR_ARG;
+        }
+        if (desc->face_kind[f] == SOMMERFELD_TAG && desc->warp_A != 0.0) {
+            set_error("wt2_ctx_create: Sommerfeld faces are built for flat meshes only");
+            return WT2_ERR_UNSUPPORTED;
+        }
+    }
+    if (!(desc->sommerfeld_inv_R >= 0) || !std::isfinite(desc->sommerfeld_inv_R)) {
+        set_error("wt2_ctx_create: sommerfeld_inv_R must be finite and >= 0");
         return WT2_ERR_ARG;
     }
     wt2_ctx* c = new wt2_ctx();
@@ -864,6 +895,11 @@ int wt2_coords(const wt2_ctx* c, double* xyz)
 int wt2_ctx_set_background(wt2_ctx* c, const double* bg10)
 {
     if (!c || !bg10) { set_error("wt2_ctx_set_background: null pointer"); return WT2_ERR_ARG; }
+    for (int a = 0; a < 3; ++a)
+        if (som_axis(c, a)) {
+            set_error("wt2_ctx_set_background: Sommerfeld faces are built for flat meshes only");
+            return WT2_ERR_UNSUPPORTED;
+        }
     if (c->gvalid) { hipGraphExecDestroy(c->gexec); c->gvalid = false; }
     int rc;
     if (!c->curved) {
@@ -925,7 +961,7 @@ static int run_prologue(wt2_ctx* c, const double* Yu, const double* Yv)
     PassDesc p = base_pass(c, 0, c->d.bdry_values);
     p.variant = is_mixed(c) ? VAR_MIX_IN64 : VAR_F64;
     const bool ko = c->d.eps_ko > 0 && c->kos > 0;
-    p.lap = 1; p.ko = ko; p.needv = ko;
+    p.lap = 1; p.ko = ko; p.needv = ko || som_axis(c, 0);
     p.Su = Yu; p.Sv = Yv; p.P = c->Pb;
     p.epi = EPI_FIRST;
     return pass_rc(launch_flat_pass(p, c->stream), "flat pass (prologue)");

@@ -40,6 +40,8 @@ struct AxisConst {
     TC kh;                   // KO scale kos / h
     TC koL, koR, tau, ih2, gL, gR;
     TC winv0, winvN;         // 1 / w at the two face nodes
+    TC hv, hiR;              // Sommerfeld faces: h and h / R
+    int somL, somR;          // 1: that outer face is a Sommerfeld face
 };
 
 template <int N, typename TC>
@@ -152,17 +154,25 @@ __device__ __forceinline__ void apply_G_ends(const TC* __restrict__ op, const TC
 //   (1/h^2){ G G u + H^-1 [ alpha G^T B du - 1/2 B d(Gu) - tau |B| du ] },
 // given g = G u and the neighbours' face values (uL, gnL: value and G u at the
 // last node of the low neighbour; uR, gnR: at the first node of the high one).
+// An outer Sommerfeld face (src/wave.jl:444-518) has no SIPG terms and instead
+// the drag -(u_t + d_n u + u/R) / (h w) on its node, d_n u = -+(1/h) G u at the
+// low / high face; v0, vN: u_t at the segment's two end nodes.  The drag rides
+// in the dg slot of the face term, (1/h^2)(1/w)(+-1/2 dg - tau du) with du = 0
+// and dg = 2 (+-G u - h u_t - (h/R) u) (the factor 2 and 1/2 are exact).
 template <int N, typename TC>
 __device__ __forceinline__ void sipg_seg(const TC (&u)[N], const TC (&g)[N], TC (&out)[N],
                                          const TC* __restrict__ op, const AxisConst<N, TC>& k,
-                                         TC uL, TC gnL, TC uR, TC gnR, bool hasL, bool hasR)
+                                         TC uL, TC gnL, TC uR, TC gnR, bool hasL, bool hasR,
+                                         TC v0, TC vN)
 {
     using L = OpLayout<N>;
     TC duL, dgL, aL, duR, dgR, aR;
-    if (hasL) { duL = u[0] - uL; dgL = g[0] - gnL; aL = TC(0.5); }
-    else      { duL = u[0] - k.gL; dgL = TC(0); aL = TC(1); }
-    if (hasR) { duR = u[N - 1] - uR; dgR = g[N - 1] - gnR; aR = TC(0.5); }
-    else      { duR = u[N - 1] - k.gR; dgR = TC(0); aR = TC(1); }
+    if (hasL)        { duL = u[0] - uL; dgL = g[0] - gnL; aL = TC(0.5); }
+    else if (k.somL) { duL = TC(0); dgL = TC(2) * (g[0] - fma(k.hv, v0, k.hiR * u[0])); aL = TC(0); }
+    else             { duL = u[0] - k.gL; dgL = TC(0); aL = TC(1); }
+    if (hasR)        { duR = u[N - 1] - uR; dgR = g[N - 1] - gnR; aR = TC(0.5); }
+    else if (k.somR) { duR = TC(0); dgR = TC(2) * (g[N - 1] + fma(k.hv, vN, k.hiR * u[N - 1])); aR = TC(0); }
+    else             { duR = u[N - 1] - k.gR; dgR = TC(0); aR = TC(1); }
     const TC tL = -aL * duL, tR = aR * duR;
     TC gg[N];
     apply_G<N, TC>(op, g, gg);
@@ -180,14 +190,14 @@ __device__ __forceinline__ void sipg_seg(const TC (&u)[N], const TC (&g)[N], TC 
 template <int N, typename TC, typename X>
 __device__ __forceinline__ void sipg_add(const TC (&u)[N], TC (&out)[N], const TC* __restrict__ op,
                                          const AxisConst<N, TC>& k, X& xch,
-                                         bool hasL, bool hasR)
+                                         bool hasL, bool hasR, TC v0, TC vN)
 {
     TC g[N];
     apply_G<N, TC>(op, u, g);
     TC uL, gnL, uR, gnR;
     xch(u[N - 1], u[0], uL, uR);
     xch(g[N - 1], g[0], gnL, gnR);
-    sipg_seg<N, TC>(u, g, out, op, k, uL, gnL, uR, gnR, hasL, hasR);
+    sipg_seg<N, TC>(u, g, out, op, k, uL, gnL, uR, gnR, hasL, hasR, v0, vN);
 }
 
 template <int N, typename TC, typename TS, typename TP>
@@ -284,8 +294,10 @@ __device__ __forceinline__ void flat_pass_body(const KArgs<N, TC>& a, const int 
         TC u[N];
 #pragma unroll
         for (int r = 0; r < N; ++r) u[r] = TC(lds[su * fstride + base + r * SR]);
+        TC v0 = TC(0), vN = TC(0);
+        if (needv) { v0 = TC(lds[sv * fstride + base]); vN = TC(lds[sv * fstride + base + (N - 1) * SR]); }
         ShflX<TC> xs{srcL, srcR};
-        sipg_add<N, TC>(u, cv, op, a.k, xs, hasL, hasR);
+        sipg_add<N, TC>(u, cv, op, a.k, xs, hasL, hasR, v0, vN);
     }
     __syncthreads();
     if (active) {
@@ -499,7 +511,7 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
 #pragma unroll
         for (int r = 0; r < N; ++r) cv[r] = x[r];
     }
-    if (a.lap) sipg_add<N, TC>(u, cv, op, a.k, xch, hasL, hasR);
+    if (a.lap) sipg_add<N, TC>(u, cv, op, a.k, xch, hasL, hasR, TC(v[0]), TC(v[N - 1]));
     const int epi0 = a.epi;
     const bool fold = a.fold && epi0 >= EPI_RK1;
     // a folding pass with the LDS exchange needs every work-item at its barriers
@@ -623,7 +635,7 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
 #pragma unroll
         for (int r = 0; r < N; ++r) fc[r] = x[r];
     }
-    if (a.lap) sipg_add<N, TC>(fu, fc, op, a.k, xch, hasL, hasR);
+    if (a.lap) sipg_add<N, TC>(fu, fc, op, a.k, xch, hasL, hasR, TC(nv[0]), TC(nv[N - 1]));
     if (active) store_seg_s<N, TP, TC>(P + g0, gs, fc);
 }
 
@@ -702,6 +714,9 @@ static hipError_t make_kargs(const PassDesc& p, KArgs<N, TC>& a, int* tiles_out,
     a.k.tau = TC(p.ops.tau); a.k.ih2 = TC(p.ops.ih2);
     a.k.gL = TC(p.ops.gL); a.k.gR = TC(p.ops.gR);
     a.k.winv0 = TC(p.ops.winv0); a.k.winvN = TC(p.ops.winvN);
+    a.k.hv = TC(p.ops.hv); a.k.hiR = TC(p.ops.hiR);
+    a.k.somL = p.ops.somL; a.k.somR = p.ops.somR;
+    if ((p.ops.somL || p.ops.somR) && p.lap && !p.needv) return hipErrorInvalidValue;   // the drag reads u_t
     a.Su = p.Su; a.Sv = p.Sv; a.P = p.P;
     a.So_u = p.So_u; a.So_v = p.So_v; a.Yu = p.Yu; a.Yv = p.Yv;
     a.Au = p.Au; a.Av = p.Av; a.Ou = p.Ou; a.Ov = p.Ov;
@@ -845,6 +860,7 @@ struct Lap1Args {
     AxisConst<N, TC> k[3];
     const TC* op;
     const double* u;
+    const double* v;   // u_t, read at Sommerfeld face nodes only; nullptr = zero
     double* out;
     int M0, M1, M2;
     int per0, per1, per2;
@@ -924,11 +940,17 @@ __device__ __forceinline__ void lap1_axis(const Lap1Args<N, TC>& a, const TC* __
         apply_G_ends<N, TC>(op, xr, gnR, yN);
         uR = xr[0];
     }
+    TC v0 = TC(0), vN = TC(0);
+    if (a.v) {
+        const double* vl = a.v + int64_t(e) * N3 + gbase;
+        if (!hasL && a.k[AX].somL) v0 = TC(vl[0]);
+        if (!hasR && a.k[AX].somR) vN = TC(vl[(N - 1) * GS]);
+    }
     TC g[N], cv[N];
     apply_G<N, TC>(op, u, g);
 #pragma unroll
     for (int m = 0; m < N; ++m) cv[m] = TC(0);
-    sipg_seg<N, TC>(u, g, cv, op, a.k[AX], uL, gnL, uR, gnR, hasL, hasR);
+    sipg_seg<N, TC>(u, g, cv, op, a.k[AX], uL, gnL, uR, gnR, hasL, hasR, v0, vN);
     if (AX == 2) {
 #pragma unroll
         for (int m = 0; m < N; ++m) Re[lbase + m * LS] = cv[m];
@@ -1005,9 +1027,11 @@ static hipError_t launch_lap1_t(const Lap1Desc& d, hipStream_t s)
         a.k[ax].tau = TC(o.tau); a.k[ax].ih2 = TC(o.ih2);
         a.k[ax].gL = TC(o.gL); a.k[ax].gR = TC(o.gR);
         a.k[ax].winv0 = TC(o.winv0); a.k[ax].winvN = TC(o.winvN);
+        a.k[ax].hv = TC(o.hv); a.k[ax].hiR = TC(o.hiR);
+        a.k[ax].somL = o.somL; a.k[ax].somR = o.somR;
     }
     a.op = static_cast<const TC*>(std::is_same<TC, float>::value ? d.op32 : d.op64);
-    a.u = d.u; a.out = d.out;
+    a.u = d.u; a.v = d.v; a.out = d.out;
     a.M0 = d.M[0]; a.M1 = d.M[1]; a.M2 = d.M[2];
     a.per0 = d.periodic[0]; a.per1 = d.periodic[1]; a.per2 = d.periodic[2];
     const int64_t Ne = int64_t(d.M[0]) * d.M[1] * d.M[2];

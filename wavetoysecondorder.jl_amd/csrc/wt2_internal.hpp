@@ -13,6 +13,7 @@ namespace wt2 {
 constexpr int NMIN = 3;
 constexpr int NMAX = 10;
 constexpr int MMAX_AXIS = 64;   // one element line per axis fits one wavefront
+constexpr int SOMMERFELD_TAG = 7;   // face_kind of a Sommerfeld face (SOMMERFELD_BDRY_TAG, src/wave.jl:23)
 
 // Epilogue of a pencil pass: what is done with the axis contribution c.
 enum Epi : int {
@@ -41,6 +42,8 @@ struct AxisOps {
     double tau, ih2;         // penalty, 1/h^2
     double gL, gR;           // Dirichlet values at the low / high outer face
     double winv0, winvN;     // 1/w_0, 1/w_{N-1}
+    double hv, hiR;          // Sommerfeld faces: h and h/R (the drag's scales)
+    int somL, somR;          // 1: the low / high outer face is a Sommerfeld face
 };
 
 // Even/odd operator table of the GLL differentiation matrix (see wt2_flat.hip).
@@ -82,6 +85,7 @@ struct Lap1Desc {
     int M[3], periodic[3];
     AxisOps ops[3];
     const double* u;
+    const double* v;    // u_t at the Sommerfeld faces' nodes; nullptr = zero
     double* out;
     const void* op64; const void* op32;
 };

@@ -21,6 +21,8 @@ struct MeshDesc
     bdry_values::NTuple{6,Float64}
     precision::Int32      # 0 = FP64, 1 = mixed
     flags::Int32
+    face_kind::NTuple{6,Int32}   # 0 = Dirichlet, 7 = SOMMERFELD_BDRY_TAG
+    sommerfeld_inv_R::Float64    # 1/R; 0 for R = Inf
 end
 
 mutable struct Context
@@ -36,18 +38,22 @@ end
 
 """
     Context(N, M; x0=0.0, x1=1.0, periodic=false, warp_A=0.0, τ, ε_KO=0.0,
-            bdry_values=ntuple(_->0.0, 6), precision=:fp64)
+            bdry_values=ntuple(_->0.0, 6), precision=:fp64,
+            sommerfeld_faces=(), sommerfeld_R=Inf)
 
 The mesh + operators + geometry + workspace bundle (make_uniform_hex +
 make_geometry + make_workspace + to_device of src/evolve.jl:910-936).
+`sommerfeld_faces` lists the faces (1..6) tagged SOMMERFELD_BDRY_TAG.
 """
 function Context(N::Integer, M::Integer; x0 = 0.0, x1 = 1.0, periodic = false,
                  warp_A = 0.0, τ, ε_KO = 0.0, bdry_values = ntuple(_ -> 0.0, 6),
-                 precision = :fp64)
+                 precision = :fp64, sommerfeld_faces = (), sommerfeld_R = Inf)
     d = MeshDesc(Int32(N), ntuple(_ -> Int32(M), 3), ntuple(_ -> Float64(x0), 3),
                  ntuple(_ -> Float64(x1), 3), ntuple(_ -> Int32(periodic), 3),
                  Float64(warp_A), Float64(τ), Float64(ε_KO), NTuple{6,Float64}(bdry_values),
-                 Int32(precision === :mixed ? 1 : 0), Int32(0))
+                 Int32(precision === :mixed ? 1 : 0), Int32(0),
+                 ntuple(f -> Int32(f in sommerfeld_faces ? 7 : 0), 6),
+                 isinf(sommerfeld_R) ? 0.0 : 1.0 / Float64(sommerfeld_R))
     out = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:wt2_ctx_create, libwt2), Cint, (Ref{MeshDesc}, Ref{Ptr{Cvoid}}), d, out))
     ctx = Context(out[], N^3 * M^3)

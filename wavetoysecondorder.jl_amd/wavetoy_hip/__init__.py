@@ -3,7 +3,7 @@ wave RHS and RK4 step of eschnett/WaveToySecondOrder.jl, behind the C ABI of
 libwt2.so (include/wt2.h) and a host mirror of the reference's call names."""
 from ._lib import EXPORTS, LIB_PATH, WT2Error, lib  # noqa: F401
 from .core import (  # noqa: F401
-    AnalyticBackground3D, Batch, Element, Geometry, Mesh, Params3d, SBPOps, Workspace,
+    SOMMERFELD_BDRY_TAG, AnalyticBackground3D, Batch, Element, Geometry, Mesh, Params3d, SBPOps, Workspace,
     adm_bump_background, ctx_footprint, mesh_desc, discrete_energy, eigenmode_cartesian, gaussian_pulse, host_coords,
     make_coef3d, make_element, make_geometry, make_operators, make_uniform_hex,
     make_warped_uniform_hex, make_workspace, recommended_dt_rk4, rhs_wave3d,

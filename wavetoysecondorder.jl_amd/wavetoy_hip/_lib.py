@@ -49,6 +49,8 @@ class MeshDesc(ctypes.Structure):
         ("bdry_values", ctypes.c_double * 6),
         ("precision", ctypes.c_int32),
         ("flags", ctypes.c_int32),
+        ("face_kind", ctypes.c_int32 * 6),
+        ("sommerfeld_inv_R", ctypes.c_double),
     ]
 
 

@@ -26,6 +26,8 @@ import numpy as np
 from . import _lib
 from ._lib import WT2_FP64, WT2_MIXED, WT2_HOST_PTRS, WT2_ALLOW_NONCOERCIVE, WT2Error, check, lib
 
+SOMMERFELD_BDRY_TAG = 7     # src/wave.jl:23
+
 try:  # torch is plumbing for device memory; numpy host arrays work without it
     import torch
 except ImportError:  # pragma: no cover
@@ -116,7 +118,11 @@ def make_warped_uniform_hex(Mx, My, Mz, x0, x1, A, periodic=False):
 # -------------------------------------------------------- device workspace
 
 def mesh_desc(mesh, elem, tau=None, eps_ko=0.0, bdry_values=(0.0,) * 6, precision="fp64",
-              allow_noncoercive=False):
+              allow_noncoercive=False, sommerfeld_faces=(), sommerfeld_R=math.inf):
+    """wt2_mesh_desc of a mesh: sommerfeld_faces lists the outer faces (1..6 =
+    -x,+x,-y,+y,-z,+z) that carry the Sommerfeld tag (SOMMERFELD_BDRY_TAG = 7,
+    src/wave.jl:23) instead of a Dirichlet value; sommerfeld_R is the radius of
+    their u/R term (inf: plane-wave condition)."""
     d = _lib.MeshDesc()
     d.N = elem.N
     for a in range(3):
@@ -131,6 +137,15 @@ def mesh_desc(mesh, elem, tau=None, eps_ko=0.0, bdry_values=(0.0,) * 6, precisio
         d.bdry_values[f] = float(bdry_values[f])
     d.precision = {"fp64": WT2_FP64, "mixed": WT2_MIXED}[precision]
     d.flags = WT2_ALLOW_NONCOERCIVE if allow_noncoercive else 0
+    som = tuple(int(f) for f in sommerfeld_faces)
+    if any(f < 1 or f > 6 for f in som):
+        raise ValueError("sommerfeld_faces are face numbers 1..6")
+    for f in range(6):
+        d.face_kind[f] = SOMMERFELD_BDRY_TAG if (f + 1) in som else 0
+    R = float(sommerfeld_R)
+    if not R > 0:
+        raise ValueError("sommerfeld_R must be > 0")
+    d.sommerfeld_inv_R = 0.0 if math.isinf(R) else 1.0 / R
     return d
 
 
@@ -153,8 +168,10 @@ class Workspace:
     (the reference's geom/ops/work bundle after to_device)."""
 
     def __init__(self, mesh, elem, tau=None, eps_ko=0.0, bdry_values=(0.0,) * 6,
-                 precision="fp64", allow_noncoercive=False, bg=None):
-        d = mesh_desc(mesh, elem, tau, eps_ko, bdry_values, precision, allow_noncoercive)
+                 precision="fp64", allow_noncoercive=False, bg=None, sommerfeld_faces=(),
+                 sommerfeld_R=math.inf):
+        d = mesh_desc(mesh, elem, tau, eps_ko, bdry_values, precision, allow_noncoercive,
+                      sommerfeld_faces, sommerfeld_R)
         self.desc = d
         self.mesh = mesh
         self.elem = elem
