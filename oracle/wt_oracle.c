@@ -827,3 +827,143 @@ int wto_assemble_1d(int N, int M, int periodic, double tau, double h, int kind, 
     free(x); free(y);
     return 0;
 }
+
+/* ------------------------------------------ first-order (Phi, Pi) system */
+
+/* coef11 from the background (sample_background3d!'s per-node kernel,
+ * src/wave3d_curved.jl:75-96): alpha, sqrt(det gamma), beta1..3 and the
+ * cofactor inverse gamma^{ij}; field q of node n at coef11[q * n_pts + n]. */
+void wto_fo_coef(long npts, const double* bg10, double* coef11)
+{
+    #pragma omp parallel for schedule(static)
+    for (long n = 0; n < npts; ++n) {
+        const double* b = bg10 + 10 * n;
+        double g11 = b[4], g12 = b[5], g13 = b[6], g22 = b[7], g23 = b[8], g33 = b[9];
+        double det = g11 * (g22 * g33 - g23 * g23) - g12 * (g12 * g33 - g23 * g13) + g13 * (g12 * g23 - g22 * g13);
+        double* o = coef11;
+        o[0 * npts + n] = b[0];
+        o[1 * npts + n] = sqrt(det);
+        o[2 * npts + n] = b[1];
+        o[3 * npts + n] = b[2];
+        o[4 * npts + n] = b[3];
+        o[5 * npts + n] = (g22 * g33 - g23 * g23) / det;
+        o[6 * npts + n] = (g13 * g23 - g12 * g33) / det;
+        o[7 * npts + n] = (g12 * g23 - g13 * g22) / det;
+        o[8 * npts + n] = (g11 * g33 - g13 * g13) / det;
+        o[9 * npts + n] = (g12 * g13 - g11 * g23) / det;
+        o[10 * npts + n] = (g11 * g22 - g12 * g12) / det;
+    }
+}
+
+/* wave3d_curved_rhs! on the axis-aligned affine path (metric === nothing,
+ * src/wave3d_curved.jl:205-258):
+ *   DPhi_d = apply_D!(Phi, d),  F^i = b^i Pi + alpha sqrtg g^{ij} DPhi_j,
+ *   dPi = sum_i apply_D!(F^i, i),  dPhi = b . DPhi + (alpha / sqrtg) Pi,
+ * then eps mu^-5 D_d^6 on both fields with the one-sided apply_D!
+ * (_ko_axis3!, :182-192), then the affine outer-boundary pass
+ * (_bc3d_affine_kernel!, src/boundaries3d.jl:76-126) when bc is given. */
+void wto_fo_rhs(const wto_ctx* c, const double* Phi, const double* Pi, const double* coef11,
+                double eps_ko, const wto_bc3d* bc, double* dPhi, double* dPi)
+{
+    long n = c->npts;
+    const double *al = coef11, *sg = coef11 + n, *b[3] = {coef11 + 2 * n, coef11 + 3 * n, coef11 + 4 * n};
+    const double* gu = coef11 + 5 * n;    /* 11 12 13 22 23 33 */
+    static const int gmap[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    double* D = (double*)malloc(sizeof(double) * 3 * n);
+    double* F = (double*)malloc(sizeof(double) * n);
+    double* s = (double*)malloc(sizeof(double) * n);
+    for (int d = 0; d < 3; ++d) apply_D_g(c, d, Phi, D + d * n, 0);
+    for (int i = 0; i < 3; ++i) {
+        #pragma omp parallel for schedule(static)
+        for (long q = 0; q < n; ++q)
+            F[q] = b[i][q] * Pi[q] + al[q] * sg[q] * (gu[gmap[i][0] * n + q] * D[q] + gu[gmap[i][1] * n + q] * D[n + q] +
+                                                     gu[gmap[i][2] * n + q] * D[2 * n + q]);
+        apply_D_g(c, i, F, s, 0);
+        #pragma omp parallel for schedule(static)
+        for (long q = 0; q < n; ++q) dPi[q] = (i == 0 ? 0.0 : dPi[q]) + s[q];
+    }
+    #pragma omp parallel for schedule(static)
+    for (long q = 0; q < n; ++q)
+        dPhi[q] = b[0][q] * D[q] + b[1][q] * D[n + q] + b[2][q] * D[2 * n + q] + (al[q] / sg[q]) * Pi[q];
+    if (eps_ko != 0.0) {
+        double koT = eps_ko * c->inv_mu5;
+        double* s2 = (double*)malloc(sizeof(double) * n);
+        for (int fld = 0; fld < 2; ++fld) {
+            const double* x = fld == 0 ? Phi : Pi;
+            double* out = fld == 0 ? dPhi : dPi;
+            for (int d = 0; d < 3; ++d) {
+                apply_D_g(c, d, x, s, 0);
+                apply_D_g(c, d, s, s2, 0);
+                apply_D_g(c, d, s2, s, 0);
+                apply_D_g(c, d, s, s2, 0);
+                apply_D_g(c, d, s2, s, 0);
+                apply_D_g(c, d, s, s2, 0);
+                #pragma omp parallel for schedule(static)
+                for (long q = 0; q < n; ++q) out[q] += koT * s2[q];
+            }
+        }
+        free(s2);
+    }
+    if (bc) {
+        int N = c->N;
+        #pragma omp parallel for schedule(static)
+        for (long q = 0; q < n; ++q) {
+            long e = q / c->npe;
+            int r = (int)(q % c->npe);
+            int nd[3] = {r % N, (r / N) % N, r / (N * N)};
+            long ec[3] = {e % c->d.M[0], (e / c->d.M[0]) % c->d.M[1], e / ((long)c->d.M[0] * c->d.M[1])};
+            double dF = 0.0, dP = 0.0;
+            for (int f = 0; f < 6; ++f) {
+                int dn = f / 2, hi = f % 2;
+                if (c->d.periodic[dn]) continue;                       /* no outer face */
+                if (ec[dn] != (hi ? c->d.M[dn] - 1 : 0)) continue;     /* bdry[f, e] == 0 */
+                int kind = bc->kinds[f];
+                if (kind == WTO_BC_EXCISION) continue;
+                int fn = hi ? N - 1 : 0;
+                if (nd[dn] != fn) continue;
+                double nh = hi ? 1.0 : -1.0;
+                double a = al[q] / sg[q];
+                double gunn = gu[gmap[dn][dn] * n + q];
+                double a_n = al[q] * sqrt(gunn);
+                double bax = b[dn][q];
+                double bn = nh * bax;
+                double wt = (1.0 / c->h[dn]) / c->w[fn];
+                if (kind == WTO_BC_FULL_DIRICHLET) {
+                    double tau = bc->sigma * (fabs(a_n - bax) + fabs(a_n + bax)) * wt;
+                    double gF = bc->gPhi ? bc->gPhi[q] : 0.0, gP = bc->gPi ? bc->gPi[q] : 0.0;
+                    dF += -tau * (Phi[q] - gF);
+                    dP += -tau * (Pi[q] - gP);
+                } else {
+                    double qn = nh * D[dn * n + q];
+                    double rr = Pi[q] + ((bn + a_n) / a) * qn;
+                    double g = (kind == WTO_BC_SOMMERFELD || !bc->gPhi) ? 0.0 : bc->gPhi[q];
+                    double s_in = a_n + bn;
+                    dP += -bc->sigma * fabs(s_in) * wt * (rr - g);
+                }
+            }
+            dPhi[q] += dF;
+            dPi[q] += dP;
+        }
+    }
+    free(D); free(F); free(s);
+}
+
+/* wave3d_energy (src/wave3d_curved.jl:268-287):
+ * E = sum 1/2 [Pi^2 / sqrtg + sqrtg g^{ij} DPhi_i DPhi_j] Hphys. */
+double wto_fo_energy(const wto_ctx* c, const double* Phi, const double* Pi, const double* coef11)
+{
+    long n = c->npts;
+    const double *sg = coef11 + n, *gu = coef11 + 5 * n;
+    double* D = (double*)malloc(sizeof(double) * 3 * n);
+    for (int d = 0; d < 3; ++d) apply_D_g(c, d, Phi, D + d * n, 0);
+    double E = 0.0;
+    #pragma omp parallel for reduction(+ : E) schedule(static)
+    for (long q = 0; q < n; ++q) {
+        double d1 = D[q], d2 = D[n + q], d3 = D[2 * n + q];
+        E += (Pi[q] * Pi[q] / sg[q] +
+              sg[q] * (gu[q] * d1 * d1 + gu[3 * n + q] * d2 * d2 + gu[5 * n + q] * d3 * d3 + 2 * gu[n + q] * d1 * d2 +
+                       2 * gu[2 * n + q] * d1 * d3 + 2 * gu[4 * n + q] * d2 * d3)) * c->hphys[q] / 2;
+    }
+    free(D);
+    return E;
+}

@@ -16,7 +16,10 @@
  *     (src/wave3d_curved.jl:158-192; METHODS.md:256-298), applied to u_t
  *     (see wto_rhs for why), outer faces closed with a zero ghost value,
  *   - classical RK4 (test/wave3d_tests.jl:32-44),
- *   - the H-norm energy (test/kernels3d_tests.jl:32-47).
+ *   - the H-norm energy (test/kernels3d_tests.jl:32-47),
+ *   - the first-order (Phi, Pi) system of wave3d_curved_rhs! with its KO,
+ *     affine boundary pass and energy (src/wave3d_curved.jl:205-287,
+ *     src/boundaries3d.jl:56-126).
  * Layout: fields are (N,N,N,Ne) column-major (i fastest), elements x-fastest
  * e = ex + Mx*(ey + My*ez); faces 1..6 = (-x,+x,-y,+y,-z,+z).
  */
@@ -75,6 +78,24 @@ void wto_apply_Dko(const wto_ctx* c, int axis, const double* in, double* out);
  * receives nsteps+1 energies (before the first step and after each). */
 void wto_rk4(const wto_ctx* c, double* u, double* v, double dt, int nsteps, double* energy_out);
 double wto_energy(const wto_ctx* c, const double* u, const double* v);
+
+/* First-order (Phi, Pi) ADM system, axis-aligned affine meshes.
+ * coef11: 11 fields of n_pts values, SoA, in the order of make_coef3d
+ * (alpha, sqrtg, b1, b2, b3, gu11, gu12, gu13, gu22, gu23, gu33). */
+#define WTO_BC_DIRICHLET      1   /* src/boundaries1d.jl:59-62 */
+#define WTO_BC_SOMMERFELD     2
+#define WTO_BC_EXCISION       3
+#define WTO_BC_FULL_DIRICHLET 4
+typedef struct wto_bc3d {
+    int kinds[6];          /* per face -x,+x,-y,+y,-z,+z                     */
+    double sigma;
+    const double* gPhi;    /* nullable targets (per node)                    */
+    const double* gPi;
+} wto_bc3d;
+void wto_fo_coef(long npts, const double* bg10, double* coef11);
+void wto_fo_rhs(const wto_ctx* c, const double* Phi, const double* Pi, const double* coef11,
+                double eps_ko, const wto_bc3d* bc, double* dPhi, double* dPi);
+double wto_fo_energy(const wto_ctx* c, const double* Phi, const double* Pi, const double* coef11);
 
 /* Dense 1D operators on a line of M elements (for spectrum / symmetry pins).
  * kind 0: SIPG Laplacian with homogeneous Dirichlet (or periodic) ends,

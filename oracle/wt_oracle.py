@@ -35,6 +35,44 @@ class Desc(ctypes.Structure):
     ]
 
 
+BC_DIRICHLET, BC_SOMMERFELD, BC_EXCISION, BC_FULL_DIRICHLET = 1, 2, 3, 4   # src/boundaries1d.jl:59-62
+_BC_NAMES = {"dirichlet": 1, "sommerfeld": 2, "excision": 3, "full_dirichlet": 4}
+
+
+class BC3d(ctypes.Structure):
+    _fields_ = [
+        ("kinds", ctypes.c_int * 6),
+        ("sigma", ctypes.c_double),
+        ("gPhi", ctypes.POINTER(ctypes.c_double)),
+        ("gPi", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+def make_bc3d(kinds, sigma=1.0, gPhi=None, gPi=None):
+    """make_bc3d (src/boundaries3d.jl:44-54): kinds per face (-x,+x,-y,+y,-z,+z),
+    names or BC_* codes; gPhi / gPi: optional per-node targets."""
+    b = BC3d()
+    for f in range(6):
+        k = kinds[f]
+        b.kinds[f] = _BC_NAMES[k] if isinstance(k, str) else int(k)
+    b.sigma = float(sigma)
+    b._keep = [None if g is None else np.ascontiguousarray(g, dtype=np.float64) for g in (gPhi, gPi)]
+    b.gPhi = _p(b._keep[0])
+    b.gPi = _p(b._keep[1])
+    return b
+
+
+def fo_coef(bg10):
+    """sample_background3d!'s coefficient bundle (src/wave3d_curved.jl:75-96)
+    from 10 background values per node: (11, npts) alpha, sqrtg, b1..3,
+    gu11, gu12, gu13, gu22, gu23, gu33."""
+    bg = np.ascontiguousarray(bg10, dtype=np.float64).reshape(-1)
+    n = bg.size // 10
+    out = np.zeros(11 * n)
+    lib().wto_fo_coef(n, _p(bg), _p(out))
+    return out.reshape(11, n)
+
+
 def build():
     """Compile libwt_oracle.so with the Makefile beside this file."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -69,6 +107,10 @@ def lib():
         L.wto_rk4.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_int, dp]
         L.wto_energy.argtypes = [vp, dp, dp]
         L.wto_energy.restype = ctypes.c_double
+        L.wto_fo_coef.argtypes = [ctypes.c_long, dp, dp]
+        L.wto_fo_rhs.argtypes = [vp, dp, dp, dp, ctypes.c_double, ctypes.POINTER(BC3d), dp, dp]
+        L.wto_fo_energy.argtypes = [vp, dp, dp, dp]
+        L.wto_fo_energy.restype = ctypes.c_double
         L.wto_assemble_1d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, dp]
         _lib = L
@@ -191,3 +233,32 @@ class Oracle:
 
     def energy(self, u, v):
         return lib().wto_energy(self.ctx, _p(u), _p(v))
+
+    # first-order (Phi, Pi) system, affine meshes
+    def fo_rhs(self, Phi, Pi, coef, eps_ko=0.1, bc=None):
+        """wave3d_curved_rhs!(Phidot, Pidot, Phi, Pi, coef; eps_KO, bc3d)
+        (src/wave3d_curved.jl:205-258); coef: (11, npts) from fo_coef."""
+        coef = np.ascontiguousarray(coef, dtype=np.float64).reshape(-1)
+        assert coef.size == 11 * self.n
+        dPhi = self._new(); dPi = self._new()
+        lib().wto_fo_rhs(self.ctx, _p(Phi), _p(Pi), _p(coef), float(eps_ko),
+                         None if bc is None else ctypes.byref(bc), _p(dPhi), _p(dPi))
+        return dPhi, dPi
+
+    def fo_energy(self, Phi, Pi, coef):
+        coef = np.ascontiguousarray(coef, dtype=np.float64).reshape(-1)
+        return lib().wto_fo_energy(self.ctx, _p(Phi), _p(Pi), _p(coef))
+
+    def fo_rk4(self, Phi, Pi, coef, dt, nsteps, eps_ko=0.1, bc=None):
+        """Classical RK4 of the first-order system in the stage order of
+        test/wave3d_tests.jl:32-44."""
+        Phi = np.array(Phi, dtype=np.float64, copy=True)
+        Pi = np.array(Pi, dtype=np.float64, copy=True)
+        for _ in range(nsteps):
+            k1 = self.fo_rhs(Phi, Pi, coef, eps_ko, bc)
+            k2 = self.fo_rhs(Phi + dt / 2 * k1[0], Pi + dt / 2 * k1[1], coef, eps_ko, bc)
+            k3 = self.fo_rhs(Phi + dt / 2 * k2[0], Pi + dt / 2 * k2[1], coef, eps_ko, bc)
+            k4 = self.fo_rhs(Phi + dt * k3[0], Pi + dt * k3[1], coef, eps_ko, bc)
+            Phi += dt / 6 * (k1[0] + 2 * k2[0] + 2 * k3[0] + k4[0])
+            Pi += dt / 6 * (k1[1] + 2 * k2[1] + 2 * k3[1] + k4[1])
+        return Phi, Pi

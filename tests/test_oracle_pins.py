@@ -530,3 +530,205 @@ def test_sommerfeld_absorbs_outgoing_plane_wave():
     assert Es[-1] < 1e-6 * Es[0]
     Ec = run((1, 2), M=6)
     assert Es[-1] / Es[0] < 1e-3 * Ec[-1] / Ec[0]
+
+
+# ------------------------------------------------------------------ first-order (Phi, Pi) system
+
+def _flat_bg(n):
+    return np.tile([1.0, 0, 0, 0, 1.0, 0, 0, 1.0, 0, 1.0], n)
+
+
+def _dxmin(N, M):
+    xi, _, _ = wo.gll(N)
+    return np.min(np.diff(xi)) / M
+
+
+def test_fo_periodic_plane_wave_energy_and_convergence():
+    """test/wave3d_tests.jl:48-69: periodic plane wave sin(k(x+y+z) - wt) on the
+    flat background, N = 4, M = 4 and 8, to t ~ 0.2 with dt = 0.1 dx_min: energy
+    drift < 1e-3 and the error falls by more than 4x."""
+    N = 4
+    kap = 2 * math.pi; om = kap * math.sqrt(3.0)
+    errs = []
+    for M in (4, 8):
+        o = wo.Oracle(N, M, periodic=True)
+        coef = wo.fo_coef(_flat_bg(o.n))
+        x = o.coords(); s = x.sum(axis=1)
+        Phi = np.sin(kap * s); Pi = -om * np.cos(kap * s)
+        E0 = o.fo_energy(Phi, Pi, coef)
+        dt = 0.1 * _dxmin(N, M); nst = int(math.ceil(0.2 / dt))
+        Phi, Pi = o.fo_rk4(Phi, Pi, coef, dt, nst, eps_ko=0.0)
+        E1 = o.fo_energy(Phi, Pi, coef)
+        assert abs(E1 / E0 - 1) < 1e-3
+        ex = np.sin(kap * s - om * nst * dt)
+        errs.append(math.sqrt(np.sum((Phi - ex) ** 2 * o.hphys())))
+    assert errs[0] / errs[1] > 4
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_fo_spectrum_stable(periodic):
+    """test/wave3d_tests.jl:71-90: N = 4, M = 2, KO 0.1, periodic or Sommerfeld on
+    all six faces: max Re(lambda) <= 1e-5 max |lambda| of the dense RHS."""
+    o = wo.Oracle(4, 2, periodic=periodic)
+    coef = wo.fo_coef(_flat_bg(o.n))
+    bc = None if periodic else wo.make_bc3d(["sommerfeld"] * 6)
+    nn = o.n
+    A = np.zeros((2 * nn, 2 * nn))
+    z = np.zeros(nn)
+    for j in range(2 * nn):
+        P = z.copy(); Q = z.copy()
+        (P if j < nn else Q)[j % nn] = 1.0
+        a, b = o.fo_rhs(P, Q, coef, 0.1, bc)
+        A[:nn, j] = a; A[nn:, j] = b
+    lam = np.linalg.eigvals(A)
+    assert np.max(lam.real) <= 1e-5 * np.max(np.abs(lam))
+
+
+def test_fo_noise_bounded_with_sommerfeld():
+    """test/wave3d_tests.jl:92-101: noise of size sqrt(eps), Sommerfeld faces, KO
+    0.1, to t = 8: stays finite and below 1000x its size."""
+    N, M = 4, 2
+    o = wo.Oracle(N, M)
+    coef = wo.fo_coef(_flat_bg(o.n))
+    rng = np.random.default_rng(20260604)
+    amp = math.sqrt(np.finfo(float).eps)
+    Phi = amp * rng.standard_normal(o.n); Pi = amp * rng.standard_normal(o.n)
+    dt = 0.1 * _dxmin(N, M); nst = int(math.ceil(8 / dt))
+    Phi, Pi = o.fo_rk4(Phi, Pi, coef, dt, nst, eps_ko=0.1, bc=wo.make_bc3d(["sommerfeld"] * 6))
+    assert np.all(np.isfinite(Phi)) and np.max(np.abs(Phi)) < 1000 * amp
+
+
+def _metric_bg(x, alpha_on=False, shift_on=False):
+    """A smooth periodic static background: gamma SPD with off-diagonal terms;
+    alpha = 1 and beta = 0 unless switched on."""
+    X, Y, Z = (2 * math.pi * x[:, d] for d in range(3))
+    n = x.shape[0]
+    bg = np.zeros((n, 10))
+    bg[:, 0] = 1.0 + (0.2 * np.sin(X) * np.cos(Y) if alpha_on else 0.0)
+    if shift_on:
+        bg[:, 1] = 0.1 * np.cos(Y); bg[:, 2] = 0.05 * np.sin(Z); bg[:, 3] = -0.08 * np.cos(X + Z)
+    bg[:, 4] = 1.2 + 0.2 * np.sin(Z)
+    bg[:, 5] = 0.1 * np.cos(X)
+    bg[:, 6] = 0.05 * np.sin(Y + Z)
+    bg[:, 7] = 1.0 + 0.15 * np.cos(X - Y)
+    bg[:, 8] = -0.07 * np.sin(X)
+    bg[:, 9] = 0.9 + 0.1 * np.sin(Y)
+    return bg.reshape(-1)
+
+
+def test_fo_energy_exactly_conserved_on_static_metric():
+    """With alpha = 1, beta = 0 and any static gamma the semi-discrete energy of
+    wave3d_energy is conserved exactly on a periodic mesh (H D skew): its rate,
+    taken as the central difference of the quadratic E along the RHS, vanishes
+    to rounding; with a varying lapse it does not (the rate is O(1))."""
+    o = wo.Oracle(4, (3, 2, 2), periodic=True)
+    x = o.coords()
+    rng = np.random.default_rng(5)
+    Phi = rng.standard_normal(o.n); Pi = rng.standard_normal(o.n)
+    rates = []
+    for alpha_on in (False, True):
+        coef = wo.fo_coef(_metric_bg(x, alpha_on=alpha_on))
+        dPhi, dPi = o.fo_rhs(Phi, Pi, coef, eps_ko=0.0)
+        d = 1e-3
+        Ep = o.fo_energy(Phi + d * dPhi, Pi + d * dPi, coef)
+        Em = o.fo_energy(Phi - d * dPhi, Pi - d * dPi, coef)
+        scale = o.fo_energy(Phi, Pi, coef) * math.sqrt(np.sum(dPhi ** 2 + dPi ** 2) / np.sum(Phi ** 2 + Pi ** 2))
+        rates.append(abs((Ep - Em) / (2 * d)) / scale)
+    assert rates[0] < 1e-11
+    assert rates[1] > 1e-3 * 1e-3 and rates[1] > 1e4 * rates[0]
+
+
+def test_fo_rhs_converges_to_the_pde():
+    """The discrete RHS against the continuous one,
+    dPhi/dt = b.grad Phi + (alpha/sqrtg) Pi, dPi/dt = div(b Pi + alpha sqrtg g^-1 grad Phi),
+    for smooth periodic data and a background with lapse, shift and an
+    off-diagonal metric (the divergence of the continuous flux by 4th-order
+    central differences of its closed form, step 1e-3): spectral convergence
+    in M at N = 6."""
+    tp = 2 * math.pi
+
+    def bgv(x):
+        X, Y, Z = x[..., 0], x[..., 1], x[..., 2]
+        al = 1 + 0.2 * np.sin(tp * X) * np.cos(tp * Y)
+        be = np.stack([0.1 * np.cos(tp * Y), 0.05 * np.sin(tp * Z), -0.08 * np.cos(tp * (X + Z))], -1)
+        g = np.empty(X.shape + (3, 3))
+        g[..., 0, 0] = 1.2 + np.sin(tp * Z) / 5
+        g[..., 0, 1] = g[..., 1, 0] = np.cos(tp * X) / 10
+        g[..., 0, 2] = g[..., 2, 0] = np.sin(tp * (Y + Z)) / 20
+        g[..., 1, 1] = 1 + 0.15 * np.cos(tp * (X - Y))
+        g[..., 1, 2] = g[..., 2, 1] = -0.07 * np.sin(tp * X)
+        g[..., 2, 2] = 0.9 + np.sin(tp * Y) / 10
+        return al, be, g
+
+    def fields(x):
+        X, Y, Z = x[..., 0], x[..., 1], x[..., 2]
+        Phi = np.sin(tp * X + 1) * np.cos(tp * Y) + np.sin(tp * Z)
+        Pi = np.cos(tp * (X + Y + Z))
+        grad = np.stack([tp * np.cos(tp * X + 1) * np.cos(tp * Y), -tp * np.sin(tp * X + 1) * np.sin(tp * Y),
+                         tp * np.cos(tp * Z)], -1)
+        return Phi, Pi, grad
+
+    def flux(x):
+        al, be, g = bgv(x)
+        _, Pi, grad = fields(x)
+        sg = np.sqrt(np.linalg.det(g))
+        return be * Pi[..., None] + (al * sg)[..., None] * np.einsum("...ij,...j->...i", np.linalg.inv(g), grad)
+
+    def div_flux(x):
+        d = 1e-3
+        out = np.zeros(x.shape[0])
+        for i in range(3):
+            e = np.zeros(3); e[i] = d
+            out += (8 * (flux(x + e)[:, i] - flux(x - e)[:, i]) - (flux(x + 2 * e)[:, i] - flux(x - 2 * e)[:, i])) / (12 * d)
+        return out
+
+    errs = []
+    for M in (2, 4):
+        o = wo.Oracle(6, M, periodic=True)
+        x = o.coords()
+        al, be, g = bgv(x)
+        bg = np.concatenate([al[:, None], be, g[:, 0, 0:3], g[:, 1, 1:3], g[:, 2, 2:3]], axis=1).reshape(-1)
+        coef = wo.fo_coef(bg)
+        Phi, Pi, grad = fields(x)
+        sg = np.sqrt(np.linalg.det(g))
+        dPhi_ex = np.sum(be * grad, axis=1) + al / sg * Pi
+        dPi_ex = div_flux(x)
+        dP, dQ = o.fo_rhs(np.ascontiguousarray(Phi), np.ascontiguousarray(Pi), coef, eps_ko=0.0)
+        errs.append((np.max(np.abs(dP - dPhi_ex)) / np.max(np.abs(dPhi_ex)),
+                     np.max(np.abs(dQ - dPi_ex)) / np.max(np.abs(dPi_ex))))
+    assert errs[1][0] < 1e-3 and errs[1][1] < 1e-2, errs
+    assert errs[0][0] / errs[1][0] > 16 and errs[0][1] / errs[1][1] > 8, errs
+
+
+def test_fo_boundary_pass_terms():
+    """The affine boundary pass (src/boundaries3d.jl:76-126) on a flat background:
+    full-state Dirichlet pulls (Phi, Pi) to the targets at the face nodes only, with
+    rate sigma (|a_n - b| + |a_n + b|) / (h w); Sommerfeld drives r = Pi + d_n Phi;
+    excision adds nothing."""
+    N, M = 4, 2
+    o = wo.Oracle(N, M)
+    coef = wo.fo_coef(_flat_bg(o.n))
+    rng = np.random.default_rng(9)
+    Phi = rng.standard_normal(o.n); Pi = rng.standard_normal(o.n)
+    base = o.fo_rhs(Phi, Pi, coef, eps_ko=0.0)
+    ex = o.fo_rhs(Phi, Pi, coef, eps_ko=0.0, bc=wo.make_bc3d(["excision"] * 6))
+    assert np.array_equal(base[0], ex[0]) and np.array_equal(base[1], ex[1])
+    x = o.coords()
+    xi, w, _ = wo.gll(N)
+    h = 1.0 / M
+    on1 = np.isclose(x[:, 0], 0.0)              # face 1 (-x)
+    gP = np.full(o.n, 0.3); gQ = np.full(o.n, -0.2)
+    fd = o.fo_rhs(Phi, Pi, coef, eps_ko=0.0,
+                  bc=wo.make_bc3d(["full_dirichlet", "excision", "excision", "excision", "excision", "excision"],
+                                  sigma=0.5, gPhi=gP, gPi=gQ))
+    rate = 0.5 * 2.0 / (h * w[0])              # a_n = 1, beta = 0
+    assert np.allclose(fd[0][on1] - base[0][on1], -rate * (Phi[on1] - 0.3), rtol=1e-12, atol=1e-12)
+    assert np.allclose(fd[1][on1] - base[1][on1], -rate * (Pi[on1] + 0.2), rtol=1e-12, atol=1e-12)
+    assert np.array_equal(fd[0][~on1], base[0][~on1]) and np.array_equal(fd[1][~on1], base[1][~on1])
+    # Sommerfeld on +x: r = Pi + d_x Phi (n = +1), rate 1 / (h w_N)
+    onx = np.isclose(x[:, 0], 1.0)
+    so = o.fo_rhs(Phi, Pi, coef, eps_ko=0.0,
+                  bc=wo.make_bc3d(["excision", "sommerfeld", "excision", "excision", "excision", "excision"]))
+    Dx = o.apply_D(0, Phi)
+    assert np.allclose(so[1][onx] - base[1][onx], -(Pi[onx] + Dx[onx]) / (h * w[-1]), rtol=1e-12, atol=1e-10)
+    assert np.array_equal(so[0], base[0])
