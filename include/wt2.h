@@ -148,6 +148,45 @@ int wt2_rk4_steps(wt2_ctx* ctx, double* u, double* ud, double dt, int32_t nsteps
  * deterministic reduction order. */
 int wt2_energy(wt2_ctx* ctx, const double* u, const double* ud, double* E, int32_t flags);
 
+/* ---- the first-order (Phi, Pi) system on affine meshes ----
+ * coef11: 11 fields of npoints values, one after the other, in the order of
+ * make_coef3d / sample_background3d! (src/wave3d_curved.jl:75-124):
+ * alpha, sqrt(det gamma), beta1, beta2, beta3, gamma^11, gamma^12, gamma^13,
+ * gamma^22, gamma^23, gamma^33.  The context must be flat (no warp, no
+ * background); its mu is the reference's (apply_D! power iteration). */
+#define WT2_BC_DIRICHLET      1   /* src/boundaries1d.jl:59-62 */
+#define WT2_BC_SOMMERFELD     2
+#define WT2_BC_EXCISION       3
+#define WT2_BC_FULL_DIRICHLET 4
+
+/* make_bc3d (src/boundaries3d.jl:44-54) for the affine pass. */
+typedef struct wt2_bc3d {
+    int32_t kinds[6];        /* WT2_BC_* per face -x,+x,-y,+y,-z,+z          */
+    int32_t reserved;
+    double  sigma;           /* penalty scale (reference default 1)           */
+    const double* gPhi;      /* field-radiation / full-state targets, per node */
+    const double* gPi;       /* (nullable: zero)                               */
+} wt2_bc3d;
+
+/* sample_background3d! (src/wave3d_curved.jl:75-115) on the device: coef11
+ * from 10 background values per node (node-major, as wt2_ctx_set_background). */
+int wt2_fo_coef(wt2_ctx* ctx, const double* bg10, double* coef11, int32_t flags);
+
+/* wave3d_curved_rhs!(Phidot, Pidot, Phi, Pi, coef; geom, ops, ws, eps_KO, bc3d)
+ * on the affine path (metric === nothing, src/wave3d_curved.jl:205-258):
+ * DPhi_d = apply_D!(Phi, d); F^i = b^i Pi + alpha sqrtg g^{ij} DPhi_j;
+ * Pidot = sum_i apply_D!(F^i, i); Phidot = b . DPhi + (alpha/sqrtg) Pi; then
+ * eps mu^-5 D_d^6 on both fields; then the affine boundary pass
+ * (src/boundaries3d.jl:76-126) when bc3d is non-null.  FP64. */
+int wt2_fo_rhs(wt2_ctx* ctx, double* dPhi, double* dPi, const double* Phi, const double* Pi,
+               const double* coef11, double eps_ko, const wt2_bc3d* bc3d, int32_t flags);
+
+/* wave3d_energy (src/wave3d_curved.jl:268-287):
+ * E = sum 1/2 [Pi^2/sqrtg + sqrtg g^{ij} DPhi_i DPhi_j] Hphys, FP64,
+ * deterministic reduction order. */
+int wt2_fo_energy(wt2_ctx* ctx, const double* Phi, const double* Pi, const double* coef11, double* E,
+                  int32_t flags);
+
 /* Use an external HIP stream (hipStream_t passed as void*); NULL restores the
  * context's own stream. */
 int wt2_set_stream(wt2_ctx* ctx, void* stream);

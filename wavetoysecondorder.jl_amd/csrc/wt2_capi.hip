@@ -9,6 +9,7 @@
 #include <mutex>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -227,6 +228,7 @@ struct wt2_ctx {
     void* op32;     // operator table (FP32)
     int Ecap;
     CurvedData cd;
+    double* fo_D;   // first-order path: DPhi_1..3 (lazy)
     int64_t bytes;
     // captured RK4 step
     hipGraphExec_t gexec;
@@ -843,6 +845,7 @@ void wt2_ctx_destroy(wt2_ctx* c)
     for (int f = 0; f < 2; ++f) { hipFree(c->Y[f]); hipFree(c->O[f]); }
     hipFree(c->Lu); hipFree(c->red); hipFree(c->Edev);
     hipFree(c->op64); hipFree(c->op32);
+    hipFree(c->fo_D);
     curved_free(c->cd);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
     delete c;
@@ -1188,6 +1191,173 @@ int wt2_energy(wt2_ctx* c, const double* u, const double* ud, double* E, int32_t
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(E, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     return sync(c);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- first-order (Phi, Pi) system
+
+namespace {
+
+int fo_ready(wt2_ctx* c, const char* who)
+{
+    if (c->curved || c->d.warp_A != 0.0) {
+        set_error(std::string(who) + ": the first-order path is built for affine meshes without a context background "
+                                     "(the coefficients come in coef11)");
+        return WT2_ERR_UNSUPPORTED;
+    }
+    if (!c->fo_D) return alloc_dev(c, reinterpret_cast<void**>(&c->fo_D), size_t(3) * size_t(c->npts) * sizeof(double));
+    return WT2_OK;
+}
+
+void fo_desc(const wt2_ctx* c, FoDesc& fd)
+{
+    std::memset(&fd, 0, sizeof(fd));
+    fd.N = c->N;
+    for (int a = 0; a < 3; ++a) { fd.M[a] = c->d.M[a]; fd.periodic[a] = c->d.periodic[a]; fd.h[a] = c->h[a]; }
+    for (int i = 0; i < c->N; ++i) fd.w[i] = c->w[i];
+    fd.op64 = c->op64;
+    fd.D = c->fo_D;
+}
+
+// Device copies of host arrays for a host-pointer call (freed by the caller).
+struct FoTemp {
+    std::vector<void*> bufs;
+    ~FoTemp() { for (void* p : bufs) hipFree(p); }
+    int in(wt2_ctx* c, const double* src, size_t count, const double** dst)
+    {
+        void* p = nullptr;
+        if (hipMalloc(&p, count * sizeof(double)) != hipSuccess) { set_error("device staging allocation failed"); return WT2_ERR_OOM; }
+        bufs.push_back(p);
+        if (src && hipMemcpyAsync(p, src, count * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+            set_error("host-to-device copy failed");
+            return WT2_ERR_HIP;
+        }
+        *dst = static_cast<const double*>(p);
+        return WT2_OK;
+    }
+};
+
+bool fo_aligned(std::initializer_list<const void*> ps)
+{
+    for (const void* p : ps)
+        if (p && misaligned(p)) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wt2_fo_coef(wt2_ctx* c, const double* bg10, double* coef11, int32_t flags)
+{
+    if (!c || !bg10 || !coef11) { set_error("wt2_fo_coef: null pointer"); return WT2_ERR_ARG; }
+    const bool host = flags & WT2_HOST_PTRS;
+    const size_t n = size_t(c->npts);
+    FoTemp tmp;
+    const double* bg = bg10;
+    double* out = coef11;
+    int rc = WT2_OK;
+    if (host) {
+        const double* o = nullptr;
+        rc = tmp.in(c, bg10, 10 * n, &bg);
+        if (!rc) rc = tmp.in(c, nullptr, 11 * n, &o);
+        out = const_cast<double*>(o);
+    } else if (!fo_aligned({bg10, coef11})) {
+        set_error("device field pointers must be 16-byte aligned");
+        return WT2_ERR_ARG;
+    }
+    if (!rc) rc = pass_rc(launch_fo_coef(bg, out, c->npts, c->stream), "first-order coefficients");
+    if (!rc && host)
+        rc = pass_rc(hipMemcpyAsync(coef11, out, 11 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy out");
+    if (!rc) rc = sync(c);
+    return rc;
+}
+
+int wt2_fo_rhs(wt2_ctx* c, double* dPhi, double* dPi, const double* Phi, const double* Pi,
+               const double* coef11, double eps_ko, const wt2_bc3d* bc3d, int32_t flags)
+{
+    if (!c || !dPhi || !dPi || !Phi || !Pi || !coef11) { set_error("wt2_fo_rhs: null pointer"); return WT2_ERR_ARG; }
+    if (!(eps_ko >= 0) || !std::isfinite(eps_ko)) { set_error("wt2_fo_rhs: eps_ko must be finite and >= 0"); return WT2_ERR_ARG; }
+    if (bc3d) {
+        for (int f = 0; f < 6; ++f)
+            if (bc3d->kinds[f] < WT2_BC_DIRICHLET || bc3d->kinds[f] > WT2_BC_FULL_DIRICHLET) {
+                set_error("wt2_fo_rhs: bc3d kinds must be WT2_BC_DIRICHLET..WT2_BC_FULL_DIRICHLET");
+                return WT2_ERR_ARG;
+            }
+        if (!(bc3d->sigma >= 0) || !std::isfinite(bc3d->sigma)) { set_error("wt2_fo_rhs: bc3d sigma must be finite and >= 0"); return WT2_ERR_ARG; }
+    }
+    int rc = fo_ready(c, "wt2_fo_rhs");
+    if (rc) return rc;
+    const bool host = flags & WT2_HOST_PTRS;
+    const size_t n = size_t(c->npts);
+    FoDesc fd;
+    fo_desc(c, fd);
+    fd.Phi = Phi; fd.Pi = Pi; fd.coef = coef11; fd.dPhi = dPhi; fd.dPi = dPi;
+    fd.gPhi = bc3d ? bc3d->gPhi : nullptr;
+    fd.gPi = bc3d ? bc3d->gPi : nullptr;
+    FoTemp tmp;
+    if (host) {
+        const double* o1 = nullptr; const double* o2 = nullptr;
+        rc = tmp.in(c, Phi, n, &fd.Phi);
+        if (!rc) rc = tmp.in(c, Pi, n, &fd.Pi);
+        if (!rc) rc = tmp.in(c, coef11, 11 * n, &fd.coef);
+        if (!rc && fd.gPhi) rc = tmp.in(c, bc3d->gPhi, n, &fd.gPhi);
+        if (!rc && fd.gPi) rc = tmp.in(c, bc3d->gPi, n, &fd.gPi);
+        if (!rc) rc = tmp.in(c, nullptr, n, &o1);
+        if (!rc) rc = tmp.in(c, nullptr, n, &o2);
+        fd.dPhi = const_cast<double*>(o1); fd.dPi = const_cast<double*>(o2);
+        if (rc) return rc;
+    } else if (!fo_aligned({dPhi, dPi, Phi, Pi, coef11, fd.gPhi, fd.gPi})) {
+        set_error("device field pointers must be 16-byte aligned");
+        return WT2_ERR_ARG;
+    }
+    fd.koT = eps_ko * c->inv_mu5;
+    if (bc3d) {
+        for (int f = 0; f < 6; ++f) fd.kinds[f] = bc3d->kinds[f];
+        fd.sigma = bc3d->sigma;
+    } else {
+        for (int f = 0; f < 6; ++f) fd.kinds[f] = WT2_BC_EXCISION;   // no boundary terms
+        fd.sigma = 0.0;
+    }
+    rc = pass_rc(launch_fo_rhs(fd, c->stream), "first-order RHS");
+    if (!rc && host) {
+        rc = pass_rc(hipMemcpyAsync(dPhi, fd.dPhi, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy out");
+        if (!rc) rc = pass_rc(hipMemcpyAsync(dPi, fd.dPi, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy out");
+    }
+    if (!rc) rc = sync(c);
+    return rc;
+}
+
+int wt2_fo_energy(wt2_ctx* c, const double* Phi, const double* Pi, const double* coef11, double* E, int32_t flags)
+{
+    if (!c || !Phi || !Pi || !coef11 || !E) { set_error("wt2_fo_energy: null pointer"); return WT2_ERR_ARG; }
+    int rc = fo_ready(c, "wt2_fo_energy");
+    if (rc) return rc;
+    const bool host = flags & WT2_HOST_PTRS;
+    const size_t n = size_t(c->npts);
+    FoDesc fd;
+    fo_desc(c, fd);
+    fd.Phi = Phi; fd.Pi = Pi; fd.coef = coef11;
+    FoTemp tmp;
+    if (host) {
+        rc = tmp.in(c, Phi, n, &fd.Phi);
+        if (!rc) rc = tmp.in(c, Pi, n, &fd.Pi);
+        if (!rc) rc = tmp.in(c, coef11, 11 * n, &fd.coef);
+        if (rc) return rc;
+    } else if (!fo_aligned({Phi, Pi, coef11})) {
+        set_error("device field pointers must be 16-byte aligned");
+        return WT2_ERR_ARG;
+    }
+    fd.grad_only = 1;
+    for (int f = 0; f < 6; ++f) fd.kinds[f] = WT2_BC_EXCISION;
+    fd.partial = c->red;
+    fd.Eout = c->red + RED_BLOCKS;
+    rc = pass_rc(launch_fo_rhs(fd, c->stream), "first-order gradient");
+    if (!rc) rc = pass_rc(launch_fo_energy(fd, c->stream), "first-order energy");
+    if (!rc) rc = pass_rc(hipMemcpyAsync(E, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy out");
+    if (!rc) rc = sync(c);
+    return rc;
 }
 
 }  // extern "C"

@@ -1,0 +1,400 @@
+// wt2_fo.hip -- the first-order (Phi, Pi) ADM right-hand side of
+// wave3d_curved_rhs! (src/wave3d_curved.jl:205-258) on axis-aligned affine
+// meshes: the per-axis SAT-coupled derivative apply_D! (one-sided at outer
+// faces), the flux F^i = b^i Pi + alpha sqrtg g^{ij} DPhi_j and its divergence,
+// eps mu^-5 D_d^6 on both fields (_ko_axis3!, :182-192), and the affine
+// outer-boundary pass (_bc3d_affine_kernel!, src/boundaries3d.jl:76-126).
+//
+// Six pencil passes per call.  Pass (axis d, phase 0): DPhi_d = D_d Phi, kept
+// in a context buffer, and the KO terms of both fields along d.  Pass (axis d,
+// phase 1): dPi += D_d F^d with F^d formed at load time from the coefficients,
+// Pi and the three DPhi; the last one (z) also adds b . DPhi + (alpha/sqrtg) Pi
+// to dPhi and the boundary terms.  A workgroup owns whole element lines along
+// its axis (LT lines x Ma elements, work-item = (element, line) with adjacent
+// lines in adjacent lanes), and the face values of neighbouring elements go
+// through LDS, as in the second-order pencil passes.
+#include "wt2_internal.hpp"
+#include "wt2_ops.hpp"
+
+#include <algorithm>
+
+namespace wt2 {
+
+namespace {
+
+constexpr int FO_MAXT = 512;
+
+struct FoArgs {
+    const double* Phi; const double* Pi;
+    const double* coef;      // 11 fields of npts (alpha, sqrtg, b1..3, gu11, gu12, gu13, gu22, gu23, gu33)
+    double* D;               // 3 fields: DPhi_1..3
+    double* dPhi; double* dPi;
+    const double* gPhi; const double* gPi;
+    const double* op;        // OpLayout<N> table
+    int64_t npts;
+    int M0, M1, M2;
+    int per0, per1, per2;
+    int axis, phase, first, last;
+    int Ma, T0, slo, shi, estride, LT, nbk, LA, LB, GS;
+    double ih, sat0, satN;   // 1/h; SAT -1/(2 w_0 h), +1/(2 w_{N-1} h) along the axis
+    double koT;              // eps mu^-5 (0: no KO)
+    int kinds[6];
+    double sigma;
+    double bw[3][2];         // boundary weight 1/(h_d w_face) per axis, low / high face
+};
+
+// Face values of the line's neighbouring elements through LDS (two
+// alternating buffers; every work-item of the workgroup makes each call).
+struct FoX {
+    double* buf;
+    int me, idxL, idxR, k;
+    __device__ __forceinline__ void operator()(double last, double first, double& fromL, double& fromR)
+    {
+        double* b = buf + (k & 1) * 2 * FO_MAXT;
+        b[me] = last;
+        b[FO_MAXT + me] = first;
+        __syncthreads();
+        fromL = b[idxL];
+        fromR = b[FO_MAXT + idxR];
+        ++k;
+    }
+};
+
+// y = apply_D!(x) on this segment: (1/h) G x plus the centred-flux SAT at
+// interior faces only (one-sided at outer faces, METHODS.md:256-264).
+template <int N>
+__device__ __forceinline__ void fo_D(const double* __restrict__ op, const FoArgs& a, FoX& xch, bool hasL, bool hasR,
+                                     const double (&x)[N], double (&y)[N])
+{
+    double xl, xr;
+    xch(x[N - 1], x[0], xl, xr);
+    double g[N];
+    apply_G<N, double>(op, x, g);
+#pragma unroll
+    for (int i = 0; i < N; ++i) y[i] = g[i] * a.ih;
+    if (hasL) y[0] = fma(a.sat0, xl - x[0], y[0]);
+    if (hasR) y[N - 1] = fma(a.satN, xr - x[N - 1], y[N - 1]);
+}
+
+template <int N>
+__device__ __forceinline__ void fo_load(const double* __restrict__ p, int gs, bool on, double (&x)[N])
+{
+#pragma unroll
+    for (int r = 0; r < N; ++r) x[r] = on ? p[int64_t(r) * gs] : 0.0;
+}
+
+template <int N>
+__device__ __forceinline__ void fo_store(double* __restrict__ p, int gs, bool on, const double (&x)[N])
+{
+    if (!on) return;
+#pragma unroll
+    for (int r = 0; r < N; ++r) p[int64_t(r) * gs] = x[r];
+}
+
+template <int N>
+__global__ void __launch_bounds__(FO_MAXT) fo_pass_kernel(FoArgs a)
+{
+    constexpr int N3 = N * N * N;
+    __shared__ double op[OpLayout<N>::SIZE];
+    __shared__ double xbuf[4 * FO_MAXT];
+    for (int q = threadIdx.x; q < OpLayout<N>::SIZE; q += blockDim.x) op[q] = a.op[q];
+    __syncthreads();
+
+    int t = blockIdx.x;
+    const int blk = t % a.nbk;
+    t /= a.nbk;
+    const int ebase = (t % a.T0) * a.slo + (t / a.T0) * a.shi;
+    const int tid = threadIdx.x;
+    const int m = tid / a.LT;
+    const int line = blk * a.LT + (tid - m * a.LT);
+    const int Ma = a.Ma;
+    const bool active = m < Ma && line < N * N;
+    const int per = a.axis == 0 ? a.per0 : (a.axis == 1 ? a.per1 : a.per2);
+    const bool hasL = active && (m > 0 || per);
+    const bool hasR = active && (m < Ma - 1 || per);
+    FoX xch;
+    xch.buf = xbuf; xch.me = tid; xch.k = 0;
+    xch.idxL = hasL ? (m > 0 ? m - 1 : Ma - 1) * a.LT + (tid - m * a.LT) : tid;
+    xch.idxR = hasR ? (m < Ma - 1 ? m + 1 : 0) * a.LT + (tid - m * a.LT) : tid;
+    const int mm = active ? m : 0, ll = active ? line : 0;
+    const int e = ebase + mm * a.estride;
+    const int64_t g0 = int64_t(e) * N3 + (ll / N) * a.LA + (ll % N) * a.LB;
+    const int gs = a.GS;
+    const int64_t n = a.npts;
+
+    if (a.phase == 0) {
+        double x[N], y[N];
+        fo_load<N>(a.Phi + g0, gs, active, x);
+        fo_D<N>(op, a, xch, hasL, hasR, x, y);
+        fo_store<N>(a.D + a.axis * n + g0, gs, active, y);
+        for (int fld = 0; fld < 2; ++fld) {
+            double* out = (fld == 0 ? a.dPhi : a.dPi) + g0;
+            if (a.koT != 0.0) {
+                if (fld == 1) fo_load<N>(a.Pi + g0, gs, active, x);
+#pragma unroll 1
+                for (int rep = 0; rep < 3; ++rep) {
+                    fo_D<N>(op, a, xch, hasL, hasR, x, y);
+                    fo_D<N>(op, a, xch, hasL, hasR, y, x);
+                }
+                double acc[N];
+                if (a.first) {
+#pragma unroll
+                    for (int r = 0; r < N; ++r) acc[r] = a.koT * x[r];
+                } else {
+                    fo_load<N>(out, gs, active, acc);
+#pragma unroll
+                    for (int r = 0; r < N; ++r) acc[r] = fma(a.koT, x[r], acc[r]);
+                }
+                fo_store<N>(out, gs, active, acc);
+            } else if (a.first) {
+                double z[N];
+#pragma unroll
+                for (int r = 0; r < N; ++r) z[r] = 0.0;
+                fo_store<N>(out, gs, active, z);
+            }
+        }
+        return;
+    }
+
+    // phase 1: dPi += D_d F^d, F^d = b^d Pi + alpha sqrtg (g^{d1} DPhi_1 + g^{d2} DPhi_2 + g^{d3} DPhi_3)
+    const int d = a.axis;
+    const int gi0 = d == 0 ? 5 : (d == 1 ? 6 : 7);     // g^{d1}
+    const int gi1 = d == 0 ? 6 : (d == 1 ? 8 : 9);     // g^{d2}
+    const int gi2 = d == 0 ? 7 : (d == 1 ? 9 : 10);    // g^{d3}
+    double F[N], DF[N];
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        const int64_t q = g0 + int64_t(r) * gs;
+        double f = 0.0;
+        if (active) {
+            const double* c = a.coef;
+            f = c[(2 + d) * n + q] * a.Pi[q] +
+                c[q] * c[n + q] * (c[gi0 * n + q] * a.D[q] + c[gi1 * n + q] * a.D[n + q] + c[gi2 * n + q] * a.D[2 * n + q]);
+        }
+        F[r] = f;
+    }
+    fo_D<N>(op, a, xch, hasL, hasR, F, DF);
+    if (!active) return;
+    const double* c = a.coef;
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        const int64_t q = g0 + int64_t(r) * gs;
+        a.dPi[q] += DF[r];
+    }
+    if (!a.last) return;
+    // dPhi += b . DPhi + (alpha / sqrtg) Pi, then the outer-boundary terms
+    const int ex = e % a.M0, ey = (e / a.M0) % a.M1, ez = e / (a.M0 * a.M1);
+    const int ec[3] = {ex, ey, ez};
+    const int Mv[3] = {a.M0, a.M1, a.M2};
+    const int pv[3] = {a.per0, a.per1, a.per2};
+    const int la = ll % N, lb = ll / N;
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        const int64_t q = g0 + int64_t(r) * gs;
+        const double al = c[q], sg = c[n + q];
+        const double D1 = a.D[q], D2 = a.D[n + q], D3 = a.D[2 * n + q];
+        const double Pq = a.Pi[q];
+        double dF = a.dPhi[q] + (c[2 * n + q] * D1 + c[3 * n + q] * D2 + c[4 * n + q] * D3 + (al / sg) * Pq);
+        double dP = 0.0;
+        // node (i, j, k) of the element
+        const int nd[3] = {d == 0 ? r : la, d == 1 ? r : (d == 0 ? la : lb), d == 2 ? r : lb};
+        for (int f = 0; f < 6; ++f) {
+            const int dn = f >> 1, hi = f & 1;
+            if (pv[dn] || ec[dn] != (hi ? Mv[dn] - 1 : 0) || nd[dn] != (hi ? N - 1 : 0)) continue;
+            const int kind = a.kinds[f];
+            if (kind == WT2_BC_EXCISION) continue;
+            const double nh = hi ? 1.0 : -1.0;
+            const double gunn = c[(dn == 0 ? 5 : (dn == 1 ? 8 : 10)) * n + q];
+            const double a_n = al * sqrt(gunn);
+            const double bax = c[(2 + dn) * n + q];
+            const double bn = nh * bax;
+            const double wt = a.bw[dn][hi];
+            if (kind == WT2_BC_FULL_DIRICHLET) {
+                const double tau = a.sigma * (fabs(a_n - bax) + fabs(a_n + bax)) * wt;
+                const double gF = a.gPhi ? a.gPhi[q] : 0.0, gP = a.gPi ? a.gPi[q] : 0.0;
+                dF += -tau * (a.Phi[q] - gF);
+                dP += -tau * (Pq - gP);
+            } else {
+                const double Dn = dn == 0 ? D1 : (dn == 1 ? D2 : D3);
+                const double rr = Pq + ((bn + a_n) / (al / sg)) * (nh * Dn);
+                const double gv = (kind == WT2_BC_SOMMERFELD || !a.gPhi) ? 0.0 : a.gPhi[q];
+                dP += -a.sigma * fabs(a_n + bn) * wt * (rr - gv);
+            }
+        }
+        a.dPhi[q] = dF;
+        if (dP != 0.0) a.dPi[q] += dP;
+    }
+}
+
+template <int N>
+hipError_t launch_fo_n(const FoDesc& fd, hipStream_t s)
+{
+    FoArgs a;
+    a.Phi = fd.Phi; a.Pi = fd.Pi; a.coef = fd.coef; a.D = fd.D;
+    a.dPhi = fd.dPhi; a.dPi = fd.dPi; a.gPhi = fd.gPhi; a.gPi = fd.gPi;
+    a.op = static_cast<const double*>(fd.op64);
+    a.M0 = fd.M[0]; a.M1 = fd.M[1]; a.M2 = fd.M[2];
+    a.per0 = fd.periodic[0]; a.per1 = fd.periodic[1]; a.per2 = fd.periodic[2];
+    a.npts = int64_t(N) * N * N * fd.M[0] * fd.M[1] * fd.M[2];
+    a.koT = fd.koT;
+    for (int f = 0; f < 6; ++f) a.kinds[f] = fd.kinds[f];
+    a.sigma = fd.sigma;
+    for (int dd = 0; dd < 3; ++dd) {
+        a.bw[dd][0] = 1.0 / (fd.h[dd] * fd.w[0]);
+        a.bw[dd][1] = 1.0 / (fd.h[dd] * fd.w[N - 1]);
+    }
+    const int Mx = fd.M[0], My = fd.M[1], Mz = fd.M[2];
+    if (fd.grad_only) a.koT = 0.0;
+    for (int phase = 0; phase < (fd.grad_only ? 1 : 2); ++phase)
+        for (int d = 0; d < 3; ++d) {
+            a.axis = d; a.phase = phase;
+            a.first = phase == 0 && d == 0 && !fd.grad_only;
+            a.last = phase == 1 && d == 2;
+            const double h = fd.h[d];
+            a.ih = 1.0 / h;
+            a.sat0 = -1.0 / (2.0 * fd.w[0] * h);
+            a.satN = 1.0 / (2.0 * fd.w[N - 1] * h);
+            int rows;
+            if (d == 0) {        // rows (ey, ez), line (j, k): base N j + N^2 k
+                a.Ma = Mx; a.T0 = My; a.slo = Mx; a.shi = Mx * My; a.estride = 1;
+                a.LB = N; a.LA = N * N; a.GS = 1; rows = My * Mz;
+            } else if (d == 1) { // rows (ex, ez), line (i, k): base i + N^2 k
+                a.Ma = My; a.T0 = Mx; a.slo = 1; a.shi = Mx * My; a.estride = Mx;
+                a.LB = 1; a.LA = N * N; a.GS = N; rows = Mx * Mz;
+            } else {             // rows (ex, ey), line (i, j): base i + N j
+                a.Ma = Mz; a.T0 = Mx; a.slo = 1; a.shi = Mx; a.estride = Mx * My;
+                a.LB = 1; a.LA = N; a.GS = N * N; rows = Mx * My;
+            }
+            a.LT = std::min(N * N, FO_MAXT / a.Ma);
+            a.nbk = (N * N + a.LT - 1) / a.LT;
+            const int nthr = (a.LT * a.Ma + 63) / 64 * 64;
+            if (a.Ma > 64 || a.LT < 1 || nthr > FO_MAXT) return hipErrorInvalidConfiguration;
+            (void)hipGetLastError();
+            hipLaunchKernelGGL(fo_pass_kernel<N>, dim3(rows * a.nbk), dim3(nthr), 0, s, a);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+    return hipSuccess;
+}
+
+// coef11 from 10 background values per node (sample_background3d!'s kernel,
+// src/wave3d_curved.jl:75-96): lapse, sqrt(det gamma), shift, cofactor inverse.
+__global__ void fo_coef_kernel(const double* __restrict__ bg, double* __restrict__ c, int64_t n)
+{
+    for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
+        const double* b = bg + 10 * q;
+        const double g11 = b[4], g12 = b[5], g13 = b[6], g22 = b[7], g23 = b[8], g33 = b[9];
+        const double det = g11 * (g22 * g33 - g23 * g23) - g12 * (g12 * g33 - g23 * g13) + g13 * (g12 * g23 - g22 * g13);
+        c[q] = b[0];
+        c[n + q] = sqrt(det);
+        c[2 * n + q] = b[1];
+        c[3 * n + q] = b[2];
+        c[4 * n + q] = b[3];
+        c[5 * n + q] = (g22 * g33 - g23 * g23) / det;
+        c[6 * n + q] = (g13 * g23 - g12 * g33) / det;
+        c[7 * n + q] = (g12 * g23 - g13 * g22) / det;
+        c[8 * n + q] = (g11 * g33 - g13 * g13) / det;
+        c[9 * n + q] = (g12 * g13 - g11 * g23) / det;
+        c[10 * n + q] = (g11 * g22 - g12 * g12) / det;
+    }
+}
+
+// Per-node energy density of wave3d_energy (src/wave3d_curved.jl:268-287),
+// 1/2 [Pi^2 / sqrtg + sqrtg g^{ij} DPhi_i DPhi_j] Hphys, summed per block in a
+// fixed order; the blocks' partial sums are added in a fixed order after.
+struct FoWeights { double w[16]; };
+
+template <int N>
+__global__ void __launch_bounds__(RED_THREADS) fo_energy_partial(const double* __restrict__ Pi,
+                                                                const double* __restrict__ c,
+                                                                const double* __restrict__ D, double hvol,
+                                                                FoWeights w, int64_t n,
+                                                                double* __restrict__ partial)
+{
+    constexpr int N3 = N * N * N;
+    __shared__ double wl[N];
+    __shared__ double red[RED_THREADS];
+    if (threadIdx.x < N) wl[threadIdx.x] = w.w[threadIdx.x];
+    __syncthreads();
+    double s = 0.0;
+    for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
+        const int r = int(q % N3);
+        const double H = hvol * wl[r % N] * wl[(r / N) % N] * wl[r / (N * N)];
+        const double d1 = D[q], d2 = D[n + q], d3 = D[2 * n + q];
+        const double sg = c[n + q];
+        s += (Pi[q] * Pi[q] / sg +
+              sg * (c[5 * n + q] * d1 * d1 + c[8 * n + q] * d2 * d2 + c[10 * n + q] * d3 * d3 + 2 * c[6 * n + q] * d1 * d2 +
+                    2 * c[7 * n + q] * d1 * d3 + 2 * c[9 * n + q] * d2 * d3)) * H / 2;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = RED_THREADS / 2; k > 0; k >>= 1) {
+        if (int(threadIdx.x) < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void fo_energy_final(const double* __restrict__ partial, int nb, double* __restrict__ out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int b = 0; b < nb; ++b) s += partial[b];
+        *out = s;
+    }
+}
+
+template <int N>
+hipError_t launch_fo_energy_n(const FoDesc& fd, hipStream_t s)
+{
+    const int64_t n = int64_t(N) * N * N * fd.M[0] * fd.M[1] * fd.M[2];
+    FoWeights w;
+    for (int i = 0; i < 16; ++i) w.w[i] = i < N ? fd.w[i] : 0.0;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(fo_energy_partial<N>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, fd.Pi, fd.coef, fd.D,
+                       fd.h[0] * fd.h[1] * fd.h[2], w, n, fd.partial);
+    hipLaunchKernelGGL(fo_energy_final, dim3(1), dim3(64), 0, s, fd.partial, RED_BLOCKS, fd.Eout);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_fo_rhs(const FoDesc& fd, hipStream_t s)
+{
+    switch (fd.N) {
+    case 3: return launch_fo_n<3>(fd, s);
+    case 4: return launch_fo_n<4>(fd, s);
+    case 5: return launch_fo_n<5>(fd, s);
+    case 6: return launch_fo_n<6>(fd, s);
+    case 7: return launch_fo_n<7>(fd, s);
+    case 8: return launch_fo_n<8>(fd, s);
+    case 9: return launch_fo_n<9>(fd, s);
+    case 10: return launch_fo_n<10>(fd, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fo_energy(const FoDesc& fd, hipStream_t s)
+{
+    switch (fd.N) {
+    case 3: return launch_fo_energy_n<3>(fd, s);
+    case 4: return launch_fo_energy_n<4>(fd, s);
+    case 5: return launch_fo_energy_n<5>(fd, s);
+    case 6: return launch_fo_energy_n<6>(fd, s);
+    case 7: return launch_fo_energy_n<7>(fd, s);
+    case 8: return launch_fo_energy_n<8>(fd, s);
+    case 9: return launch_fo_energy_n<9>(fd, s);
+    case 10: return launch_fo_energy_n<10>(fd, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fo_coef(const double* bg10, double* coef11, int64_t npts, hipStream_t s)
+{
+    const int64_t blocks = std::min<int64_t>((npts + 255) / 256, 4096);
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(fo_coef_kernel, dim3(unsigned(std::max<int64_t>(blocks, 1))), dim3(256), 0, s, bg10, coef11, npts);
+    return hipGetLastError();
+}
+
+}  // namespace wt2

@@ -91,6 +91,30 @@ struct Lap1Desc {
 };
 hipError_t launch_flat_lap1(const Lap1Desc& d, hipStream_t s);
 
+// The first-order (Phi, Pi) system (wt2_fo.hip).  launch_fo_rhs: the six
+// passes of one right-hand side (with grad_only: the three that fill D);
+// launch_fo_energy: the energy of (Phi, Pi) from D; launch_fo_coef: the
+// coefficient fields from bg10.
+struct FoDesc {
+    int N;
+    int M[3], periodic[3];
+    double h[3], w[16];
+    const double* Phi; const double* Pi;
+    const double* coef;          // 11 fields
+    double* D;                   // 3 fields
+    double* dPhi; double* dPi;
+    const double* gPhi; const double* gPi;
+    const void* op64;
+    double koT;
+    int kinds[6];
+    double sigma;
+    double* partial; double* Eout;
+    int grad_only;               // phase-0 passes only, D alone (for the energy)
+};
+hipError_t launch_fo_rhs(const FoDesc& d, hipStream_t s);
+hipError_t launch_fo_energy(const FoDesc& d, hipStream_t s);
+hipError_t launch_fo_coef(const double* bg10, double* coef11, int64_t npts, hipStream_t s);
+
 // A batch of pencil passes: one launch over every instance of a group (same N,
 // precision variant, axis, kernel kind and epilogue), driven by a device table
 // of per-instance descriptors and a packed (instance, tile) work list.

@@ -33,7 +33,14 @@ EXPORTS = (
     "wt2_batch_create", "wt2_batch_destroy", "wt2_batch_add", "wt2_batch_remove", "wt2_batch_step",
     "wt2_batch_step_async", "wt2_batch_sync",
     "wt2_batch_slot", "wt2_ctx_autotune", "wt2_ctx_tiles", "wt2_ctx_set_tiles", "wt2_launch_pass", "wt2_last_error", "wt2_version",
+    "wt2_fo_coef", "wt2_fo_rhs", "wt2_fo_energy",
 )
+
+# first-order boundary kinds (src/boundaries1d.jl:59-62)
+WT2_BC_DIRICHLET = 1
+WT2_BC_SOMMERFELD = 2
+WT2_BC_EXCISION = 3
+WT2_BC_FULL_DIRICHLET = 4
 
 
 class MeshDesc(ctypes.Structure):
@@ -51,6 +58,16 @@ class MeshDesc(ctypes.Structure):
         ("flags", ctypes.c_int32),
         ("face_kind", ctypes.c_int32 * 6),
         ("sommerfeld_inv_R", ctypes.c_double),
+    ]
+
+
+class BC3d(ctypes.Structure):
+    _fields_ = [
+        ("kinds", ctypes.c_int32 * 6),
+        ("reserved", ctypes.c_int32),
+        ("sigma", ctypes.c_double),
+        ("gPhi", ctypes.c_void_p),
+        ("gPi", ctypes.c_void_p),
     ]
 
 
@@ -118,6 +135,9 @@ def lib():
         L.wt2_ctx_autotune.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_int32]
         L.wt2_ctx_tiles.argtypes = [vp, dp, dp]
         L.wt2_ctx_set_tiles.argtypes = [vp, dp, dp]
+        L.wt2_fo_coef.argtypes = [vp, dp, dp, ctypes.c_int32]
+        L.wt2_fo_rhs.argtypes = [vp, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.POINTER(BC3d), ctypes.c_int32]
+        L.wt2_fo_energy.argtypes = [vp, dp, dp, dp, dp, ctypes.c_int32]
         L.wt2_last_error.restype = ctypes.c_char_p
         L.wt2_version.restype = ctypes.c_char_p
         _lib = L

@@ -223,27 +223,27 @@ class Workspace:
         return x
 
     # -- pointer plumbing
-    def _ptr(self, a, writable=False):
+    def _ptr(self, a, writable=False, mult=1):
         if a is None:
             return None, False
         if torch is not None and isinstance(a, torch.Tensor):
-            if a.dtype != torch.float64 or not a.is_contiguous() or a.numel() != self.npoints:
-                raise ValueError("field tensors must be contiguous float64 with N^3*Ne entries")
+            if a.dtype != torch.float64 or not a.is_contiguous() or a.numel() != mult * self.npoints:
+                raise ValueError(f"field tensors must be contiguous float64 with {mult} x N^3*Ne entries")
             if not a.is_cuda:
                 raise ValueError("torch field tensors must live on the GPU")
             return a.data_ptr(), False
         if isinstance(a, np.ndarray):
-            if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"] or a.size != self.npoints:
-                raise ValueError("field arrays must be C-contiguous float64 with N^3*Ne entries")
+            if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"] or a.size != mult * self.npoints:
+                raise ValueError(f"field arrays must be C-contiguous float64 with {mult} x N^3*Ne entries")
             if writable and not a.flags["WRITEABLE"]:
                 raise ValueError("output array is read-only")
             return a.ctypes.data, True
         raise TypeError("fields must be torch CUDA tensors or numpy arrays")
 
-    def _ptrs(self, *arrs, writable=()):
+    def _ptrs(self, *arrs, writable=(), mult=None):
         ptrs, hosts = [], set()
         for i, a in enumerate(arrs):
-            p, h = self._ptr(a, i in writable)
+            p, h = self._ptr(a, i in writable, (mult or {}).get(i, 1))
             ptrs.append(p)
             if a is not None:
                 hosts.add(h)
@@ -319,6 +319,50 @@ class Workspace:
         E = ctypes.c_double()
         self._sync_torch()
         check(lib().wt2_energy(self.ctx, pu, pud, ctypes.byref(E), fl))
+        return E.value
+
+    # -- the first-order (Phi, Pi) system (affine meshes)
+    def fo_coef(self, bg10, out=None):
+        """sample_background3d! (src/wave3d_curved.jl:75-115) on the device:
+        the 11 coefficient fields (alpha, sqrtg, b1..3, gu11, gu12, gu13, gu22,
+        gu23, gu33), each npoints long, one after the other, from 10 background
+        values per node.  bg10: (npoints, 10) host array or device tensor."""
+        if out is None:
+            if torch is not None and isinstance(bg10, torch.Tensor):
+                out = torch.empty(11 * self.npoints, dtype=torch.float64, device=bg10.device)
+            else:
+                out = np.empty(11 * self.npoints)
+        if isinstance(bg10, np.ndarray):
+            bg10 = np.ascontiguousarray(bg10, dtype=np.float64).reshape(-1)
+        (pb, po), fl = self._ptrs(bg10, out, writable=(1,), mult={0: 10, 1: 11})
+        self._sync_torch()
+        check(lib().wt2_fo_coef(self.ctx, pb, po, fl))
+        return out
+
+    def fo_rhs(self, dPhi, dPi, Phi, Pi, coef, eps_ko=0.1, bc3d=None):
+        """wave3d_curved_rhs!(Phidot, Pidot, Phi, Pi, coef; eps_KO, bc3d) on the
+        affine path (src/wave3d_curved.jl:205-258)."""
+        g = (None, None) if bc3d is None else (bc3d.gPhi_arr, bc3d.gPi_arr)
+        (p1, p2, p3, p4, pc, pg1, pg2), fl = self._ptrs(dPhi, dPi, Phi, Pi, coef, g[0], g[1],
+                                                        writable=(0, 1), mult={4: 11})
+        bc = None
+        if bc3d is not None:
+            bc = _lib.BC3d()
+            for f in range(6):
+                bc.kinds[f] = bc3d.kinds[f]
+            bc.sigma = bc3d.sigma
+            bc.gPhi = pg1
+            bc.gPi = pg2
+        self._sync_torch()
+        check(lib().wt2_fo_rhs(self.ctx, p1, p2, p3, p4, pc, float(eps_ko),
+                               None if bc is None else ctypes.byref(bc), fl))
+        return dPhi, dPi
+
+    def fo_energy(self, Phi, Pi, coef):
+        (p1, p2, pc), fl = self._ptrs(Phi, Pi, coef, mult={2: 11})
+        E = ctypes.c_double()
+        self._sync_torch()
+        check(lib().wt2_fo_energy(self.ctx, p1, p2, pc, ctypes.byref(E), fl))
         return E.value
 
 
@@ -400,6 +444,40 @@ def rhs_wave3d(udd, u, ud, params, geom=None, ops=None, work=None):
 def wave3d_rhs_full(du, dud, u, ud, work):
     """(du, dud) = (ud + KO(u), L u + KO(ud)) with KO = eps mu^-5 sum_d D_d^6."""
     return work.rhs_full(du, dud, u, ud)
+
+
+class BC3d:
+    """make_bc3d(kinds; sigma, gPhi, gPi) (src/boundaries3d.jl:44-54): a boundary
+    kind per face (-x,+x,-y,+y,-z,+z) as a name (:dirichlet, :sommerfeld,
+    :excision, :full_dirichlet) or a WT2_BC_* code, the penalty scale and the
+    optional per-node targets (arrays on the same side as the fields)."""
+
+    _NAMES = {"dirichlet": _lib.WT2_BC_DIRICHLET, "sommerfeld": _lib.WT2_BC_SOMMERFELD,
+              "excision": _lib.WT2_BC_EXCISION, "full_dirichlet": _lib.WT2_BC_FULL_DIRICHLET}
+
+    def __init__(self, kinds, sigma=1.0, gPhi=None, gPi=None):
+        if len(kinds) != 6:
+            raise ValueError("make_bc3d needs six face kinds")
+        self.kinds = tuple(self._NAMES[k] if isinstance(k, str) else int(k) for k in kinds)
+        self.sigma = float(sigma)
+        self.gPhi_arr = gPhi
+        self.gPi_arr = gPi
+
+
+def make_bc3d(kinds, sigma=1.0, gPhi=None, gPi=None):
+    return BC3d(kinds, sigma, gPhi, gPi)
+
+
+def wave3d_curved_rhs(dPhi, dPi, Phi, Pi, coef, work, eps_ko=0.1, bc3d=None):
+    """wave3d_curved_rhs!(Phidot, Pidot, Phi, Pi, coef; geom, ops, ws, eps_KO, bc3d)
+    (src/wave3d_curved.jl:205-258), the affine first-order path; coef from
+    work.fo_coef."""
+    return work.fo_rhs(dPhi, dPi, Phi, Pi, coef, eps_ko, bc3d)
+
+
+def wave3d_energy(Phi, Pi, coef, work):
+    """wave3d_energy (src/wave3d_curved.jl:268-287)."""
+    return work.fo_energy(Phi, Pi, coef)
 
 
 def discrete_energy(u, ud, work):
