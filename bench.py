@@ -405,6 +405,76 @@ def passes_run(wh, torch, args):
             "passes": out}
 
 
+FO_ALG_BYTES = 120   # first-order RHS: read Phi, Pi and 11 coefficients, write Phidot, Pidot (FP64)
+
+
+def fo_run(wh, torch, args):
+    """The first-order (Phi, Pi) ADM right-hand side (wt2_fo_rhs) on the flat
+    32^3 x 8^3 mesh with the ADM bump background, KO 0.1 and Sommerfeld faces:
+    RHS evaluations per second, with the CPU oracle timed on 16^3 x 8^3."""
+    mesh = wh.make_uniform_hex(M_ELEMS, 0.0, 1.0)
+    elem = wh.make_element(N_NODES)
+    ws = wh.Workspace(mesh, elem)
+    x = wh.host_coords(mesh, elem)
+    bg = wh.sample_background3d(wh.adm_bump_background(), 0.0, x[:, 0], x[:, 1], x[:, 2])
+    coef = ws.fo_coef(torch.from_numpy(bg.reshape(-1).copy()).cuda())
+    Phi0, _ = wh.gaussian_pulse(x, (0.5, 0.45, 0.55), width=0.1)
+    Phi = torch.from_numpy(Phi0).cuda()
+    Pi = torch.zeros_like(Phi)
+    dP = torch.empty_like(Phi); dQ = torch.empty_like(Phi)
+    bc = wh.make_bc3d(["sommerfeld"] * 6)
+    stream = torch.cuda.Stream()
+    ws.set_stream(stream.cuda_stream)
+    K = max(1, min(args.steps, 200))
+    with torch.cuda.stream(stream):
+        for _ in range(max(args.warmup, 2)):
+            ws.fo_rhs(dP, dQ, Phi, Pi, coef, 0.1, bc)
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True); b = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        a.record(stream)
+        for _ in range(K):
+            ws.fo_rhs(dP, dQ, Phi, Pi, coef, 0.1, bc)
+        b.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    ev_ms = a.elapsed_time(b) / K
+    ws.set_stream(None)
+    npts = ws.npoints
+    out = {"metric": "first-order (Phi, Pi) RHS evaluations: grid points/s", "value": npts * K / wall,
+           "unit": "grid points/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+           "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic (Gaussian Phi, Pi = 0, ADM bump background)",
+           "config": {"workload": "wave3d_curved_rhs! affine path, flat cube 32^3 elements x 8^3 GLL nodes "
+                                  "(16,777,216 points), ADM bump background, KO eps=0.1, Sommerfeld on six faces",
+                      "points": npts},
+           "roofline": {"bound": "hbm", "achieved": FO_ALG_BYTES * npts / (ev_ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+                        "unit": "GB/s", "frac": FO_ALG_BYTES * npts / (ev_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                        "traffic": None, "kernel": "fo_pass_kernel<8> (six passes per RHS)",
+                        "algorithmic_bytes_per_point": FO_ALG_BYTES, "event_ms_per_rhs": ev_ms}}
+    if not args.no_cpu:
+        import wt_oracle as wo
+        Mc = 16
+        o = wo.Oracle(N_NODES, Mc)
+        xc = o.coords()
+        bgc = wh.sample_background3d(wh.adm_bump_background(), 0.0, xc[:, 0], xc[:, 1], xc[:, 2])
+        cc = wo.fo_coef(bgc.reshape(-1))
+        Pc = np.exp(-np.sum((xc - 0.5) ** 2, axis=1) / 0.02)
+        Qc = np.zeros_like(Pc)
+        bcc = wo.make_bc3d(["sommerfeld"] * 6)
+        o.fo_rhs(Pc, Qc, cc, 0.1, bcc)
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            o.fo_rhs(Pc, Qc, cc, 0.1, bcc)
+        tc = (time.perf_counter() - t0) / reps
+        out["cpu_baseline"] = {"value": o.n / tc, "unit": "grid points/s",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                               "kind": "port", "sample": f"CPU oracle (C, OpenMP, FP64), {reps} first-order RHS "
+                                                         f"evaluations on 16^3 x 8^3 elements, {tc:.2f} s each"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -414,7 +484,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default tile shapes instead of the autotuner's")
-    ap.add_argument("--config", choices=["plain", "curved", "batch", "serving", "passes"], default="plain",
+    ap.add_argument("--config", choices=["plain", "curved", "batch", "serving", "passes", "fo"], default="plain",
                     help="plain: the FP64 flat workload (headline); curved: the warped-mesh "
                          "ADM workload, FP64 vs mixed, error at --t-final")
     ap.add_argument("--instances", type=int, default=512, help="batch config: number of instances")
@@ -455,6 +525,9 @@ def main():
         return
     if args.config == "passes":
         print(json.dumps(passes_run(wh, torch, args)), flush=True)
+        return
+    if args.config == "fo":
+        print(json.dumps(fo_run(wh, torch, args)), flush=True)
         return
 
     mesh = wh.make_uniform_hex(M_ELEMS, 0.0, 1.0)

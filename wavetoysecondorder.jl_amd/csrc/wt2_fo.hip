@@ -5,14 +5,14 @@
 // eps mu^-5 D_d^6 on both fields (_ko_axis3!, :182-192), and the affine
 // outer-boundary pass (_bc3d_affine_kernel!, src/boundaries3d.jl:76-126).
 //
-// Six pencil passes per call.  Pass (axis d, phase 0): DPhi_d = D_d Phi, kept
-// in a context buffer, and the KO terms of both fields along d.  Pass (axis d,
-// phase 1): dPi += D_d F^d with F^d formed at load time from the coefficients,
-// Pi and the three DPhi; the last one (z) also adds b . DPhi + (alpha/sqrtg) Pi
-// to dPhi and the boundary terms.  A workgroup owns whole element lines along
-// its axis (LT lines x Ma elements, work-item = (element, line) with adjacent
-// lines in adjacent lanes), and the face values of neighbouring elements go
-// through LDS, as in the second-order pencil passes.
+// Seven launches per call.  Pencil pass (axis d, phase 0): DPhi_d = D_d Phi,
+// kept in a context buffer, and the KO terms of both fields along d.  A
+// pointwise kernel then forms the fluxes F^i over that buffer, adds
+// b . DPhi + (alpha/sqrtg) Pi to dPhi and the boundary terms.  Pencil pass
+// (axis d, phase 1): dPi += D_d F^d.  A pass workgroup owns whole element
+// lines along its axis (LT lines x Ma elements, work-item = (element, line)
+// with adjacent lines in adjacent lanes), and the face values of neighbouring
+// elements go through LDS, as in the second-order pencil passes.
 #include "wt2_internal.hpp"
 #include "wt2_ops.hpp"
 
@@ -34,8 +34,8 @@ struct FoArgs {
     int64_t npts;
     int M0, M1, M2;
     int per0, per1, per2;
-    int axis, phase, first, last;
-    int Ma, T0, slo, shi, estride, LT, nbk, LA, LB, GS;
+    int axis, first, anybc;
+    int Ma, T0, slo, shi, estride, LT, nbk;
     double ih, sat0, satN;   // 1/h; SAT -1/(2 w_0 h), +1/(2 w_{N-1} h) along the axis
     double koT;              // eps mu^-5 (0: no KO)
     int kinds[6];
@@ -76,25 +76,52 @@ __device__ __forceinline__ void fo_D(const double* __restrict__ op, const FoArgs
     if (hasR) y[N - 1] = fma(a.satN, xr - x[N - 1], y[N - 1]);
 }
 
-template <int N>
-__device__ __forceinline__ void fo_load(const double* __restrict__ p, int gs, bool on, double (&x)[N])
+// N values of a segment at node stride GS (compile-time: contiguous x-line
+// segments load as 16-byte vectors).
+template <int N, int GS>
+__device__ __forceinline__ void fo_load(const double* __restrict__ p, bool on, double (&x)[N])
 {
+    if (!on) {
 #pragma unroll
-    for (int r = 0; r < N; ++r) x[r] = on ? p[int64_t(r) * gs] : 0.0;
+        for (int r = 0; r < N; ++r) x[r] = 0.0;
+        return;
+    }
+    if constexpr (GS == 1 && N % 2 == 0) {
+#pragma unroll
+        for (int r = 0; r < N; r += 2) {
+            const double2 t = *reinterpret_cast<const double2*>(p + r);
+            x[r] = t.x;
+            x[r + 1] = t.y;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < N; ++r) x[r] = p[r * GS];
+    }
 }
 
-template <int N>
-__device__ __forceinline__ void fo_store(double* __restrict__ p, int gs, bool on, const double (&x)[N])
+template <int N, int GS>
+__device__ __forceinline__ void fo_store(double* __restrict__ p, bool on, const double (&x)[N])
 {
     if (!on) return;
+    if constexpr (GS == 1 && N % 2 == 0) {
 #pragma unroll
-    for (int r = 0; r < N; ++r) p[int64_t(r) * gs] = x[r];
+        for (int r = 0; r < N; r += 2) *reinterpret_cast<double2*>(p + r) = make_double2(x[r], x[r + 1]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < N; ++r) p[r * GS] = x[r];
+    }
 }
 
-template <int N>
+// One pencil pass along axis AX.  PHASE 0: D[AX] = D_AX Phi and the KO terms
+// of both fields along AX into dPhi, dPi (the first pass writes them).
+// PHASE 1: dPi += D_AX F^AX, F^AX held in D[AX] (written by fo_flux_kernel).
+template <int N, int AX, int PHASE>
 __global__ void __launch_bounds__(FO_MAXT) fo_pass_kernel(FoArgs a)
 {
     constexpr int N3 = N * N * N;
+    constexpr int GS = AX == 0 ? 1 : (AX == 1 ? N : N * N);
+    constexpr int LA = AX == 2 ? N : N * N;          // line -> (line / N) * LA + (line % N) * LB
+    constexpr int LB = AX == 0 ? N : 1;
     __shared__ double op[OpLayout<N>::SIZE];
     __shared__ double xbuf[4 * FO_MAXT];
     for (int q = threadIdx.x; q < OpLayout<N>::SIZE; q += blockDim.x) op[q] = a.op[q];
@@ -106,31 +133,31 @@ __global__ void __launch_bounds__(FO_MAXT) fo_pass_kernel(FoArgs a)
     const int ebase = (t % a.T0) * a.slo + (t / a.T0) * a.shi;
     const int tid = threadIdx.x;
     const int m = tid / a.LT;
-    const int line = blk * a.LT + (tid - m * a.LT);
+    const int lane_line = tid - m * a.LT;
+    const int line = blk * a.LT + lane_line;
     const int Ma = a.Ma;
     const bool active = m < Ma && line < N * N;
-    const int per = a.axis == 0 ? a.per0 : (a.axis == 1 ? a.per1 : a.per2);
+    const int per = AX == 0 ? a.per0 : (AX == 1 ? a.per1 : a.per2);
     const bool hasL = active && (m > 0 || per);
     const bool hasR = active && (m < Ma - 1 || per);
     FoX xch;
     xch.buf = xbuf; xch.me = tid; xch.k = 0;
-    xch.idxL = hasL ? (m > 0 ? m - 1 : Ma - 1) * a.LT + (tid - m * a.LT) : tid;
-    xch.idxR = hasR ? (m < Ma - 1 ? m + 1 : 0) * a.LT + (tid - m * a.LT) : tid;
+    xch.idxL = hasL ? (m > 0 ? m - 1 : Ma - 1) * a.LT + lane_line : tid;
+    xch.idxR = hasR ? (m < Ma - 1 ? m + 1 : 0) * a.LT + lane_line : tid;
     const int mm = active ? m : 0, ll = active ? line : 0;
     const int e = ebase + mm * a.estride;
-    const int64_t g0 = int64_t(e) * N3 + (ll / N) * a.LA + (ll % N) * a.LB;
-    const int gs = a.GS;
+    const int64_t g0 = int64_t(e) * N3 + (ll / N) * LA + (ll % N) * LB;
     const int64_t n = a.npts;
 
-    if (a.phase == 0) {
+    if constexpr (PHASE == 0) {
         double x[N], y[N];
-        fo_load<N>(a.Phi + g0, gs, active, x);
+        fo_load<N, GS>(a.Phi + g0, active, x);
         fo_D<N>(op, a, xch, hasL, hasR, x, y);
-        fo_store<N>(a.D + a.axis * n + g0, gs, active, y);
+        fo_store<N, GS>(a.D + AX * n + g0, active, y);
         for (int fld = 0; fld < 2; ++fld) {
             double* out = (fld == 0 ? a.dPhi : a.dPi) + g0;
             if (a.koT != 0.0) {
-                if (fld == 1) fo_load<N>(a.Pi + g0, gs, active, x);
+                if (fld == 1) fo_load<N, GS>(a.Pi + g0, active, x);
 #pragma unroll 1
                 for (int rep = 0; rep < 3; ++rep) {
                     fo_D<N>(op, a, xch, hasL, hasR, x, y);
@@ -141,89 +168,108 @@ __global__ void __launch_bounds__(FO_MAXT) fo_pass_kernel(FoArgs a)
 #pragma unroll
                     for (int r = 0; r < N; ++r) acc[r] = a.koT * x[r];
                 } else {
-                    fo_load<N>(out, gs, active, acc);
+                    fo_load<N, GS>(out, active, acc);
 #pragma unroll
                     for (int r = 0; r < N; ++r) acc[r] = fma(a.koT, x[r], acc[r]);
                 }
-                fo_store<N>(out, gs, active, acc);
+                fo_store<N, GS>(out, active, acc);
             } else if (a.first) {
                 double z[N];
 #pragma unroll
                 for (int r = 0; r < N; ++r) z[r] = 0.0;
-                fo_store<N>(out, gs, active, z);
+                fo_store<N, GS>(out, active, z);
             }
         }
-        return;
+    } else {
+        double F[N], DF[N], acc[N];
+        fo_load<N, GS>(a.D + AX * n + g0, active, F);
+        fo_D<N>(op, a, xch, hasL, hasR, F, DF);
+        fo_load<N, GS>(a.dPi + g0, active, acc);
+#pragma unroll
+        for (int r = 0; r < N; ++r) acc[r] += DF[r];
+        fo_store<N, GS>(a.dPi + g0, active, acc);
     }
+}
 
-    // phase 1: dPi += D_d F^d, F^d = b^d Pi + alpha sqrtg (g^{d1} DPhi_1 + g^{d2} DPhi_2 + g^{d3} DPhi_3)
-    const int d = a.axis;
-    const int gi0 = d == 0 ? 5 : (d == 1 ? 6 : 7);     // g^{d1}
-    const int gi1 = d == 0 ? 6 : (d == 1 ? 8 : 9);     // g^{d2}
-    const int gi2 = d == 0 ? 7 : (d == 1 ? 9 : 10);    // g^{d3}
-    double F[N], DF[N];
-#pragma unroll
-    for (int r = 0; r < N; ++r) {
-        const int64_t q = g0 + int64_t(r) * gs;
-        double f = 0.0;
-        if (active) {
-            const double* c = a.coef;
-            f = c[(2 + d) * n + q] * a.Pi[q] +
-                c[q] * c[n + q] * (c[gi0 * n + q] * a.D[q] + c[gi1 * n + q] * a.D[n + q] + c[gi2 * n + q] * a.D[2 * n + q]);
-        }
-        F[r] = f;
-    }
-    fo_D<N>(op, a, xch, hasL, hasR, F, DF);
-    if (!active) return;
-    const double* c = a.coef;
-#pragma unroll
-    for (int r = 0; r < N; ++r) {
-        const int64_t q = g0 + int64_t(r) * gs;
-        a.dPi[q] += DF[r];
-    }
-    if (!a.last) return;
-    // dPhi += b . DPhi + (alpha / sqrtg) Pi, then the outer-boundary terms
-    const int ex = e % a.M0, ey = (e / a.M0) % a.M1, ez = e / (a.M0 * a.M1);
-    const int ec[3] = {ex, ey, ez};
-    const int Mv[3] = {a.M0, a.M1, a.M2};
-    const int pv[3] = {a.per0, a.per1, a.per2};
-    const int la = ll % N, lb = ll / N;
-#pragma unroll
-    for (int r = 0; r < N; ++r) {
-        const int64_t q = g0 + int64_t(r) * gs;
+// Pointwise, between the two phases: F^i = b^i Pi + alpha sqrtg g^{ij} DPhi_j
+// written over D (each work-item reads and writes only its own node),
+// dPhi += b . DPhi + (alpha / sqrtg) Pi, and the affine outer-boundary terms
+// (src/boundaries3d.jl:76-126) added to dPhi and dPi.
+template <int N>
+__global__ void __launch_bounds__(256) fo_flux_kernel(FoArgs a)
+{
+    constexpr int N3 = N * N * N;
+    const int64_t n = a.npts;
+    const double* __restrict__ c = a.coef;
+    for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
         const double al = c[q], sg = c[n + q];
+        const double b1 = c[2 * n + q], b2 = c[3 * n + q], b3 = c[4 * n + q];
+        const double g11 = c[5 * n + q], g12 = c[6 * n + q], g13 = c[7 * n + q];
+        const double g22 = c[8 * n + q], g23 = c[9 * n + q], g33 = c[10 * n + q];
         const double D1 = a.D[q], D2 = a.D[n + q], D3 = a.D[2 * n + q];
         const double Pq = a.Pi[q];
-        double dF = a.dPhi[q] + (c[2 * n + q] * D1 + c[3 * n + q] * D2 + c[4 * n + q] * D3 + (al / sg) * Pq);
+        const double as = al * sg;
+        a.D[q] = b1 * Pq + as * (g11 * D1 + g12 * D2 + g13 * D3);
+        a.D[n + q] = b2 * Pq + as * (g12 * D1 + g22 * D2 + g23 * D3);
+        a.D[2 * n + q] = b3 * Pq + as * (g13 * D1 + g23 * D2 + g33 * D3);
+        double dF = a.dPhi[q] + (b1 * D1 + b2 * D2 + b3 * D3 + (al / sg) * Pq);
         double dP = 0.0;
-        // node (i, j, k) of the element
-        const int nd[3] = {d == 0 ? r : la, d == 1 ? r : (d == 0 ? la : lb), d == 2 ? r : lb};
-        for (int f = 0; f < 6; ++f) {
-            const int dn = f >> 1, hi = f & 1;
-            if (pv[dn] || ec[dn] != (hi ? Mv[dn] - 1 : 0) || nd[dn] != (hi ? N - 1 : 0)) continue;
-            const int kind = a.kinds[f];
-            if (kind == WT2_BC_EXCISION) continue;
-            const double nh = hi ? 1.0 : -1.0;
-            const double gunn = c[(dn == 0 ? 5 : (dn == 1 ? 8 : 10)) * n + q];
-            const double a_n = al * sqrt(gunn);
-            const double bax = c[(2 + dn) * n + q];
-            const double bn = nh * bax;
-            const double wt = a.bw[dn][hi];
-            if (kind == WT2_BC_FULL_DIRICHLET) {
-                const double tau = a.sigma * (fabs(a_n - bax) + fabs(a_n + bax)) * wt;
-                const double gF = a.gPhi ? a.gPhi[q] : 0.0, gP = a.gPi ? a.gPi[q] : 0.0;
-                dF += -tau * (a.Phi[q] - gF);
-                dP += -tau * (Pq - gP);
-            } else {
-                const double Dn = dn == 0 ? D1 : (dn == 1 ? D2 : D3);
-                const double rr = Pq + ((bn + a_n) / (al / sg)) * (nh * Dn);
-                const double gv = (kind == WT2_BC_SOMMERFELD || !a.gPhi) ? 0.0 : a.gPhi[q];
-                dP += -a.sigma * fabs(a_n + bn) * wt * (rr - gv);
+        if (a.anybc) {
+            const int64_t e = q / N3;
+            const int r = int(q - e * N3);
+            const int nd[3] = {r % N, (r / N) % N, r / (N * N)};
+            const int ec[3] = {int(e % a.M0), int((e / a.M0) % a.M1), int(e / (int64_t(a.M0) * a.M1))};
+            const int Mv[3] = {a.M0, a.M1, a.M2};
+            const int pv[3] = {a.per0, a.per1, a.per2};
+            const double Dv[3] = {D1, D2, D3};
+            const double bv[3] = {b1, b2, b3};
+            const double gnn[3] = {g11, g22, g33};
+            for (int f = 0; f < 6; ++f) {
+                const int dn = f >> 1, hi = f & 1;
+                if (pv[dn] || ec[dn] != (hi ? Mv[dn] - 1 : 0) || nd[dn] != (hi ? N - 1 : 0)) continue;
+                const int kind = a.kinds[f];
+                if (kind == WT2_BC_EXCISION) continue;
+                const double nh = hi ? 1.0 : -1.0;
+                const double a_n = al * sqrt(gnn[dn]);
+                const double bax = bv[dn];
+                const double bn = nh * bax;
+                const double wt = a.bw[dn][hi];
+                if (kind == WT2_BC_FULL_DIRICHLET) {
+                    const double tau = a.sigma * (fabs(a_n - bax) + fabs(a_n + bax)) * wt;
+                    const double gF = a.gPhi ? a.gPhi[q] : 0.0, gP = a.gPi ? a.gPi[q] : 0.0;
+                    dF += -tau * (a.Phi[q] - gF);
+                    dP += -tau * (Pq - gP);
+                } else {
+                    const double rr = Pq + ((bn + a_n) / (al / sg)) * (nh * Dv[dn]);
+                    const double gv = (kind == WT2_BC_SOMMERFELD || !a.gPhi) ? 0.0 : a.gPhi[q];
+                    dP += -a.sigma * fabs(a_n + bn) * wt * (rr - gv);
+                }
             }
         }
         a.dPhi[q] = dF;
         if (dP != 0.0) a.dPi[q] += dP;
     }
+}
+
+template <int N, int AX, int PHASE>
+hipError_t fo_launch_pass(FoArgs& a, hipStream_t s)
+{
+    const int Mx = a.M0, My = a.M1, Mz = a.M2;
+    int rows;
+    if (AX == 0) {        // rows (ey, ez), line (j, k): base N j + N^2 k
+        a.Ma = Mx; a.T0 = My; a.slo = Mx; a.shi = Mx * My; a.estride = 1; rows = My * Mz;
+    } else if (AX == 1) { // rows (ex, ez), line (i, k): base i + N^2 k
+        a.Ma = My; a.T0 = Mx; a.slo = 1; a.shi = Mx * My; a.estride = Mx; rows = Mx * Mz;
+    } else {              // rows (ex, ey), line (i, j): base i + N j
+        a.Ma = Mz; a.T0 = Mx; a.slo = 1; a.shi = Mx; a.estride = Mx * My; rows = Mx * My;
+    }
+    a.LT = std::min(N * N, FO_MAXT / a.Ma);
+    a.nbk = (N * N + a.LT - 1) / a.LT;
+    const int nthr = (a.LT * a.Ma + 63) / 64 * 64;
+    if (a.Ma > 64 || a.LT < 1 || nthr > FO_MAXT) return hipErrorInvalidConfiguration;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL((fo_pass_kernel<N, AX, PHASE>), dim3(rows * a.nbk), dim3(nthr), 0, s, a);
+    return hipGetLastError();
 }
 
 template <int N>
@@ -236,45 +282,45 @@ hipError_t launch_fo_n(const FoDesc& fd, hipStream_t s)
     a.M0 = fd.M[0]; a.M1 = fd.M[1]; a.M2 = fd.M[2];
     a.per0 = fd.periodic[0]; a.per1 = fd.periodic[1]; a.per2 = fd.periodic[2];
     a.npts = int64_t(N) * N * N * fd.M[0] * fd.M[1] * fd.M[2];
-    a.koT = fd.koT;
-    for (int f = 0; f < 6; ++f) a.kinds[f] = fd.kinds[f];
+    a.koT = fd.grad_only ? 0.0 : fd.koT;
+    a.anybc = 0;
+    for (int f = 0; f < 6; ++f) {
+        a.kinds[f] = fd.kinds[f];
+        if (fd.kinds[f] != WT2_BC_EXCISION && !fd.periodic[f / 2]) a.anybc = 1;
+    }
     a.sigma = fd.sigma;
     for (int dd = 0; dd < 3; ++dd) {
         a.bw[dd][0] = 1.0 / (fd.h[dd] * fd.w[0]);
         a.bw[dd][1] = 1.0 / (fd.h[dd] * fd.w[N - 1]);
     }
-    const int Mx = fd.M[0], My = fd.M[1], Mz = fd.M[2];
-    if (fd.grad_only) a.koT = 0.0;
-    for (int phase = 0; phase < (fd.grad_only ? 1 : 2); ++phase)
-        for (int d = 0; d < 3; ++d) {
-            a.axis = d; a.phase = phase;
-            a.first = phase == 0 && d == 0 && !fd.grad_only;
-            a.last = phase == 1 && d == 2;
-            const double h = fd.h[d];
-            a.ih = 1.0 / h;
-            a.sat0 = -1.0 / (2.0 * fd.w[0] * h);
-            a.satN = 1.0 / (2.0 * fd.w[N - 1] * h);
-            int rows;
-            if (d == 0) {        // rows (ey, ez), line (j, k): base N j + N^2 k
-                a.Ma = Mx; a.T0 = My; a.slo = Mx; a.shi = Mx * My; a.estride = 1;
-                a.LB = N; a.LA = N * N; a.GS = 1; rows = My * Mz;
-            } else if (d == 1) { // rows (ex, ez), line (i, k): base i + N^2 k
-                a.Ma = My; a.T0 = Mx; a.slo = 1; a.shi = Mx * My; a.estride = Mx;
-                a.LB = 1; a.LA = N * N; a.GS = N; rows = Mx * Mz;
-            } else {             // rows (ex, ey), line (i, j): base i + N j
-                a.Ma = Mz; a.T0 = Mx; a.slo = 1; a.shi = Mx; a.estride = Mx * My;
-                a.LB = 1; a.LA = N; a.GS = N * N; rows = Mx * My;
-            }
-            a.LT = std::min(N * N, FO_MAXT / a.Ma);
-            a.nbk = (N * N + a.LT - 1) / a.LT;
-            const int nthr = (a.LT * a.Ma + 63) / 64 * 64;
-            if (a.Ma > 64 || a.LT < 1 || nthr > FO_MAXT) return hipErrorInvalidConfiguration;
-            (void)hipGetLastError();
-            hipLaunchKernelGGL(fo_pass_kernel<N>, dim3(rows * a.nbk), dim3(nthr), 0, s, a);
-            hipError_t e = hipGetLastError();
-            if (e != hipSuccess) return e;
-        }
-    return hipSuccess;
+    auto axis_consts = [&](int d) {
+        a.axis = d;
+        a.ih = 1.0 / fd.h[d];
+        a.sat0 = -1.0 / (2.0 * fd.w[0] * fd.h[d]);
+        a.satN = 1.0 / (2.0 * fd.w[N - 1] * fd.h[d]);
+    };
+    hipError_t e;
+    a.first = !fd.grad_only;
+    axis_consts(0);
+    if ((e = fo_launch_pass<N, 0, 0>(a, s)) != hipSuccess) return e;
+    a.first = 0;
+    axis_consts(1);
+    if ((e = fo_launch_pass<N, 1, 0>(a, s)) != hipSuccess) return e;
+    axis_consts(2);
+    if ((e = fo_launch_pass<N, 2, 0>(a, s)) != hipSuccess) return e;
+    if (fd.grad_only) return hipSuccess;
+    {
+        const int64_t blocks = std::min<int64_t>((a.npts + 255) / 256, 8192);
+        (void)hipGetLastError();
+        hipLaunchKernelGGL(fo_flux_kernel<N>, dim3(unsigned(std::max<int64_t>(blocks, 1))), dim3(256), 0, s, a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    axis_consts(0);
+    if ((e = fo_launch_pass<N, 0, 1>(a, s)) != hipSuccess) return e;
+    axis_consts(1);
+    if ((e = fo_launch_pass<N, 1, 1>(a, s)) != hipSuccess) return e;
+    axis_consts(2);
+    return fo_launch_pass<N, 2, 1>(a, s);
 }
 
 // coef11 from 10 background values per node (sample_background3d!'s kernel,
