@@ -92,4 +92,68 @@ function rk4_steps!(u::Array{Float64,4}, u̇::Array{Float64,4}, dt::Float64, nst
     return ne > 0 ? E[1:ne] : nothing
 end
 
+# ---- first-order (Φ, Π) path (wt2_fo_*)
+
+const _BC = Dict(:dirichlet => Int32(1), :sommerfeld => Int32(2), :excision => Int32(3),
+                 :full_dirichlet => Int32(4))
+
+struct BC3d
+    kinds::NTuple{6,Int32}
+    reserved::Int32
+    σ::Float64
+    gΦ::Ptr{Float64}
+    gΠ::Ptr{Float64}
+end
+
+"""
+    make_bc3d(kinds; σ = 1, gΦ = nothing, gΠ = nothing)
+
+make_bc3d of src/boundaries3d.jl:44-54 for the affine pass (host targets must
+stay alive for the call).
+"""
+make_bc3d(kinds; σ = 1.0, gΦ = nothing, gΠ = nothing) =
+    BC3d(ntuple(i -> kinds[i] isa Symbol ? _BC[kinds[i]] : Int32(kinds[i]), 6), Int32(0), Float64(σ),
+         gΦ === nothing ? Ptr{Float64}(C_NULL) : pointer(gΦ), gΠ === nothing ? Ptr{Float64}(C_NULL) : pointer(gΠ))
+
+"""
+    fo_coef(bg10; ctx) -> coef (11 fields of N^3 Ne values)
+
+make_coef3d + sample_background3d! (src/wave3d_curved.jl:75-124) from 10
+background values per node.
+"""
+function fo_coef(bg10::Vector{Float64}; ctx::Context)
+    coef = zeros(11 * ctx.npoints)
+    check(ccall((:wt2_fo_coef, libwt2), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint),
+                ctx.ptr, bg10, coef, WT2_HOST_PTRS))
+    return coef
+end
+
+"""
+    wave3d_curved_rhs!(Φ̇, Π̇, Φ, Π, coef; ctx, ε_KO = 0.1, bc3d = nothing) -> (Φ̇, Π̇)
+
+Same arguments and result as WaveToySecondOrder.wave3d_curved_rhs! on its affine
+path (src/wave3d_curved.jl:205-258).
+"""
+function wave3d_curved_rhs!(Φ̇::Array{Float64,4}, Π̇::Array{Float64,4}, Φ::Array{Float64,4},
+                            Π::Array{Float64,4}, coef::Vector{Float64}; ctx::Context,
+                            ε_KO::Real = 0.1, bc3d = nothing)
+    bc = bc3d === nothing ? Ptr{BC3d}(C_NULL) : Ref(bc3d)
+    check(ccall((:wt2_fo_rhs, libwt2), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble,
+                 Ptr{BC3d}, Cint),
+                ctx.ptr, Φ̇, Π̇, Φ, Π, coef, Float64(ε_KO), bc, WT2_HOST_PTRS))
+    return Φ̇, Π̇
+end
+
+"""
+    wave3d_energy(Φ, Π, coef; ctx) -> E   (src/wave3d_curved.jl:268-287)
+"""
+function wave3d_energy(Φ::Array{Float64,4}, Π::Array{Float64,4}, coef::Vector{Float64}; ctx::Context)
+    E = Ref(0.0)
+    check(ccall((:wt2_fo_energy, libwt2), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint),
+                ctx.ptr, Φ, Π, coef, E, WT2_HOST_PTRS))
+    return E[]
+end
+
 end # module
