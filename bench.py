@@ -475,6 +475,45 @@ def fo_run(wh, torch, args):
     return out
 
 
+def mixed_error_run(wh, torch, args):
+    """Growth in time of the mixed-vs-FP64 relative H-norm error of u and u_t on
+    N = 10 meshes of --elems^3 elements, the warped ADM-bump configuration and
+    its flat counterpart (KO 0.1, Dirichlet, dt = 0.2 h_min, Gaussian width 0.1),
+    sampled at t = 0.125 ... 2."""
+    N, M = 10, args.elems
+    out = {"metric": "mixed-vs-FP64 relative H-norm error against time", "config": {
+        "workload": f"N = 10, {M}^3 elements, warped A=0.05 + ADM bump background and flat, KO 0.1, "
+                    "Gaussian width 0.1 centred at (0.55, 0.41, 0.32), dt = 0.2 h_min", "elements": M}}
+    for kind in ("curved", "flat"):
+        mesh = (wh.make_warped_uniform_hex(M, M, M, 0.0, 1.0, 0.05) if kind == "curved"
+                else wh.make_uniform_hex(M, 0.0, 1.0))
+        elem = wh.make_element(N)
+        x = wh.host_coords(mesh, elem)
+        bg = (wh.sample_background3d(wh.adm_bump_background(), 0.0, x[:, 0], x[:, 1], x[:, 2])
+              if kind == "curved" else None)
+        u0, v0 = wh.gaussian_pulse(x, (0.55, 0.41, 0.32), width=0.1)
+        tau = wh.tau_effective(TAU_LITERAL, N)
+        ws = {p: wh.Workspace(mesh, elem, tau=tau, eps_ko=EPS_KO, precision=p, bg=bg) for p in ("fp64", "mixed")}
+        dt = wh.recommended_dt_rk4(ws["fp64"], 0.2)
+        st = {p: (torch.from_numpy(u0).cuda(), torch.from_numpy(v0).cuda()) for p in ws}
+        o = wh.Workspace(mesh, elem, tau=tau, bg=bg)
+        z = torch.zeros_like(st["fp64"][0])
+        rows, t = [], 0.0
+        for T in (0.125, 0.25, 0.5, 1.0, 2.0):
+            k = int(round((T - t) / dt))
+            for p in ws:
+                ws[p].rk4_steps(st[p][0], st[p][1], dt, k)
+            t += k * dt
+            du = (st["mixed"][0] - st["fp64"][0]).contiguous()
+            dv = (st["mixed"][1] - st["fp64"][1]).contiguous()
+            rows.append({"t": t, "steps": int(round(t / dt)),
+                         "u": math.sqrt(o.energy(z, du) / o.energy(z, st["fp64"][0])),
+                         "u_t": math.sqrt(o.energy(z, dv) / o.energy(z, st["fp64"][1]))})
+            print(f"{kind} t={t:.3f} u {rows[-1]['u']:.3e} u_t {rows[-1]['u_t']:.3e}", file=sys.stderr, flush=True)
+        out[kind] = {"dt": dt, "samples": rows}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -484,10 +523,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default tile shapes instead of the autotuner's")
-    ap.add_argument("--config", choices=["plain", "curved", "batch", "serving", "passes", "fo"], default="plain",
+    ap.add_argument("--config", choices=["plain", "curved", "batch", "serving", "passes", "fo", "mixed_error"],
+                    default="plain",
                     help="plain: the FP64 flat workload (headline); curved: the warped-mesh "
                          "ADM workload, FP64 vs mixed, error at --t-final")
     ap.add_argument("--instances", type=int, default=512, help="batch config: number of instances")
+    ap.add_argument("--elems", type=int, default=12, help="mixed_error config: elements per axis")
     ap.add_argument("--requests", type=int, default=64, help="serving config: requests per rate")
     ap.add_argument("--no-sweep", action="store_true",
                     help="passes config: only the default tiles, each pass launched 3 times (for counter runs)")
@@ -528,6 +569,9 @@ def main():
         return
     if args.config == "fo":
         print(json.dumps(fo_run(wh, torch, args)), flush=True)
+        return
+    if args.config == "mixed_error":
+        print(json.dumps(mixed_error_run(wh, torch, args)), flush=True)
         return
 
     mesh = wh.make_uniform_hex(M_ELEMS, 0.0, 1.0)
