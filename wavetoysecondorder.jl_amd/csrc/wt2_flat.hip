@@ -1126,6 +1126,14 @@ static hipError_t batch_build_t(const PassDesc* d, int n, FlatBatchPlan& plan)
         shmem = std::max(shmem, sh);
         for (int t = 0; t < tiles; ++t) work.push_back(make_int2(i, t));
     }
+    if (n > 0 && tab[0].xcd) {
+        // the XCD-contiguous order of the single-context z passes: the
+        // workgroup with index b (XCD b % 8) takes entry (b % 8) W/8 + b / 8
+        const int W = int(work.size()), full = (W / 8) * 8;
+        std::vector<int2> perm(work);
+        for (int b = 0; b < full; ++b) perm[b] = work[(b % 8) * (W / 8) + b / 8];
+        work.swap(perm);
+    }
     const size_t tb = sizeof(KArgs<N, TC>) * size_t(n);
     const size_t wb = sizeof(int2) * work.size();
     if (tb > plan.tab_cap) {
