@@ -206,6 +206,7 @@ struct wt2_ctx {
     int N, Ne;
     int64_t npts;
     double xi[16], w[16], G[256];
+    float G32[256];   // the FP32 G of the FP32 operator table (build_op_table32)
     double h[3];
     double mu, inv_mu5, kos, max_sym_eig, h_min;
     int curved;
@@ -817,7 +818,7 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
         std::vector<double> tab(ts);
         std::vector<float> tab32(ts);
         build_op_table(N, c->G, c->w, tab.data());
-        for (int q = 0; q < ts; ++q) tab32[q] = float(tab[q]);
+        build_op_table32(N, c->G, c->w, tab32.data(), c->G32);
         rc = alloc_dev(c, &c->op64, ts * sizeof(double));
         if (rc == WT2_OK) rc = alloc_dev(c, &c->op32, ts * sizeof(float));
         if (rc == WT2_OK && (hipMemcpy(c->op64, tab.data(), ts * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
@@ -828,7 +829,10 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
     }
     if (rc == WT2_OK && c->curved)
         rc = curved_init(c->cd, *desc, c->xi, c->w, c->G, nullptr, c->stream, &c->bytes);
-    if (rc == WT2_OK && c->curved) { c->cd.op64 = c->op64; c->cd.op32 = c->op32; }
+    if (rc == WT2_OK && c->curved) {
+        c->cd.op64 = c->op64; c->cd.op32 = c->op32;
+        std::memcpy(c->cd.G32, c->G32, sizeof(c->G32));
+    }
     if (rc != WT2_OK) {
         wt2_ctx_destroy(c);
         return rc;
@@ -907,7 +911,10 @@ int wt2_ctx_set_background(wt2_ctx* c, const double* bg10)
     int rc;
     if (!c->curved) {
         rc = curved_init(c->cd, c->d, c->xi, c->w, c->G, bg10, c->stream, &c->bytes);
-        if (rc == WT2_OK) { c->cd.op64 = c->op64; c->cd.op32 = c->op32; }
+        if (rc == WT2_OK) {
+            c->cd.op64 = c->op64; c->cd.op32 = c->op32;
+            std::memcpy(c->cd.G32, c->G32, sizeof(c->G32));
+        }
         if (rc == WT2_OK) c->curved = 1;
     } else {
         rc = curved_set_background(c->cd, c->d, c->xi, c->w, c->G, bg10, c->stream);

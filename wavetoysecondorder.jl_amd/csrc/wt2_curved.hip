@@ -602,7 +602,9 @@ curved_volume_batch_kernel(const CArgs<N, TC>* __restrict__ tab, const int2* __r
 template <int N, typename TC>
 static void make_cargs(const CurvedData& cd, const CurvedCall& cc, CArgs<N, TC>& a)
 {
-    for (int i = 0; i < N * N; ++i) a.k.G[i] = TC(cd.G[i]);
+    // FP32: the matrix the FP32 even/odd table applies (z derivatives), so
+    // that all three axes and the lifts use one G
+    for (int i = 0; i < N * N; ++i) a.k.G[i] = std::is_same<TC, float>::value ? TC(cd.G32[i]) : TC(cd.G[i]);
     for (int i = 0; i < N; ++i) { a.k.w[i] = TC(cd.w[i]); a.k.winv[i] = TC(1.0 / cd.w[i]); }
     a.k.tau = TC(cd.tau);
     for (int f = 0; f < 6; ++f) a.k.bv[f] = TC(cc.bv[f]);

@@ -20,6 +20,7 @@ struct CurvedData {
     double tau = 0.0;
     double w[16] = {0};
     double G[256] = {0};
+    float G32[256] = {0};       // FP32 G, the same matrix as the FP32 even/odd table
     double h_min = 0.0;
     double* coef64 = nullptr;   // 13 per node, SoA: coef64[q*npts + n]
     float* coef32 = nullptr;    // same, FP32 copy (mixed precision)

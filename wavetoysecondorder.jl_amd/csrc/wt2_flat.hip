@@ -705,6 +705,53 @@ void build_op_table(int N, const double* G, const double* w, double* tab)
     }
 }
 
+// The FP32 table and the FP32 matrix it stands for.  The even/odd form
+// applies the matrix with rows y_i = (AE_i + AO_i)/2 . x_p + (AE_i - AO_i)/2 .
+// x_{N-1-p} (i, p < N/2); each (AE, AO) pair is rounded onto one grid, twice
+// the FP32 spacing of the larger, so that these halves are FP32 numbers and
+// G32 -- used where a kernel applies G row by row -- is exactly the matrix the
+// even/odd form applies.  Two different FP32 roundings of G in one operator
+// break its discrete SBP symmetry by the rounding.
+void build_op_table32(int N, const double* G, const double* w, float* tab, float* G32)
+{
+    const int H = N / 2, E = N - N / 2;
+    auto quant = [](double a, double b, double& qa, double& qb) {
+        const double m = std::max(std::fabs(a), std::fabs(b));
+        if (m == 0.0) { qa = qb = 0.0; return; }
+        const double d = std::ldexp(1.0, int(std::floor(std::log2(m))) - 22);
+        qa = std::nearbyint(a / d) * d;
+        qb = std::nearbyint(b / d) * d;
+    };
+    for (int q = 0; q < N * N; ++q) G32[q] = 0.0f;
+    for (int i = 0; i < H; ++i)
+        for (int p = 0; p < H; ++p) {
+            double ae, ao;
+            quant(G[i * N + p] + G[i * N + (N - 1 - p)], G[i * N + p] - G[i * N + (N - 1 - p)], ae, ao);
+            tab[i * E + p] = float(ae);
+            tab[E * H + i * H + p] = float(ao);
+            G32[i * N + p] = float(0.5 * (ae + ao));
+            G32[i * N + (N - 1 - p)] = float(0.5 * (ae - ao));
+        }
+    if (E > H) {                       // odd N: middle column and middle row
+        for (int i = 0; i < H; ++i) {
+            tab[i * E + H] = float(G[i * N + H]);
+            G32[i * N + H] = tab[i * E + H];
+        }
+        for (int p = 0; p < H; ++p) {
+            const float ao = float(G[H * N + p] - G[H * N + (N - 1 - p)]);
+            tab[E * H + H * H + p] = ao;
+            G32[H * N + p] = 0.5f * ao;
+            G32[H * N + (N - 1 - p)] = -0.5f * ao;
+        }
+    }
+    for (int i = 0; i < H; ++i)        // the lower rows by centro-antisymmetry
+        for (int p = 0; p < N; ++p) G32[(N - 1 - i) * N + (N - 1 - p)] = -G32[i * N + p];
+    for (int i = 0; i < N; ++i) {
+        tab[2 * E * H + i] = float(double(G32[i]) / w[i]);
+        tab[2 * E * H + N + i] = float(double(G32[(N - 1) * N + i]) / w[i]);
+    }
+}
+
 static int pick_lpw(int Ma) { return 64 / Ma; }
 
 // Upper bound of a pass kernel's static LDS (the operator table): the

@@ -49,6 +49,7 @@ struct AxisOps {
 // Even/odd operator table of the GLL differentiation matrix (see wt2_flat.hip).
 int op_table_size(int N);
 void build_op_table(int N, const double* G, const double* w, double* tab);
+void build_op_table32(int N, const double* G, const double* w, float* tab, float* G32);
 
 // Everything one pencil-pass launch needs (type-erased pointers).
 struct PassDesc {
