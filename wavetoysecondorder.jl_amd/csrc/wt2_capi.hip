@@ -226,7 +226,8 @@ struct wt2_ctx {
     double* red;    // reduction partials + result
     double* Edev;   // energy history (device)
     void* op64;     // operator table (FP64)
-    void* op32;     // operator table (FP32)
+    void* op32;     // operator table (FP32, each entry rounded)
+    void* op32c;    // FP32 table of the curved operator: the same matrix as G32
     int Ecap;
     CurvedData cd;
     double* fo_D;   // first-order path: DPhi_1..3 (lazy)
@@ -642,7 +643,7 @@ int wt2_ctx_footprint(const wt2_mesh_desc* desc, int32_t with_background, int64_
     // the same allocations as wt2_ctx_create: S, P, A register blocks, Lu,
     // reduction partials, operator tables; plus the curved coefficients
     int64_t b = 3 * (2 * n * es) + n * int64_t(sizeof(double)) + (RED_BLOCKS + 8) * int64_t(sizeof(double)) +
-                ts * int64_t(sizeof(double) + sizeof(float));
+                ts * int64_t(sizeof(double) + 2 * sizeof(float));
     if (desc->warp_A != 0.0 || with_background)
         b += 13 * n * int64_t(sizeof(double) + sizeof(float)) + n * int64_t(sizeof(double)) +
              Ne * 6 * N * N * int64_t(sizeof(double));
@@ -816,13 +817,16 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
     if (rc == WT2_OK) {
         const int ts = op_table_size(N);
         std::vector<double> tab(ts);
-        std::vector<float> tab32(ts);
+        std::vector<float> tab32(ts), tab32c(ts);
         build_op_table(N, c->G, c->w, tab.data());
-        build_op_table32(N, c->G, c->w, tab32.data(), c->G32);
+        for (int q = 0; q < ts; ++q) tab32[q] = float(tab[q]);
+        build_op_table32(N, c->G, c->w, tab32c.data(), c->G32);
         rc = alloc_dev(c, &c->op64, ts * sizeof(double));
         if (rc == WT2_OK) rc = alloc_dev(c, &c->op32, ts * sizeof(float));
+        if (rc == WT2_OK) rc = alloc_dev(c, &c->op32c, ts * sizeof(float));
         if (rc == WT2_OK && (hipMemcpy(c->op64, tab.data(), ts * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-                             hipMemcpy(c->op32, tab32.data(), ts * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)) {
+                             hipMemcpy(c->op32, tab32.data(), ts * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+                             hipMemcpy(c->op32c, tab32c.data(), ts * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)) {
             set_error("wt2_ctx_create: operator table upload failed");
             rc = WT2_ERR_HIP;
         }
@@ -830,7 +834,7 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
     if (rc == WT2_OK && c->curved)
         rc = curved_init(c->cd, *desc, c->xi, c->w, c->G, nullptr, c->stream, &c->bytes);
     if (rc == WT2_OK && c->curved) {
-        c->cd.op64 = c->op64; c->cd.op32 = c->op32;
+        c->cd.op64 = c->op64; c->cd.op32 = c->op32c;
         std::memcpy(c->cd.G32, c->G32, sizeof(c->G32));
     }
     if (rc != WT2_OK) {
@@ -848,7 +852,7 @@ void wt2_ctx_destroy(wt2_ctx* c)
     hipFree(c->Sb); hipFree(c->Pb); hipFree(c->Ab);
     for (int f = 0; f < 2; ++f) { hipFree(c->Y[f]); hipFree(c->O[f]); }
     hipFree(c->Lu); hipFree(c->red); hipFree(c->Edev);
-    hipFree(c->op64); hipFree(c->op32);
+    hipFree(c->op64); hipFree(c->op32); hipFree(c->op32c);
     hipFree(c->fo_D);
     curved_free(c->cd);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
@@ -912,7 +916,7 @@ int wt2_ctx_set_background(wt2_ctx* c, const double* bg10)
     if (!c->curved) {
         rc = curved_init(c->cd, c->d, c->xi, c->w, c->G, bg10, c->stream, &c->bytes);
         if (rc == WT2_OK) {
-            c->cd.op64 = c->op64; c->cd.op32 = c->op32;
+            c->cd.op64 = c->op64; c->cd.op32 = c->op32c;
             std::memcpy(c->cd.G32, c->G32, sizeof(c->G32));
         }
         if (rc == WT2_OK) c->curved = 1;
