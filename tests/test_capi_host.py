@@ -100,6 +100,28 @@ def test_null_pointers_rejected():
     assert L.wt2_rk4_steps(None, None, None, 0.1, 1, 0, None, 0) == _lib.WT2_ERR_ARG
     assert L.wt2_energy(None, None, None, None, 0) == _lib.WT2_ERR_ARG
     assert L.wt2_ctx_info(None, None) == _lib.WT2_ERR_ARG
+    assert L.wt2_fo_rhs(None, None, None, None, None, None, 0.1, None, 0) == _lib.WT2_ERR_ARG
+    assert L.wt2_fo_energy(None, None, None, None, None, 0) == _lib.WT2_ERR_ARG
+    assert L.wt2_fo_coef(None, None, None, 0) == _lib.WT2_ERR_ARG
+
+
+def test_host_descriptors_validate_their_arguments():
+    import wavetoy_hip as wh
+    mesh = wh.make_uniform_hex(2, 0.0, 1.0)
+    elem = wh.make_element(4)
+    d = wh.mesh_desc(mesh, elem, sommerfeld_faces=(2, 5), sommerfeld_R=4.0)
+    assert list(d.face_kind) == [0, 7, 0, 0, 7, 0] and d.sommerfeld_inv_R == 0.25
+    assert wh.mesh_desc(mesh, elem).sommerfeld_inv_R == 0.0
+    with pytest.raises(ValueError):
+        wh.mesh_desc(mesh, elem, sommerfeld_faces=(7,))
+    with pytest.raises(ValueError):
+        wh.mesh_desc(mesh, elem, sommerfeld_faces=(1,), sommerfeld_R=0.0)
+    bc = wh.make_bc3d(["sommerfeld", "dirichlet", "excision", "full_dirichlet", 2, 1], sigma=0.5)
+    assert bc.kinds == (2, 1, 3, 4, 2, 1) and bc.sigma == 0.5
+    with pytest.raises(KeyError):
+        wh.make_bc3d(["absorbing"] * 6)
+    with pytest.raises(ValueError):
+        wh.make_bc3d(["sommerfeld"] * 5)
 
 
 def test_host_coords_match_oracle():
