@@ -603,7 +603,8 @@ def main():
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_flat_pass_fp64_32x8.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+        tj = json.load(open(tfile))
+        traffic = tj.get("hbm_bytes_per_launch_by_launches_per_step", {}).get(str(lps), tj.get("hbm_bytes_per_launch"))
 
     out = {
         "metric": "grid-point updates/s per RK4 step, mixed vs FP64 at <=1e-5 err; p99 latency",
@@ -628,7 +629,7 @@ def main():
                      "kernel": f"flat_xpass_kernel<8,double,...> (all {lps} pencil passes of a step)",
                      "launches_per_step": lps,
                      "algorithmic_bytes_per_point_step": FUSED_MIN_BYTES["fp64"],
-                     "pass_design_bytes_per_point_step": step_bytes("fp64"),
+                     "pass_design_bytes_per_point_step": step_bytes("fp64", lps),
                      "pass_design_GBs": design_GBs},
         "rhs_roofline": {"bound": "hbm", "kernel": "flat_lap1_kernel<8,double> (wt2_rhs3d)",
                          "ms": r64["rhs_ms"], "achieved": 24 * npts / rhs_s / 1e9, "peak": PEAK_HBM_GBS,

@@ -310,17 +310,18 @@ def test_exchange_kernels_drop_in_and_rhs_outputs(kind):
     assert relerr(du, du_ref) <= TOL64 and relerr(dv, dv_ref) <= TOL64
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_folded_schedule(prec, monkeypatch):
-    """The opt-in folded RK4 schedule (WT2_FOLD=1: 8 passes per step, the first
-    pass of each stage computed by the previous stage's RK pass): 6 steps with
-    KO against the oracle, and a batch of it equal to the single run."""
+@pytest.mark.parametrize("prec,fold", [("fp64", 1), ("mixed", 1), ("fp64", 0), ("mixed", 0)])
+def test_folded_and_unfolded_schedules(prec, fold, monkeypatch):
+    """The folded RK4 schedule (the default, WT2_FOLD=1: 8 passes per step, the
+    first pass of each stage computed by the previous stage's RK pass) and the
+    12-pass one (WT2_FOLD=0): 6 steps with KO against the oracle, and a batch of
+    it equal to the single run."""
     import torch
-    _say(f"folded schedule {prec}")
-    monkeypatch.setenv("WT2_FOLD", "1")
+    _say(f"schedule {prec} fold={fold}")
+    monkeypatch.setenv("WT2_FOLD", str(fold))
     N, M = 6, (5, 3, 4)
     ws, o = _pair(N, M, eps_ko=0.1, precision=prec)
-    assert ws.info()["launches_per_step"] == 8
+    assert ws.info()["launches_per_step"] == (8 if fold else 12)
     x = o.coords()
     u0 = np.exp(-np.sum((x - 0.5) ** 2, axis=1) / 0.02)
     v0 = 0.1 * np.sin(math.pi * x[:, 0])
