@@ -601,7 +601,7 @@ TuneKey tune_key(const wt2_ctx* c)
     k.v[0] = c->N; k.v[1] = c->d.M[0]; k.v[2] = c->d.M[1]; k.v[3] = c->d.M[2];
     k.v[4] = c->d.precision; k.v[5] = c->curved; k.v[6] = c->d.eps_ko > 0;
     k.v[7] = c->d.periodic[0] + 2 * c->d.periodic[1] + 4 * c->d.periodic[2];
-    k.v[8] = int(som_axis(c, 0)) + 2 * int(som_axis(c, 1)) + 4 * int(som_axis(c, 2));
+    k.v[8] = int(som_axis(c, 0)) + 2 * int(som_axis(c, 1)) + 4 * int(som_axis(c, 2)) + 8 * (c->fold != 0);
     return k;
 }
 }  // namespace
@@ -788,17 +788,17 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
     }
 
     {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto it = g_tune.find(tune_key(c));
-        if (it != g_tune.end())
-            for (int a = 0; a < 3; ++a) { c->nb[a] = it->second.nb[a]; c->direct[a] = it->second.direct[a]; }
-    }
-    {
         // the folded 8-pass schedule by default (32^3 x 8^3: 1.66 ms per FP64
         // step against 1.91 ms for the 12-pass one, 1.16 against 1.29 ms mixed,
         // with the XCD-ordered z passes); WT2_FOLD=0 selects the 12-pass one
         const char* ev = std::getenv("WT2_FOLD");
         c->fold = ev ? std::atoi(ev) != 0 : 1;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(tune_key(c));
+        if (it != g_tune.end())
+            for (int a = 0; a < 3; ++a) { c->nb[a] = it->second.nb[a]; c->direct[a] = it->second.direct[a]; }
     }
     if (hipGetDevice(&c->device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1124,6 +1124,9 @@ int wt2_ctx_autotune(wt2_ctx* c, double* u, double* ud, double dt, int32_t flags
         int bnb = c->nb[a], bdir = c->direct[a];
         const int lpw = 64 / c->d.M[a];
         for (int dir = 2; dir >= 0; --dir) {
+            // the folded schedule needs LDS-free x and y passes: an LDS-staged
+            // candidate there would switch it off (8 -> 12 launches per step)
+            if (dir == 0 && a <= 1 && c->fold && !c->curved) continue;
             for (int nb = 1; nb <= N; ++nb) {
                 if (!tiles_ok(c, a, nb, dir)) continue;
                 c->direct[a] = dir;
