@@ -69,3 +69,93 @@ def test_curved_gaussian_evolution_energy(precision):
     E = r["energy"]
     assert np.all(np.isfinite(E))
     assert np.all(np.diff(E) <= 1e-10 * E[0])
+
+
+def test_outgoing_pulse_converges_and_radial_mode_starts_exact():
+    """ic = "outgoing" (src/wave.jl:292-320) on the Dirichlet cube while the
+    pulse is far from the faces: l2_err against the exact pulse falls by > 4x
+    per element doubling.  ic = "radial": l2_err is 0 at t0 and the sampled
+    line holds the eigenmode."""
+    print("outgoing pulse convergence", flush=True)
+    errs = []
+    for M in (4, 8):
+        r = wh.evolve3d(N=6, M=M, ic="outgoing", ic_pulse_offset=0.2, ic_pulse_width=0.05, eps_ko=0.0,
+                        t1=0.05, cfl=0.1, Nt=2)
+        assert r["l2_err"][0] == 0.0
+        errs.append(r["l2_err"][-1])
+    print("outgoing l2 errors", errs, flush=True)
+    assert errs[0] / errs[1] > 4
+    r = wh.evolve3d(N=6, M=4, ic="radial", eps_ko=0.1, t1=0.1, cfl=0.2, Nt=3)
+    assert r["l2_err"][0] == 0.0 and np.all(np.isfinite(r["l2_err"]))
+    xs = r["xs_line"]
+    line = np.stack([xs, np.full_like(xs, r["y_target"]), np.full_like(xs, r["z_target"])], axis=1)
+    assert np.allclose(r["us"][:, 0], wh.eigenmode_radial(line, 0.0, R=0.5, centre=(0.5, 0.5, 0.5))[0], atol=1e-15)
+    assert np.all(np.diff(r["energy"]) <= 1e-12 * r["energy"][0])
+
+
+def test_conservative_periodic_plane_wave_convergence_and_energy():
+    """evolve3d(formulation = "conservative"): the periodic Minkowski plane wave
+    converges (> 4x per element doubling) and keeps its energy to 1e-3
+    (test/evolve3d_tests.jl:13-27); also with a constant shift."""
+    print("conservative convergence", flush=True)
+    for bgk, shift in (("minkowski", (0.0, 0.0, 0.0)), ("constant_shift", (0.3, -0.2, 0.1))):
+        errs = []
+        for M in (4, 8):
+            r = wh.evolve3d(formulation="conservative", N=4, M=M, background=bgk, shift=shift, ic="exact",
+                            bc="periodic", t1=0.3, Nt=4)
+            errs.append(np.max(r["l2_err"]))
+        print(bgk, "l2 errors", errs, flush=True)
+        assert errs[0] / errs[1] > 4
+    r = wh.evolve3d(formulation="conservative", N=4, M=6, background="minkowski", ic="exact", bc="periodic",
+                    t1=0.5, Nt=5)
+    assert abs(r["energy"][-1] / r["energy"][0] - 1) < 1e-3
+    assert r["integrator_name"] == "RK4"
+    assert r["ts_actual"][-1] == 0.5 and np.all(r["ts_actual"] >= r["ts"])
+    assert r["Phis"].shape == (len(r["xs_line"]), 5) and np.all(np.diff(r["xs_line"]) > 0)
+
+
+def test_conservative_noise_with_auto_faces_stays_bounded():
+    """Noise of amplitude sqrt(eps) with bc = "auto" (Sommerfeld everywhere for
+    Minkowski) and KO 0.1: finite, energy never above its start by more than
+    1e-6 (test/evolve3d_tests.jl:29-37)."""
+    print("conservative noise", flush=True)
+    r = wh.evolve3d(formulation="conservative", N=4, M=4, background="minkowski", ic="noise", bc="auto",
+                    eps_ko=0.1, t1=0.5, Nt=5)
+    assert r["kinds"] == (wh.BC3d._NAMES["sommerfeld"],) * 6
+    assert np.all(np.isfinite(r["Phi_final"])) and np.all(np.isfinite(r["Pi_final"]))
+    assert np.max(r["energy"]) <= r["energy"][0] * (1 + 1e-6)
+
+
+def test_conservative_superluminal_shift_faces():
+    """A shift of 2 along x: the -x side is excised, the +x side takes the
+    exact state (full Dirichlet), the others are Sommerfeld; the run stays
+    finite and the error stays at the level of the Sommerfeld faces'
+    reflection of the oblique plane wave."""
+    print("conservative superluminal", flush=True)
+    r = wh.evolve3d(formulation="conservative", N=4, M=4, background="constant_shift", shift=(2.0, 0.0, 0.0),
+                    ic="exact", bc="auto", eps_ko=0.1, t1=0.2, Nt=3)
+    BC = wh.BC3d._NAMES
+    assert r["kinds"] == (BC["excision"], BC["full_dirichlet"]) + (BC["sommerfeld"],) * 4
+    print("superluminal l2", r["l2_err"], "energy", r["energy"], flush=True)
+    assert np.all(np.isfinite(r["Phi_final"])) and np.all(np.isfinite(r["energy"]))
+
+
+def test_conservative_driver_matches_oracle_rk4():
+    """Four driver steps on a periodic mesh equal the oracle's RK4 of the
+    first-order system (FP64 bar 1e-12)."""
+    print("conservative driver vs oracle", flush=True)
+    N, M = 4, 3
+    r0 = wh.evolve3d(formulation="conservative", N=N, M=M, ic="gaussian", bc="periodic", eps_ko=0.1,
+                     t1=1e-9, Nt=2, slice_y=0.0, slice_z=0.0)
+    dt = r0["dt"]
+    r = wh.evolve3d(formulation="conservative", N=N, M=M, ic="gaussian", bc="periodic", eps_ko=0.1,
+                    t1=4 * dt, Nt=2, slice_y=0.0, slice_z=0.0)
+    assert r["nsteps"] == 4
+    o = wo.Oracle(N, M, periodic=True)
+    x = o.coords()
+    Phi0 = np.exp(-np.sum((x - 0.5) ** 2, axis=1) / (2 * 0.15 ** 2))
+    bg = np.zeros((o.n, 10)); bg[:, 0] = 1.0; bg[:, 4] = 1.0; bg[:, 7] = 1.0; bg[:, 9] = 1.0
+    coef = wo.fo_coef(bg.reshape(-1))
+    Pr, Qr = o.fo_rk4(Phi0, np.zeros(o.n), coef, dt, 4, eps_ko=0.1)
+    assert np.max(np.abs(r["Phi_final"] - Pr)) <= 1e-12 * np.max(np.abs(Pr))
+    assert np.max(np.abs(r["Pi_final"] - Qr)) <= 1e-12 * np.max(np.abs(Qr))

@@ -8,6 +8,8 @@ from .core import (  # noqa: F401
     make_coef3d, make_element, make_geometry, make_operators, make_uniform_hex,
     make_warped_uniform_hex, make_workspace, recommended_dt_rk4, rhs_wave3d,
     sample_background3d, tau_effective, to_device, wave3d_rhs_full,
-    BC3d, make_bc3d, wave3d_curved_rhs, wave3d_energy,
+    BC3d, make_bc3d, wave3d_curved_rhs, wave3d_energy, eigenmode_radial, outgoing_pulse,
+    FACE_SUBLUMINAL, FACE_OUTFLOW, FACE_INFLOW, FACE_SONIC, classify_face1d, classify_face3d,
+    validate_bc1d,
 )
 from .evolve import evolve3d  # noqa: F401

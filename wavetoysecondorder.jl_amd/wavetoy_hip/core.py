@@ -468,6 +468,54 @@ def make_bc3d(kinds, sigma=1.0, gPhi=None, gPi=None):
     return BC3d(kinds, sigma, gPhi, gPi)
 
 
+# characteristic classes of a boundary face (src/boundaries1d.jl:79-83)
+FACE_SUBLUMINAL, FACE_OUTFLOW, FACE_INFLOW, FACE_SONIC = 1, 2, 3, 4
+_FACE_CLASS_NAMES = {FACE_SUBLUMINAL: "subluminal", FACE_OUTFLOW: "superluminal outflow",
+                     FACE_INFLOW: "superluminal inflow", FACE_SONIC: "sonic"}
+
+
+def classify_face1d(a, beta, nsign, sonic_tol=np.finfo(np.float64).eps ** 0.25):
+    """classify_face1d (src/boundaries1d.jl:97-108): the class of a face with
+    normal characteristic speed a = alpha sqrt(gamma^nn), shift component beta
+    along the normal axis and outward sign nsign = +-1."""
+    s_R = a - beta
+    s_L = -a - beta
+    tol = sonic_tol * a
+    if abs(s_R) <= tol or abs(s_L) <= tol:
+        return FACE_SONIC
+    out_R = s_R * nsign > 0
+    out_L = s_L * nsign > 0
+    if out_R and out_L:
+        return FACE_OUTFLOW
+    if not out_R and not out_L:
+        return FACE_INFLOW
+    return FACE_SUBLUMINAL
+
+
+def classify_face3d(alpha, b1, b2, b3, gu11, gu22, gu33, axis, nsign,
+                    sonic_tol=np.finfo(np.float64).eps ** 0.25):
+    """classify_face3d (src/boundaries3d.jl:25-32); axis 1..3."""
+    gunn = (gu11, gu22, gu33)[axis - 1]
+    bax = (b1, b2, b3)[axis - 1]
+    return classify_face1d(alpha * math.sqrt(gunn), bax, nsign, sonic_tol)
+
+
+def validate_bc1d(cls, kind, face):
+    """validate_bc1d (src/boundaries1d.jl:120-138): subluminal faces take
+    Dirichlet or Sommerfeld, outflow faces excision, inflow faces the full
+    state; sonic faces none.  Raises ValueError."""
+    if cls == FACE_SONIC:
+        raise ValueError(f"{face} boundary face is at a sonic point (|beta| ~ alpha/sqrt(gamma)): a characteristic "
+                         "speed vanishes there and no boundary condition is well-posed")
+    ok = ((cls == FACE_SUBLUMINAL and kind in (_lib.WT2_BC_DIRICHLET, _lib.WT2_BC_SOMMERFELD))
+          or (cls == FACE_OUTFLOW and kind == _lib.WT2_BC_EXCISION)
+          or (cls == FACE_INFLOW and kind == _lib.WT2_BC_FULL_DIRICHLET))
+    if not ok:
+        adm = {FACE_SUBLUMINAL: "dirichlet or sommerfeld", FACE_OUTFLOW: "excision",
+               FACE_INFLOW: "full_dirichlet"}[cls]
+        raise ValueError(f"{face} boundary face is {_FACE_CLASS_NAMES[cls]}; admissible boundary conditions: {adm}")
+
+
 def wave3d_curved_rhs(dPhi, dPi, Phi, Pi, coef, work, eps_ko=0.1, bc3d=None):
     """wave3d_curved_rhs!(Phidot, Pidot, Phi, Pi, coef; geom, ops, ws, eps_KO, bc3d)
     (src/wave3d_curved.jl:205-258), the affine first-order path; coef from
@@ -563,6 +611,43 @@ def gaussian_pulse(coords, centre, width=None, denom=None, amplitude=1.0):
     c = np.asarray(centre, dtype=np.float64)
     r2 = np.sum((coords - c[None, :]) ** 2, axis=1)
     return amplitude * np.exp(-r2 / denom), np.zeros(len(coords))
+
+
+def eigenmode_radial(coords, t, A=1.0, R=0.5, n=1, centre=(0.0, 0.0, 0.0)):
+    """eigenmode_radial! (src/wave.jl:207-226): u = A sinc(n r / R) cos(omega t),
+    u_t = -A omega sinc(n r / R) sin(omega t), omega = n pi / R, with
+    sinc(y) = sin(pi y) / (pi y) (1 at y = 0)."""
+    c = np.asarray(centre, dtype=np.float64)
+    r = np.sqrt(np.sum((coords - c[None, :]) ** 2, axis=1))
+    omega = n * math.pi / R
+    s = A * np.sinc(n * r / R)
+    return s * math.cos(omega * t), -omega * s * math.sin(omega * t)
+
+
+def _sinhc(x):
+    """sinh(x)/x, 1 at x = 0 (series below x^2 = 1e-3)."""
+    x = np.asarray(x, dtype=np.float64)
+    x2 = x * x
+    small = x2 < 1e-3
+    safe = np.where(small, 1.0, x)
+    return np.where(small, 1.0 + x2 * (1.0 / 6 + x2 * (1.0 / 120 + x2 / 5040)), np.sinh(safe) / safe)
+
+
+def outgoing_pulse(coords, t, A=1.0, s0=0.25, sigma=0.05, centre=(0.0, 0.0, 0.0)):
+    """outgoing_pulse! (src/wave.jl:292-320): the outgoing spherical pulse
+    u = [F(t - r) - F(t + r)] / r, F(s) = A exp(-(s + s0)^2 / (2 sigma^2)), in
+    the closed form with b = t + s0:
+    u = 2 A b / sigma^2 sinhc(b r / sigma^2) exp(-(b^2 + r^2) / (2 sigma^2)),
+    u_t = -2 A / sigma^2 exp(...) (b^2 / sigma^2 sinhc(b r / sigma^2) - cosh(b r / sigma^2))."""
+    c = np.asarray(centre, dtype=np.float64)
+    r = np.sqrt(np.sum((coords - c[None, :]) ** 2, axis=1))
+    is2 = 1.0 / (sigma * sigma)
+    b = t + s0
+    e = np.exp(-0.5 * (b * b + r * r) * is2)
+    sh = _sinhc(b * r * is2)
+    u = 2.0 * A * b * is2 * sh * e
+    ud = -2.0 * A * is2 * e * (b * b * is2 * sh - np.cosh(b * r * is2))
+    return u, ud
 
 
 def recommended_dt_rk4(work, cfl=0.2):

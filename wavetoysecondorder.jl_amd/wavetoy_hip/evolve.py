@@ -30,10 +30,34 @@ def build_slice(coords, y_target, z_target, atol=math.sqrt(np.finfo(np.float64).
     return np.asarray(idx, dtype=np.int64)[order], np.asarray(xs)[order]
 
 
-def evolve3d(N=8, M=4, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="cartesian",
-             ic_wavenumber=math.pi, ic_centre=(0.5, 0.5, 0.5), ic_width=None, ic_denom=None,
-             tau=None, eps_ko=0.1, t0=0.0, t1=None, nsteps=None, dt=None, cfl=0.2, Nt=2,
-             precision="fp64", energy_every=None, device=True, slice_y=None, slice_z=None):
+def evolve3d(formulation="strong", **kwargs):
+    """evolve3d(; formulation, kwargs...) (src/evolve.jl:1083-1089): the
+    second-order (u, u_t) driver (formulation "strong") or the first-order
+    (Phi, Pi) ADM driver ("conservative")."""
+    if formulation == "conservative":
+        return _evolve3d_conservative(**kwargs)
+    if formulation != "strong":
+        raise ValueError(f"evolve3d: formulation must be 'strong' or 'conservative'; got {formulation!r}")
+    return _evolve3d_strong(**kwargs)
+
+
+def _evolve3d_strong(N=8, M=4, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="cartesian",
+                     ic_wavenumber=math.pi, ic_centre=(0.5, 0.5, 0.5), ic_width=None, ic_denom=None,
+                     ic_radial_mode=1, ic_radius=None, ic_pulse_offset=None, ic_pulse_width=None,
+                     outer_bc="dirichlet", tau=None, eps_ko=0.1, t0=0.0, t1=None, nsteps=None, dt=None,
+                     cfl=0.2, Nt=2, precision="fp64", energy_every=None, device=True, slice_y=None,
+                     slice_z=None):
+    """The (u, u_t) driver (_evolve3d_strong, src/evolve.jl:877-1073) on the
+    cube mesh (warp_A = 0) or the warped cube.  ic: "cartesian" (standing
+    wave), "radial" (eigenmode_radial, radius ic_radius = L/2 and mode
+    ic_radial_mode about the cube centre), "outgoing" (outgoing_pulse,
+    offset ic_pulse_offset = L/4 and width ic_pulse_width = offset/5 about the
+    cube centre) -- these three carry the exact solution for l2_err -- or
+    "gaussian".  Outer faces are Dirichlet (the reference takes
+    outer_bc = "sommerfeld" only on its spherical meshes)."""
+    if outer_bc != "dirichlet":
+        raise ValueError(f"evolve3d: outer_bc = {outer_bc!r} is only supported on the spherical meshes "
+                         "(inflated cube, radial shell); the cube takes 'dirichlet'")
     mesh = (core.make_warped_uniform_hex(M, M, M, x0, x1, warp_A) if warp_A
             else core.make_uniform_hex(M, x0, x1))
     elem = core.make_element(N)
@@ -60,10 +84,23 @@ def evolve3d(N=8, M=4, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="cartesia
         omega = math.sqrt(3.0) * k / (x1 - x0)
         exact = lambda t: core.eigenmode_cartesian(coords, t, kx=k, ky=k, kz=k, omega=omega,
                                                    x0=x0, x1=x1)[0]
+    elif ic == "radial":
+        L = x1 - x0
+        R = L / 2 if ic_radius is None else float(ic_radius)
+        ctr = ((x0 + x1) / 2,) * 3
+        u0, v0 = core.eigenmode_radial(coords, t0, R=R, n=ic_radial_mode, centre=ctr)
+        exact = lambda t: core.eigenmode_radial(coords, t, R=R, n=ic_radial_mode, centre=ctr)[0]
+    elif ic == "outgoing":
+        L = x1 - x0
+        s0 = L / 4 if ic_pulse_offset is None else float(ic_pulse_offset)
+        sig = s0 / 5 if ic_pulse_width is None else float(ic_pulse_width)
+        ctr = ((x0 + x1) / 2,) * 3
+        u0, v0 = core.outgoing_pulse(coords, t0, s0=s0, sigma=sig, centre=ctr)
+        exact = lambda t: core.outgoing_pulse(coords, t, s0=s0, sigma=sig, centre=ctr)[0]
     elif ic == "gaussian":
         u0, v0 = core.gaussian_pulse(coords, ic_centre, width=ic_width, denom=ic_denom)
     else:
-        raise ValueError(f"unknown ic {ic}")
+        raise ValueError(f"evolve3d: unknown ic {ic!r} (cartesian, radial, outgoing, gaussian)")
 
     if device:
         import torch
@@ -118,3 +155,191 @@ def evolve3d(N=8, M=4, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="cartesia
                 ud_final=vf, dt=dt, nsteps=nsteps, dx=work.h_min, tau=tau_used, mu=work.mu,
                 y_target=y_target, z_target=z_target,
                 integrator_name="RK4", work=work, coords=coords)
+
+
+# ------------------------------------------------ first-order (Phi, Pi) driver
+
+def _background3d(kind, shift=(0.0, 0.0, 0.0)):
+    """The built-in static backgrounds of the cube (_background3d,
+    src/evolve.jl:1109-1128): Minkowski or a constant shift, alpha = 1,
+    gamma = I.  Returns the 10 background values of every node, the exact
+    plane wave Phi = sin(k (x + y + z) - omega t) with k = 2 pi,
+    omega = -k (b1 + b2 + b3) + k sqrt(3), its Pi = -k sqrt(3) cos(...), and
+    the largest characteristic speed |b1| + |b2| + |b3| + sqrt(3)."""
+    if kind == "minkowski":
+        b = (0.0, 0.0, 0.0)
+    elif kind == "constant_shift":
+        b = tuple(float(v) for v in shift)
+    elif kind == "radial_shift":
+        raise ValueError("evolve3d (conservative): background 'radial_shift' needs the radial-shell mesh, "
+                         "which this package does not build")
+    else:
+        raise ValueError(f"evolve3d (conservative): unknown background {kind!r} (minkowski, constant_shift)")
+    k = 2.0 * math.pi
+    s = b[0] + b[1] + b[2]
+    omega = -k * s + k * math.sqrt(3.0)
+    row = np.array([1.0, b[0], b[1], b[2], 1.0, 0.0, 0.0, 1.0, 0.0, 1.0])
+    phi_e = lambda t, x, y, z, lib=np: lib.sin(k * (x + y + z) - omega * t)
+    pi_e = lambda t, x, y, z, lib=np: (-omega - k * s) * lib.cos(k * (x + y + z) - omega * t)
+    return row, phi_e, pi_e, abs(b[0]) + abs(b[1]) + abs(b[2]) + math.sqrt(3.0)
+
+
+def _side_node(M, N, f):
+    """Flat node index of the representative node of boundary side f (0..5 =
+    -x, +x, -y, +y, -z, +z): the first element on that side, the side's row,
+    the middle tangential node (_side_node_3d, src/evolve.jl:1094-1106)."""
+    ax, hi = f // 2, f % 2
+    ec = [0, 0, 0]
+    if hi:
+        ec[ax] = M[ax] - 1
+    e = ec[0] + M[0] * (ec[1] + M[1] * ec[2])
+    mid = (N + 1) // 2 - 1
+    ijk = [mid, mid, mid]
+    ijk[ax] = N - 1 if hi else 0
+    return e * N ** 3 + ijk[0] + N * (ijk[1] + N * ijk[2])
+
+
+def _inverse_metric_diag(g):
+    """gamma^11, gamma^22, gamma^33 from (g11, g12, g13, g22, g23, g33)."""
+    g11, g12, g13, g22, g23, g33 = g
+    c11 = g22 * g33 - g23 * g23
+    c22 = g11 * g33 - g13 * g13
+    c33 = g11 * g22 - g12 * g12
+    det = g11 * c11 - g12 * (g12 * g33 - g23 * g13) + g13 * (g12 * g23 - g22 * g13)
+    return c11 / det, c22 / det, c33 / det
+
+
+def _face_kinds(bc, bg10, M, N):
+    """The six boundary kinds of a non-periodic cube (src/evolve.jl:1218-1243):
+    each face is classified from the background at its representative node;
+    bc = "auto" picks Sommerfeld / excision / full Dirichlet by class, a
+    6-tuple of names or WT2_BC_* codes is checked against the class."""
+    BC = core.BC3d._NAMES
+    kinds = []
+    for f in range(6):
+        q = bg10[_side_node(M, N, f)]
+        gu = _inverse_metric_diag(q[4:10])
+        cls = core.classify_face3d(q[0], q[1], q[2], q[3], gu[0], gu[1], gu[2], f // 2 + 1, 1 if f % 2 else -1)
+        if bc == "auto":
+            kind = {core.FACE_SUBLUMINAL: BC["sommerfeld"], core.FACE_OUTFLOW: BC["excision"]}.get(
+                cls, BC["full_dirichlet"])
+        else:
+            kind = BC[bc[f]] if isinstance(bc[f], str) else int(bc[f])
+        core.validate_bc1d(cls, kind, ("-x", "+x", "-y", "+y", "-z", "+z")[f] + " side")
+        kinds.append(kind)
+    return tuple(kinds)
+
+
+def _evolve3d_conservative(N=4, M=8, x0=0.0, x1=1.0, mesh_kind="cubical", background="minkowski",
+                           shift=(0.0, 0.0, 0.0), ic="exact", bc="periodic", ic_width=0.15,
+                           noise_amp=math.sqrt(np.finfo(np.float64).eps), eps_ko=0.0, t0=0.0, t1=1.0,
+                           Nt=200, cfl=0.1, slice_y=None, slice_z=None, seed=0):
+    """The first-order (Phi, Pi) ADM driver (_evolve3d_conservative,
+    src/evolve.jl:1149-1366) on the cube mesh: wave3d_curved_rhs! on the
+    device (wt2_fo_rhs) under classical RK4 with fixed dt = cfl h_min /
+    max_speed (capped by 1.4 / (eps_ko mu)), the last step cut to end at t1.
+    The reference picks RK4 for N <= 4 and higher-order adaptive-free
+    Runge-Kutta methods above; this driver runs RK4 for every N.
+
+    bc: "periodic", "auto" (classify each face) or a 6-tuple of kinds.  ic:
+    "exact" (plane wave; l2_err against it, full-Dirichlet faces driven by
+    it), "gaussian" (width ic_width about the cube centre) or "noise"
+    (noise_amp times normal deviates from numpy's generator with `seed`).
+    Returns the reference's outputs: ts, ts_actual, xs_line, perm, Phis, Pis
+    (line samples), l2_err, energy, Phi_final, Pi_final, dt, dx, kinds."""
+    import torch
+    if mesh_kind != "cubical":
+        raise ValueError(f"evolve3d (conservative): mesh_kind {mesh_kind!r} is not built here (cubical only)")
+    periodic = bc == "periodic"
+    if not (periodic or bc == "auto" or (isinstance(bc, (tuple, list)) and len(bc) == 6)):
+        raise ValueError(f"evolve3d: bc must be 'periodic', 'auto' or a 6-tuple; got {bc!r}")
+    if ic not in ("exact", "gaussian", "noise"):
+        raise ValueError(f"evolve3d: unknown ic {ic!r} (exact, gaussian, noise)")
+    mesh = core.make_uniform_hex(M, x0, x1, periodic)
+    elem = core.make_element(N)
+    coords = core.host_coords(mesh, elem)
+    n = coords.shape[0]
+    row, phi_e, pi_e, max_speed = _background3d(background, shift)
+    bg10 = np.broadcast_to(row, (n, 10)).copy()
+    kinds = None if periodic else _face_kinds(bc, bg10, mesh.M, N)
+    work = core.Workspace(mesh, elem)
+    coef = work.fo_coef(torch.from_numpy(bg10.reshape(-1)).cuda())
+    dx = work.h_min
+    dt = cfl * dx / max_speed
+    if eps_ko != 0:
+        dt = min(dt, 1.4 / (eps_ko * work.mu))
+
+    xg, yg, zg = (torch.from_numpy(coords[:, a].copy()).cuda() for a in range(3))
+    if ic == "exact":
+        Phi0 = phi_e(t0, coords[:, 0], coords[:, 1], coords[:, 2])
+        Pi0 = pi_e(t0, coords[:, 0], coords[:, 1], coords[:, 2])
+    elif ic == "gaussian":
+        c0 = (x0 + x1) / 2
+        r2 = np.sum((coords - c0) ** 2, axis=1)
+        Phi0 = np.exp(-r2 / (2 * ic_width * ic_width))
+        Pi0 = np.zeros(n)
+    elif ic == "noise":
+        rng = np.random.default_rng(seed)
+        Phi0 = noise_amp * rng.standard_normal(n)
+        Pi0 = noise_amp * rng.standard_normal(n)
+    Phi = torch.from_numpy(np.ascontiguousarray(Phi0)).cuda()
+    Pi = torch.from_numpy(np.ascontiguousarray(Pi0)).cuda()
+
+    withdata = ic == "exact"
+    needdata = kinds is not None and core.BC3d._NAMES["full_dirichlet"] in kinds
+    gPhi = torch.zeros_like(Phi) if needdata else None
+    gPi = torch.zeros_like(Phi) if needdata else None
+    bc3d = None if kinds is None else core.make_bc3d(kinds, gPhi=gPhi, gPi=gPi)
+
+    def rhs(dP, dQ, P, Q, t):
+        if needdata and withdata:
+            gPhi.copy_(phi_e(t, xg, yg, zg, lib=torch))
+            gPi.copy_(pi_e(t, xg, yg, zg, lib=torch))
+        work.fo_rhs(dP, dQ, P, Q, coef, eps_ko, bc3d)
+
+    k = [torch.empty_like(Phi) for _ in range(8)]
+    sP, sQ = torch.empty_like(Phi), torch.empty_like(Phi)
+
+    def rk4_step(t, h):
+        rhs(k[0], k[1], Phi, Pi, t)
+        torch.add(Phi, k[0], alpha=h / 2, out=sP); torch.add(Pi, k[1], alpha=h / 2, out=sQ)
+        rhs(k[2], k[3], sP, sQ, t + h / 2)
+        torch.add(Phi, k[2], alpha=h / 2, out=sP); torch.add(Pi, k[3], alpha=h / 2, out=sQ)
+        rhs(k[4], k[5], sP, sQ, t + h / 2)
+        torch.add(Phi, k[4], alpha=h, out=sP); torch.add(Pi, k[5], alpha=h, out=sQ)
+        rhs(k[6], k[7], sP, sQ, t + h)
+        for y, a, b, c, d in ((Phi, k[0], k[2], k[4], k[6]), (Pi, k[1], k[3], k[5], k[7])):
+            y.add_(a + 2 * b + 2 * c + d, alpha=h / 6)
+
+    c_mid = (x0 + x1) / 2
+    y_target = c_mid if slice_y is None else float(slice_y)
+    z_target = c_mid if slice_z is None else float(slice_z)
+    sidx, xs_line = build_slice(coords, y_target, z_target)
+    if sidx.size == 0:
+        raise ValueError(f"evolve3d: slice y={y_target} z={z_target} hit no nodes")
+    H = core.Geometry(mesh, elem).Hphys
+    ts = np.linspace(t0, t1, Nt)
+    Phis = np.empty((len(sidx), Nt)); Pis = np.empty_like(Phis)
+    ts_actual = np.empty(Nt); l2_err = np.empty(Nt); energy = np.empty(Nt)
+    t = float(t0)
+    nsteps = 0
+    for q, tq in enumerate(ts):
+        while t < tq:
+            last = t1 - t <= dt * (1 + 1e-12)   # the step that ends the interval lands on t1
+            h = t1 - t if last else dt
+            rk4_step(t, h)
+            t = t1 if last else t + h
+            nsteps += 1
+        ts_actual[q] = t
+        Ph, Qh = Phi.cpu().numpy(), Pi.cpu().numpy()
+        if not (np.all(np.isfinite(Ph)) and np.all(np.isfinite(Qh))):
+            raise FloatingPointError("evolve3d: non-finite state")
+        Phis[:, q] = Ph[sidx]; Pis[:, q] = Qh[sidx]
+        ref = phi_e(t, coords[:, 0], coords[:, 1], coords[:, 2]) if withdata else 0.0
+        l2_err[q] = math.sqrt(float(np.sum((Ph - ref) ** 2 * H)))
+        energy[q] = work.fo_energy(Phi, Pi, coef)
+    return dict(ts=ts, ts_actual=ts_actual, xs_line=xs_line, perm=np.arange(len(sidx)), Phis=Phis, Pis=Pis,
+                l2_err=l2_err, energy=energy, Phi_final=Phi.cpu().numpy(), Pi_final=Pi.cpu().numpy(),
+                dt=dt, dx=dx, nsteps=nsteps, kinds=kinds, background=background, ic=ic, bc=bc,
+                mesh_kind=mesh_kind, x0=x0, x1=x1, y_target=y_target, z_target=z_target,
+                integrator_name="RK4", work=work, coords=coords, coef=coef)
