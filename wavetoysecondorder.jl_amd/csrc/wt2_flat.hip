@@ -149,8 +149,8 @@ __device__ __forceinline__ void apply_G_ends(const TC* __restrict__ op, const TC
     TC e[E], o[H];
 #pragma unroll
     for (int p = 0; p < H; ++p) {
-        e[p] = TC(0.5) * (x[p] + x[N - 1 - p]);
-        o[p] = TC(0.5) * (x[p] - x[N - 1 - p]);
+        e[p] = x[p] + x[N - 1 - p];
+        o[p] = x[p] - x[N - 1 - p];
     }
     if (E > H) e[E - 1] = x[H];
     TC b = TC(0), av = TC(0);
@@ -696,9 +696,9 @@ void build_op_table(int N, const double* G, const double* w, double* tab)
     const int H = N / 2, E = N - N / 2;
     for (int i = 0; i < H; ++i)
         for (int p = 0; p < E; ++p)
-            tab[i * E + p] = (p < H) ? G[i * N + p] + G[i * N + (N - 1 - p)] : G[i * N + p];
+            tab[i * E + p] = (p < H) ? 0.5 * (G[i * N + p] + G[i * N + (N - 1 - p)]) : G[i * N + p];
     for (int i = 0; i < E; ++i)
-        for (int p = 0; p < H; ++p) tab[E * H + i * H + p] = G[i * N + p] - G[i * N + (N - 1 - p)];
+        for (int p = 0; p < H; ++p) tab[E * H + i * H + p] = 0.5 * (G[i * N + p] - G[i * N + (N - 1 - p)]);
     for (int i = 0; i < N; ++i) {
         tab[2 * E * H + i] = G[i] / w[i];
         tab[2 * E * H + N + i] = G[(N - 1) * N + i] / w[i];
@@ -707,8 +707,9 @@ void build_op_table(int N, const double* G, const double* w, double* tab)
 
 // The FP32 table and the FP32 matrix it stands for.  The even/odd form
 // applies the matrix with rows y_i = (AE_i + AO_i)/2 . x_p + (AE_i - AO_i)/2 .
-// x_{N-1-p} (i, p < N/2); each (AE, AO) pair is rounded onto one grid, twice
-// the FP32 spacing of the larger, so that these halves are FP32 numbers and
+// x_{N-1-p} (i, p < N/2; the table holds AE/2, AO/2); each (AE, AO) pair is
+// rounded onto one grid, twice the FP32 spacing of the larger, so that these
+// halves are FP32 numbers and
 // G32 -- used where a kernel applies G row by row -- is exactly the matrix the
 // even/odd form applies.  Two different FP32 roundings of G in one operator
 // break its discrete SBP symmetry by the rounding.
@@ -727,8 +728,8 @@ void build_op_table32(int N, const double* G, const double* w, float* tab, float
         for (int p = 0; p < H; ++p) {
             double ae, ao;
             quant(G[i * N + p] + G[i * N + (N - 1 - p)], G[i * N + p] - G[i * N + (N - 1 - p)], ae, ao);
-            tab[i * E + p] = float(ae);
-            tab[E * H + i * H + p] = float(ao);
+            tab[i * E + p] = float(0.5 * ae);
+            tab[E * H + i * H + p] = float(0.5 * ao);
             G32[i * N + p] = float(0.5 * (ae + ao));
             G32[i * N + (N - 1 - p)] = float(0.5 * (ae - ao));
         }
@@ -739,7 +740,7 @@ void build_op_table32(int N, const double* G, const double* w, float* tab, float
         }
         for (int p = 0; p < H; ++p) {
             const float ao = float(G[H * N + p] - G[H * N + (N - 1 - p)]);
-            tab[E * H + H * H + p] = ao;
+            tab[E * H + H * H + p] = 0.5f * ao;
             G32[H * N + p] = 0.5f * ao;
             G32[H * N + (N - 1 - p)] = -0.5f * ao;
         }

@@ -12,7 +12,8 @@ namespace wt2 {
 //   [E*H, 2*E*H)      Ao[i][p] (i < E, p < H)  odd part
 //   [2EH, 2EH+N)      G[0][i] / w_i
 //   [2EH+N, 2EH+2N)   G[N-1][i] / w_i
-// with H = floor(N/2), E = ceil(N/2).
+// with H = floor(N/2), E = ceil(N/2), Ae[i][p] = (G[i][p] + G[i][N-1-p]) / 2,
+// Ao[i][p] = (G[i][p] - G[i][N-1-p]) / 2 and, for odd N, Ae[i][H] = G[i][H].
 template <int N>
 struct OpLayout {
     static constexpr int H = N / 2;
@@ -31,9 +32,11 @@ __device__ __forceinline__ T shfl64(T v, int src)
 }
 
 // y = G x on one element segment by the even/odd split of the GLL
-// differentiation matrix (G[N-1-i][p] = -G[i][N-1-p]): with e_p, o_p the
-// symmetric / antisymmetric halves of x, y_i = a_i + b_i and
+// differentiation matrix (G[N-1-i][p] = -G[i][N-1-p]): with e_p = x_p +
+// x_{N-1-p}, o_p = x_p - x_{N-1-p} (e_H = x_H for odd N), y_i = a_i + b_i and
 // y_{N-1-i} = b_i - a_i, a = Ae e, b = Ao o -- half the multiply-adds of G x.
+// The halves sit in the table, so no product by 1/2 is formed here; the
+// result is bit for bit that of halving e and o (both halvings are exact).
 template <int N, typename TC>
 __device__ __forceinline__ void apply_G(const TC* __restrict__ op, const TC (&x)[N], TC (&y)[N])
 {
@@ -42,8 +45,8 @@ __device__ __forceinline__ void apply_G(const TC* __restrict__ op, const TC (&x)
     TC e[E], o[H];
 #pragma unroll
     for (int p = 0; p < H; ++p) {
-        e[p] = TC(0.5) * (x[p] + x[N - 1 - p]);
-        o[p] = TC(0.5) * (x[p] - x[N - 1 - p]);
+        e[p] = x[p] + x[N - 1 - p];
+        o[p] = x[p] - x[N - 1 - p];
     }
     if (E > H) e[E - 1] = x[H];
 #pragma unroll
