@@ -5,7 +5,7 @@ exp(-|x-x0|^2/0.01) with x0 ~ U[0.3,0.7]^3 from the seed, u_t = 0,
 dt = 0.2 h_min), with the mixed FP32-RHS / FP64-state path measured beside it
 on the same initial data, and the CPU oracle timed on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--seed 0] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--seed 0] [--no-cpu] [--repeats 5]
 
 Prints one JSON line.  Only one GPU is used; --gpus > 1 exits with a message.
 """
@@ -60,7 +60,7 @@ def initial_data(wh, mesh, elem, seed):
     return u0, v0, x0
 
 
-def run_gpu(wh, torch, prec, u0, v0, steps, warmup, stream, tune=True):
+def run_gpu(wh, torch, prec, u0, v0, steps, warmup, stream, tune=True, repeats=0):
     mesh = wh.make_uniform_hex(M_ELEMS, 0.0, 1.0)
     elem = wh.make_element(N_NODES)
     tau = wh.tau_effective(TAU_LITERAL, N_NODES)
@@ -118,11 +118,33 @@ def run_gpu(wh, torch, prec, u0, v0, steps, warmup, stream, tune=True):
     b.record(stream)
     torch.cuda.synchronize()
     rhs_ms = a.elapsed_time(b) / rreps
+    # repeats (diagnostics, after the timed region): `repeats` more runs of
+    # `steps` steps on a copy of the state, for the median and minimum
+    rep_ms = []
+    if repeats > 0:
+        with torch.cuda.stream(stream):
+            ur, vr = u.clone(), v.clone()
+        ws.rk4_steps(ur, vr, dt, 1)          # graph capture for these pointers
+        for _ in range(repeats):
+            a.record(stream)
+            ws.rk4_steps(ur, vr, dt, steps)
+            b.record(stream)
+            torch.cuda.synchronize()
+            rep_ms.append(a.elapsed_time(b) / steps)
+        del ur, vr
     ws.set_stream(None)
     E = ws.energy(u, v)
     lps = ws.info()["launches_per_step"]
     return dict(ws=ws, u=u.cpu().numpy(), v=v.cpu().numpy(), dt=dt, wall=wall, ev_ms=ev_ms, lps=lps,
-                pass_ms=pass_ms, rhs_ms=rhs_ms, energy=E, npts=ws.npoints, h_min=ws.h_min, mu=ws.mu, tau=tau)
+                pass_ms=pass_ms, rhs_ms=rhs_ms, energy=E, npts=ws.npoints, h_min=ws.h_min, mu=ws.mu, tau=tau,
+                rep_ms=rep_ms)
+
+
+def repeat_summary(rep_ms):
+    """Median and minimum ms per step over the repeat runs (None without repeats)."""
+    if not rep_ms:
+        return None
+    return {"runs": len(rep_ms), "median": float(np.median(rep_ms)), "min": float(min(rep_ms))}
 
 
 def cpu_baseline(u0, v0, dt, budget_s=12.0):
@@ -525,6 +547,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="plain config: further runs of --steps steps after the timed one (median, minimum)")
     ap.add_argument("--no-tune", action="store_true", help="default tile shapes instead of the autotuner's")
     ap.add_argument("--config", choices=["plain", "curved", "batch", "serving", "passes", "fo", "mixed_error"],
                     default="plain",
@@ -582,7 +606,7 @@ def main():
     u0, v0, x0 = initial_data(wh, mesh, elem, args.seed)
     stream = torch.cuda.Stream()
 
-    r64 = run_gpu(wh, torch, "fp64", u0, v0, args.steps, args.warmup, stream, not args.no_tune)
+    r64 = run_gpu(wh, torch, "fp64", u0, v0, args.steps, args.warmup, stream, not args.no_tune, args.repeats)
     npts, K = r64["npts"], args.steps
     value = npts * K / r64["wall"]
     # dominant kernel: the pencil-pass kernel (every launch of a step: 8 with
@@ -636,11 +660,12 @@ def main():
                          "unit": "GB/s", "frac": 24 * npts / rhs_s / 1e9 / PEAK_HBM_GBS,
                          "algorithmic_bytes_per_point": 24, "compulsory_bytes_per_point": 16,
                          "compulsory_GBs": 16 * npts / rhs_s / 1e9},
+        "ms_per_step_repeats": repeat_summary(r64["rep_ms"]),
         "pass_ms_fp64": r64["pass_ms"],
         "energy_final": r64["energy"],
     }
     if not args.no_mixed:
-        rmx = run_gpu(wh, torch, "mixed", u0, v0, K, args.warmup, stream, not args.no_tune)
+        rmx = run_gpu(wh, torch, "mixed", u0, v0, K, args.warmup, stream, not args.no_tune, args.repeats)
         vmx = npts * K / rmx["wall"]
         H = wh.Geometry(mesh, elem).Hphys
         d = rmx["u"] - r64["u"]
@@ -649,7 +674,7 @@ def main():
         out["mixed"] = {"value": vmx, "ms_per_step": rmx["wall"] * 1e3 / K,
                         "speedup_vs_fp64": vmx / value, "hnorm_rel_err_vs_fp64": herr,
                         "max_rel_err_vs_fp64": maxerr, "dtype": "f32 RHS / f64 state",
-                        "pass_ms": rmx["pass_ms"],
+                        "pass_ms": rmx["pass_ms"], "ms_per_step_repeats": repeat_summary(rmx["rep_ms"]),
                         "roofline_achieved_GBs": FUSED_MIN_BYTES["mixed"] * npts * K / (rmx["ev_ms"] * 1e-3) / 1e9}
         del rmx
     if not args.no_cpu:
