@@ -661,6 +661,7 @@ def main():
                          "algorithmic_bytes_per_point": 24, "compulsory_bytes_per_point": 16,
                          "compulsory_GBs": 16 * npts / rhs_s / 1e9},
         "ms_per_step_repeats": repeat_summary(r64["rep_ms"]),
+        "tiles_fp64": r64["ws"].tiles(),
         "pass_ms_fp64": r64["pass_ms"],
         "energy_final": r64["energy"],
     }
@@ -675,6 +676,7 @@ def main():
                         "speedup_vs_fp64": vmx / value, "hnorm_rel_err_vs_fp64": herr,
                         "max_rel_err_vs_fp64": maxerr, "dtype": "f32 RHS / f64 state",
                         "pass_ms": rmx["pass_ms"], "ms_per_step_repeats": repeat_summary(rmx["rep_ms"]),
+                        "tiles": rmx["ws"].tiles(),
                         "roofline_achieved_GBs": FUSED_MIN_BYTES["mixed"] * npts * K / (rmx["ev_ms"] * 1e-3) / 1e9}
         del rmx
     if not args.no_cpu:
