@@ -347,3 +347,50 @@ def test_folded_and_unfolded_schedules(prec, fold, monkeypatch):
     b.remove(slot)
     b.close()
     assert done == 6 and np.array_equal(u2.cpu().numpy(), uh) and np.array_equal(E2, E)
+
+
+EDGE_CASES = [
+    # N, M, periodic: the longest element lines the pass kernels take (64 per axis,
+    # 32 for N >= 9), and a single element along y or z (not x: mu comes from
+    # the x lines, and on a single element G is nilpotent, so mu -> 0 and the
+    # KO scale mu^-5 is unbounded, in the oracle as in the reference's formula)
+    (3, (64, 2, 2), False),
+    (4, (2, 64, 2), (False, True, False)),
+    (8, (2, 2, 64), False),
+    (9, (32, 2, 2), False),
+    (10, (2, 32, 2), False),
+    (5, (3, 1, 2), False),
+    (6, (3, 1, 1), (True, False, False)),
+]
+
+
+@pytest.mark.parametrize("N,M,per", EDGE_CASES, ids=[f"N{c[0]}-M{'x'.join(map(str, c[1]))}" for c in EDGE_CASES])
+def test_longest_and_shortest_element_lines(N, M, per):
+    """Drop-in RHS and 3 RK4 steps with KO at the per-axis line-length limits and
+    with one element along an axis, FP64 against the oracle (1e-12) and the
+    mixed path within its H-norm bar (1e-5).  The box's sides are proportional
+    to M (cubic elements): mu, and with it the KO scale and the dt cap, comes
+    from the x axis alone (src/wave3d_curved.jl:158-178), so strongly
+    anisotropic elements would make KO along a finer axis unstable at that dt,
+    in the oracle as on the device."""
+    _say("edge case", N, M, per)
+    x1 = tuple(m / max(M) for m in M)
+    ws, o = _pair(N, M, x1=x1, periodic=per, eps_ko=0.1)
+    x = o.coords()
+    c = 0.4 * np.asarray(x1)
+    u0 = np.exp(-np.sum(((x - c) / np.asarray(x1)) ** 2, axis=1) / 0.05)
+    v0 = 0.2 * np.cos(2 * math.pi * x[:, 2] / x1[2]) * np.sin(math.pi * x[:, 0] / x1[0])
+    out = np.zeros(o.n)
+    ws.rhs(out, u0, None, (0.0,) * 6)
+    assert relerr(out, o.lap(u0)) <= TOL64
+    dt = 0.2 * o.hmin
+    uo, vo = o.rk4(u0, v0, dt, 3)
+    u = u0.copy(); v = v0.copy()
+    ws.rk4_steps(u, v, dt, 3)
+    assert relerr(u, uo) <= TOL64 and relerr(v, vo) <= TOL64
+    wm, _ = _pair(N, M, x1=x1, periodic=per, eps_ko=0.1, precision="mixed")
+    u = u0.copy(); v = v0.copy()
+    wm.rk4_steps(u, v, dt, 3)
+    H = o.hphys()
+    d = u - uo
+    assert math.sqrt(np.sum(H * d * d) / np.sum(H * uo * uo)) <= 1e-5
