@@ -52,8 +52,9 @@ def test_curved_rhs3d_matches_oracle(case):
     assert abs(ws.h_min / o.hmin - 1) < 1e-13
 
 
-@pytest.mark.parametrize("N,M", [(4, 2), (8, 2)])
+@pytest.mark.parametrize("N,M", [(4, 2), (8, 2), (4, (64, 2, 2)), (10, (32, 2, 2))])
 def test_curved_rhs_full_and_rk4_match_oracle(N, M):
+    """Also at the longest element lines of the KO passes (64; 32 for N >= 9)."""
     _say("curved rhs_full/rk4", N, M)
     ws, o, x = _pair(N, M, eps_ko=0.1)
     rng = np.random.default_rng(7)
@@ -69,7 +70,12 @@ def test_curved_rhs_full_and_rk4_match_oracle(N, M):
     uo, vo, Eo = o.rk4(u0, v0, dt, 5, energy=True)
     uu = u0.copy(); vv = v0.copy()
     E = ws.rk4_steps(uu, vv, dt, 5, energy_every=1)
-    assert relerr(uu, uo) <= TOL64 and relerr(vv, vo) <= TOL64
+    # five steps: the RHS bar per evaluation (checked above), and 1e-11 for the
+    # state at N = 10 on 32-element lines, where the six chained KO derivatives
+    # amplify the rounding differences of the two operation orders over the
+    # steps (measured 4.6e-12 on u_t, 4.5e-16 on u)
+    tol = TOL64 if N < 10 else 1e-11
+    assert relerr(uu, uo) <= TOL64 and relerr(vv, vo) <= tol
     assert np.max(np.abs(E - Eo)) <= TOL64 * np.max(np.abs(Eo))
 
 
