@@ -65,8 +65,8 @@ typedef struct wt2_mesh_desc {
     int32_t face_kind[6];    /* per face: 0 = Dirichlet (SIPG, tag f), 7 =
                                 Sommerfeld (free for SIPG plus the
                                 Bayliss-Turkel drag u_t + d_n u + u/R -> 0,
-                                src/wave.jl:444-518); flat meshes, bounded
-                                axes only                                   */
+                                src/wave.jl:444-518); bounded axes, flat
+                                or warped meshes without a background       */
     double  sommerfeld_inv_R;/* 1/R of that drag; 0 = plane-wave (BGT-0)     */
 } wt2_mesh_desc;
 

@@ -185,10 +185,11 @@ wto_ctx* wto_create(const wto_desc* d, const double* bg)
     wto_gll(N, c->xi, c->w, c->G);
     for (int a = 0; a < 3; ++a) c->h[a] = (d->x1[a] - d->x0[a]) / d->M[a];
     c->curved = (d->warp_A != 0.0) || (bg != NULL);
-    /* Sommerfeld faces: affine meshes, non-periodic axes only */
+    /* Sommerfeld faces: non-periodic axes, no background (the drag is the
+     * flat-spacetime condition of rhs_wave3d!) */
     for (int f = 0; f < 6; ++f) {
         if (d->face_kind[f] != 0 && d->face_kind[f] != 7) { free(c); return NULL; }
-        if (d->face_kind[f] == 7 && (c->curved || d->periodic[f / 2])) { free(c); return NULL; }
+        if (d->face_kind[f] == 7 && (bg != NULL || d->periodic[f / 2])) { free(c); return NULL; }
     }
     c->coords = (double*)malloc(sizeof(double) * 3 * c->npts);
     c->hphys = (double*)malloc(sizeof(double) * c->npts);
@@ -640,11 +641,27 @@ static void op_curved(const wto_ctx* c, const double* u, const double* v,
                 const double* un = en >= 0 ? u + en * npe : NULL;
                 const double* vn = (en >= 0 && with_adv) ? v + en * npe : NULL;
                 const double* con = en >= 0 ? c->coef + 13 * en * (long)npe : NULL;
+                int som = en < 0 && c->d.face_kind[2 * a + sd] == 7;
                 for (int q = 0; q < N; ++q)
                 for (int p = 0; p < N; ++p) {
                     int n = fidx(N, a, r, p, q);
                     int nn = fidx(N, a, rn, p, q);
                     const double* o = co + 13 * n;
+                    if (som) {
+                        /* Sommerfeld face (src/wave.jl:444-518): no SIPG terms,
+                         * and the drag -(wF/Hphys)(u_t + d_n u + u/R).  Unweighted
+                         * (times wgt = 1/|J| later): |n_u| d_n u = n K^ab G_b u
+                         * (outward normal n), |n_u| = |J| |grad xi_a| =
+                         * sqrt(K^aa / wgt), and wF/Hphys = |n_u| / (w_r |J|). */
+                        double gs[3];
+                        int i0 = n % N, j0 = (n / N) % N, k0 = n / (N * N);
+                        grad_node(c, ue, i0, j0, k0, gs);
+                        double Fs = Kab(o, a, 0) * gs[0] + Kab(o, a, 1) * gs[1] + Kab(o, a, 2) * gs[2];
+                        double nu = sqrt(Kab(o, a, a) / o[9]);
+                        double vv = v ? v[e * npe + n] : 0.0;
+                        ye[n] -= (nrm * Fs + nu * (vv + c->d.sommerfeld_inv_R * ue[n])) / w[r];
+                        continue;
+                    }
                     double du, dF = 0.0, kap = o[10 + a];
                     if (en >= 0) {
                         du = ue[n] - un[nn];

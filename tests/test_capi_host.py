@@ -82,7 +82,6 @@ def _create(d):
     (dict(N=10, tau=20.0), _lib.WT2_ERR_TAU),
     (dict(face_kind=(0, 3, 0, 0, 0, 0)), _lib.WT2_ERR_ARG),          # only 0 and 7 are face kinds
     (dict(face_kind=(7, 0, 0, 0, 0, 0), periodic_x=True), _lib.WT2_ERR_ARG),
-    (dict(face_kind=(0, 0, 0, 7, 0, 0), warp_A=0.05), _lib.WT2_ERR_UNSUPPORTED),
     (dict(face_kind=(7,) * 6, inv_R=-1.0), _lib.WT2_ERR_ARG),
 ])
 def test_ctx_create_rejects(kw, code):

@@ -214,3 +214,99 @@ def test_outgoing_plane_wave_leaves_at_size(prec):
     assert Es[-1] < 1e-8 * Es[0]
     assert np.all(np.diff(Es) <= (1e-12 if prec == "fp64" else 1e-8) * Es[0])
     assert abs(Ed[-1] / Ed[0] - 1) < 1e-4
+
+
+# ------------------------------------------------------- warped meshes
+
+CURVED = [
+    # N, M, warp A, faces, R
+    (3, 2, 0.05, (1, 2, 3, 4, 5, 6), math.inf),
+    (4, (2, 3, 2), 0.05, (2, 3, 6), 2.5),
+    (6, 2, 0.03, (1, 4), 1.0),
+    (8, (2, 2, 3), 0.05, (5, 6), math.inf),
+    (10, 2, 0.05, (1, 2, 3, 4, 5, 6), 3.0),
+]
+CIDS = [f"N{c[0]}-M{c[1]}-f{''.join(map(str, c[3]))}" for c in CURVED]
+
+
+def _cpair(N, M, A, faces, R, eps_ko=0.0, precision="fp64"):
+    Mt = M if isinstance(M, tuple) else (M, M, M)
+    mesh = wh.make_warped_uniform_hex(*Mt, 0.0, 1.0, A)
+    tau = wh.tau_effective(None, N)
+    ws = wh.Workspace(mesh, wh.make_element(N), tau=tau, eps_ko=eps_ko, precision=precision,
+                      sommerfeld_faces=faces, sommerfeld_R=R)
+    o = wo.Oracle(N, Mt, warp_A=A, tau=tau, eps_ko=eps_ko, sommerfeld_faces=faces, sommerfeld_R=R)
+    return ws, o
+
+
+@pytest.mark.parametrize("case", CURVED, ids=CIDS)
+def test_warped_rhs3d_matches_oracle(case):
+    """The curved operator's Sommerfeld faces (drag with the physical normal)
+    in the drop-in RHS, with u_t given and with u_t = NULL."""
+    N, M, A, faces, R = case
+    _say("warped sommerfeld rhs3d", case)
+    ws, o = _cpair(N, M, A, faces, R)
+    rng = np.random.default_rng(500 + N)
+    u = rng.standard_normal(o.n)
+    ud = rng.standard_normal(o.n)
+    out = np.zeros(o.n)
+    ws.rhs(out, u, ud, (0.0,) * 6)
+    assert relerr(out, o.lap(u, ud)) <= TOL64
+    ws.rhs(out, u, None, (0.0,) * 6)
+    assert relerr(out, o.lap(u)) <= TOL64
+
+
+@pytest.mark.parametrize("case", CURVED[1:4], ids=CIDS[1:4])
+def test_warped_rk4_steps_and_energy_match_oracle(case):
+    """5 RK4 steps with KO 0.1 and the energy at every step (FP64, 1e-12), and
+    the mixed path within the H-norm bar."""
+    N, M, A, faces, R = case
+    _say("warped sommerfeld rk4", case)
+    ws, o = _cpair(N, M, A, faces, R, eps_ko=0.1)
+    x = o.coords()
+    u0 = np.exp(-np.sum((x - 0.55) ** 2, axis=1) / 0.04)
+    v0 = 0.2 * np.sin(2 * math.pi * x[:, 0]) * np.cos(math.pi * x[:, 2])
+    dt = 0.2 * o.hmin
+    uo, vo, Eo = o.rk4(u0, v0, dt, 5, energy=True)
+    u = u0.copy(); v = v0.copy()
+    E = ws.rk4_steps(u, v, dt, 5, energy_every=1)
+    assert relerr(u, uo) <= TOL64 and relerr(v, vo) <= TOL64
+    assert np.max(np.abs(E - Eo)) <= TOL64 * np.max(np.abs(Eo))
+    assert np.all(np.diff(Eo) <= 1e-12 * Eo[0])
+    wm, _ = _cpair(N, M, A, faces, R, eps_ko=0.1, precision="mixed")
+    u = u0.copy(); v = v0.copy()
+    wm.rk4_steps(u, v, dt, 5)
+    H = o.hphys()
+    d = u - uo
+    assert math.sqrt(np.sum(H * d * d) / np.sum(H * uo * uo)) <= 1e-5
+
+
+def test_warped_sommerfeld_batch_equals_single_runs_and_background_refused():
+    import torch
+    _say("warped sommerfeld batch")
+    N, M, A, faces, R = CURVED[1]
+    runs = []
+    for rep in range(2):
+        ws, o = _cpair(N, M, A, faces, R, eps_ko=0.1)
+        x = o.coords()
+        u = torch.from_numpy(np.exp(-np.sum((x - 0.5) ** 2, axis=1) / 0.03)).cuda()
+        v = torch.from_numpy(0.1 * np.cos(math.pi * x[:, 1])).cuda()
+        runs.append((ws, u, v, 0.2 * o.hmin))
+    ws, u, v, dt = runs[0]
+    E = ws.rk4_steps(u, v, dt, 6, energy_every=2)
+    b = wh.Batch()
+    ws2, u2, v2, _ = runs[1]
+    s = b.add(ws2, u2, v2, dt, 6, energy_every=2)
+    fin = 0
+    while fin < 1:
+        fin += b.step(4)
+    torch.cuda.synchronize()
+    _, E2 = b.slot_info(s)
+    assert np.array_equal(u2.cpu().numpy(), u.cpu().numpy()) and np.array_equal(v2.cpu().numpy(), v.cpu().numpy())
+    assert np.array_equal(E2, E)
+    b.close()
+    mesh = wh.make_warped_uniform_hex(2, 2, 2, 0.0, 1.0, 0.05)
+    x = wh.host_coords(mesh, wh.make_element(4))
+    bg = wh.sample_background3d(wh.adm_bump_background(), 0.0, x[:, 0], x[:, 1], x[:, 2])
+    with pytest.raises(wh.WT2Error):
+        wh.Workspace(mesh, wh.make_element(4), bg=bg, sommerfeld_faces=(1,))

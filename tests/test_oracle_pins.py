@@ -466,11 +466,11 @@ def test_sommerfeld_surface_terms_closed_form():
     assert np.max(np.abs(oi.lap(one, np.zeros(oi.n)))) < 1e-10
     od = wo.Oracle(5, 2)
     assert np.max(np.abs(od.lap(one))) > 1.0
-    # Sommerfeld faces only on bounded axes of flat meshes
+    # Sommerfeld faces only on bounded axes, and not with a background
     with pytest.raises(ValueError):
         wo.Oracle(4, 2, periodic=(True, False, False), sommerfeld_faces=(2,))
     with pytest.raises(ValueError):
-        wo.Oracle(4, 2, warp_A=0.05, sommerfeld_faces=(1,))
+        wo.Oracle(4, 2, bg=_flat_bg(4 ** 3 * 8), sommerfeld_faces=(1,))
 
 
 def test_sommerfeld_operator_symmetric_nsd():
@@ -732,3 +732,72 @@ def test_fo_boundary_pass_terms():
     Dx = o.apply_D(0, Phi)
     assert np.allclose(so[1][onx] - base[1][onx], -(Pi[onx] + Dx[onx]) / (h * w[-1]), rtol=1e-12, atol=1e-10)
     assert np.array_equal(so[0], base[0])
+
+
+def _warped_face_weights(o, N, M, A, faces):
+    """Per node: sum over the Sommerfeld faces through it of w_p w_q |n_u|, with
+    n_u = J_p x J_q from the warp map x = x0 + L (xi + A s(xi)),
+    s = sin(2 pi xi_1) sin(2 pi xi_2) sin(2 pi xi_3), xi the global unit
+    coordinates (element e_a covers [e_a, e_a + 1] / M_a)."""
+    xi, w, _ = wo.gll(N)
+    out = np.zeros(o.n)
+    tp = 2 * math.pi
+    for e in range(M[0] * M[1] * M[2]):
+        ec = (e % M[0], (e // M[0]) % M[1], e // (M[0] * M[1]))
+        for f in faces:
+            a, side = (f - 1) // 2, (f - 1) % 2
+            if ec[a] != (M[a] - 1 if side else 0):
+                continue
+            r = N - 1 if side else 0
+            pax = [b for b in range(3) if b != a]
+            for q in range(N):
+                for pp in range(N):
+                    idx = [0, 0, 0]
+                    idx[a] = r; idx[pax[0]] = pp; idx[pax[1]] = q
+                    g = [(ec[b] + xi[idx[b]]) / M[b] for b in range(3)]
+                    sn = [math.sin(tp * g[b]) for b in range(3)]
+                    cs = [math.cos(tp * g[b]) for b in range(3)]
+                    ds = [tp * cs[0] * sn[1] * sn[2], tp * sn[0] * cs[1] * sn[2], tp * sn[0] * sn[1] * cs[2]]
+                    J = np.array([[((1.0 if i == b else 0.0) + A * ds[b]) / M[b] for b in range(3)] for i in range(3)])
+                    nu = np.linalg.norm(np.cross(J[:, pax[0]], J[:, pax[1]]))
+                    n = e * N ** 3 + idx[0] + N * (idx[1] + N * idx[2])
+                    out[n] += w[pp] * w[q] * nu
+    return out
+
+
+def test_sommerfeld_on_warped_meshes():
+    """Sommerfeld faces on a warped mesh (no background): the drag
+    -(wF/Hphys)(u_t + d_n u + u/R) of src/wave.jl:444-518 with the physical
+    normal.  (1) dE/dt = <v, L(u, v) - L(u, 0)>_W equals minus the sum of
+    w_p w_q |J_p x J_q| v^2 over the Sommerfeld face nodes, |J_p x J_q| taken from
+    the warp map itself; (2) with u_t = 0 the operator stays W-symmetric and
+    negative semidefinite; (3) as A -> 0 it tends to the flat Sommerfeld
+    operator linearly in A."""
+    N, M, A = 4, (2, 3, 2), 0.05
+    faces = (1, 2, 3, 4, 5, 6)
+    o = wo.Oracle(N, M, warp_A=A, sommerfeld_faces=faces, sommerfeld_R=1.5)
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(o.n); v = rng.standard_normal(o.n)
+    W = o.hphys()
+    rate = np.dot(W * v, o.lap(u, v) - o.lap(u))
+    ref = np.dot(_warped_face_weights(o, N, M, A, faces), v * v)
+    assert abs(rate + ref) < 1e-11 * ref
+    # a subset of faces and R = inf
+    sub = (2, 3, 5)
+    o2 = wo.Oracle(N, M, warp_A=A, sommerfeld_faces=sub)
+    rate2 = np.dot(o2.hphys() * v, o2.lap(u, v) - o2.lap(u))
+    assert abs(rate2 + np.dot(_warped_face_weights(o2, N, M, A, sub), v * v)) < 1e-11 * abs(rate2)
+    # (2)
+    o3 = wo.Oracle(3, (2, 2, 1), warp_A=A, sommerfeld_faces=(1, 4, 5), sommerfeld_R=2.0)
+    S = o3.hphys()[:, None] * _dense(o3, o3.lap, o3.n)
+    assert np.max(np.abs(S - S.T)) < 1e-10 * np.max(np.abs(S))
+    ev = np.linalg.eigvalsh(0.5 * (S + S.T))
+    assert ev.max() < 1e-10 * np.abs(ev).max()
+    # (3)
+    of = wo.Oracle(4, (3, 2, 2), sommerfeld_faces=(2, 3, 6), sommerfeld_R=2.0)
+    d = []
+    for eps in (1e-7, 1e-8):
+        oc = wo.Oracle(4, (3, 2, 2), warp_A=eps, sommerfeld_faces=(2, 3, 6), sommerfeld_R=2.0)
+        a = of.lap(u[:of.n], v[:of.n]); b = oc.lap(u[:of.n], v[:of.n])
+        d.append(np.max(np.abs(a - b)) / np.max(np.abs(a)))
+    assert d[0] < 1e-5 and 8 < d[0] / d[1] < 12

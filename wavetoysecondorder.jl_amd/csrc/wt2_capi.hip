@@ -692,10 +692,6 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
             set_error("wt2_ctx_create: a Sommerfeld face on a periodic axis");
             return WT2_ERR_ARG;
         }
-        if (desc->face_kind[f] == SOMMERFELD_TAG && desc->warp_A != 0.0) {
-            set_error("wt2_ctx_create: Sommerfeld faces are built for flat meshes only");
-            return WT2_ERR_UNSUPPORTED;
-        }
     }
     if (!(desc->sommerfeld_inv_R >= 0) || !std::isfinite(desc->sommerfeld_inv_R)) {
         set_error("wt2_ctx_create: sommerfeld_inv_R must be finite and >= 0");
@@ -908,7 +904,7 @@ int wt2_ctx_set_background(wt2_ctx* c, const double* bg10)
     if (!c || !bg10) { set_error("wt2_ctx_set_background: null pointer"); return WT2_ERR_ARG; }
     for (int a = 0; a < 3; ++a)
         if (som_axis(c, a)) {
-            set_error("wt2_ctx_set_background: Sommerfeld faces are built for flat meshes only");
+            set_error("wt2_ctx_set_background: Sommerfeld faces are built for meshes without a background");
             return WT2_ERR_UNSUPPORTED;
         }
     if (c->gvalid) { hipGraphExecDestroy(c->gexec); c->gvalid = false; }

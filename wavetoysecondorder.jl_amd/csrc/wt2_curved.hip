@@ -202,6 +202,9 @@ int curved_init(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const 
     cd.Ne = int64_t(d.M[0]) * d.M[1] * d.M[2];
     cd.npts = cd.Ne * d.N * d.N * d.N;
     cd.tau = d.tau;
+    for (int f = 0; f < 6; ++f)
+        if (d.face_kind[f] == SOMMERFELD_TAG && !d.periodic[f / 2]) cd.som |= 1 << f;
+    cd.inv_R = d.sommerfeld_inv_R;
     for (int i = 0; i < d.N; ++i) cd.w[i] = w[i];
     for (int i = 0; i < d.N * d.N; ++i) cd.G[i] = G[i];
     const size_t n = size_t(cd.npts);
@@ -251,6 +254,8 @@ struct CConst {
     TC winv[N];
     TC tau;
     TC bv[6];
+    TC iR;      // 1/R of the Sommerfeld faces
+    int som;    // bit f: outer face f is a Sommerfeld face
 };
 
 template <int N, typename TC>
@@ -475,6 +480,19 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
             const int n = fidx<N>(ax, r, p, qq);
             const int64_t gn = eb + n;
             const int64_t en = nbr_elem(e, ax, side, a.M0, a.M1, a.M2, a.per0, a.per1, a.per2);
+            if (en < 0 && ((a.k.som >> f) & 1)) {
+                // Sommerfeld face (src/wave.jl:444-518): no SIPG terms; the drag
+                // -(wF/Hphys)(u_t + d_n u + u/R), here unweighted (the output
+                // is multiplied by wgt = 1/|J|): |n_u| d_n u = n F^a with this
+                // element's own normal flux, |n_u| = sqrt(K^aa / wgt), and
+                // wF/Hphys = |n_u| / (w_r |J|).  B holds u_t (zero without it).
+#pragma unroll
+                for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = TC(0);
+                const TC nu = sqrt(kab<N, TC>(a.coef, np, gn, ax, ax) / a.coef[9 * np + gn]);
+                const TC drag = fma(nrm, a.fbuf[e * S::F2 + q], nu * fma(a.k.iR, A[n], B[n]));
+                Pt[q] = -drag * T.winv[r];
+                continue;
+            }
             TC du, dF = TC(0), kap = a.coef[(10 + ax) * np + gn], dv = TC(0), alpha;
             if (en >= 0) {
                 const int64_t gnn = en * N3 + fidx<N>(ax, rn, p, qq);
@@ -608,6 +626,8 @@ static void make_cargs(const CurvedData& cd, const CurvedCall& cc, CArgs<N, TC>&
     for (int i = 0; i < N; ++i) { a.k.w[i] = TC(cd.w[i]); a.k.winv[i] = TC(1.0 / cd.w[i]); }
     a.k.tau = TC(cd.tau);
     for (int f = 0; f < 6; ++f) a.k.bv[f] = TC(cc.bv[f]);
+    a.k.iR = TC(cd.inv_R);
+    a.k.som = cd.som;
     a.u = cc.u; a.v = cc.v;
     a.coef = reinterpret_cast<const TC*>(std::is_same<TC, float>::value ? static_cast<const void*>(cd.coef32)
                                                                           : static_cast<const void*>(cd.coef64));

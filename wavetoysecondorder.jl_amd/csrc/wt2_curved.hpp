@@ -18,6 +18,8 @@ struct CurvedData {
     int periodic[3] = {0, 0, 0};
     int64_t Ne = 0, npts = 0;
     double tau = 0.0;
+    int som = 0;                // bit f: outer face f (0..5) is a Sommerfeld face
+    double inv_R = 0.0;         // 1/R of the Sommerfeld faces' u/R term
     double w[16] = {0};
     double G[256] = {0};
     float G32[256] = {0};       // FP32 G, the same matrix as the FP32 even/odd table

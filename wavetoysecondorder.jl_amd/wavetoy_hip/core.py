@@ -433,11 +433,16 @@ def rhs_wave3d(udd, u, ud, params, geom=None, ops=None, work=None):
     """rhs_wave3d!(udd, u, ud, params; geom, ops, work) (src/wave.jl:573-611):
     udd = L_SIPG u with Dirichlet values params.bdry_values[tag-1].  work is a
     Workspace (its tau is the context's; params.tau must agree).  Tag-7
-    Sommerfeld faces do not exist on these meshes."""
+    Sommerfeld faces are the workspace's (Workspace(sommerfeld_faces=...,
+    sommerfeld_R=...)); params.sommerfeld_R must agree with its R."""
     if work is None:
         raise ValueError("rhs_wave3d needs work= (a Workspace from make_workspace)")
     if params.tau is not None and abs(params.tau - work.tau) > 1e-12 * work.tau:
         raise ValueError(f"params.tau={params.tau} differs from the workspace tau={work.tau}")
+    inv_R = work.desc.sommerfeld_inv_R
+    pR = 0.0 if math.isinf(params.sommerfeld_R) else 1.0 / params.sommerfeld_R
+    if any(work.desc.face_kind[f] == SOMMERFELD_BDRY_TAG for f in range(6)) and abs(pR - inv_R) > 1e-12 * max(inv_R, 1.0):
+        raise ValueError(f"params.sommerfeld_R={params.sommerfeld_R} differs from the workspace's Sommerfeld R")
     return work.rhs(udd, u, ud, params.bdry_values)
 
 
