@@ -41,11 +41,11 @@ def evolve3d(formulation="strong", **kwargs):
     return _evolve3d_strong(**kwargs)
 
 
-def _evolve3d_strong(N=8, M=4, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="cartesian",
-                     ic_wavenumber=math.pi, ic_centre=(0.5, 0.5, 0.5), ic_width=None, ic_denom=None,
+def _evolve3d_strong(N=5, M=8, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="cartesian",
+                     ic_wavenumber=3 * math.pi, ic_centre=(0.5, 0.5, 0.5), ic_width=None, ic_denom=None,
                      ic_radial_mode=1, ic_radius=None, ic_pulse_offset=None, ic_pulse_width=None,
-                     outer_bc="dirichlet", tau=None, eps_ko=0.1, t0=0.0, t1=None, nsteps=None, dt=None,
-                     cfl=0.2, Nt=2, precision="fp64", energy_every=None, device=True, slice_y=None,
+                     outer_bc="dirichlet", tau=None, eps_ko=0.0, t0=0.0, t1=1.0, nsteps=None, dt=None,
+                     cfl=0.2, Nt=200, precision="fp64", energy_every=None, device=True, slice_y=None,
                      slice_z=None):
     """The (u, u_t) driver (_evolve3d_strong, src/evolve.jl:877-1073) on the
     cube mesh (warp_A = 0) or the warped cube.  ic: "cartesian" (standing
@@ -54,7 +54,12 @@ def _evolve3d_strong(N=8, M=4, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="
     offset ic_pulse_offset = L/4 and width ic_pulse_width = offset/5 about the
     cube centre) -- these three carry the exact solution for l2_err -- or
     "gaussian".  Outer faces are Dirichlet (the reference takes
-    outer_bc = "sommerfeld" only on its spherical meshes)."""
+    outer_bc = "sommerfeld" only on its spherical meshes).  The defaults are
+    the reference's (N = 5, M = 8, wavenumber 3 pi, t1 = 1, Nt = 200 samples,
+    no KO: rhs_wave3d! has none); the time stepper is classical RK4 with dt =
+    cfl h_min (the reference picks a symplectic method by N, src/evolve.jl:27-36,
+    and its dt from a spectral-radius estimate); nsteps, when given, overrides
+    t1."""
     if outer_bc != "dirichlet":
         raise ValueError(f"evolve3d: outer_bc = {outer_bc!r} is only supported on the spherical meshes "
                          "(inflated cube, radial shell); the cube takes 'dirichlet'")
