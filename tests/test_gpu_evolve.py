@@ -16,7 +16,8 @@ def test_plain_cpu_configuration_full_run():
     """4^3 elements x 8^3 nodes (32,768 points), Dirichlet, tau_eff, KO eps 0.1,
     u0 = sin(pi x) sin(pi y) sin(pi z), u_t = 0, RK4 dt = 0.2 h_min, 200 steps."""
     print("plain CPU configuration, 200 steps", flush=True)
-    r = wh.evolve3d(N=8, M=4, ic="cartesian", tau=20.0, eps_ko=0.1, cfl=0.2, nsteps=200, Nt=5,
+    r = wh.evolve3d(N=8, M=4, ic="cartesian", ic_wavenumber=math.pi, tau=20.0, eps_ko=0.1, cfl=0.2,
+                    nsteps=200, Nt=5,
                     device=True)
     assert r["tau"] == 73.5
     assert abs(r["dt"] - 0.2 * r["dx"]) < 1e-15
@@ -52,7 +53,7 @@ def test_convergence_order_against_exact_solution():
     reference's convergence criterion (test/evolve3d_tests.jl:14-21)."""
     errs = []
     for M in (2, 4, 8):
-        r = wh.evolve3d(N=4, M=M, ic="cartesian", eps_ko=0.0, t1=0.25, cfl=0.1, Nt=2)
+        r = wh.evolve3d(N=4, M=M, ic="cartesian", ic_wavenumber=math.pi, eps_ko=0.0, t1=0.25, cfl=0.1, Nt=2)
         errs.append(r["l2_err"][-1])
     print("l2 errors", errs, flush=True)
     assert errs[0] / errs[1] > 4 and errs[1] / errs[2] > 4
@@ -159,3 +160,16 @@ def test_conservative_driver_matches_oracle_rk4():
     Pr, Qr = o.fo_rk4(Phi0, np.zeros(o.n), coef, dt, 4, eps_ko=0.1)
     assert np.max(np.abs(r["Phi_final"] - Pr)) <= 1e-12 * np.max(np.abs(Pr))
     assert np.max(np.abs(r["Pi_final"] - Qr)) <= 1e-12 * np.max(np.abs(Qr))
+
+
+def test_strong_driver_defaults_run():
+    """evolve3d() with the reference's defaults (N = 5, M = 8, standing wave
+    3 pi, t1 = 1, 200 samples, no KO): runs to t1 and tracks the exact
+    solution (error well below the mode's amplitude)."""
+    print("evolve3d defaults", flush=True)
+    r = wh.evolve3d()
+    assert r["work"].N == 5 and r["work"].mesh.M == (8, 8, 8)
+    assert abs(r["ts"][-1] - 1.0) < 1e-12 and len(r["ts"]) == 200
+    assert r["l2_err"][0] == 0.0 and np.all(np.isfinite(r["l2_err"]))
+    print("max l2 error", float(np.max(r["l2_err"])), flush=True)
+    assert np.max(r["l2_err"]) < 0.05
