@@ -575,8 +575,8 @@ def main():
         print(json.dumps({"calibrate_copy_bytes_read": src.numel() * 8, "bytes_written": dst.numel() * 8}))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
-              f"the multi-GPU sharding is described in DESIGN.md, not run.", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for {args.gpus}); instances or element slabs "
+              f"would be sharded one per GPU, which this benchmark does not run.", file=sys.stderr)
         sys.exit(2)
 
     import torch

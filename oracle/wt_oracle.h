@@ -2,8 +2,9 @@
  * wt_oracle.h -- CPU FP64 oracle for the 3D SBP-SAT wave RHS and RK4 step.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the checker
- * (or the timed CPU baseline).  The product path (libwt2.so) never calls it.
+ * cpu_baseline leg of bench.py may load this library, and only to check results
+ * against (or as the timed CPU baseline).  The product path (libwt2.so) never
+ * calls it.
  *
  * It restates, element by element, the reference's hot path:
  *   - GLL element operators (HexSBPSAT make_element/make_operators, used at

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU FP64 oracle (libwt_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product library never loads
+cpu_baseline leg of bench.py, to check results against.  The product library never loads
 it.  The C source (wt_oracle.c) cites the reference file:line each function
 restates.
 """

@@ -1,7 +1,7 @@
 """GPU parity of the flat (affine) path through the C ABI against the CPU oracle.
 
-FP64 bar: max |gpu - oracle| <= 1e-12 * max |oracle| (the RHS tolerance the
-north star states); mixed precision (FP32 RHS bulk, FP64 state): relative
+FP64 bar: max |gpu - oracle| <= 1e-12 * max |oracle| (the stated RHS
+tolerance); mixed precision (FP32 RHS bulk, FP64 state): relative
 RHS error <= 2e-5 and RK4 H-norm error <= 1e-5.
 """
 import math
