@@ -173,3 +173,15 @@ def test_strong_driver_defaults_run():
     assert r["l2_err"][0] == 0.0 and np.all(np.isfinite(r["l2_err"]))
     print("max l2 error", float(np.max(r["l2_err"])), flush=True)
     assert np.max(r["l2_err"]) < 0.05
+
+
+def test_unstable_run_is_reported_not_aborted():
+    """A time step far beyond the stability limit blows the state up; the driver
+    raises FloatingPointError at the first sample with non-finite values (the
+    reference asserts at every sample, src/evolve.jl:1039) and the library
+    keeps working."""
+    print("unstable run", flush=True)
+    with pytest.raises(FloatingPointError):
+        wh.evolve3d(N=4, M=2, ic="cartesian", ic_wavenumber=math.pi, dt=0.5, nsteps=400, Nt=5)
+    r = wh.evolve3d(N=4, M=2, ic="cartesian", ic_wavenumber=math.pi, t1=0.1, Nt=2)
+    assert np.all(np.isfinite(r["u_final"]))

@@ -138,8 +138,11 @@ def _evolve3d_strong(N=5, M=8, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="
     def sample(step):
         t = t0 + step * dt
         ts.append(t)
-        energies.append(work.energy(u, v))
         uh, vh = fetch(u), fetch(v)
+        # every sample, as the reference's driver asserts (src/evolve.jl:1039)
+        if not (np.all(np.isfinite(uh)) and np.all(np.isfinite(vh))):
+            raise FloatingPointError(f"evolve3d: non-finite state at t = {t} (step {step})")
+        energies.append(work.energy(u, v))
         us.append(uh[slice_idx])
         uds.append(vh[slice_idx])
         if exact is not None and H is not None:
