@@ -183,3 +183,26 @@ def test_fo_rejects_curved_contexts_and_bad_boundary_kinds():
         ws2.fo_rhs(np.zeros(n), np.zeros(n), z, z, c, 0.1, wh.make_bc3d([0, 1, 1, 1, 1, 1]))
     with pytest.raises(wh.WT2Error):
         ws2.fo_rhs(np.zeros(n), np.zeros(n), z, z, c, -1.0)
+
+
+def test_fused_and_separate_pointwise_step_agree_bitwise(monkeypatch):
+    """The z pass doing the pointwise flux step (default) and the same step as a
+    kernel of its own (WT2_FO_FUSE=0) give the same bits, with every boundary
+    kind and KO."""
+    import torch
+    _say("fo fused vs separate")
+    N, M, per, eps, kinds = CASES[5]
+    ws, o = _pair(N, M, per)
+    x = o.coords()
+    coef = ws.fo_coef(torch.from_numpy(_bg(x, 3).reshape(-1).copy()).cuda())
+    rng = np.random.default_rng(9)
+    t = lambda a: torch.from_numpy(a).cuda()
+    Phi = t(rng.standard_normal(o.n)); Pi = t(rng.standard_normal(o.n))
+    bc = wh.make_bc3d(kinds, sigma=0.8, gPhi=t(rng.standard_normal(o.n)), gPi=t(rng.standard_normal(o.n)))
+    out = []
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("WT2_FO_FUSE", fuse)
+        dP = torch.empty_like(Phi); dQ = torch.empty_like(Phi)
+        ws.fo_rhs(dP, dQ, Phi, Pi, coef, eps, bc)
+        out.append((dP.cpu().numpy(), dQ.cpu().numpy()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])

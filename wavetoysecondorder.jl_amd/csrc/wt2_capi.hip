@@ -1228,6 +1228,9 @@ void fo_desc(const wt2_ctx* c, FoDesc& fd)
     for (int i = 0; i < c->N; ++i) fd.w[i] = c->w[i];
     fd.op64 = c->op64;
     fd.D = c->fo_D;
+    // WT2_FO_FUSE=0: the pointwise step as a kernel of its own (same results)
+    const char* ev = std::getenv("WT2_FO_FUSE");
+    fd.fuse = ev ? (std::atoi(ev) != 0) : 1;
 }
 
 // Device copies of host arrays for a host-pointer call (freed by the caller).

@@ -5,11 +5,12 @@
 // eps mu^-5 D_d^6 on both fields (_ko_axis3!, :182-192), and the affine
 // outer-boundary pass (_bc3d_affine_kernel!, src/boundaries3d.jl:76-126).
 //
-// Seven launches per call.  Pencil pass (axis d, phase 0): DPhi_d = D_d Phi,
-// kept in a context buffer, and the KO terms of both fields along d.  A
-// pointwise kernel then forms the fluxes F^i over that buffer, adds
-// b . DPhi + (alpha/sqrtg) Pi to dPhi and the boundary terms.  Pencil pass
-// (axis d, phase 1): dPi += D_d F^d.  A pass workgroup owns whole element
+// Six launches per call.  Pencil pass (axis d, phase 0): DPhi_d = D_d Phi,
+// kept in a context buffer, and the KO terms of both fields along d; the last
+// of them (z) then does the pointwise step on its own nodes: the fluxes F^i
+// over that buffer (DPhi_3 still in registers), b . DPhi + (alpha/sqrtg) Pi
+// into dPhi and the boundary terms.  Pencil pass (axis d, phase 1):
+// dPi += D_d F^d.  A pass workgroup owns whole element
 // lines along its axis (LT lines x Ma elements, work-item = (element, line)
 // with adjacent lines in adjacent lanes), and the face values of neighbouring
 // elements go through LDS, as in the second-order pencil passes.
@@ -17,6 +18,7 @@
 #include "wt2_ops.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace wt2 {
 
@@ -35,6 +37,7 @@ struct FoArgs {
     int M0, M1, M2;
     int per0, per1, per2;
     int axis, first, anybc;
+    int fuse;                // z pass of phase 0 also does the pointwise flux step
     int Ma, T0, slo, shi, estride, LT, nbk;
     double ih, sat0, satN;   // 1/h; SAT -1/(2 w_0 h), +1/(2 w_{N-1} h) along the axis
     double koT;              // eps mu^-5 (0: no KO)
@@ -112,6 +115,56 @@ __device__ __forceinline__ void fo_store(double* __restrict__ p, bool on, const 
     }
 }
 
+// The pointwise step at node q (element e, node (i, j, k)): F^i = b^i Pi +
+// alpha sqrtg g^{ij} DPhi_j, dF += b . DPhi + (alpha / sqrtg) Pi, and the
+// affine outer-boundary terms (src/boundaries3d.jl:76-126) into dF and dP.
+template <int N>
+__device__ __forceinline__ void fo_point(const FoArgs& a, int64_t q, int64_t e, int i, int j, int k,
+                                         double D1, double D2, double D3, double Pq,
+                                         double& dF, double& dP, double& F1, double& F2, double& F3)
+{
+    const int64_t n = a.npts;
+    const double* __restrict__ c = a.coef;
+    const double al = c[q], sg = c[n + q];
+    const double b1 = c[2 * n + q], b2 = c[3 * n + q], b3 = c[4 * n + q];
+    const double g11 = c[5 * n + q], g12 = c[6 * n + q], g13 = c[7 * n + q];
+    const double g22 = c[8 * n + q], g23 = c[9 * n + q], g33 = c[10 * n + q];
+    const double as = al * sg;
+    F1 = b1 * Pq + as * (g11 * D1 + g12 * D2 + g13 * D3);
+    F2 = b2 * Pq + as * (g12 * D1 + g22 * D2 + g23 * D3);
+    F3 = b3 * Pq + as * (g13 * D1 + g23 * D2 + g33 * D3);
+    dF = dF + (b1 * D1 + b2 * D2 + b3 * D3 + (al / sg) * Pq);
+    if (!a.anybc) return;
+    const int nd[3] = {i, j, k};
+    const int ec[3] = {int(e % a.M0), int((e / a.M0) % a.M1), int(e / (int64_t(a.M0) * a.M1))};
+    const int Mv[3] = {a.M0, a.M1, a.M2};
+    const int pv[3] = {a.per0, a.per1, a.per2};
+    const double Dv[3] = {D1, D2, D3};
+    const double bv[3] = {b1, b2, b3};
+    const double gnn[3] = {g11, g22, g33};
+    for (int f = 0; f < 6; ++f) {
+        const int dn = f >> 1, hi = f & 1;
+        if (pv[dn] || ec[dn] != (hi ? Mv[dn] - 1 : 0) || nd[dn] != (hi ? N - 1 : 0)) continue;
+        const int kind = a.kinds[f];
+        if (kind == WT2_BC_EXCISION) continue;
+        const double nh = hi ? 1.0 : -1.0;
+        const double a_n = al * sqrt(gnn[dn]);
+        const double bax = bv[dn];
+        const double bn = nh * bax;
+        const double wt = a.bw[dn][hi];
+        if (kind == WT2_BC_FULL_DIRICHLET) {
+            const double tau = a.sigma * (fabs(a_n - bax) + fabs(a_n + bax)) * wt;
+            const double gF = a.gPhi ? a.gPhi[q] : 0.0, gP = a.gPi ? a.gPi[q] : 0.0;
+            dF += -tau * (a.Phi[q] - gF);
+            dP += -tau * (Pq - gP);
+        } else {
+            const double rr = Pq + ((bn + a_n) / (al / sg)) * (nh * Dv[dn]);
+            const double gv = (kind == WT2_BC_SOMMERFELD || !a.gPhi) ? 0.0 : a.gPhi[q];
+            dP += -a.sigma * fabs(a_n + bn) * wt * (rr - gv);
+        }
+    }
+}
+
 // One pencil pass along axis AX.  PHASE 0: D[AX] = D_AX Phi and the KO terms
 // of both fields along AX into dPhi, dPi (the first pass writes them).
 // PHASE 1: dPi += D_AX F^AX, F^AX held in D[AX] (written by fo_flux_kernel).
@@ -149,6 +202,46 @@ __global__ void __launch_bounds__(FO_MAXT) fo_pass_kernel(FoArgs a)
     const int64_t g0 = int64_t(e) * N3 + (ll / N) * LA + (ll % N) * LB;
     const int64_t n = a.npts;
 
+    if constexpr (PHASE == 0 && AX == 2) {
+        if (a.fuse) {
+            // DPhi_3 stays in registers; KO sums are kept for the pointwise step
+            double x[N], d3[N], pi[N], acc[2][N];
+            fo_load<N, GS>(a.Phi + g0, active, x);
+            fo_D<N>(op, a, xch, hasL, hasR, x, d3);
+            fo_load<N, GS>(a.Pi + g0, active, pi);
+            for (int fld = 0; fld < 2; ++fld) {
+                double* out = (fld == 0 ? a.dPhi : a.dPi) + g0;
+                fo_load<N, GS>(out, active, acc[fld]);
+                if (a.koT != 0.0) {
+                    double y[N];
+                    if (fld == 1) {
+#pragma unroll
+                        for (int r = 0; r < N; ++r) x[r] = pi[r];
+                    }
+#pragma unroll 1
+                    for (int rep = 0; rep < 3; ++rep) {
+                        fo_D<N>(op, a, xch, hasL, hasR, x, y);
+                        fo_D<N>(op, a, xch, hasL, hasR, y, x);
+                    }
+#pragma unroll
+                    for (int r = 0; r < N; ++r) acc[fld][r] = fma(a.koT, x[r], acc[fld][r]);
+                }
+            }
+            if (!active) return;
+            const int i = ll % N, j = ll / N;
+#pragma unroll
+            for (int r = 0; r < N; ++r) {
+                const int64_t q = g0 + int64_t(r) * GS;
+                double dP = 0.0, F1, F2, F3;
+                fo_point<N>(a, q, e, i, j, r, a.D[q], a.D[n + q], d3[r], pi[r], acc[0][r], dP, F1, F2, F3);
+                a.D[q] = F1; a.D[n + q] = F2; a.D[2 * n + q] = F3;
+                if (dP != 0.0) acc[1][r] += dP;
+            }
+            fo_store<N, GS>(a.dPhi + g0, true, acc[0]);
+            fo_store<N, GS>(a.dPi + g0, true, acc[1]);
+            return;
+        }
+    }
     if constexpr (PHASE == 0) {
         double x[N], y[N];
         fo_load<N, GS>(a.Phi + g0, active, x);
@@ -191,61 +284,21 @@ __global__ void __launch_bounds__(FO_MAXT) fo_pass_kernel(FoArgs a)
     }
 }
 
-// Pointwise, between the two phases: F^i = b^i Pi + alpha sqrtg g^{ij} DPhi_j
-// written over D (each work-item reads and writes only its own node),
-// dPhi += b . DPhi + (alpha / sqrtg) Pi, and the affine outer-boundary terms
-// (src/boundaries3d.jl:76-126) added to dPhi and dPi.
+// The pointwise step as a kernel of its own, between the two phases (the
+// unfused schedule, WT2_FO_FUSE=0): each work-item reads and writes only its
+// own node.
 template <int N>
 __global__ void __launch_bounds__(256) fo_flux_kernel(FoArgs a)
 {
     constexpr int N3 = N * N * N;
     const int64_t n = a.npts;
-    const double* __restrict__ c = a.coef;
     for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
-        const double al = c[q], sg = c[n + q];
-        const double b1 = c[2 * n + q], b2 = c[3 * n + q], b3 = c[4 * n + q];
-        const double g11 = c[5 * n + q], g12 = c[6 * n + q], g13 = c[7 * n + q];
-        const double g22 = c[8 * n + q], g23 = c[9 * n + q], g33 = c[10 * n + q];
-        const double D1 = a.D[q], D2 = a.D[n + q], D3 = a.D[2 * n + q];
-        const double Pq = a.Pi[q];
-        const double as = al * sg;
-        a.D[q] = b1 * Pq + as * (g11 * D1 + g12 * D2 + g13 * D3);
-        a.D[n + q] = b2 * Pq + as * (g12 * D1 + g22 * D2 + g23 * D3);
-        a.D[2 * n + q] = b3 * Pq + as * (g13 * D1 + g23 * D2 + g33 * D3);
-        double dF = a.dPhi[q] + (b1 * D1 + b2 * D2 + b3 * D3 + (al / sg) * Pq);
-        double dP = 0.0;
-        if (a.anybc) {
-            const int64_t e = q / N3;
-            const int r = int(q - e * N3);
-            const int nd[3] = {r % N, (r / N) % N, r / (N * N)};
-            const int ec[3] = {int(e % a.M0), int((e / a.M0) % a.M1), int(e / (int64_t(a.M0) * a.M1))};
-            const int Mv[3] = {a.M0, a.M1, a.M2};
-            const int pv[3] = {a.per0, a.per1, a.per2};
-            const double Dv[3] = {D1, D2, D3};
-            const double bv[3] = {b1, b2, b3};
-            const double gnn[3] = {g11, g22, g33};
-            for (int f = 0; f < 6; ++f) {
-                const int dn = f >> 1, hi = f & 1;
-                if (pv[dn] || ec[dn] != (hi ? Mv[dn] - 1 : 0) || nd[dn] != (hi ? N - 1 : 0)) continue;
-                const int kind = a.kinds[f];
-                if (kind == WT2_BC_EXCISION) continue;
-                const double nh = hi ? 1.0 : -1.0;
-                const double a_n = al * sqrt(gnn[dn]);
-                const double bax = bv[dn];
-                const double bn = nh * bax;
-                const double wt = a.bw[dn][hi];
-                if (kind == WT2_BC_FULL_DIRICHLET) {
-                    const double tau = a.sigma * (fabs(a_n - bax) + fabs(a_n + bax)) * wt;
-                    const double gF = a.gPhi ? a.gPhi[q] : 0.0, gP = a.gPi ? a.gPi[q] : 0.0;
-                    dF += -tau * (a.Phi[q] - gF);
-                    dP += -tau * (Pq - gP);
-                } else {
-                    const double rr = Pq + ((bn + a_n) / (al / sg)) * (nh * Dv[dn]);
-                    const double gv = (kind == WT2_BC_SOMMERFELD || !a.gPhi) ? 0.0 : a.gPhi[q];
-                    dP += -a.sigma * fabs(a_n + bn) * wt * (rr - gv);
-                }
-            }
-        }
+        const int64_t e = q / N3;
+        const int r = int(q - e * N3);
+        double dF = a.dPhi[q], dP = 0.0, F1, F2, F3;
+        fo_point<N>(a, q, e, r % N, (r / N) % N, r / (N * N), a.D[q], a.D[n + q], a.D[2 * n + q], a.Pi[q],
+                    dF, dP, F1, F2, F3);
+        a.D[q] = F1; a.D[n + q] = F2; a.D[2 * n + q] = F3;
         a.dPhi[q] = dF;
         if (dP != 0.0) a.dPi[q] += dP;
     }
@@ -300,6 +353,7 @@ hipError_t launch_fo_n(const FoDesc& fd, hipStream_t s)
         a.satN = 1.0 / (2.0 * fd.w[N - 1] * fd.h[d]);
     };
     hipError_t e;
+    a.fuse = 0;
     a.first = !fd.grad_only;
     axis_consts(0);
     if ((e = fo_launch_pass<N, 0, 0>(a, s)) != hipSuccess) return e;
@@ -307,9 +361,10 @@ hipError_t launch_fo_n(const FoDesc& fd, hipStream_t s)
     axis_consts(1);
     if ((e = fo_launch_pass<N, 1, 0>(a, s)) != hipSuccess) return e;
     axis_consts(2);
+    a.fuse = fd.grad_only ? 0 : fd.fuse;   // the z pass does the pointwise step
     if ((e = fo_launch_pass<N, 2, 0>(a, s)) != hipSuccess) return e;
     if (fd.grad_only) return hipSuccess;
-    {
+    if (!a.fuse) {
         const int64_t blocks = std::min<int64_t>((a.npts + 255) / 256, 8192);
         (void)hipGetLastError();
         hipLaunchKernelGGL(fo_flux_kernel<N>, dim3(unsigned(std::max<int64_t>(blocks, 1))), dim3(256), 0, s, a);

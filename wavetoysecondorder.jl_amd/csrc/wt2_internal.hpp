@@ -93,7 +93,7 @@ struct Lap1Desc {
 hipError_t launch_flat_lap1(const Lap1Desc& d, hipStream_t s);
 
 // The first-order (Phi, Pi) system (wt2_fo.hip).  launch_fo_rhs: the six
-// passes of one right-hand side (with grad_only: the three that fill D);
+// launches of one right-hand side (seven with fuse = 0) (with grad_only: the three that fill D);
 // launch_fo_energy: the energy of (Phi, Pi) from D; launch_fo_coef: the
 // coefficient fields from bg10.
 struct FoDesc {
@@ -111,6 +111,7 @@ struct FoDesc {
     double sigma;
     double* partial; double* Eout;
     int grad_only;               // phase-0 passes only, D alone (for the energy)
+    int fuse;                    // the z pass of phase 0 does the pointwise step
 };
 hipError_t launch_fo_rhs(const FoDesc& d, hipStream_t s);
 hipError_t launch_fo_energy(const FoDesc& d, hipStream_t s);
