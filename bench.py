@@ -206,9 +206,20 @@ def curved_run(wh, torch, args):
             done += k
         wall = time.perf_counter() - t0
         E1 = ws.energy(u, v)
+        # the drop-in RHS alone (wt2_rhs3d's two kernels: face fluxes, volume)
+        ea = torch.cuda.Event(enable_timing=True)
+        eb = torch.cuda.Event(enable_timing=True)
+        for _ in range(3):
+            ws.launch_pass(3, 1, u, v, dt)
+        ea.record(stream)
+        for _ in range(20):
+            ws.launch_pass(3, 1, u, v, dt)
+        eb.record(stream)
+        torch.cuda.synchronize()
+        rhs_ms = ea.elapsed_time(eb) / 20
         res[prec] = dict(ms_per_step=wall * 1e3 / steps, steps=steps, dt=dt,
                          value=ws.npoints * steps / wall, energy0=E0, energy1=E1,
-                         u=u.cpu().numpy(), h_min=ws.h_min)
+                         u=u.cpu().numpy(), h_min=ws.h_min, rhs_ms=rhs_ms)
         ws.set_stream(None)
         ws.close()
     o = wh.Workspace(mesh, elem, tau=tau, bg=bg)
@@ -227,6 +238,13 @@ def curved_run(wh, torch, args):
         "mixed_rel_err_weighted_l2_vs_fp64": math.sqrt(hn_d / hn_u),
         "mixed_max_rel_err_vs_fp64": float(np.max(np.abs(d)) / np.max(np.abs(res["fp64"]["u"]))),
         "t_final": res["fp64"]["steps"] * res["fp64"]["dt"],
+        # drop-in RHS: u, u_t in, u_tt out (24 B) + 13 static coefficients per
+        # node (104 B FP64, 52 B FP32)
+        "rhs_roofline": {prec: {"ms": res[prec]["rhs_ms"], "algorithmic_bytes_per_point": bpp,
+                                "achieved": bpp * o.npoints / (res[prec]["rhs_ms"] * 1e-3) / 1e9,
+                                "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                "frac": bpp * o.npoints / (res[prec]["rhs_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS}
+                         for prec, bpp in (("fp64", 128), ("mixed", 76))},
     }
     return out
 
