@@ -912,13 +912,17 @@ hipError_t launch_flat_pass(const PassDesc& p, hipStream_t s)
 // ------------------------------------------------- single-pass Laplacian
 
 // Lines of one element: N^2 per axis, one work-item each.  A workgroup holds
-// EW elements (about 256 work-items).  LDS per element: the element's u and
-// the running sum, both with x-lines padded to N+1 so that the N^2 lanes
-// reading x-lines fall on distinct banks.
+// EW elements: about 256 work-items for N <= 7 (128 for N = 5), one element
+// for N >= 8 (a single wavefront at N = 8, so its barriers wait on nobody
+// else) -- the fastest of 64/128/256-work-item targets measured per N at
+// about 16M points (N = 8: 117 against 127 us).  LDS per element: the
+// element's u and the running sum, both with x-lines padded to N+1 so that
+// the N^2 lanes reading x-lines fall on distinct banks.
 template <int N>
 struct Lap1Shape {
     static constexpr int L = N * N;
-    static constexpr int EW = (256 / L) > 0 ? 256 / L : 1;
+    static constexpr int TGT = N >= 8 ? 64 : (N == 5 ? 128 : 256);
+    static constexpr int EW = (TGT / L) > 0 ? TGT / L : 1;
     static constexpr int NT = EW * L;
     static constexpr int RS = N + 1;              // padded x-line stride
     static constexpr int ES = RS * N * N;         // per-element LDS stride
