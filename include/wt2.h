@@ -97,6 +97,26 @@ int wt2_gll(int32_t N, double* xi, double* w, double* G);
  * (src/wave3d_curved.jl:158-178).  Fails with WT2_ERR_TAU when the 1D SIPG
  * operator is indefinite at desc->tau unless WT2_ALLOW_NONCOERCIVE is set. */
 int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out);
+/* Create a context on a mesh given by node coordinates and face connectivity
+ * (multi-patch meshes such as the cubed cube or inflated cube, whose patches
+ * meet with orientation != 0).  Replaces make_geometry + make_workspace +
+ * to_device of such a mesh (src/evolve.jl:916-936) and the connectivity reads
+ * of rhs_wave3d! (src/wave.jl:471-473; src/wave_strong_rhs.jl:41-73,
+ * 316-324).  Arrays are the reference's MeshGeometry/MeshConnectivity fields,
+ * column-major: coords (3,N,N,N,Ne); neighbour, neighbour_face, bdry (6,Ne)
+ * with 1-based element and face numbers (neighbour 0 = outer face).
+ * orientation may be NULL: each shared face's D4 map is then found from the
+ * face nodes' coordinates (they must coincide to 1e-8 of the element size).
+ * Otherwise orientation[f,e] = s + 2 rp + 4 rq maps this face's node (p,q)
+ * to the neighbour face's (p',q') = (q,p) if s else (p,q), then p' -> N-1-p'
+ * if rp, q' -> N-1-q' if rq.  Outer faces carry bdry tag 1..6 (Dirichlet
+ * value desc->bdry_values[tag-1]) or 7 (Sommerfeld).  desc supplies N, tau,
+ * precision, flags, bdry_values and sommerfeld_inv_R; eps_ko must be 0.  The
+ * geometry J = dx/dxi is G applied to coords.  Ne must factor as
+ * M0*M1*M2 with each factor within wt2_ctx_create's per-axis limits. */
+int wt2_ctx_create_mesh(const wt2_mesh_desc* desc, int64_t Ne, const double* coords,
+                        const int32_t* neighbour, const int32_t* neighbour_face,
+                        const int32_t* orientation, const int32_t* bdry, wt2_ctx** out);
 /* Device bytes that wt2_ctx_create (and, with with_background != 0 on a flat
  * mesh, wt2_ctx_set_background) will allocate for desc, without allocating:
  * the figure wt2_info.device_bytes reports afterwards.  Caller-owned state and

@@ -138,3 +138,72 @@ def test_tau_effective_matches_reference_driver_scaling():
     assert wh.tau_effective(20.0, 8) == 73.5
     assert wh.tau_effective(20.0, 4) == 20.0
     assert wh.tau_effective(None, 10) == 121.5
+
+
+def _ball(N=3, M=1, tag=1):
+    return wh.make_cubed_ball_mesh(N, M, R=1.0, outer_tag=tag)
+
+
+def _create_mesh(m, eps=0.0, orientation=None, nbr=None, nface=None, bdry=None):
+    d = _desc(N=m.N, eps=eps)
+    ctx = ctypes.c_void_p()
+    arr = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+    nbr, nface, bdry = arr(m.neighbour if nbr is None else nbr), arr(m.neighbour_face if nface is None else nface), \
+        arr(m.bdry if bdry is None else bdry)
+    ori = None if orientation is None else arr(orientation)
+    rc = _lib.lib().wt2_ctx_create_mesh(ctypes.byref(d), m.Ne, m.coords.ctypes.data, nbr.ctypes.data,
+                                        nface.ctypes.data, None if ori is None else ori.ctypes.data,
+                                        bdry.ctypes.data, ctypes.byref(ctx))
+    if rc == 0:
+        _lib.lib().wt2_ctx_destroy(ctx)
+    return rc
+
+
+def test_cubed_ball_mesh_connectivity():
+    m = wh.make_cubed_ball_mesh(4, 2, R=2.0, outer_tag=7)
+    assert m.Ne == 7 * 8
+    r = np.linalg.norm(m.coords.reshape(-1, 3), axis=1)
+    assert abs(r.max() - 2.0) < 1e-14
+    outer = m.neighbour == 0
+    assert outer.sum() == 6 * 4 and np.all(m.bdry[outer] == 7) and np.all(m.bdry[~outer] == 0)
+    e, f = np.nonzero(~outer)
+    en, fn = m.neighbour[e, f] - 1, m.neighbour_face[e, f] - 1
+    assert np.all(m.neighbour[en, fn] == e + 1) and np.all(m.neighbour_face[en, fn] == f + 1)
+    # the outer faces lie on the sphere
+    for e, f in zip(*np.nonzero(outer)):
+        x = wh.core._face_nodes(m.coords[e:e + 1], f).reshape(-1, 3)
+        assert np.max(np.abs(np.linalg.norm(x, axis=1) - 2.0)) < 1e-14
+
+
+def test_ctx_create_mesh_validates_before_touching_a_device():
+    m = _ball()
+    ok = (0, _lib.WT2_ERR_HIP)        # valid input: created, or no device here
+    assert _create_mesh(m) in ok
+    assert _create_mesh(m, eps=0.1) == _lib.WT2_ERR_UNSUPPORTED
+    bad = m.bdry.copy()
+    bad[m.neighbour == 0] = 9
+    assert _create_mesh(m, bdry=bad) == _lib.WT2_ERR_ARG
+    nb = m.neighbour.copy()
+    e, f = map(int, np.argwhere(nb > 0)[0])
+    nb[e, f] = m.Ne + 1
+    assert _create_mesh(m, nbr=nb) == _lib.WT2_ERR_ARG
+    # a face named as the neighbour of a face it does not touch
+    nf = m.neighbour_face.copy()
+    nf[e, f] = nf[e, f] % 6 + 1
+    assert _create_mesh(m, nface=nf) == _lib.WT2_ERR_ARG
+    assert "does not meet" in _lib.lib().wt2_last_error().decode() or \
+        "name each other" in _lib.lib().wt2_last_error().decode()
+    # orientations that are not each other's inverse
+    ori = np.zeros((m.Ne, 6), dtype=np.int32)
+    ori[e, f] = 2
+    assert _create_mesh(m, orientation=ori) == _lib.WT2_ERR_ARG
+    ori[e, f] = 8
+    assert _create_mesh(m, orientation=ori) == _lib.WT2_ERR_ARG
+    # 67 elements (prime > 64) cannot be tiled as M0 x M1 x M2
+    m67 = wh.HexMesh(3, np.tile(m.coords[:1], (67, 1, 1, 1, 1)), np.zeros((67, 6)), np.zeros((67, 6)),
+                     np.ones((67, 6)))
+    assert _create_mesh(m67) == _lib.WT2_ERR_UNSUPPORTED
+    with pytest.raises(ValueError):
+        wh.Workspace(m, wh.make_element(3), sommerfeld_faces=(1,))
+    with pytest.raises(ValueError):
+        wh.Workspace(m, wh.make_element(4))

@@ -1,0 +1,225 @@
+"""Meshes given by coordinates and connectivity (wt2_ctx_create_mesh): the
+multi-patch cubed ball, whose patches meet with orientation != 0, and
+relabelled copies of uniform meshes.
+
+The oracle has no restatement of general connectivity, so these tests pin the
+path by what the mathematics fixes:
+  * on affine elements the operator is the uniform-mesh one (oracle-pinned)
+    whatever the element numbering and the orientation of each element's
+    local axes;
+  * relabelling the elements of a curved mesh and turning each element's
+    axes by a symmetry of the cube changes nothing but rounding;
+  * the SIPG operator is symmetric in the H3 |J| inner product and negative
+    semi-definite at zero Dirichlet data on the cubed ball;
+  * the standing radial mode on the cubed ball converges to the exact
+    solution as N grows;
+  * Sommerfeld outer faces only dissipate energy.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import wavetoy_hip as wh
+
+pytestmark = pytest.mark.gpu
+
+
+def _orient_pq(ori, N, p, q):
+    a, b = (q, p) if ori & 1 else (p, q)
+    if ori & 2:
+        a = N - 1 - a
+    if ori & 4:
+        b = N - 1 - b
+    return a, b
+
+
+def _face_node_index(N, f, p, q):
+    ax, r = f // 2, (N - 1 if f % 2 else 0)
+    if ax == 0:
+        return r + N * (p + N * q)
+    if ax == 1:
+        return p + N * (r + N * q)
+    return p + N * (q + N * r)
+
+
+def _orientations(mesh):
+    """The D4 code of every shared face, found by brute force over the 8
+    maps of the face nodes (encoding of wt2_ctx_create_mesh)."""
+    N, X = mesh.N, mesh.coords.reshape(mesh.Ne, -1, 3)
+    ori = np.zeros((mesh.Ne, 6), dtype=np.int32)
+    for e in range(mesh.Ne):
+        for f in range(6):
+            if mesh.neighbour[e, f] == 0:
+                continue
+            en, fn = mesh.neighbour[e, f] - 1, mesh.neighbour_face[e, f] - 1
+            best = None
+            for o in range(8):
+                d = 0.0
+                for q in range(N):
+                    for p in range(N):
+                        pn, qn = _orient_pq(o, N, p, q)
+                        d = max(d, np.max(np.abs(X[e, _face_node_index(N, f, p, q)]
+                                                 - X[en, _face_node_index(N, fn, pn, qn)])))
+                if best is None or d < best[0]:
+                    best = (d, o)
+            ori[e, f] = best[1]
+    return ori
+
+
+def _relabel(X, rng):
+    """Permute the elements and apply a random symmetry of the cube (axis
+    permutation and reversals) to each element's node array.  Returns the new
+    coordinates and, per new node, the flat index of the old node it is."""
+    Ne, N = X.shape[0], X.shape[1]
+    perm = rng.permutation(Ne)
+    idx = np.arange(Ne * N ** 3).reshape(Ne, N, N, N)
+    Xn = np.empty_like(X)
+    src = np.empty_like(idx)
+    for en, eo in enumerate(perm):
+        axes = tuple(rng.permutation(3))
+        flips = rng.integers(0, 2, size=3)
+        A, I = np.transpose(X[eo], axes + (3,)), np.transpose(idx[eo], axes)
+        for d in range(3):
+            if flips[d]:
+                A, I = np.flip(A, axis=d), np.flip(I, axis=d)
+        Xn[en], src[en] = A, I
+    return Xn, src.reshape(-1)
+
+
+def _rhs(work, u, ud, bv):
+    out = np.zeros_like(u)
+    work.rhs(out, u, ud, bv)
+    return out
+
+
+def _abs_detJ(mesh):
+    """|det dx/dxi| per node, G applied to the coordinates."""
+    N = mesh.N
+    G = wh.make_element(N).G
+    X = mesh.coords                                  # [e, k, j, i, c]
+    J = np.stack([np.einsum("ip,ekjpc->ekjic", G, X),
+                  np.einsum("jp,ekpic->ekjic", G, X),
+                  np.einsum("kp,epjic->ekjic", G, X)], axis=-1)
+    return np.abs(np.linalg.det(J)).reshape(-1)
+
+
+def _h3(N, Ne):
+    w = wh.make_element(N).w
+    return np.tile(np.einsum("k,j,i->kji", w, w, w).reshape(-1), Ne)
+
+
+@pytest.mark.parametrize("N,M,precision", [(4, (3, 2, 2), "fp64"), (6, (2, 2, 3), "fp64"), (5, (2, 3, 2), "mixed")])
+def test_affine_mesh_any_labelling_matches_uniform_operator(N, M, precision):
+    rng = np.random.default_rng(N)
+    mesh = wh.make_uniform_hex(M, 0.0, 1.0)
+    elem = wh.make_element(N)
+    bv = (0.3, -0.2, 0.1, 0.4, -0.5, 0.25)
+    ref = wh.Workspace(mesh, elem, bdry_values=bv, precision=precision)
+    X = ref.coords().reshape(mesh.Ne, N, N, N, 3)
+    Xn, src = _relabel(X, rng)
+    # outer face tags: the face's side of the unit cube, so Dirichlet values follow
+    nbr, nface, bdry = wh.connect_by_coords(N, Xn)
+    for e in range(mesh.Ne):
+        for f in range(6):
+            if nbr[e, f] == 0:
+                c = wh.core._face_nodes(Xn[e:e + 1], f).reshape(-1, 3).mean(axis=0)
+                a = int(np.argmin(np.minimum(np.abs(c), np.abs(1 - c))))
+                bdry[e, f] = 2 * a + (1 if c[a] > 0.5 else 0) + 1
+    hm = wh.HexMesh(N, Xn, nbr, nface, bdry)
+    work = wh.Workspace(hm, elem, bdry_values=bv, precision=precision)
+    assert work.info()["curved"] == 1
+    u = rng.standard_normal(ref.npoints)
+    ud = rng.standard_normal(ref.npoints)
+    want = _rhs(ref, u, ud, bv)
+    got = _rhs(work, u[src], ud[src], bv)
+    tol = (1e-11 if precision == "fp64" else 2e-5) * np.max(np.abs(want))
+    assert np.max(np.abs(got - want[src])) < tol
+    assert np.array_equal(work.coords().reshape(-1, 3), Xn.reshape(-1, 3))
+
+
+@pytest.mark.parametrize("explicit", [False, True])
+def test_cubed_ball_relabelling_invariance(explicit):
+    N, M = 5, 2
+    rng = np.random.default_rng(11)
+    ball = wh.make_cubed_ball_mesh(N, M, R=1.0, outer_tag=7)
+    Xn, src = _relabel(ball.coords, rng)
+    nbr, nface, bdry = wh.connect_by_coords(N, Xn, outer_tag=7)
+    hm = wh.HexMesh(N, Xn, nbr, nface, bdry)
+    ori = _orientations(ball)
+    assert np.any(ori[ball.neighbour > 0] != 0)       # the patches meet turned
+    if explicit:
+        ball.orientation = ori
+        hm.orientation = _orientations(hm)
+    elem = wh.make_element(N)
+    a = wh.Workspace(ball, elem, sommerfeld_R=1.0)
+    b = wh.Workspace(hm, elem, sommerfeld_R=1.0)
+    u = rng.standard_normal(a.npoints)
+    ud = rng.standard_normal(a.npoints)
+    want = _rhs(a, u, ud, (0.0,) * 6)
+    got = _rhs(b, u[src], ud[src], (0.0,) * 6)
+    assert np.max(np.abs(got - want[src])) < 1e-11 * np.max(np.abs(want))
+
+
+@pytest.mark.parametrize("N", [4, 6])
+def test_cubed_ball_operator_symmetric_and_nonpositive(N):
+    ball = wh.make_cubed_ball_mesh(N, 2, R=1.0)
+    work = wh.Workspace(ball, wh.make_element(N))
+    W = _h3(N, ball.Ne) * _abs_detJ(ball)            # H3 / wgt with wgt = 1 / |J|
+    rng = np.random.default_rng(N)
+    u, v = rng.standard_normal(work.npoints), rng.standard_normal(work.npoints)
+    z = np.zeros_like(u)
+    Lu, Lv = _rhs(work, u, z, (0.0,) * 6), _rhs(work, v, z, (0.0,) * 6)
+    a, b = np.dot(v * W, Lu), np.dot(u * W, Lv)
+    scale = math.sqrt(abs(np.dot(u * W, Lu) * np.dot(v * W, Lv)))
+    assert abs(a - b) < 1e-11 * scale
+    assert np.dot(u * W, Lu) < 0 and np.dot(v * W, Lv) < 0
+
+
+def _radial_mode_error(N, M, t1):
+    ball = wh.make_cubed_ball_mesh(N, M, R=1.0)
+    work = wh.Workspace(ball, wh.make_element(N))
+    X = ball.coords.reshape(-1, 3)
+    u, ud = wh.eigenmode_radial(X, 0.0, R=1.0)
+    dt0 = wh.recommended_dt_rk4(work, cfl=0.2)
+    n = max(1, math.ceil(t1 / dt0))
+    work.rk4_steps(u, ud, t1 / n, n)
+    ue, _ = wh.eigenmode_radial(X, t1, R=1.0)
+    return np.max(np.abs(u - ue)), u
+
+
+def test_cubed_ball_standing_wave_converges():
+    e4, _ = _radial_mode_error(4, 2, 0.25)
+    e7, u7 = _radial_mode_error(7, 2, 0.25)
+    assert np.all(np.isfinite(u7))
+    assert e7 < 2e-4 and e7 < e4 / 10, (e4, e7)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cubed_ball_sommerfeld_dissipates(precision):
+    N = 6
+    ball = wh.make_cubed_ball_mesh(N, 2, R=1.0, outer_tag=7)
+    work = wh.Workspace(ball, wh.make_element(N), sommerfeld_R=1.0, precision=precision)
+    X = ball.coords.reshape(-1, 3)
+    u, ud = wh.outgoing_pulse(X, 0.0, s0=0.3, sigma=0.1)
+    dt = wh.recommended_dt_rk4(work, cfl=0.2)
+    n = 20 * math.ceil(1.2 / dt / 20)                # to t ~ 1.2: the pulse has crossed r = 1
+    E = work.rk4_steps(u, ud, dt, n, energy_every=20)
+    assert np.all(np.isfinite(u))
+    assert np.all(np.diff(E) <= 1e-12 * E[0]), E
+    assert E[-1] < 0.5 * E[0], E                     # it left through the sphere
+
+
+def test_cubed_ball_mixed_matches_fp64():
+    N = 6
+    ball = wh.make_cubed_ball_mesh(N, 2, R=1.0)
+    X = ball.coords.reshape(-1, 3)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        work = wh.Workspace(ball, wh.make_element(N), precision=prec)
+        u, ud = wh.eigenmode_radial(X, 0.0, R=1.0)
+        n = math.ceil(0.25 / wh.recommended_dt_rk4(work, cfl=0.2))
+        work.rk4_steps(u, ud, 0.25 / n, n)
+        res[prec] = u
+    assert np.all(np.isfinite(res["fp64"]))
+    assert np.max(np.abs(res["mixed"] - res["fp64"])) < 1e-5 * np.max(np.abs(res["fp64"]))

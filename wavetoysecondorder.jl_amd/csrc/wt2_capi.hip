@@ -210,6 +210,7 @@ struct wt2_ctx {
     double h[3];
     double mu, inv_mu5, kos, max_sym_eig, h_min;
     int curved;
+    bool mesh_som;  // general mesh with Sommerfeld-tagged outer faces
     int fold;       // RK4 schedule with the first pass of each stage folded into the previous stage
     int nb[3];
     int direct[3];  // per axis: LDS-free pass kernel
@@ -651,7 +652,191 @@ int wt2_ctx_footprint(const wt2_mesh_desc* desc, int32_t with_background, int64_
     return WT2_OK;
 }
 
+// General-mesh input of wt2_ctx_create_mesh: node coordinates and the
+// kernels' connectivity table.
+struct MeshIn {
+    const double* xyz;
+    std::vector<int2> conn;
+    bool som;           // some outer face carries the Sommerfeld tag
+};
+
+static int ctx_create_impl(const wt2_mesh_desc* desc, const MeshIn* mi, wt2_ctx** out);
+
 int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
+{
+    return ctx_create_impl(desc, nullptr, out);
+}
+
+// D4 map of a face's node (p, q) to the neighbour face's node (the encoding of
+// wt2_ctx_create_mesh's orientation argument)
+static void orient_pq(int ori, int N, int p, int q, int* pn, int* qn)
+{
+    int a = p, b = q;
+    if (ori & 1) { a = q; b = p; }
+    if (ori & 2) a = N - 1 - a;
+    if (ori & 4) b = N - 1 - b;
+    *pn = a; *qn = b;
+}
+
+// element-local node index of face f's node (p, q)
+static int face_node(int N, int f, int p, int q)
+{
+    const int ax = f >> 1, r = (f & 1) ? N - 1 : 0;
+    if (ax == 0) return r + N * (p + N * q);
+    if (ax == 1) return p + N * (r + N * q);
+    return p + N * (q + N * r);
+}
+
+int wt2_ctx_create_mesh(const wt2_mesh_desc* desc, int64_t Ne, const double* coords,
+                        const int32_t* neighbour, const int32_t* neighbour_face,
+                        const int32_t* orientation, const int32_t* bdry, wt2_ctx** out)
+{
+    if (!desc || !coords || !neighbour || !neighbour_face || !bdry || !out) {
+        set_error("wt2_ctx_create_mesh: null pointer");
+        return WT2_ERR_ARG;
+    }
+    *out = nullptr;
+    const int N = desc->N;
+    if (N < NMIN || N > NMAX) { set_error("wt2_ctx_create_mesh: N must be in 3..10"); return WT2_ERR_SHAPE; }
+    if (Ne < 1 || Ne > (int64_t(1) << 30)) { set_error("wt2_ctx_create_mesh: Ne out of range"); return WT2_ERR_SHAPE; }
+    if (desc->eps_ko != 0.0) {
+        set_error("wt2_ctx_create_mesh: Kreiss-Oliger dissipation is built for uniform meshes only (eps_ko must be 0)");
+        return WT2_ERR_UNSUPPORTED;
+    }
+    // the element count as M0*M1*M2 within the per-axis limits (the RK4
+    // update kernels tile the element array by that shape)
+    const int mmax = N >= 9 ? 32 : MMAX_AXIS;
+    int Mf[3] = {0, 0, 0};
+    for (int m2 = 1; m2 <= mmax && !Mf[0]; ++m2) {
+        if (Ne % m2) continue;
+        const int64_t r = Ne / m2;
+        for (int m1 = 1; m1 <= mmax; ++m1)
+            if (r % m1 == 0 && r / m1 <= mmax) { Mf[0] = int(r / m1); Mf[1] = m1; Mf[2] = m2; break; }
+    }
+    if (!Mf[0]) {
+        set_error("wt2_ctx_create_mesh: Ne does not factor as M0*M1*M2 within the per-axis element limits");
+        return WT2_ERR_UNSUPPORTED;
+    }
+    const int N2 = N * N;
+    const int64_t N3 = int64_t(N2) * N;
+    auto X = [&](int64_t e, int n, int a) { return coords[size_t(3 * (e * N3 + n) + a)]; };
+    MeshIn mi;
+    mi.xyz = coords;
+    mi.som = false;
+    mi.conn.resize(size_t(Ne) * 6);
+    char buf[256];
+    for (int64_t e = 0; e < Ne; ++e) {
+        // element size for the coordinate tolerance: its longest corner-to-corner span
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int n = 0; n < N3; ++n)
+            for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], X(e, n, a)); hi[a] = std::max(hi[a], X(e, n, a)); }
+        const double size = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+        if (!(size > 0) || !std::isfinite(size)) {
+            std::snprintf(buf, sizeof(buf), "wt2_ctx_create_mesh: element %lld has degenerate or non-finite coordinates",
+                          (long long)(e + 1));
+            set_error(buf);
+            return WT2_ERR_ARG;
+        }
+        for (int f = 0; f < 6; ++f) {
+            const size_t k = size_t(6 * e + f);
+            const int32_t nb = neighbour[k];
+            int2 cf;
+            if (nb == 0) {
+                const int tag = bdry[k];
+                if (tag < 1 || tag > SOMMERFELD_TAG) {
+                    std::snprintf(buf, sizeof(buf),
+                                  "wt2_ctx_create_mesh: outer face %d of element %lld has tag %d (1..6 Dirichlet, 7 Sommerfeld)",
+                                  f + 1, (long long)(e + 1), tag);
+                    set_error(buf);
+                    return WT2_ERR_ARG;
+                }
+                mi.som |= tag == SOMMERFELD_TAG;
+                cf.x = -1;
+                cf.y = tag << 6;
+                mi.conn[k] = cf;
+                continue;
+            }
+            const int fn = neighbour_face[k] - 1;
+            if (nb < 1 || nb > Ne || fn < 0 || fn > 5 || (nb - 1 == e && fn == f)) {
+                std::snprintf(buf, sizeof(buf), "wt2_ctx_create_mesh: face %d of element %lld: bad neighbour %d / face %d",
+                              f + 1, (long long)(e + 1), nb, fn + 1);
+                set_error(buf);
+                return WT2_ERR_ARG;
+            }
+            const int64_t en = nb - 1;
+            int ori = -1;
+            if (orientation) {
+                ori = orientation[k];
+                if (ori < 0 || ori > 7) {
+                    std::snprintf(buf, sizeof(buf), "wt2_ctx_create_mesh: face %d of element %lld: orientation %d not in 0..7",
+                                  f + 1, (long long)(e + 1), ori);
+                    set_error(buf);
+                    return WT2_ERR_ARG;
+                }
+            } else {
+                // the D4 map under which the two faces' nodes coincide
+                double best = 1e300;
+                for (int o = 0; o < 8; ++o) {
+                    double dmax = 0.0;
+                    for (int q = 0; q < N && dmax < best; ++q)
+                        for (int p = 0; p < N; ++p) {
+                            int pn, qn;
+                            orient_pq(o, N, p, q, &pn, &qn);
+                            const int n0 = face_node(N, f, p, q), n1 = face_node(N, fn, pn, qn);
+                            for (int a = 0; a < 3; ++a) dmax = std::max(dmax, std::fabs(X(e, n0, a) - X(en, n1, a)));
+                        }
+                    if (dmax < best) { best = dmax; ori = o; }
+                }
+                if (!(best <= 1e-8 * size)) {
+                    std::snprintf(buf, sizeof(buf),
+                                  "wt2_ctx_create_mesh: face %d of element %lld does not meet face %d of element %d "
+                                  "(closest node distance %g)", f + 1, (long long)(e + 1), fn + 1, nb, best);
+                    set_error(buf);
+                    return WT2_ERR_ARG;
+                }
+            }
+            cf.x = int(en);
+            cf.y = fn | (ori << 3);
+            mi.conn[k] = cf;
+        }
+    }
+    // every shared face is shared both ways, with inverse node maps
+    for (int64_t e = 0; e < Ne; ++e)
+        for (int f = 0; f < 6; ++f) {
+            const int2 cf = mi.conn[size_t(6 * e + f)];
+            if (cf.x < 0) continue;
+            const int fn = cf.y & 7, ori = (cf.y >> 3) & 7;
+            const int2 back = mi.conn[size_t(6 * int64_t(cf.x) + fn)];
+            bool ok = back.x == e && (back.y & 7) == f;
+            const int orib = (back.y >> 3) & 7;
+            for (int q = 0; q < N && ok; ++q)
+                for (int p = 0; p < N && ok; ++p) {
+                    int pn, qn, pb, qb;
+                    orient_pq(ori, N, p, q, &pn, &qn);
+                    orient_pq(orib, N, pn, qn, &pb, &qb);
+                    ok = pb == p && qb == q;
+                }
+            if (!ok) {
+                std::snprintf(buf, sizeof(buf),
+                              "wt2_ctx_create_mesh: face %d of element %lld and face %d of element %d do not name each other "
+                              "with inverse orientations", f + 1, (long long)(e + 1), fn + 1, cf.x + 1);
+                set_error(buf);
+                return WT2_ERR_ARG;
+            }
+        }
+    wt2_mesh_desc d = *desc;
+    for (int a = 0; a < 3; ++a) {
+        d.M[a] = Mf[a];
+        d.x0[a] = 0.0;
+        d.x1[a] = 1.0;
+        d.periodic[a] = 0;
+    }
+    for (int f = 0; f < 6; ++f) d.face_kind[f] = 0;
+    d.warp_A = 0.0;
+    return ctx_create_impl(&d, &mi, out);
+}
+
+static int ctx_create_impl(const wt2_mesh_desc* desc, const MeshIn* mi, wt2_ctx** out)
 {
     if (!desc || !out) { set_error("wt2_ctx_create: null pointer"); return WT2_ERR_ARG; }
     *out = nullptr;
@@ -704,7 +889,8 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
     c->npts = int64_t(c->N) * c->N * c->N * c->Ne;
     gll(c->N, c->xi, c->w, c->G);
     for (int a = 0; a < 3; ++a) c->h[a] = (desc->x1[a] - desc->x0[a]) / desc->M[a];
-    c->curved = desc->warp_A != 0.0;
+    c->curved = desc->warp_A != 0.0 || mi != nullptr;
+    c->mesh_som = mi && mi->som;
     const int N = c->N;
 
     // Coercivity of the 1D SIPG operator: the flat operator is a Kronecker sum
@@ -827,7 +1013,9 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out)
             rc = WT2_ERR_HIP;
         }
     }
-    if (rc == WT2_OK && c->curved)
+    if (rc == WT2_OK && mi)
+        rc = curved_init_mesh(c->cd, *desc, c->xi, c->w, c->G, mi->xyz, mi->conn, c->stream, &c->bytes);
+    else if (rc == WT2_OK && c->curved)
         rc = curved_init(c->cd, *desc, c->xi, c->w, c->G, nullptr, c->stream, &c->bytes);
     if (rc == WT2_OK && c->curved) {
         c->cd.op64 = c->op64; c->cd.op32 = c->op32c;
@@ -895,15 +1083,18 @@ int wt2_set_stream(wt2_ctx* c, void* stream)
 int wt2_coords(const wt2_ctx* c, double* xyz)
 {
     if (!c || !xyz) { set_error("wt2_coords: null pointer"); return WT2_ERR_ARG; }
-    mesh_coords(c->d, c->xi, xyz);
+    if (!c->cd.xyz.empty())
+        std::memcpy(xyz, c->cd.xyz.data(), c->cd.xyz.size() * sizeof(double));
+    else
+        mesh_coords(c->d, c->xi, xyz);
     return WT2_OK;
 }
 
 int wt2_ctx_set_background(wt2_ctx* c, const double* bg10)
 {
     if (!c || !bg10) { set_error("wt2_ctx_set_background: null pointer"); return WT2_ERR_ARG; }
-    for (int a = 0; a < 3; ++a)
-        if (som_axis(c, a)) {
+    for (int a = 0; a < 4; ++a)
+        if (a == 3 ? c->mesh_som : som_axis(c, a)) {
             set_error("wt2_ctx_set_background: Sommerfeld faces are built for meshes without a background");
             return WT2_ERR_UNSUPPORTED;
         }

@@ -73,9 +73,11 @@ static int upload(CurvedData& cd, const std::vector<double>& coef, const std::ve
 
 // Per-node coefficients (SoA, 13 x npts) and the energy weight H3/wgt, from the
 // analytic warp Jacobian and the background (nullptr = Minkowski).
+// xyz != nullptr: node coordinates (3,N,N,N,Ne) of a general mesh, with
+// J = dx/dxi from G applied along each reference axis; otherwise the warp map.
 static void compute_coefs(const wt2_mesh_desc& d, const double* xi_n, const double* w,
                           const double* bg10, std::vector<double>& coef, std::vector<double>& wv,
-                          double* h_min)
+                          double* h_min, const double* xyz = nullptr, const double* Gm = nullptr)
 {
     const int N = d.N;
     const int* M = d.M;
@@ -94,6 +96,12 @@ static void compute_coefs(const wt2_mesh_desc& d, const double* xi_n, const doub
     for (int64_t e = tid; e < Ne; e += nt) {
         const int ec[3] = {int(e % M[0]), int((e / M[0]) % M[1]), int(e / (int64_t(M[0]) * M[1]))};
         double xl[16][16][16][3];
+        if (xyz)
+            for (int k = 0; k < N; ++k)
+                for (int j = 0; j < N; ++j)
+                    for (int i = 0; i < N; ++i)
+                        for (int a = 0; a < 3; ++a)
+                            xl[k][j][i][a] = xyz[size_t(3 * (i + int64_t(N) * (j + int64_t(N) * (k + int64_t(N) * e))) + a)];
         for (int k = 0; k < N; ++k)
             for (int j = 0; j < N; ++j)
                 for (int i = 0; i < N; ++i) {
@@ -105,10 +113,23 @@ static void compute_coefs(const wt2_mesh_desc& d, const double* xi_n, const doub
                     const double A = d.warp_A;
                     const double sv = A * s1 * s2 * s3;
                     const double ds[3] = {A * tp * c1 * s2 * s3, A * tp * s1 * c2 * s3, A * tp * s1 * s2 * c3};
-                    for (int a = 0; a < 3; ++a) xl[k][j][i][a] = d.x0[a] + L[a] * (xi[a] + sv);
                     double J[3][3];
-                    for (int a = 0; a < 3; ++a)
-                        for (int b = 0; b < 3; ++b) J[a][b] = L[a] * ((a == b ? 1.0 : 0.0) + ds[b]) / M[b];
+                    if (xyz) {
+                        for (int a = 0; a < 3; ++a)
+                            for (int b = 0; b < 3; ++b) {
+                                double s = 0.0;
+                                for (int p = 0; p < N; ++p) {
+                                    int jj[3] = {i, j, k};
+                                    jj[b] = p;
+                                    s += Gm[ii[b] * N + p] * xl[jj[2]][jj[1]][jj[0]][a];
+                                }
+                                J[a][b] = s;
+                            }
+                    } else {
+                        for (int a = 0; a < 3; ++a) xl[k][j][i][a] = d.x0[a] + L[a] * (xi[a] + sv);
+                        for (int a = 0; a < 3; ++a)
+                            for (int b = 0; b < 3; ++b) J[a][b] = L[a] * ((a == b ? 1.0 : 0.0) + ds[b]) / M[b];
+                    }
                     const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
                                        J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                                        J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
@@ -230,18 +251,40 @@ int curved_init(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const 
 }
 
 int curved_set_background(CurvedData& cd, const wt2_mesh_desc& d, const double* xi,
-                          const double* w, const double*, const double* bg10, hipStream_t s)
+                          const double* w, const double* G, const double* bg10, hipStream_t s)
 {
     std::vector<double> coef, wv;
-    compute_coefs(d, xi, w, bg10, coef, wv, &cd.h_min);
+    compute_coefs(d, xi, w, bg10, coef, wv, &cd.h_min, cd.xyz.empty() ? nullptr : cd.xyz.data(), G);
     const int rc = upload(cd, coef, wv, s);
     if (rc == WT2_OK) cd.pending = false;
     return rc;
 }
 
+int curved_init_mesh(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const double* w,
+                     const double* G, const double* xyz, const std::vector<int2>& conn, hipStream_t s,
+                     int64_t* bytes)
+{
+    int rc = curved_init(cd, d, xi, w, G, nullptr, s, bytes);
+    if (rc != WT2_OK) return rc;
+    cd.xyz.assign(xyz, xyz + size_t(3 * cd.npts));
+    cd.som = 0;    // Sommerfeld faces come from the per-face tags
+    if (hipMalloc(&cd.conn, conn.size() * sizeof(int2)) != hipSuccess) {
+        curved_free(cd);
+        set_error("curved set-up: device allocation failed");
+        return WT2_ERR_OOM;
+    }
+    *bytes += int64_t(conn.size() * sizeof(int2));
+    if (hipMemcpy(cd.conn, conn.data(), conn.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
+        curved_free(cd);
+        set_error("curved set-up: connectivity upload failed");
+        return WT2_ERR_HIP;
+    }
+    return WT2_OK;
+}
+
 void curved_free(CurvedData& cd)
 {
-    hipFree(cd.coef64); hipFree(cd.coef32); hipFree(cd.wu); hipFree(cd.wv); hipFree(cd.fbuf);
+    hipFree(cd.coef64); hipFree(cd.coef32); hipFree(cd.wu); hipFree(cd.wv); hipFree(cd.fbuf); hipFree(cd.conn);
     cd = CurvedData();
 }
 
@@ -269,6 +312,7 @@ struct CArgs {
     int64_t npts;
     int M0, M1, M2, per0, per1, per2;
     int adv, weighted;
+    const int2* conn;               // general connectivity, or nullptr (uniform mesh)
 };
 
 template <int N>
@@ -475,12 +519,34 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
             const int f = q / N2, pq = q - f * N2;
             const int p = pq % N, qq = pq / N;
             const int ax = f >> 1, side = f & 1;
-            const int r = side ? N - 1 : 0, rn = side ? 0 : N - 1;
+            const int r = side ? N - 1 : 0;
             const TC nrm = side ? TC(1) : TC(-1);
             const int n = fidx<N>(ax, r, p, qq);
             const int64_t gn = eb + n;
-            const int64_t en = nbr_elem(e, ax, side, a.M0, a.M1, a.M2, a.per0, a.per1, a.per2);
-            if (en < 0 && ((a.k.som >> f) & 1)) {
+            // neighbour element en (-1: outer face), its face fn, the neighbour
+            // face's node (pn, qn) that meets this face's (p, qq), and whether
+            // this outer face is a Sommerfeld face; bvf: its Dirichlet value
+            int64_t en;
+            int fn, pn = p, qn = qq;
+            bool som;
+            TC bvf;
+            if (a.conn) {
+                const int2 cf = a.conn[e * 6 + f];
+                en = cf.x;
+                fn = cf.y & 7;
+                const int ori = (cf.y >> 3) & 7, tag = cf.y >> 6;
+                if (ori & 1) { pn = qq; qn = p; }
+                if (ori & 2) pn = N - 1 - pn;
+                if (ori & 4) qn = N - 1 - qn;
+                som = tag == 7;
+                bvf = (tag >= 1 && tag <= 6) ? a.k.bv[tag - 1] : TC(0);
+            } else {
+                en = nbr_elem(e, ax, side, a.M0, a.M1, a.M2, a.per0, a.per1, a.per2);
+                fn = (ax << 1) | (1 - side);
+                som = (a.k.som >> f) & 1;
+                bvf = a.k.bv[f];
+            }
+            if (en < 0 && som) {
                 // Sommerfeld face (src/wave.jl:444-518): no SIPG terms; the drag
                 // -(wF/Hphys)(u_t + d_n u + u/R), here unweighted (the output
                 // is multiplied by wgt = 1/|J|): |n_u| d_n u = n F^a with this
@@ -495,15 +561,19 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
             }
             TC du, dF = TC(0), kap = a.coef[(10 + ax) * np + gn], dv = TC(0), alpha;
             if (en >= 0) {
-                const int64_t gnn = en * N3 + fidx<N>(ax, rn, p, qq);
+                // outward normal fluxes of the two sides: nrm F (own) and
+                // nrm_n F_n (neighbour); dF = F + (nrm nrm_n) F_n
+                const int axn = fn >> 1;
+                const int64_t gnn = en * N3 + fidx<N>(axn, (fn & 1) ? N - 1 : 0, pn, qn);
                 du = A[n] - TC(u[gnn]);
-                const int fn = (ax << 1) | (1 - side);
-                dF = a.fbuf[e * S::F2 + q] - a.fbuf[en * S::F2 + fn * N2 + pq];
-                kap = TC(0.5) * (kap + a.coef[(10 + ax) * np + gnn]);
+                const TC Fn = a.fbuf[en * S::F2 + fn * N2 + pn + N * qn];
+                const TC Fo = a.fbuf[e * S::F2 + q];
+                dF = ((fn & 1) == side) ? Fo + Fn : Fo - Fn;
+                kap = TC(0.5) * (kap + a.coef[(10 + axn) * np + gnn]);
                 if (adv) dv = B[n] - TC(v[gnn]);
                 alpha = TC(0.5);
             } else {
-                du = A[n] - a.k.bv[f];
+                du = A[n] - bvf;
                 if (adv) dv = B[n];
                 alpha = TC(1);
             }
@@ -639,6 +709,7 @@ static void make_cargs(const CurvedData& cd, const CurvedCall& cc, CArgs<N, TC>&
     a.per0 = cd.periodic[0]; a.per1 = cd.periodic[1]; a.per2 = cd.periodic[2];
     a.adv = cc.v != nullptr;
     a.weighted = cc.weighted;
+    a.conn = cd.conn;
 }
 
 template <int N, typename TC, typename TS, typename TO>

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "../../include/wt2.h"
 
@@ -32,6 +33,9 @@ struct CurvedData {
     bool pending = false;       // allocated; coefficients not computed yet
     const void* op64 = nullptr; // even/odd operator tables (owned by the context)
     const void* op32 = nullptr;
+    // general connectivity (wt2_ctx_create_mesh); empty / nullptr on uniform meshes
+    std::vector<double> xyz;    // node coordinates (3,N,N,N,Ne)
+    int2* conn = nullptr;       // per (e, f): (neighbour or -1, fn | ori << 3 | tag << 6)
 };
 
 struct CurvedCall {
@@ -63,6 +67,11 @@ int curved_init(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const 
 int curved_set_background(CurvedData& cd, const wt2_mesh_desc& d, const double* xi,
                           const double* w, const double* G, const double* bg10, hipStream_t s);
 void curved_free(CurvedData& cd);
+// The same for a mesh given by node coordinates xyz (3,N,N,N,Ne) and the
+// per-face connectivity table conn (Ne*6 entries, the kernels' encoding).
+int curved_init_mesh(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const double* w,
+                     const double* G, const double* xyz, const std::vector<int2>& conn, hipStream_t s,
+                     int64_t* bytes);
 hipError_t curved_operator(const CurvedData& cd, const CurvedCall& cc, hipStream_t s);
 
 // Node coordinates (3,N,N,N,Ne) of the (possibly warped) uniform hex mesh.

@@ -28,7 +28,7 @@ WT2_NO_GRAPH = 4
 
 #: every entry point include/wt2.h declares
 EXPORTS = (
-    "wt2_gll", "wt2_ctx_create", "wt2_ctx_footprint", "wt2_ctx_destroy", "wt2_ctx_info", "wt2_ctx_set_background",
+    "wt2_gll", "wt2_ctx_create", "wt2_ctx_create_mesh", "wt2_ctx_footprint", "wt2_ctx_destroy", "wt2_ctx_info", "wt2_ctx_set_background",
     "wt2_coords", "wt2_rhs3d", "wt2_rhs_full", "wt2_rk4_steps", "wt2_energy", "wt2_set_stream",
     "wt2_batch_create", "wt2_batch_destroy", "wt2_batch_add", "wt2_batch_remove", "wt2_batch_step",
     "wt2_batch_step_async", "wt2_batch_sync",
@@ -108,6 +108,8 @@ def lib():
         vp = ctypes.c_void_p
         L.wt2_gll.argtypes = [ctypes.c_int32, dp, dp, dp]
         L.wt2_ctx_create.argtypes = [ctypes.POINTER(MeshDesc), ctypes.POINTER(vp)]
+        L.wt2_ctx_create_mesh.argtypes = [ctypes.POINTER(MeshDesc), ctypes.c_int64, dp, dp, dp, dp, dp,
+                                          ctypes.POINTER(vp)]
         L.wt2_ctx_footprint.argtypes = [ctypes.POINTER(MeshDesc), ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]
         L.wt2_ctx_destroy.argtypes = [vp]
         L.wt2_ctx_destroy.restype = None

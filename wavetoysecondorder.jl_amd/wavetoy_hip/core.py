@@ -115,6 +115,115 @@ def make_warped_uniform_hex(Mx, My, Mz, x0, x1, A, periodic=False):
     return Mesh((Mx, My, Mz), _triple(x0), _triple(x1), _triple(periodic), warp_A=A)
 
 
+def _face_nodes(X, f):
+    """Nodes of face f (0..5 = -x,+x,-y,+y,-z,+z) of element arrays X
+    (Ne, N, N, N, 3) indexed [e, k, j, i], as (Ne, q, p, 3): the face's (p, q)
+    run over the two tangential axes in increasing order, as in the
+    reference's face extraction (src/wave_strong_rhs.jl:41-73)."""
+    ax, r = f // 2, (0 if f % 2 == 0 else -1)
+    if ax == 0:
+        return X[:, :, :, r, :]
+    if ax == 1:
+        return X[:, :, r, :, :]
+    return X[:, r, :, :, :]
+
+
+class HexMesh:
+    """A hex mesh given by node coordinates and face connectivity: the
+    reference's MeshGeometry.coords and MeshConnectivity fields (neighbour,
+    neighbour_face, orientation, bdry; 1-based, column-major (6, Ne), here as
+    (Ne, 6) arrays), for meshes that are not one uniform block, such as the
+    multi-patch cubed ball.  coords: (Ne, N, N, N, 3) or (N^3 Ne, 3), node
+    index i + N (j + N (k + N e)).  orientation None: found from the face
+    nodes' coordinates by the library."""
+
+    M = (1, 1, 1)
+    x0 = (0.0, 0.0, 0.0)
+    x1 = (1.0, 1.0, 1.0)
+    periodic = (False, False, False)
+    warp_A = 0.0
+
+    def __init__(self, N, coords, neighbour, neighbour_face, bdry, orientation=None):
+        self.N = int(N)
+        X = np.ascontiguousarray(coords, dtype=np.float64)
+        if X.size % (3 * self.N ** 3):
+            raise ValueError("coords must hold 3 N^3 values per element")
+        self.Ne = X.size // (3 * self.N ** 3)
+        self.coords = X.reshape(self.Ne, self.N, self.N, self.N, 3)
+
+        def tab(a):
+            t = np.ascontiguousarray(a, dtype=np.int32)
+            if t.shape != (self.Ne, 6):
+                raise ValueError("connectivity arrays are (Ne, 6)")
+            return t
+        self.neighbour = tab(neighbour)
+        self.neighbour_face = tab(neighbour_face)
+        self.bdry = tab(bdry)
+        self.orientation = None if orientation is None else tab(orientation)
+
+
+def connect_by_coords(N, coords, outer_tag=1):
+    """neighbour, neighbour_face and bdry of elements that meet face to face:
+    two faces are shared when their node centroids coincide (to 1e-9 of the
+    smallest element); every other face is an outer face with outer_tag."""
+    from scipy.spatial import cKDTree
+    X = np.asarray(coords, dtype=np.float64).reshape(-1, N, N, N, 3)
+    Ne = X.shape[0]
+    cen = np.stack([_face_nodes(X, f).mean(axis=(1, 2)) for f in range(6)], axis=1).reshape(-1, 3)
+    span = (X.reshape(Ne, -1, 3).max(axis=1) - X.reshape(Ne, -1, 3).min(axis=1)).max(axis=1)
+    tol = 1e-9 * float(span.min())
+    nbr = np.zeros((Ne, 6), dtype=np.int32)
+    nface = np.zeros((Ne, 6), dtype=np.int32)
+    bdry = np.full((Ne, 6), int(outer_tag), dtype=np.int32)
+    for a, b in cKDTree(cen).query_pairs(tol):
+        ea, fa, eb, fb = a // 6, a % 6, b // 6, b % 6
+        if nbr[ea, fa] or nbr[eb, fb]:
+            raise ValueError("a face meets more than one other face")
+        nbr[ea, fa], nface[ea, fa], nbr[eb, fb], nface[eb, fb] = eb + 1, fb + 1, ea + 1, fa + 1
+    bdry[nbr > 0] = 0
+    return nbr, nface, bdry
+
+
+def make_cubed_ball_mesh(N, M, R=1.0, a=None, outer_tag=1):
+    """A ball of radius R as 7 patches of M^3 elements each (the layout of the
+    reference's make_cubed_cube_mesh, src/evolve.jl:916-918): the cube
+    [-a, a]^3 (default a = R/2) and six shells between its faces and the
+    sphere.  Shell (axis d, sign sg) has local axes (s, t, rho): s, t in
+    [-1, 1] along the two other axes in cyclic order, rho in [0, 1] outward;
+    x = (1 - rho) a (s, t, sg) + rho R (tan(pi s / 4), tan(pi t / 4), sg) / |.|
+    (components in the order (axis d+1, d+2, d)).  The patches meet with
+    orientation != 0; outer faces (on the sphere) carry outer_tag."""
+    a = 0.5 * R if a is None else float(a)
+    xs = make_element(N).xs
+    g = (np.arange(M)[:, None] + xs[None, :]) / M          # (M, N): element, node -> [0, 1]
+    els = []
+    # the inner cube: element (ex, ey, ez) x-fastest, nodes [k, j, i]
+    c = -a + 2.0 * a * g
+    for ez in range(M):
+        for ey in range(M):
+            for ex in range(M):
+                Z, Y, Xc = np.meshgrid(c[ez], c[ey], c[ex], indexing="ij")
+                els.append(np.stack([Xc, Y, Z], axis=-1))
+    st = -1.0 + 2.0 * g
+    for d in range(3):
+        p1, p2 = (d + 1) % 3, (d + 2) % 3
+        for sg in (-1.0, 1.0):
+            for er in range(M):
+                for et in range(M):
+                    for es in range(M):
+                        RHO, T, S = np.meshgrid(g[er], st[et], st[es], indexing="ij")
+                        tu, tv = np.tan(0.25 * np.pi * S), np.tan(0.25 * np.pi * T)
+                        nrm = np.sqrt(1.0 + tu * tu + tv * tv)
+                        x = np.empty(S.shape + (3,))
+                        x[..., p1] = (1 - RHO) * a * S + RHO * R * tu / nrm
+                        x[..., p2] = (1 - RHO) * a * T + RHO * R * tv / nrm
+                        x[..., d] = sg * ((1 - RHO) * a + RHO * R / nrm)
+                        els.append(x)
+    X = np.stack(els)
+    nbr, nface, bdry = connect_by_coords(N, X, outer_tag)
+    return HexMesh(N, X, nbr, nface, bdry)
+
+
 # -------------------------------------------------------- device workspace
 
 def mesh_desc(mesh, elem, tau=None, eps_ko=0.0, bdry_values=(0.0,) * 6, precision="fp64",
@@ -178,7 +287,17 @@ class Workspace:
         self.N = elem.N
         self.precision = precision
         self.ctx = ctypes.c_void_p()
-        check(lib().wt2_ctx_create(ctypes.byref(d), ctypes.byref(self.ctx)))
+        if isinstance(mesh, HexMesh):
+            if mesh.N != elem.N:
+                raise ValueError("mesh coordinates are for another N")
+            if sommerfeld_faces:
+                raise ValueError("on a HexMesh, Sommerfeld faces are the outer faces with bdry tag 7")
+            ori = None if mesh.orientation is None else mesh.orientation.ctypes.data
+            check(lib().wt2_ctx_create_mesh(ctypes.byref(d), mesh.Ne, mesh.coords.ctypes.data,
+                                            mesh.neighbour.ctypes.data, mesh.neighbour_face.ctypes.data,
+                                            ori, mesh.bdry.ctypes.data, ctypes.byref(self.ctx)))
+        else:
+            check(lib().wt2_ctx_create(ctypes.byref(d), ctypes.byref(self.ctx)))
         self.npoints = elem.N ** 3 * mesh.Ne
         if bg is not None:
             self.set_background(bg)
