@@ -207,3 +207,19 @@ def test_ctx_create_mesh_validates_before_touching_a_device():
         wh.Workspace(m, wh.make_element(3), sommerfeld_faces=(1,))
     with pytest.raises(ValueError):
         wh.Workspace(m, wh.make_element(4))
+
+
+@pytest.mark.parametrize("make,patches,tags", [
+    (lambda: wh.make_cubed_cube_mesh(3, 2, 0.3), 7, list(range(7))),
+    (lambda: wh.make_inflated_cube_mesh(3, 0.1, 0.3, 1.0, 2), 13, [0, 1]),
+])
+def test_patch_mesh_families(make, patches, tags):
+    # test/dirichlet_cubed_cube_tests.jl:21-32, test/dirichlet_cubed_sphere_tests.jl:44-82
+    m = make()
+    assert m.Ne == patches * 8
+    assert sorted(set(m.bdry.reshape(-1).tolist())) == tags
+    if patches == 13:
+        for e, f in zip(*np.nonzero(m.neighbour == 0)):
+            x = wh.core._face_nodes(m.coords[e:e + 1], f).reshape(-1, 3)
+            assert np.max(np.abs(np.linalg.norm(x, axis=1) - 1.0)) < 1e-12
+    assert _create_mesh(m) in (0, _lib.WT2_ERR_HIP)

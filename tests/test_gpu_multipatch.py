@@ -223,3 +223,39 @@ def test_cubed_ball_mixed_matches_fp64():
         res[prec] = u
     assert np.all(np.isfinite(res["fp64"]))
     assert np.max(np.abs(res["mixed"] - res["fp64"])) < 1e-5 * np.max(np.abs(res["fp64"]))
+
+
+def _rayleigh(mesh, u):
+    """-<u, L u>_Hphys / <u, u>_Hphys with Hphys = w_i w_j w_k |J|, zero
+    Dirichlet data (test/dirichlet_cubed_cube_tests.jl:58-73)."""
+    work = wh.Workspace(mesh, wh.make_element(mesh.N))
+    W = _h3(mesh.N, mesh.Ne) * _abs_detJ(mesh)
+    Lu = _rhs(work, u, np.zeros_like(u), (0.0,) * 6)
+    return -np.dot(u * W, Lu) / np.dot(u * W, u)
+
+
+def test_cubed_cube_rayleigh_quotient_converges():
+    # test/dirichlet_cubed_cube_tests.jl:54-81: u = cos(pi x/2) cos(pi y/2)
+    # cos(pi z/2) on [-1, 1]^3, lambda = 3 pi^2 / 4, N = 4, M = 2, 4, 8
+    lam = 3 * math.pi ** 2 / 4
+    errs = []
+    for M in (2, 4, 8):
+        mesh = wh.make_cubed_cube_mesh(4, M, 0.3)
+        X = mesh.coords.reshape(-1, 3)
+        u = np.prod(np.cos(0.5 * math.pi * X), axis=1)
+        errs.append(abs(_rayleigh(mesh, u) - lam))
+    assert all(e > 0 for e in errs) and errs[-1] < errs[0], errs
+    assert (errs[0] / errs[-1]) ** 0.5 > 4.0, errs
+    assert errs[-1] < 1e-3, errs
+
+
+def test_inflated_cube_rayleigh_quotient():
+    # test/dirichlet_cubed_sphere_tests.jl:84-125: u = j0(pi r), lambda = pi^2
+    # on the ball R2 = 1 (L = 0.1, R1 = 0.3), N = 4
+    errs = []
+    for M in (2, 4):
+        mesh = wh.make_inflated_cube_mesh(4, 0.1, 0.3, 1.0, M)
+        u, _ = wh.eigenmode_radial(mesh.coords.reshape(-1, 3), 0.0, R=1.0)
+        errs.append(abs(_rayleigh(mesh, u) - math.pi ** 2))
+    assert errs[1] / math.pi ** 2 < 1e-3, errs
+    assert errs[1] < errs[0] / 10, errs

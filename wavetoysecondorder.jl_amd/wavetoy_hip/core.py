@@ -184,20 +184,31 @@ def connect_by_coords(N, coords, outer_tag=1):
     return nbr, nface, bdry
 
 
-def make_cubed_ball_mesh(N, M, R=1.0, a=None, outer_tag=1):
-    """A ball of radius R as 7 patches of M^3 elements each (the layout of the
-    reference's make_cubed_cube_mesh, src/evolve.jl:916-918): the cube
-    [-a, a]^3 (default a = R/2) and six shells between its faces and the
-    sphere.  Shell (axis d, sign sg) has local axes (s, t, rho): s, t in
-    [-1, 1] along the two other axes in cyclic order, rho in [0, 1] outward;
-    x = (1 - rho) a (s, t, sg) + rho R (tan(pi s / 4), tan(pi t / 4), sg) / |.|
-    (components in the order (axis d+1, d+2, d)).  The patches meet with
-    orientation != 0; outer faces (on the sphere) carry outer_tag."""
-    a = 0.5 * R if a is None else float(a)
+def _cube_surface(a):
+    """Point (s, t) of the face (axis d, sign sg) of the cube [-a, a]^3, in the
+    patch frame (components along axes d+1, d+2, d)."""
+    return lambda S, T, sg: (a * S, a * T, sg * a * np.ones_like(S))
+
+
+def _sphere_surface(r):
+    """Point (s, t) of the sphere of radius r over the cube face (axis d, sign
+    sg), equiangular: direction (tan(pi s / 4), tan(pi t / 4), sg)."""
+    def f(S, T, sg):
+        tu, tv = np.tan(0.25 * np.pi * S), np.tan(0.25 * np.pi * T)
+        k = r / np.sqrt(1.0 + tu * tu + tv * tv)
+        return k * tu, k * tv, sg * k
+    return f
+
+
+def _patch_mesh(N, M, a, layers):
+    """Coordinates (Ne, N, N, N, 3) of a patch mesh: the cube [-a, a]^3 of M^3
+    affine elements (x fastest), then for each layer (inner, outer surface)
+    six patches of M^3 elements, one per cube face (axis d, sign -, +), with
+    local axes (s, t, rho): s, t in [-1, 1] along axes d+1, d+2 (cyclic), rho
+    in [0, 1] outward, x = (1 - rho) inner(s, t) + rho outer(s, t)."""
     xs = make_element(N).xs
-    g = (np.arange(M)[:, None] + xs[None, :]) / M          # (M, N): element, node -> [0, 1]
+    g = (np.arange(M)[:, None] + xs[None, :]) / M          # (element, node) -> [0, 1]
     els = []
-    # the inner cube: element (ex, ey, ez) x-fastest, nodes [k, j, i]
     c = -a + 2.0 * a * g
     for ez in range(M):
         for ey in range(M):
@@ -205,23 +216,60 @@ def make_cubed_ball_mesh(N, M, R=1.0, a=None, outer_tag=1):
                 Z, Y, Xc = np.meshgrid(c[ez], c[ey], c[ex], indexing="ij")
                 els.append(np.stack([Xc, Y, Z], axis=-1))
     st = -1.0 + 2.0 * g
-    for d in range(3):
-        p1, p2 = (d + 1) % 3, (d + 2) % 3
-        for sg in (-1.0, 1.0):
-            for er in range(M):
-                for et in range(M):
-                    for es in range(M):
-                        RHO, T, S = np.meshgrid(g[er], st[et], st[es], indexing="ij")
-                        tu, tv = np.tan(0.25 * np.pi * S), np.tan(0.25 * np.pi * T)
-                        nrm = np.sqrt(1.0 + tu * tu + tv * tv)
-                        x = np.empty(S.shape + (3,))
-                        x[..., p1] = (1 - RHO) * a * S + RHO * R * tu / nrm
-                        x[..., p2] = (1 - RHO) * a * T + RHO * R * tv / nrm
-                        x[..., d] = sg * ((1 - RHO) * a + RHO * R / nrm)
-                        els.append(x)
-    X = np.stack(els)
-    nbr, nface, bdry = connect_by_coords(N, X, outer_tag)
-    return HexMesh(N, X, nbr, nface, bdry)
+    for inner, outer in layers:
+        for d in range(3):
+            p1, p2 = (d + 1) % 3, (d + 2) % 3
+            for sg in (-1.0, 1.0):
+                for er in range(M):
+                    for et in range(M):
+                        for es in range(M):
+                            RHO, T, S = np.meshgrid(g[er], st[et], st[es], indexing="ij")
+                            pi_, po = inner(S, T, sg), outer(S, T, sg)
+                            x = np.empty(S.shape + (3,))
+                            for comp, ax in enumerate((p1, p2, d)):
+                                x[..., ax] = (1 - RHO) * pi_[comp] + RHO * po[comp]
+                            els.append(x)
+    return np.stack(els)
+
+
+def _tag_cube_sides(mesh, half):
+    """Outer faces on the cube [-half, half]^3 get the reference's tags 1..6
+    (-x, +x, -y, +y, -z, +z)."""
+    for e, f in zip(*np.nonzero(mesh.neighbour == 0)):
+        cen = _face_nodes(mesh.coords[e:e + 1], f).reshape(-1, 3).mean(axis=0)
+        ax = int(np.argmax(np.abs(cen)))
+        mesh.bdry[e, f] = 2 * ax + (2 if cen[ax] > 0 else 1)
+    return mesh
+
+
+def make_cubed_cube_mesh(N, M, R=0.3):
+    """The cube [-1, 1]^3 as 7 patches of M^3 elements (make_cubed_cube_mesh,
+    src/evolve.jl:916-918; test/dirichlet_cubed_cube_tests.jl): the inner cube
+    [-R, R]^3 and six wedges between its faces and the outer cube's.  Outer
+    faces carry tags 1..6 by side (-x, +x, -y, +y, -z, +z)."""
+    X = _patch_mesh(N, M, R, [(_cube_surface(R), _cube_surface(1.0))])
+    return _tag_cube_sides(HexMesh(N, X, *connect_by_coords(N, X)), 1.0)
+
+
+def make_inflated_cube_mesh(N, L=0.1, R1=0.3, R2=1.0, M=2, outer_tag=1):
+    """The ball |x| <= R2 as 13 patches of M^3 elements
+    (make_inflated_cube_mesh, src/evolve.jl:919-921;
+    test/dirichlet_cubed_sphere_tests.jl): the cube [-L, L]^3, six inflation
+    patches from its faces to the sphere R1, six shells from R1 to R2.  The
+    outer faces lie on the sphere R2 and carry outer_tag (1 Dirichlet, 7
+    Sommerfeld)."""
+    X = _patch_mesh(N, M, L, [(_cube_surface(L), _sphere_surface(R1)),
+                              (_sphere_surface(R1), _sphere_surface(R2))])
+    return HexMesh(N, X, *connect_by_coords(N, X, outer_tag))
+
+
+def make_cubed_ball_mesh(N, M, R=1.0, a=None, outer_tag=1):
+    """The ball |x| <= R as 7 patches of M^3 elements: the cube [-a, a]^3
+    (default a = R/2) and six patches from its faces to the sphere R.  The
+    patches meet with orientation != 0; outer faces carry outer_tag."""
+    a = 0.5 * R if a is None else float(a)
+    X = _patch_mesh(N, M, a, [(_cube_surface(a), _sphere_surface(R))])
+    return HexMesh(N, X, *connect_by_coords(N, X, outer_tag))
 
 
 # -------------------------------------------------------- device workspace
