@@ -62,6 +62,37 @@ function Context(N::Integer, M::Integer; x0 = 0.0, x1 = 1.0, periodic = false,
 end
 
 """
+    Context(geom, conn; τ, bdry_values=ntuple(_->0.0, 6), precision=:fp64, sommerfeld_R=Inf)
+
+The same bundle on a multi-patch mesh (make_cubed_cube_mesh /
+make_inflated_cube_mesh + make_geometry + to_device, src/evolve.jl:916-936):
+`geom.coords` (3,N,N,N,Ne) and `conn` = mesh.conn (`neighbour`,
+`neighbour_face`, `bdry`; (6,Ne)).  The D4 orientation of every shared face is
+found by the library from the face nodes, so HexMeshes' `orientation` codes
+are not passed.  Outer faces with bdry 7 are Sommerfeld faces.
+"""
+function Context(geom, conn; τ, bdry_values = ntuple(_ -> 0.0, 6), precision = :fp64,
+                 sommerfeld_R = Inf)
+    N = size(geom.coords, 2)
+    Ne = size(geom.coords, 5)
+    d = MeshDesc(Int32(N), (Int32(1), Int32(1), Int32(1)), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0),
+                 (Int32(0), Int32(0), Int32(0)), 0.0, Float64(τ), 0.0, NTuple{6,Float64}(bdry_values),
+                 Int32(precision === :mixed ? 1 : 0), Int32(0), ntuple(_ -> Int32(0), 6),
+                 isinf(sommerfeld_R) ? 0.0 : 1.0 / Float64(sommerfeld_R))
+    coords = Array{Float64}(geom.coords)
+    nbr = Array{Int32}(conn.neighbour); nface = Array{Int32}(conn.neighbour_face)
+    bdry = Array{Int32}(conn.bdry)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:wt2_ctx_create_mesh, libwt2), Cint,
+                (Ref{MeshDesc}, Int64, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32},
+                 Ref{Ptr{Cvoid}}),
+                d, Ne, coords, nbr, nface, C_NULL, bdry, out))
+    ctx = Context(out[], N^3 * Ne)
+    finalizer(c -> ccall((:wt2_ctx_destroy, libwt2), Cvoid, (Ptr{Cvoid},), c.ptr), ctx)
+    return ctx
+end
+
+"""
     rhs_wave3d!(ü, u, u̇, bdry_values; ctx) -> ü
 
 Same arguments and result as WaveToySecondOrder.rhs_wave3d! (src/wave.jl:583-611);
