@@ -141,3 +141,12 @@ def test_driver_argument_errors():
         wh.evolve3d(formulation="conservative", N=4, M=2, bc=("excision",) * 6)
     with pytest.raises(ValueError):
         wh.evolve3d(N=4, M=2, outer_bc="sommerfeld", nsteps=1)
+
+
+def test_evolve3d_mesh_kind_arguments():
+    with pytest.raises(ValueError, match="mesh_kind"):
+        wh.evolve3d(mesh_kind="radial_shell", device=False)
+    with pytest.raises(ValueError, match="spherical"):
+        wh.evolve3d(mesh_kind="cubed_cube", outer_bc="sommerfeld", device=False)
+    with pytest.raises(ValueError, match="outer_bc"):
+        wh.evolve3d(mesh_kind="inflated_cube", outer_bc="absorbing", device=False)

@@ -259,3 +259,24 @@ def test_inflated_cube_rayleigh_quotient():
         errs.append(abs(_rayleigh(mesh, u) - math.pi ** 2))
     assert errs[1] / math.pi ** 2 < 1e-3, errs
     assert errs[1] < errs[0] / 10, errs
+
+
+def test_evolve3d_on_patch_meshes():
+    # src/evolve.jl:913-928, 961-968: the inflated cube with the j0 standing
+    # mode (exact solution for l2_err) and the cubed cube with the standing
+    # cartesian wave; tau = 8 (N-1)^2 on both
+    r = wh.evolve3d(mesh_kind="inflated_cube", ic="radial", N=6, M=2, t1=0.2, Nt=3)
+    assert r["tau"] == 8 * 25 and r["x0"] == -1.0 and r["sommerfeld_R"] == math.inf
+    assert np.all(np.isfinite(r["l2_err"])) and r["l2_err"][0] < 1e-12 and r["l2_err"][-1] < 2e-3
+    assert r["xs_line"].size > 0 and abs(r["y_target"]) < 1e-12 and abs(r["z_target"]) < 1e-12
+    r = wh.evolve3d(mesh_kind="cubed_cube", ic="cartesian", ic_wavenumber=math.pi, N=6, M=2, t1=0.1, Nt=2)
+    assert np.all(np.isfinite(r["u_final"])) and r["l2_err"][-1] < 1e-3, r["l2_err"]
+
+
+def test_evolve3d_inflated_cube_sommerfeld_lets_the_pulse_out():
+    kw = dict(mesh_kind="inflated_cube", ic="outgoing", N=6, M=2, t1=1.2, Nt=4)
+    som = wh.evolve3d(outer_bc="sommerfeld", **kw)
+    dirichlet = wh.evolve3d(outer_bc="dirichlet", **kw)
+    assert som["sommerfeld_R"] == 1.0
+    assert np.all(np.diff(som["energy"]) <= 1e-12 * som["energy"][0])
+    assert som["energy"][-1] < 0.2 * dirichlet["energy"][-1], (som["energy"], dirichlet["energy"])

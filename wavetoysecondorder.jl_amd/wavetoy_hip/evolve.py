@@ -41,12 +41,39 @@ def evolve3d(formulation="strong", **kwargs):
     return _evolve3d_strong(**kwargs)
 
 
+def _hphys(N, X):
+    """Hphys = w_i w_j w_k |det dx/dxi| per node of coordinates X (Ne, N, N, N, 3),
+    dx/dxi by G (the geometry wt2_ctx_create_mesh builds)."""
+    e = core.make_element(N)
+    J = np.stack([np.einsum("ip,ekjpc->ekjic", e.G, X),
+                  np.einsum("jp,ekpic->ekjic", e.G, X),
+                  np.einsum("kp,epjic->ekjic", e.G, X)], axis=-1)
+    w3 = np.einsum("k,j,i->kji", e.w, e.w, e.w)
+    return (np.abs(np.linalg.det(J)) * w3[None]).reshape(-1)
+
+
+def _spectral_radius(work, iters=40, seed=0):
+    """Largest |eigenvalue| of the u -> u_tt operator at u_t = 0 and zero
+    boundary data, by power iteration (the role of spectral_radius_estimate in
+    the reference's recommended_dt, src/evolve.jl:27-36)."""
+    v = np.random.default_rng(seed).standard_normal(work.npoints)
+    z = np.zeros_like(v)
+    out = np.empty_like(v)
+    lam = 0.0
+    for _ in range(iters):
+        v /= np.linalg.norm(v)
+        work.rhs(out, v, z, (0.0,) * 6)
+        lam = float(np.linalg.norm(out))
+        v, out = out, v
+    return lam
+
+
 def _evolve3d_strong(N=5, M=8, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="cartesian",
                      ic_wavenumber=3 * math.pi, ic_centre=(0.5, 0.5, 0.5), ic_width=None, ic_denom=None,
                      ic_radial_mode=1, ic_radius=None, ic_pulse_offset=None, ic_pulse_width=None,
                      outer_bc="dirichlet", tau=None, eps_ko=0.0, t0=0.0, t1=1.0, nsteps=None, dt=None,
                      cfl=0.2, Nt=200, precision="fp64", energy_every=None, device=True, slice_y=None,
-                     slice_z=None):
+                     slice_z=None, mesh_kind="cubical", R=0.1, L=0.1, R1=0.3, R2=1.0):
     """The (u, u_t) driver (_evolve3d_strong, src/evolve.jl:877-1073) on the
     cube mesh (warp_A = 0) or the warped cube.  ic: "cartesian" (standing
     wave), "radial" (eigenmode_radial, radius ic_radius = L/2 and mode
@@ -59,24 +86,55 @@ def _evolve3d_strong(N=5, M=8, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="
     no KO: rhs_wave3d! has none); the time stepper is classical RK4 with dt =
     cfl h_min (the reference picks a symplectic method by N, src/evolve.jl:27-36,
     and its dt from a spectral-radius estimate); nsteps, when given, overrides
-    t1."""
-    if outer_bc != "dirichlet":
+    t1.
+
+    mesh_kind (src/evolve.jl:913-928): "cubical" (above), "cubed_cube" (the
+    cube [-1, 1]^3 as 7 patches around the inner cube [-R, R]^3, outer faces
+    Dirichlet) or "inflated_cube" (the ball of radius R2 as 13 patches: cube
+    [-L, L]^3, inflation to the sphere R1, shells to R2; outer_bc "dirichlet"
+    or "sommerfeld" with sommerfeld_R = R2).  On those the initial data are
+    centred at the origin, the sampled line is y = z = 0 and tau defaults to
+    8 (N-1)^2 (the reference's tau_mult, src/evolve.jl:962-968)."""
+    if outer_bc not in ("dirichlet", "sommerfeld"):
+        raise ValueError(f"evolve3d: unknown outer_bc {outer_bc!r}")
+    if outer_bc == "sommerfeld" and mesh_kind != "inflated_cube":
         raise ValueError(f"evolve3d: outer_bc = {outer_bc!r} is only supported on the spherical meshes "
-                         "(inflated cube, radial shell); the cube takes 'dirichlet'")
-    mesh = (core.make_warped_uniform_hex(M, M, M, x0, x1, warp_A) if warp_A
-            else core.make_uniform_hex(M, x0, x1))
+                         "(inflated cube); the cube takes 'dirichlet'")
     elem = core.make_element(N)
-    tau_used = core.tau_effective(tau, N)
+    som_R = math.inf
+    if mesh_kind == "cubical":
+        mesh = (core.make_warped_uniform_hex(M, M, M, x0, x1, warp_A) if warp_A
+                else core.make_uniform_hex(M, x0, x1))
+        tau_used = core.tau_effective(tau, N)
+        coords = core.host_coords(mesh, elem)
+    elif mesh_kind in ("cubed_cube", "inflated_cube"):
+        if mesh_kind == "cubed_cube":
+            x0, x1 = -1.0, 1.0
+            mesh = core.make_cubed_cube_mesh(N, M, R)
+        else:
+            x0, x1 = -float(R2), float(R2)
+            mesh = core.make_inflated_cube_mesh(N, L, R1, R2, M, outer_tag=7 if outer_bc == "sommerfeld" else 1)
+            som_R = float(R2) if outer_bc == "sommerfeld" else math.inf
+        tau_used = 8.0 * (N - 1) ** 2 if tau is None else float(tau)
+        coords = mesh.coords.reshape(-1, 3)
+    else:
+        raise ValueError(f"evolve3d: unknown mesh_kind {mesh_kind!r} (cubical, cubed_cube, inflated_cube)")
+    patch = mesh_kind != "cubical"
     bg = None
-    coords = core.host_coords(mesh, elem)
     if background == "adm_bump":
         bgobj = core.adm_bump_background()
         bg = core.sample_background3d(bgobj, 0.0, coords[:, 0], coords[:, 1], coords[:, 2])
     elif background is not None:
         bg = np.asarray(background, dtype=np.float64)
-    work = core.Workspace(mesh, elem, tau=tau_used, eps_ko=eps_ko, precision=precision, bg=bg)
+    work = core.Workspace(mesh, elem, tau=tau_used, eps_ko=eps_ko, precision=precision, bg=bg,
+                          sommerfeld_R=som_R)
     if dt is None:
         dt = core.recommended_dt_rk4(work, cfl)
+        if patch:
+            # the penalty 8 (N-1)^2 raises the spectral radius above what
+            # cfl h_min allows for: RK4's stability interval on the imaginary
+            # axis is 2 sqrt(2), kept at 2.5 / sqrt(rho)
+            dt = min(dt, 2.5 / math.sqrt(_spectral_radius(work)))
     if nsteps is None:
         if t1 is None:
             raise ValueError("give nsteps or t1")
@@ -120,7 +178,7 @@ def _evolve3d_strong(N=5, M=8, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="
     # the sampled line along x: the reference's y = z = 1/4 of the cube, or
     # (when that is not a node line of this mesh) the node line nearest to it
     def nearest(col):
-        q = x0 + 0.25 * (x1 - x0)
+        q = 0.0 if patch else x0 + 0.25 * (x1 - x0)
         vals = coords[:, col]
         return float(vals[np.argmin(np.abs(vals - q))])
     y_target = nearest(1) if slice_y is None else float(slice_y)
@@ -132,7 +190,9 @@ def _evolve3d_strong(N=5, M=8, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="
     bounds = np.unique(np.round(np.linspace(0, nsteps, Nt)).astype(int))
     ts, l2, energies = [], [], []
     H = None
-    if mesh.warp_A == 0.0:
+    if patch:
+        H = _hphys(N, mesh.coords)
+    elif mesh.warp_A == 0.0:
         H = core.Geometry(mesh, elem).Hphys
 
     def sample(step):
@@ -161,7 +221,8 @@ def _evolve3d_strong(N=5, M=8, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="
     return dict(ts=np.array(ts), xs_line=xs_line, us=np.stack(us, axis=1), uds=np.stack(uds, axis=1),
                 l2_err=np.array(l2), energy=np.array(energies), u_final=uf,
                 ud_final=vf, dt=dt, nsteps=nsteps, dx=work.h_min, tau=tau_used, mu=work.mu,
-                y_target=y_target, z_target=z_target,
+                y_target=y_target, z_target=z_target, mesh_kind=mesh_kind, outer_bc=outer_bc,
+                sommerfeld_R=som_R, x0=x0, x1=x1,
                 integrator_name="RK4", work=work, coords=coords)
 
 
