@@ -518,6 +518,69 @@ def fo_run(wh, torch, args):
     return out
 
 
+def patch_run(wh, torch, args):
+    """The multi-patch inflated cube (13 patches of 8^3 elements, N = 8:
+    3,407,872 points; patches meet with orientation != 0) with a Sommerfeld
+    outer sphere: RK4 steps of an outgoing pulse, FP64 and mixed, and the
+    drop-in RHS (wt2_rhs3d) on its own.  Roofline of the curved operator on
+    128 algorithmic bytes per point (FP64: u, u_t, 13 coefficients read, the
+    output written) and 76 (mixed: the coefficients in FP32)."""
+    ball = wh.make_inflated_cube_mesh(N_NODES, 0.1, 0.3, 1.0, 8, outer_tag=7)
+    elem = wh.make_element(N_NODES)
+    x = ball.coords.reshape(-1, 3)
+    u0, v0 = wh.outgoing_pulse(x, 0.0, s0=0.5, sigma=0.1)
+    K = max(1, min(args.steps, 200))
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ws = wh.Workspace(ball, elem, precision=prec, sommerfeld_R=1.0)
+        stream = torch.cuda.Stream()
+        ws.set_stream(stream.cuda_stream)
+        dt = wh.recommended_dt_rk4(ws, 0.2)
+        u = torch.from_numpy(u0).cuda(); v = torch.from_numpy(v0).cuda(); o = torch.empty_like(u)
+        with torch.cuda.stream(stream):
+            ws.rk4_steps(u, v, dt, max(args.warmup, 2))
+            torch.cuda.synchronize()
+            a = torch.cuda.Event(enable_timing=True); b = torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            a.record(stream)
+            ws.rk4_steps(u, v, dt, K)
+            b.record(stream)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            step_ms = a.elapsed_time(b) / K
+            for _ in range(3):
+                ws.rhs(o, u, v, (0.0,) * 6)
+            torch.cuda.synchronize()
+            c = torch.cuda.Event(enable_timing=True); d = torch.cuda.Event(enable_timing=True)
+            c.record(stream)
+            for _ in range(K):
+                ws.rhs(o, u, v, (0.0,) * 6)
+            d.record(stream)
+            torch.cuda.synchronize()
+            rhs_ms = c.elapsed_time(d) / K
+        assert bool(torch.isfinite(u).all())
+        ws.set_stream(None)
+        alg = 128 if prec == "fp64" else 76
+        gbs = alg * ws.npoints / (rhs_ms * 1e-3) / 1e9
+        res[prec] = {"ms_per_step": wall * 1e3 / K, "event_ms_per_step": step_ms,
+                     "point_updates_per_s": ws.npoints * K / wall, "rhs_ms": rhs_ms,
+                     "rhs_roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                      "frac": gbs / PEAK_HBM_GBS, "algorithmic_bytes_per_point": alg}}
+        npts = ws.npoints
+        ws.close()
+    return {"metric": "grid-point updates per second (RK4 step, multi-patch mesh)",
+            "value": res["fp64"]["point_updates_per_s"], "unit": "grid-point updates/s", "n_gpus": 1,
+            "steps": K, "warmup": args.warmup, "ms_per_step": res["fp64"]["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (outgoing pulse, s0 = 0.5, sigma = 0.1)",
+            "config": {"workload": "inflated cube, 13 patches of 8^3 elements (6656), N = 8 (3,407,872 points), "
+                                   "Sommerfeld sphere R = 1, tau = 1.5 (N-1)^2", "points": npts},
+            "fp64": res["fp64"], "mixed": res["mixed"],
+            "mixed_speedup": res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"],
+            "cpu_baseline": None,
+            "cpu_baseline_note": "the CPU oracle has no general-connectivity restatement"}
+
+
 def mixed_error_run(wh, torch, args):
     """Growth in time of the mixed-vs-FP64 relative H-norm error of u and u_t on
     N = 10 meshes of --elems^3 elements, the warped ADM-bump configuration and
@@ -568,7 +631,7 @@ def main():
     ap.add_argument("--repeats", type=int, default=5,
                     help="plain config: further runs of --steps steps after the timed one (median, minimum)")
     ap.add_argument("--no-tune", action="store_true", help="default tile shapes instead of the autotuner's")
-    ap.add_argument("--config", choices=["plain", "curved", "batch", "serving", "passes", "fo", "mixed_error"],
+    ap.add_argument("--config", choices=["plain", "curved", "batch", "serving", "passes", "fo", "mixed_error", "patch"],
                     default="plain",
                     help="plain: the FP64 flat workload (headline); curved: the warped-mesh "
                          "ADM workload, FP64 vs mixed, error at --t-final")
@@ -614,6 +677,9 @@ def main():
         return
     if args.config == "fo":
         print(json.dumps(fo_run(wh, torch, args)), flush=True)
+        return
+    if args.config == "patch":
+        print(json.dumps(patch_run(wh, torch, args)), flush=True)
         return
     if args.config == "mixed_error":
         print(json.dumps(mixed_error_run(wh, torch, args)), flush=True)

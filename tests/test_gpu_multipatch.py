@@ -280,3 +280,36 @@ def test_evolve3d_inflated_cube_sommerfeld_lets_the_pulse_out():
     assert som["sommerfeld_R"] == 1.0
     assert np.all(np.diff(som["energy"]) <= 1e-12 * som["energy"][0])
     assert som["energy"][-1] < 0.2 * dirichlet["energy"][-1], (som["energy"], dirichlet["energy"])
+
+
+def test_batch_of_patch_meshes_matches_single_runs():
+    # batches (one launch per kernel group per stage) give the bits of rk4_steps
+    import torch
+    specs = [(wh.make_inflated_cube_mesh(5, 0.1, 0.3, 1.0, 2, outer_tag=7), "fp64"),
+             (wh.make_cubed_cube_mesh(5, 2, 0.3), "fp64"),
+             (wh.make_cubed_ball_mesh(5, 2, outer_tag=7), "mixed")]
+    singles, runs = [], []
+    for mesh, prec in specs:
+        x = mesh.coords.reshape(-1, 3)
+        u0 = np.exp(-np.sum((x - 0.1) ** 2, axis=1) / 0.05)
+        v0 = 0.1 * np.cos(math.pi * x[:, 2])
+        for rep in range(2):
+            ws = wh.Workspace(mesh, wh.make_element(5), precision=prec, sommerfeld_R=1.0)
+            u, v = torch.from_numpy(u0.copy()).cuda(), torch.from_numpy(v0.copy()).cuda()
+            dt = wh.recommended_dt_rk4(ws, 0.2)
+            if rep == 0:
+                E = ws.rk4_steps(u, v, dt, 6, energy_every=2)
+                singles.append((u.cpu().numpy(), v.cpu().numpy(), E))
+            else:
+                runs.append((ws, u, v, dt))
+    b = wh.Batch()
+    slots = [b.add(ws, u, v, dt, 6, energy_every=2) for ws, u, v, dt in runs]
+    fin = 0
+    while fin < len(slots):
+        fin += b.step(4)
+    torch.cuda.synchronize()
+    for s, (ws, u, v, _), (us, vs, Es) in zip(slots, runs, singles):
+        _, E = b.slot_info(s)
+        assert np.array_equal(u.cpu().numpy(), us) and np.array_equal(v.cpu().numpy(), vs)
+        assert np.array_equal(E, Es)
+    b.close()
