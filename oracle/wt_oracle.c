@@ -26,6 +26,7 @@ struct wto_ctx {
     double* coords;   /* (3,N,N,N,Ne) */
     double* hphys;    /* H3*|J| per node */
     double mu, inv_mu5;
+    int* fconn;       /* general mesh: per (e, f) neighbour (-1 outer), its face, D4 code, tag */
 };
 
 /* ------------------------------------------------------------------ GLL */
@@ -172,7 +173,78 @@ static void inv3(const double J[3][3], double det, double R[3][3])
 static void apply_D_line1d(const wto_ctx* c, int M, int periodic, double h,
                            const double* in, double* out);
 
+static wto_ctx* create_impl(const wto_desc* d, const double* bg, const double* mesh_xyz);
+
 wto_ctx* wto_create(const wto_desc* d, const double* bg)
+{
+    return create_impl(d, bg, NULL);
+}
+
+/* D4 map of face node (p, q) to the neighbour face's node: swap first, then
+ * reverse p' (bit 1), then reverse q' (bit 2). */
+static void orient_map(int ori, int N, int p, int q, int* pn, int* qn)
+{
+    int a = (ori & 1) ? q : p, b = (ori & 1) ? p : q;
+    *pn = (ori & 2) ? N - 1 - a : a;
+    *qn = (ori & 4) ? N - 1 - b : b;
+}
+
+/* A mesh given by node coordinates xyz (3,N,N,N,Ne) and 1-based (6,Ne)
+ * neighbour / neighbour_face / bdry arrays (the reference's MeshGeometry and
+ * MeshConnectivity fields, src/wave.jl:471-473, src/wave_strong_rhs.jl:41-73):
+ * the D4 map of each shared face is the one under which the face nodes
+ * coincide; J = dx/dxi is G applied to xyz.  NULL on inconsistent input. */
+wto_ctx* wto_create_mesh(const wto_desc* d, long Ne, const double* xyz, const int* nbr,
+                         const int* nface, const int* bdry)
+{
+    if (d->N < 2 || d->N > WTO_NMAX || Ne < 1 || d->eps_ko != 0.0) return NULL;
+    wto_desc dm = *d;
+    dm.M[0] = (int)Ne; dm.M[1] = dm.M[2] = 1;
+    for (int a = 0; a < 3; ++a) { dm.x0[a] = 0.0; dm.x1[a] = 1.0; dm.periodic[a] = 0; }
+    dm.warp_A = 0.0;
+    for (int f = 0; f < 6; ++f) dm.face_kind[f] = 0;
+    const int N = d->N, npe = N * N * N;
+    int* fc = (int*)malloc(sizeof(int) * 24 * Ne);
+    for (long e = 0; e < Ne; ++e)
+        for (int f = 0; f < 6; ++f) {
+            int* o = fc + 4 * (6 * e + f);
+            int nb = nbr[6 * e + f];
+            if (nb == 0) {
+                int tag = bdry[6 * e + f];
+                if (tag < 1 || tag > 7) { free(fc); return NULL; }
+                o[0] = -1; o[1] = 0; o[2] = 0; o[3] = tag;
+                continue;
+            }
+            int fn = nface[6 * e + f] - 1;
+            if (nb < 1 || nb > Ne || fn < 0 || fn > 5) { free(fc); return NULL; }
+            long en = nb - 1;
+            double span = 0.0, best = 1e300;
+            int bo = -1;
+            for (int n = 0; n < npe; ++n)
+                for (int a = 0; a < 3; ++a)
+                    span = fmax(span, fabs(xyz[3 * (e * npe + n) + a] - xyz[3 * e * npe + a]));
+            for (int ori = 0; ori < 8; ++ori) {
+                double dmax = 0.0;
+                for (int q = 0; q < N; ++q)
+                    for (int p = 0; p < N; ++p) {
+                        int pn, qn;
+                        orient_map(ori, N, p, q, &pn, &qn);
+                        long n0 = e * npe + fidx(N, f / 2, (f & 1) ? N - 1 : 0, p, q);
+                        long n1 = en * npe + fidx(N, fn / 2, (fn & 1) ? N - 1 : 0, pn, qn);
+                        for (int a = 0; a < 3; ++a) dmax = fmax(dmax, fabs(xyz[3 * n0 + a] - xyz[3 * n1 + a]));
+                    }
+                if (dmax < best) { best = dmax; bo = ori; }
+            }
+            if (!(best <= 1e-8 * span)) { free(fc); return NULL; }
+            o[0] = (int)en; o[1] = fn; o[2] = bo; o[3] = 0;
+        }
+    wto_ctx* c = create_impl(&dm, NULL, xyz);
+    if (!c) { free(fc); return NULL; }
+    c->fconn = fc;
+    return c;
+}
+
+static wto_ctx* create_impl(const wto_desc* d, const double* bg, const double* mesh_xyz)
 {
     if (d->N < 2 || d->N > WTO_NMAX) return NULL;
     for (int a = 0; a < 3; ++a) if (d->M[a] < 1) return NULL;
@@ -184,7 +256,7 @@ wto_ctx* wto_create(const wto_desc* d, const double* bg)
     c->npts = (long)c->npe * c->Ne;
     wto_gll(N, c->xi, c->w, c->G);
     for (int a = 0; a < 3; ++a) c->h[a] = (d->x1[a] - d->x0[a]) / d->M[a];
-    c->curved = (d->warp_A != 0.0) || (bg != NULL);
+    c->curved = (d->warp_A != 0.0) || (bg != NULL) || (mesh_xyz != NULL);
     /* Sommerfeld faces: non-periodic axes, no background (the drag is the
      * flat-spacetime condition of rhs_wave3d!) */
     for (int f = 0; f < 6; ++f) {
@@ -212,10 +284,25 @@ wto_ctx* wto_create(const wto_desc* d, const double* bg)
             double s = 0.0, ds[3] = {0, 0, 0};
             if (d->warp_A != 0.0) warp(d->warp_A, xi, &s, ds);
             double J[3][3];
-            for (int a = 0; a < 3; ++a) {
-                c->coords[3 * n + a] = d->x0[a] + L[a] * (xi[a] + s);
-                for (int b = 0; b < 3; ++b)
-                    J[a][b] = L[a] * ((a == b ? 1.0 : 0.0) + ds[b]) / d->M[b];
+            if (mesh_xyz) {
+                for (int a = 0; a < 3; ++a) {
+                    c->coords[3 * n + a] = mesh_xyz[3 * n + a];
+                    for (int b = 0; b < 3; ++b) {
+                        double acc = 0.0;
+                        for (int p = 0; p < N; ++p) {
+                            int jj[3] = {i, j, k};
+                            jj[b] = p;
+                            acc += c->G[ii[b] * N + p] * mesh_xyz[3 * nidx(N, jj[0], jj[1], jj[2], e) + a];
+                        }
+                        J[a][b] = acc;
+                    }
+                }
+            } else {
+                for (int a = 0; a < 3; ++a) {
+                    c->coords[3 * n + a] = d->x0[a] + L[a] * (xi[a] + s);
+                    for (int b = 0; b < 3; ++b)
+                        J[a][b] = L[a] * ((a == b ? 1.0 : 0.0) + ds[b]) / d->M[b];
+                }
             }
             double detJ = det3(J);
             double H3 = c->w[i] * c->w[j] * c->w[k];
@@ -309,7 +396,7 @@ wto_ctx* wto_create(const wto_desc* d, const double* bg)
 void wto_destroy(wto_ctx* c)
 {
     if (!c) return;
-    free(c->coef); free(c->coords); free(c->hphys); free(c);
+    free(c->coef); free(c->coords); free(c->hphys); free(c->fconn); free(c);
 }
 
 long wto_npoints(const wto_ctx* c) { return c->npts; }
@@ -633,19 +720,32 @@ static void op_curved(const wto_ctx* c, const double* u, const double* v,
             /* faces */
             for (int a = 0; a < 3; ++a)
             for (int sd = 0; sd < 2; ++sd) {
-                long en = neighbour(c, e, a, sd);
-                int r = sd ? N - 1 : 0, rn = sd ? 0 : N - 1;
-                double nrm = sd ? 1.0 : -1.0;
+                /* neighbour element, its face fn (axis an, row rn, outward
+                 * sign nrm_n), the D4 code of the face node map, the tag */
+                long en;
+                int fn, ori = 0, tag = 0;
+                if (c->fconn) {
+                    const int* fc = c->fconn + 4 * (6 * e + 2 * a + sd);
+                    en = fc[0]; fn = fc[1]; ori = fc[2]; tag = fc[3];
+                } else {
+                    en = neighbour(c, e, a, sd);
+                    fn = 2 * a + (1 - sd);
+                }
+                int an = fn / 2;
+                int r = sd ? N - 1 : 0, rn = (fn & 1) ? N - 1 : 0;
+                double nrm = sd ? 1.0 : -1.0, nrm_n = (fn & 1) ? 1.0 : -1.0;
                 double alpha = en >= 0 ? 0.5 : 1.0;
-                double gval = bv[2 * a + sd];
+                double gval = c->fconn ? (tag >= 1 && tag <= 6 ? bv[tag - 1] : 0.0) : bv[2 * a + sd];
                 const double* un = en >= 0 ? u + en * npe : NULL;
                 const double* vn = (en >= 0 && with_adv) ? v + en * npe : NULL;
                 const double* con = en >= 0 ? c->coef + 13 * en * (long)npe : NULL;
-                int som = en < 0 && c->d.face_kind[2 * a + sd] == 7;
+                int som = en < 0 && (c->fconn ? tag == 7 : c->d.face_kind[2 * a + sd] == 7);
                 for (int q = 0; q < N; ++q)
                 for (int p = 0; p < N; ++p) {
                     int n = fidx(N, a, r, p, q);
-                    int nn = fidx(N, a, rn, p, q);
+                    int pn, qn;
+                    orient_map(ori, N, p, q, &pn, &qn);
+                    int nn = fidx(N, an, rn, pn, qn);
                     const double* o = co + 13 * n;
                     if (som) {
                         /* Sommerfeld face (src/wave.jl:444-518): no SIPG terms,
@@ -672,9 +772,11 @@ static void op_curved(const wto_ctx* c, const double* u, const double* v,
                         grad_node(c, un, ii, jj, kk, gn);
                         const double* on = con + 13 * nn;
                         double Fs = Kab(o, a, 0) * gs[0] + Kab(o, a, 1) * gs[1] + Kab(o, a, 2) * gs[2];
-                        double Fn = Kab(on, a, 0) * gn[0] + Kab(on, a, 1) * gn[1] + Kab(on, a, 2) * gn[2];
-                        dF = Fs - Fn;
-                        kap = 0.5 * (o[10 + a] + on[10 + a]);
+                        double Fn = Kab(on, an, 0) * gn[0] + Kab(on, an, 1) * gn[1] + Kab(on, an, 2) * gn[2];
+                        /* average outward flux: the neighbour's outward normal
+                         * is nrm_n along its own axis an */
+                        dF = Fs + nrm * nrm_n * Fn;
+                        kap = 0.5 * (o[10 + a] + on[10 + an]);
                     } else {
                         du = ue[n] - gval;
                     }

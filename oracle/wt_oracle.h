@@ -54,6 +54,11 @@ int wto_gll(int N, double* xi, double* w, double* G);
 /* bg: NULL for Minkowski, else 10 doubles per node, node-major
  * (alpha, beta1, beta2, beta3, g11, g12, g13, g22, g23, g33). */
 wto_ctx* wto_create(const wto_desc* d, const double* bg);
+/* A mesh given by coordinates (3,N,N,N,Ne) and 1-based (6,Ne) neighbour,
+ * neighbour_face and bdry arrays (d's N, tau, bdry_values, sommerfeld_inv_R
+ * apply; eps_ko must be 0).  NULL when faces do not meet. */
+wto_ctx* wto_create_mesh(const wto_desc* d, long Ne, const double* xyz, const int* nbr,
+                         const int* nface, const int* bdry);
 void wto_destroy(wto_ctx* c);
 
 long wto_npoints(const wto_ctx* c);

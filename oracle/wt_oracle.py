@@ -89,6 +89,8 @@ def lib():
         L.wto_gll.argtypes = [ctypes.c_int, dp, dp, dp]
         L.wto_create.argtypes = [ctypes.POINTER(Desc), dp]
         L.wto_create.restype = vp
+        L.wto_create_mesh.argtypes = [ctypes.POINTER(Desc), ctypes.c_long, dp, vp, vp, vp]
+        L.wto_create_mesh.restype = vp
         L.wto_destroy.argtypes = [vp]
         L.wto_npoints.argtypes = [vp]
         L.wto_npoints.restype = ctypes.c_long
@@ -171,6 +173,33 @@ class Oracle:
         self.ctx = lib().wto_create(ctypes.byref(d), _p(self._bg))
         if not self.ctx:
             raise ValueError("wto_create failed")
+
+    @classmethod
+    def from_mesh(cls, N, coords, neighbour, neighbour_face, bdry, tau=None, bdry_values=(0.0,) * 6,
+                  sommerfeld_R=math.inf):
+        """A mesh given by node coordinates (N^3 Ne, 3) and (Ne, 6) 1-based
+        neighbour / neighbour_face / bdry arrays (tag 1..6 Dirichlet, 7
+        Sommerfeld on outer faces)."""
+        self = cls.__new__(cls)
+        d = Desc()
+        d.N = N
+        d.tau = 1.5 * (N - 1) ** 2 if tau is None else tau
+        d.eps_ko = 0.0
+        for f in range(6):
+            d.bdry_values[f] = bdry_values[f]
+        d.sommerfeld_inv_R = 0.0 if math.isinf(sommerfeld_R) else 1.0 / sommerfeld_R
+        X = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+        self.Ne = X.size // (3 * N ** 3)
+        tabs = [np.ascontiguousarray(t, dtype=np.int32).reshape(-1) for t in (neighbour, neighbour_face, bdry)]
+        self.desc, self.N, self.M = d, N, (self.Ne, 1, 1)
+        self.n = N ** 3 * self.Ne
+        self._bg = None
+        self.h = (X, tabs)  # keep alive
+        self.ctx = lib().wto_create_mesh(ctypes.byref(d), self.Ne, _p(X), tabs[0].ctypes.data, tabs[1].ctypes.data,
+                                           tabs[2].ctypes.data)
+        if not self.ctx:
+            raise ValueError("wto_create_mesh failed")
+        return self
 
     def __del__(self):
         if getattr(self, "ctx", None):

@@ -313,3 +313,23 @@ def test_batch_of_patch_meshes_matches_single_runs():
         assert np.array_equal(u.cpu().numpy(), us) and np.array_equal(v.cpu().numpy(), vs)
         assert np.array_equal(E, Es)
     b.close()
+
+
+@pytest.mark.parametrize("kind,prec", [("inflated_som", "fp64"), ("cubed_cube", "fp64"), ("ball", "fp64"),
+                                       ("inflated_som", "mixed")])
+def test_patch_mesh_rhs_matches_oracle(kind, prec):
+    import wt_oracle as wo
+    N = 5
+    mesh = {"inflated_som": lambda: wh.make_inflated_cube_mesh(N, 0.1, 0.3, 1.0, 2, outer_tag=7),
+            "cubed_cube": lambda: wh.make_cubed_cube_mesh(N, 2, 0.3),
+            "ball": lambda: wh.make_cubed_ball_mesh(N, 2)}[kind]()
+    bv = (0.3, -0.2, 0.1, 0.4, -0.5, 0.25)
+    o = wo.Oracle.from_mesh(N, mesh.coords, mesh.neighbour, mesh.neighbour_face, mesh.bdry,
+                            bdry_values=bv, sommerfeld_R=2.0)
+    work = wh.Workspace(mesh, wh.make_element(N), bdry_values=bv, sommerfeld_R=2.0, precision=prec)
+    rng = np.random.default_rng(7)
+    u, ud = rng.standard_normal(o.n), rng.standard_normal(o.n)
+    want = o.lap(u, ud)
+    got = _rhs(work, u, ud, bv)
+    tol = (1e-12 if prec == "fp64" else 2e-5) * np.max(np.abs(want))
+    assert np.max(np.abs(got - want)) < tol

@@ -801,3 +801,67 @@ def test_sommerfeld_on_warped_meshes():
         a = of.lap(u[:of.n], v[:of.n]); b = oc.lap(u[:of.n], v[:of.n])
         d.append(np.max(np.abs(a - b)) / np.max(np.abs(a)))
     assert d[0] < 1e-5 and 8 < d[0] / d[1] < 12
+
+
+def _mesh_oracle(mesh, **kw):
+    return wo.Oracle.from_mesh(mesh.N, mesh.coords, mesh.neighbour, mesh.neighbour_face, mesh.bdry, **kw)
+
+
+def test_oracle_general_connectivity_reduces_to_uniform_operator():
+    # an affine mesh given by coordinates, with its elements renumbered and each
+    # element's axes turned by a symmetry of the cube, is the uniform operator
+    import wavetoy_hip as wh
+    N, M = 4, (3, 2, 2)
+    rng = np.random.default_rng(3)
+    bv = (0.3, -0.2, 0.1, 0.4, -0.5, 0.25)
+    ref = wo.Oracle(N, M, bdry_values=bv)
+    X = ref.coords().reshape(-1, N, N, N, 3)
+    Ne = X.shape[0]
+    perm = rng.permutation(Ne)
+    idx = np.arange(Ne * N ** 3).reshape(Ne, N, N, N)
+    Xn, src = np.empty_like(X), np.empty_like(idx)
+    for en, eo in enumerate(perm):
+        axes = tuple(rng.permutation(3))
+        A, I = np.transpose(X[eo], axes + (3,)), np.transpose(idx[eo], axes)
+        for d in range(3):
+            if rng.integers(0, 2):
+                A, I = np.flip(A, axis=d), np.flip(I, axis=d)
+        Xn[en], src[en] = A, I
+    src = src.reshape(-1)
+    nbr, nface, bdry = wh.connect_by_coords(N, Xn)
+    for e in range(Ne):
+        for f in range(6):
+            if nbr[e, f] == 0:
+                c = wh.core._face_nodes(Xn[e:e + 1], f).reshape(-1, 3).mean(axis=0)
+                a = int(np.argmin(np.minimum(np.abs(c), np.abs(1 - c))))
+                bdry[e, f] = 2 * a + (2 if c[a] > 0.5 else 1)
+    om = wo.Oracle.from_mesh(N, Xn, nbr, nface, bdry, bdry_values=bv)
+    u = rng.standard_normal(ref.n)
+    want, got = ref.lap(u), om.lap(u[src])
+    # the turned elements meet with every D4 code and with like and unlike
+    # sides, so an unmapped neighbour node or a wrong outward sign of the
+    # neighbour's flux breaks this equality
+    assert np.max(np.abs(got - want[src])) < 1e-11 * np.max(np.abs(want))
+
+
+def test_oracle_cubed_cube_rayleigh_and_symmetry():
+    import wavetoy_hip as wh
+    # test/dirichlet_cubed_cube_tests.jl:54-81 (N = 4, M = 2, 4): lambda = 3 pi^2 / 4
+    lam = 3 * math.pi ** 2 / 4
+    errs = []
+    for M in (2, 4):
+        mesh = wh.make_cubed_cube_mesh(4, M, 0.3)
+        o = _mesh_oracle(mesh)
+        x = o.coords()
+        u = np.prod(np.cos(0.5 * math.pi * x), axis=1)
+        H = o.hphys()
+        errs.append(abs(-np.dot(u * H, o.lap(u)) / np.dot(u * H, u) - lam))
+    assert errs[1] < errs[0] / 4 and errs[1] < 1e-2, errs
+    mesh = wh.make_inflated_cube_mesh(3, 0.1, 0.3, 1.0, 1)
+    o = _mesh_oracle(mesh)
+    H = o.hphys()
+    rng = np.random.default_rng(0)
+    u, v = rng.standard_normal(o.n), rng.standard_normal(o.n)
+    a, b = np.dot(v * H, o.lap(u)), np.dot(u * H, o.lap(v))
+    assert abs(a - b) < 1e-11 * math.sqrt(abs(np.dot(u * H, o.lap(u)) * np.dot(v * H, o.lap(v))))
+    assert np.dot(u * H, o.lap(u)) < 0
