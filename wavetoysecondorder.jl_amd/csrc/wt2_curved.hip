@@ -424,7 +424,9 @@ struct VolTables {
     TC w[N], winv[N];
 };
 
-template <int N, typename TC, typename TS, typename TO>
+// GEN: the mesh may carry a connectivity table (a.conn); false compiles the
+// uniform mesh's neighbour arithmetic alone
+template <int N, typename TC, typename TS, typename TO, bool GEN>
 __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t e0, const int nel)
 {
     using S = VolShape<N>;
@@ -530,7 +532,7 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
             int fn, pn = p, qn = qq;
             bool som;
             TC bvf;
-            if (a.conn) {
+            if (GEN && a.conn) {
                 const int2 cf = a.conn[e * 6 + f];
                 en = cf.x;
                 fn = cf.y & 7;
@@ -660,12 +662,12 @@ __global__ void __launch_bounds__(256) curved_faceflux_kernel(CArgs<N, TC> a)
 template <typename TC>
 struct VolWaves { static constexpr int value = std::is_same<TC, float>::value ? 4 : 1; };
 
-template <int N, typename TC, typename TS, typename TO>
+template <int N, typename TC, typename TS, typename TO, bool GEN>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(VolWaves<TC>::value)))
 curved_volume_kernel(CArgs<N, TC> a, int64_t Ne)
 {
     const int64_t e0 = int64_t(blockIdx.x) * VolShape<N>::EW;
-    volume_body<N, TC, TS, TO>(a, e0, int(min<int64_t>(VolShape<N>::EW, Ne - e0)));
+    volume_body<N, TC, TS, TO, GEN>(a, e0, int(min<int64_t>(VolShape<N>::EW, Ne - e0)));
 }
 
 template <int N, typename TC, typename TS>
@@ -684,7 +686,7 @@ curved_volume_batch_kernel(const CArgs<N, TC>* __restrict__ tab, const int2* __r
     const int2 w = work[blockIdx.x];
     const CArgs<N, TC>& a = tab[w.x];
     const int64_t Ne = a.npts / (N * N * N);
-    volume_body<N, TC, TS, TO>(a, w.y, int(min<int64_t>(VolShape<N>::EW, Ne - w.y)));
+    volume_body<N, TC, TS, TO, true>(a, w.y, int(min<int64_t>(VolShape<N>::EW, Ne - w.y)));
 }
 
 template <int N, typename TC>
@@ -723,13 +725,14 @@ static hipError_t launch_c(const CurvedData& cd, const CurvedCall& cc, hipStream
     if (err != hipSuccess) return err;
     using VS = VolShape<N>;
     const size_t shmem = size_t(VS::EW) * VS::PER * sizeof(TC);
-    auto kern = curved_volume_kernel<N, TC, TS, TO>;
-    static bool raised = false;
-    if (shmem + sizeof(VolTables<N, TC>) + 256 > 64 * 1024 && !raised) {
+    const int gen = cd.conn != nullptr;
+    auto kern = gen ? curved_volume_kernel<N, TC, TS, TO, true> : curved_volume_kernel<N, TC, TS, TO, false>;
+    static bool raised[2] = {false, false};
+    if (shmem + sizeof(VolTables<N, TC>) + 256 > 64 * 1024 && !raised[gen]) {
         err = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
         if (err != hipSuccess) return err;
-        raised = true;
+        raised[gen] = true;
     }
     const int64_t nblk = (cd.Ne + VS::EW - 1) / VS::EW;
     hipLaunchKernelGGL(kern, dim3(unsigned(nblk)), dim3(VS::NT), shmem, st, a, int64_t(cd.Ne));
