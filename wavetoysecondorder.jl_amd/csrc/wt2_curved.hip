@@ -678,7 +678,7 @@ __global__ void __launch_bounds__(256) curved_faceflux_batch_kernel(const CArgs<
     faceflux_body<N, TC, TS>(tab[w.x], w.y);
 }
 
-template <int N, typename TC, typename TS, typename TO>
+template <int N, typename TC, typename TS, typename TO, bool GEN>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(VolWaves<TC>::value)))
 curved_volume_batch_kernel(const CArgs<N, TC>* __restrict__ tab, const int2* __restrict__ work)
 {
@@ -686,7 +686,7 @@ curved_volume_batch_kernel(const CArgs<N, TC>* __restrict__ tab, const int2* __r
     const int2 w = work[blockIdx.x];
     const CArgs<N, TC>& a = tab[w.x];
     const int64_t Ne = a.npts / (N * N * N);
-    volume_body<N, TC, TS, TO, true>(a, w.y, int(min<int64_t>(VolShape<N>::EW, Ne - w.y)));
+    volume_body<N, TC, TS, TO, GEN>(a, w.y, int(min<int64_t>(VolShape<N>::EW, Ne - w.y)));
 }
 
 template <int N, typename TC>
@@ -759,8 +759,10 @@ static hipError_t cbatch_build_t(const CurvedData* const* cds, const CurvedCall*
 {
     std::vector<CArgs<N, TC>> tab(n);
     std::vector<int2> work, work2;
+    plan.gen = 0;
     for (int i = 0; i < n; ++i) {
         make_cargs<N, TC>(*cds[i], ccs[i], tab[i]);
+        plan.gen |= cds[i]->conn != nullptr;
         for (int64_t e = 0; e < cds[i]->Ne; ++e) work.push_back(make_int2(i, int(e)));
         for (int64_t e = 0; e < cds[i]->Ne; e += VolShape<N>::EW) work2.push_back(make_int2(i, int(e)));
     }
@@ -807,13 +809,14 @@ static hipError_t cbatch_launch_t(const CurvedBatchPlan& plan, hipStream_t st)
     if (err != hipSuccess) return err;
     using VS = VolShape<N>;
     const size_t shmem = size_t(VS::EW) * VS::PER * sizeof(TC);
-    auto kern = curved_volume_batch_kernel<N, TC, TS, TO>;
-    static bool raised = false;
-    if (shmem + sizeof(VolTables<N, TC>) + 256 > 64 * 1024 && !raised) {
+    const int gen = plan.gen != 0;
+    auto kern = gen ? curved_volume_batch_kernel<N, TC, TS, TO, true> : curved_volume_batch_kernel<N, TC, TS, TO, false>;
+    static bool raised[2] = {false, false};
+    if (shmem + sizeof(VolTables<N, TC>) + 256 > 64 * 1024 && !raised[gen]) {
         err = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
         if (err != hipSuccess) return err;
-        raised = true;
+        raised[gen] = true;
     }
     hipLaunchKernelGGL(kern, dim3(plan.nwork2), dim3(VS::NT), shmem, st, tab, plan.work2);
     return hipGetLastError();

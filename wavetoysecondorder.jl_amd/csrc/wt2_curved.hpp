@@ -53,6 +53,7 @@ struct CurvedCall {
 // element of every instance in the group (same N and precision).
 struct CurvedBatchPlan {
     int N = 0, kind = 0, nwork = 0, nwork2 = 0;
+    int gen = 0;                // some instance has a connectivity table
     void* tab = nullptr;
     int2* work = nullptr;       // (instance, element): face-flux kernel
     int2* work2 = nullptr;      // (instance, first element of a block): element kernel
