@@ -578,7 +578,7 @@ def patch_run(wh, torch, args):
             "fp64": res["fp64"], "mixed": res["mixed"],
             "mixed_speedup": res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"],
             "cpu_baseline": None,
-            "cpu_baseline_note": "the CPU oracle has no general-connectivity restatement"}
+            "cpu_baseline_note": "timed on the headline (plain) configuration only"}
 
 
 def mixed_error_run(wh, torch, args):
