@@ -175,6 +175,12 @@ static void apply_D_line1d(const wto_ctx* c, int M, int periodic, double h,
 
 static wto_ctx* create_impl(const wto_desc* d, const double* bg, const double* mesh_xyz);
 
+/* Planted mistake in the ADM coefficients (0 = none), for the pin tests that
+ * must show they catch it; read when a context is created. */
+static int g_plant = 0;
+
+void wto_plant_mistake(int code) { g_plant = code; }
+
 wto_ctx* wto_create(const wto_desc* d, const double* bg)
 {
     return create_impl(d, bg, NULL);
@@ -335,9 +341,9 @@ static wto_ctx* create_impl(const wto_desc* d, const double* bg, const double* m
              * c^i = sqrt(g) b^i / alpha; pulled back to the unit element. */
             double K[3][3], cv[3];
             for (int p = 0; p < 3; ++p) {
-                cv[p] = sg * be[p] / al;
+                cv[p] = (g_plant == WTO_PLANT_FLIP_C ? -1.0 : 1.0) * sg * be[p] / al;
                 for (int q = 0; q < 3; ++q)
-                    K[p][q] = sg * (al * gu[p][q] - be[p] * be[q] / al);
+                    K[p][q] = sg * (al * gu[p][q] - (g_plant == WTO_PLANT_DROP_BB ? 0.0 : be[p] * be[q] / al));
             }
             double Kh[3][3], ch[3];
             for (int a = 0; a < 3; ++a) {
@@ -354,7 +360,7 @@ static wto_ctx* create_impl(const wto_desc* d, const double* bg, const double* m
             o[0] = Kh[0][0]; o[1] = Kh[0][1]; o[2] = Kh[0][2];
             o[3] = Kh[1][1]; o[4] = Kh[1][2]; o[5] = Kh[2][2];
             o[6] = ch[0]; o[7] = ch[1]; o[8] = ch[2];
-            o[9] = al / (sg * aJ);
+            o[9] = g_plant == WTO_PLANT_WGT_SQRTG ? al * sg / aJ : al / (sg * aJ);
             for (int a = 0; a < 3; ++a) {
                 double kap = Kh[a][a];
                 for (int b = 0; b < 3; ++b) if (b != a) kap += fabs(Kh[a][b]);

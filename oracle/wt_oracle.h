@@ -103,6 +103,17 @@ void wto_fo_rhs(const wto_ctx* c, const double* Phi, const double* Pi, const dou
                 double eps_ko, const wto_bc3d* bc, double* dPhi, double* dPi);
 double wto_fo_energy(const wto_ctx* c, const double* Phi, const double* Pi, const double* coef11);
 
+/* Plant a known mistake in the ADM coefficients of contexts created after the
+ * call (0 restores the correct formulas).  Only for pin tests, which must show
+ * that they fail on each of these:
+ *   WTO_PLANT_DROP_BB    K^ij = sqrt(g) alpha g^ij (the -b^i b^j / alpha term dropped),
+ *   WTO_PLANT_WGT_SQRTG  wgt = alpha sqrt(g) / |J| instead of alpha / (sqrt(g) |J|),
+ *   WTO_PLANT_FLIP_C     c^i = -sqrt(g) b^i / alpha. */
+#define WTO_PLANT_DROP_BB   1
+#define WTO_PLANT_WGT_SQRTG 2
+#define WTO_PLANT_FLIP_C    3
+void wto_plant_mistake(int code);
+
 /* Dense 1D operators on a line of M elements (for spectrum / symmetry pins).
  * kind 0: SIPG Laplacian with homogeneous Dirichlet (or periodic) ends,
  * kind 1: SAT-coupled first derivative D (one-sided ends), kind 2: the KO

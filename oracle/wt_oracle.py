@@ -113,6 +113,7 @@ def lib():
         L.wto_fo_rhs.argtypes = [vp, dp, dp, dp, ctypes.c_double, ctypes.POINTER(BC3d), dp, dp]
         L.wto_fo_energy.argtypes = [vp, dp, dp, dp]
         L.wto_fo_energy.restype = ctypes.c_double
+        L.wto_plant_mistake.argtypes = [ctypes.c_int]
         L.wto_assemble_1d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, dp]
         _lib = L
@@ -131,6 +132,25 @@ def gll(N):
     if lib().wto_gll(N, _p(xi), _p(w), _p(G)) != 0:
         raise ValueError("N out of range")
     return xi, w, G
+
+
+PLANT_DROP_BB, PLANT_WGT_SQRTG, PLANT_FLIP_C = 1, 2, 3
+
+
+class planted_mistake:
+    """Context manager: oracle contexts created inside it carry a known mistake
+    in their ADM coefficients (PLANT_*), so a pin test can show it catches it."""
+
+    def __init__(self, code):
+        self.code = int(code)
+
+    def __enter__(self):
+        lib().wto_plant_mistake(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib().wto_plant_mistake(0)
+        return False
 
 
 def assemble_1d(N, M, periodic, tau, h, kind):

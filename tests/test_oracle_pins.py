@@ -423,6 +423,143 @@ def test_curved_energy_conserved_without_ko():
     assert drift[1] < drift[0] / 16
 
 
+# ------------------------------------------------------------------ ADM coefficients against the PDE
+
+_TP = 2 * math.pi
+
+
+def _adm_bg(x):
+    """A smooth periodic static background with a varying lapse, a shift of
+    |beta| ~ 0.25 and an off-diagonal metric (subluminal: K stays SPD)."""
+    X, Y, Z = x[..., 0], x[..., 1], x[..., 2]
+    al = 1 + 0.2 * np.sin(_TP * X) * np.cos(_TP * Y)
+    be = 0.25 * np.stack([np.cos(_TP * Y), 0.6 * np.sin(_TP * Z), -0.8 * np.cos(_TP * (X + Z))], -1)
+    g = np.empty(X.shape + (3, 3))
+    g[..., 0, 0] = 1.2 + np.sin(_TP * Z) / 5
+    g[..., 0, 1] = g[..., 1, 0] = np.cos(_TP * X) / 10
+    g[..., 0, 2] = g[..., 2, 0] = np.sin(_TP * (Y + Z)) / 20
+    g[..., 1, 1] = 1 + 0.15 * np.cos(_TP * (X - Y))
+    g[..., 1, 2] = g[..., 2, 1] = -0.07 * np.sin(_TP * X)
+    g[..., 2, 2] = 0.9 + np.sin(_TP * Y) / 10
+    return al, be, g
+
+
+def _adm_bg10(x):
+    al, be, g = _adm_bg(x)
+    return np.concatenate([al[:, None], be, g[:, 0, 0:3], g[:, 1, 1:3], g[:, 2, 2:3]], axis=1).reshape(-1)
+
+
+def _adm_fields(x):
+    """Periodic closed-form u, u_t and their gradients."""
+    X, Y, Z = x[..., 0], x[..., 1], x[..., 2]
+    u = np.sin(_TP * X + 1) * np.cos(_TP * Y) + np.sin(_TP * Z)
+    ut = np.cos(_TP * (X + Y + Z)) + 0.5 * np.sin(_TP * (X - Z))
+    gu = np.stack([_TP * np.cos(_TP * X + 1) * np.cos(_TP * Y), -_TP * np.sin(_TP * X + 1) * np.sin(_TP * Y),
+                   _TP * np.cos(_TP * Z)], -1)
+    s = -_TP * np.sin(_TP * (X + Y + Z))
+    c = 0.5 * _TP * np.cos(_TP * (X - Z))
+    gut = np.stack([s + c, s, s - c], -1)
+    return u, ut, gu, gut
+
+
+def _adm_K_c(x):
+    """The continuous coefficients of the covariant wave equation
+    d_mu(sqrt(-g) g^{mu nu} d_nu u) = 0 with sqrt(-g) = alpha sqrt(gamma):
+    K^{ij} = sqrt(-g) g^{ij} = sqrt(gamma)(alpha gamma^{ij} - b^i b^j / alpha),
+    c^i = sqrt(-g) g^{ti} = sqrt(gamma) b^i / alpha
+    (src/wave3d_curved.jl:1-12; METHODS.md:142-150)."""
+    al, be, g = _adm_bg(x)
+    gi = np.linalg.inv(g)
+    sg = np.sqrt(np.linalg.det(g))
+    K = sg[..., None, None] * (al[..., None, None] * gi - be[..., :, None] * be[..., None, :] / al[..., None, None])
+    return al, be, sg, K, (sg / al)[..., None] * be
+
+
+def _adm_rhs_exact(x):
+    """u_tt = b^i d_i u_t + (alpha / sqrt(gamma)) d_i(K^{ij} d_j u + c^i u_t), the
+    divergence by 4th-order central differences (step 1e-3) of the flux's closed form."""
+    def flux(y):
+        _, _, _, K, c = _adm_K_c(y)
+        _, ut, gu, _ = _adm_fields(y)
+        return np.einsum("...ij,...j->...i", K, gu) + c * ut[..., None]
+
+    d = 1e-3
+    div = np.zeros(x.shape[0])
+    for i in range(3):
+        e = np.zeros(3)
+        e[i] = d
+        div += (8 * (flux(x + e)[:, i] - flux(x - e)[:, i]) - (flux(x + 2 * e)[:, i] - flux(x - 2 * e)[:, i])) / (12 * d)
+    al, be, sg, _, _ = _adm_K_c(x)
+    _, _, _, gut = _adm_fields(x)
+    return np.sum(be * gut, axis=1) + al / sg * div
+
+
+def _adm_pde_errors(N, Ms, plant=0, warp_A=0.05):
+    errs = []
+    for M in Ms:
+        x = wo.Oracle(N, M, periodic=True, warp_A=warp_A).coords()
+        with wo.planted_mistake(plant):
+            o = wo.Oracle(N, M, periodic=True, warp_A=warp_A, bg=_adm_bg10(x))
+        u, ut, _, _ = _adm_fields(x)
+        got = o.lap(np.ascontiguousarray(u), np.ascontiguousarray(ut))
+        ex = _adm_rhs_exact(x)
+        errs.append(np.max(np.abs(got - ex)) / np.max(np.abs(ex)))
+    return errs
+
+
+def test_adm_second_order_rhs_converges_to_the_pde():
+    """The second-order ADM operator (wto_lap with a background on a periodic
+    warped mesh) against the continuous right-hand side of the covariant wave
+    equation on a static background,
+    u_tt = b^i d_i u_t + (alpha/sqrt(gamma)) [d_i(K^{ij} d_j u) + d_i(c^i u_t)],
+    with lapse, shift and an off-diagonal metric: spectral convergence in M at
+    N = 8.  Each of three plausible coefficient mistakes planted in the oracle
+    (the -b b / alpha term of K dropped, sqrt(gamma) inverted in the weight, the
+    sign of c flipped) leaves an O(1e-2..1) error that does not converge."""
+    ok = _adm_pde_errors(8, (2, 4))
+    assert ok[1] < 5e-3 and ok[0] / ok[1] > 16, ok
+    for plant in (wo.PLANT_DROP_BB, wo.PLANT_WGT_SQRTG, wo.PLANT_FLIP_C):
+        bad = _adm_pde_errors(8, (2, 4), plant)
+        assert bad[1] > 5 * ok[1] and bad[0] / bad[1] < 4, (plant, bad, ok)
+
+
+def _adm_fo_minus_so(N, M, plant=0):
+    o0 = wo.Oracle(N, M, periodic=True)
+    x = o0.coords()
+    bg = _adm_bg10(x)
+    with wo.planted_mistake(plant):
+        so = wo.Oracle(N, M, periodic=True, bg=bg)
+    coef = wo.fo_coef(bg)
+    al, be, g = _adm_bg(x)
+    sg = np.sqrt(np.linalg.det(g))
+    u, ut, _, _ = _adm_fields(x)
+    u = np.ascontiguousarray(u)
+    ut = np.ascontiguousarray(ut)
+    Du = np.stack([o0.apply_D(a, u) for a in range(3)], 1)
+    Dut = np.stack([o0.apply_D(a, ut) for a in range(3)], 1)
+    Pi = np.ascontiguousarray(sg / al * (ut - np.sum(be * Du, 1)))
+    _, dPi = o0.fo_rhs(u, Pi, coef, eps_ko=0.0)
+    utt_fo = np.sum(be * Dut, 1) + al / sg * dPi
+    utt_so = so.lap(u, ut)
+    return np.max(np.abs(utt_fo - utt_so)) / np.max(np.abs(utt_so))
+
+
+def test_adm_second_order_equals_first_order_form():
+    """The second-order ADM operator against the reference's own first-order
+    system (wave3d_curved_rhs!, src/wave3d_curved.jl:205-258, pinned to the PDE
+    by test_fo_rhs_converges_to_the_pde): with Phi = u and
+    Pi = (sqrt(gamma)/alpha)(u_t - b^i D_i u), u_tt = b^i D_i u_t +
+    (alpha/sqrt(gamma)) dPi.  On a periodic affine mesh and data continuous across
+    element faces (a smooth function at the GLL nodes) the SIPG jump terms
+    vanish and the centred-flux SAT of apply_D! carries the same face term as
+    SIPG's flux average, so the two forms agree to rounding; each planted
+    coefficient mistake breaks that by 4e-2 or more."""
+    for M in (2, 3):
+        assert _adm_fo_minus_so(8, M) < 1e-11
+    for plant in (wo.PLANT_DROP_BB, wo.PLANT_WGT_SQRTG, wo.PLANT_FLIP_C):
+        assert _adm_fo_minus_so(8, 2, plant) > 1e-2, plant
+
+
 # ------------------------------------------------------------------ Sommerfeld faces
 
 def _face_measure(o, f, faces):
