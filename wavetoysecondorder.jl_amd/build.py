@@ -19,12 +19,34 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    cmd = [HIPCC] + FLAGS + ["-o", OUT + ".tmp"] + srcs
+def build(verbose=False, jobs=None):
+    """Compile each source to an object in parallel (they share no device
+    symbols), then link the shared library."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in FLAGS if f != "-shared"]
+    cmds, objs = [], []
+    for src in SOURCES:
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        objs.append(obj)
+        cmds.append([HIPCC] + cflags + ["-c", "-o", obj, os.path.join(CSRC, src)])
+    jobs = jobs or min(len(cmds), os.cpu_count() or 1)
+    running = []
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        running.append(subprocess.Popen(cmd))
+        while len(running) >= jobs:
+            p = running.pop(0)
+            if p.wait() != 0:
+                raise subprocess.CalledProcessError(p.returncode, p.args)
+    for p in running:
+        if p.wait() != 0:
+            raise subprocess.CalledProcessError(p.returncode, p.args)
+    link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp"] + objs
     if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+        print(" ".join(link), flush=True)
+    subprocess.run(link, check=True)
     os.replace(OUT + ".tmp", OUT)
     return OUT
 
