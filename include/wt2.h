@@ -163,6 +163,24 @@ int wt2_rhs_full(wt2_ctx* ctx, double* du, double* dud, const double* u,
 int wt2_rk4_steps(wt2_ctx* ctx, double* u, double* ud, double dt, int32_t nsteps,
                   int32_t energy_every, double* energy_out, int32_t flags);
 
+/* No counterpart in the reference: wt2_rk4_steps for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same steps, kernels and results as
+ * wt2_rk4_steps on device pointers, but it allocates nothing: the context's
+ * workspace (allocated by wt2_ctx_create) holds the stage registers, and the
+ * energy history and its reduction partials live in a fixed device array of
+ * the library, sized for WT2_NOALLOC_MAX_ENERGIES samples (about 0.5 MiB,
+ * held by every GPU the library runs on until the process ends); more samples
+ * are refused with WT2_ERR_ARG.  Kernels are launched one by one (no graph is
+ * instantiated).  u, ud are device pointers (16-byte aligned); energy_out is a
+ * host array as in wt2_rk4_steps.  A curved context must have its
+ * coefficients computed first (any wt2_ctx_info call does it), else
+ * WT2_ERR_STATE.  All calls on one GPU share the one array: calls that overlap
+ * in time on one GPU (from several host threads) race. */
+#define WT2_NOALLOC_MAX_ENERGIES 65536
+int wt2_rk4_steps_noalloc(wt2_ctx* ctx, double* u, double* ud, double dt, int32_t nsteps,
+                          int32_t energy_every, double* energy_out);
+
 /* Discrete energy E = 1/2<ud,ud>_W + 1/2<u,-L u>_H (zero boundary data)
  * (test/kernels3d_tests.jl:32-47; src/wave3d_curved.jl:268-287), FP64,
  * deterministic reduction order. */

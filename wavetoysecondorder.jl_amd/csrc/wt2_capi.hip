@@ -502,7 +502,8 @@ static int run_stage(wt2_ctx* c, int stage, const double* u, const double* v, do
     return WT2_OK;
 }
 
-static int run_energy(wt2_ctx* c, const double* u, const double* v, double* Edst)
+// partial: the reduction's RED_BLOCKS partial sums (nullptr: the context's own).
+static int run_energy(wt2_ctx* c, const double* u, const double* v, double* Edst, double* partial = nullptr)
 {
     static const double zero6[6] = {0, 0, 0, 0, 0, 0};
     if (c->curved) {
@@ -523,7 +524,7 @@ static int run_energy(wt2_ctx* c, const double* u, const double* v, double* Edst
     e.wv = c->curved ? c->cd.wv : nullptr;
     e.hvol = c->curved ? 1.0 : c->h[0] * c->h[1] * c->h[2];
     for (int i = 0; i < c->N; ++i) e.w[i] = c->w[i];
-    e.partial = c->red;
+    e.partial = partial ? partial : c->red;
     e.out = Edst;
     return pass_rc(launch_energy(e, c->stream), "energy");
 }
@@ -1254,6 +1255,92 @@ int wt2_rk4_steps(wt2_ctx* c, double* u, double* ud, double dt, int32_t nsteps,
                 return WT2_ERR_NONFINITE;
             }
     }
+    return WT2_OK;
+}
+
+// ---- the entry point that allocates nothing (wt2_rk4_steps_noalloc) ----
+//
+// Its energy history and the reduction partials live in `scratch`, a fixed
+// array of this code object that every GPU holds from the library's load until
+// the process ends.  Host code never passes `scratch` itself to a kernel or a
+// copy: the array's device address, per device, comes from
+// hipGetSymbolAddress on the first call on that device.
+constexpr int NOALLOC_SCRATCH = RED_BLOCKS + 8 + WT2_NOALLOC_MAX_ENERGIES;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on `dev`, looked up once per device.
+int scratch_on(int dev, double** out)
+{
+    if (dev < 0 || dev >= MAX_DEVICES) { set_error("wt2_rk4_steps_noalloc: device index out of range"); return WT2_ERR_ARG; }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = -1;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != dev) HIPCHK(hipSetDevice(dev));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) {
+            set_error(std::string("wt2_rk4_steps_noalloc: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+            return WT2_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return WT2_OK;
+}
+}  // namespace
+
+int wt2_rk4_steps_noalloc(wt2_ctx* c, double* u, double* ud, double dt, int32_t nsteps,
+                          int32_t energy_every, double* energy_out)
+{
+    if (!c || !u || !ud) { set_error("wt2_rk4_steps_noalloc: null pointer"); return WT2_ERR_ARG; }
+    if (nsteps < 0 || !(dt > 0) || !std::isfinite(dt)) { set_error("wt2_rk4_steps_noalloc: bad dt / nsteps"); return WT2_ERR_ARG; }
+    if (misaligned(u) || misaligned(ud)) { set_error("wt2_rk4_steps_noalloc: device pointers must be 16-byte aligned"); return WT2_ERR_ARG; }
+    if (c->curved && c->cd.pending) {
+        set_error("wt2_rk4_steps_noalloc: the context's curved coefficients are not computed yet "
+                  "(call wt2_ctx_info once after creating it)");
+        return WT2_ERR_STATE;
+    }
+    const bool want_e = energy_out && energy_every > 0;
+    const int64_t ne = want_e ? 1 + int64_t(nsteps) / energy_every : 0;
+    if (ne > WT2_NOALLOC_MAX_ENERGIES) {
+        set_error("wt2_rk4_steps_noalloc: " + std::to_string(ne) + " energy samples exceed the maximum of " +
+                  std::to_string(WT2_NOALLOC_MAX_ENERGIES) + " (WT2_NOALLOC_MAX_ENERGIES)");
+        return WT2_ERR_ARG;
+    }
+    double* sp = nullptr;
+    int rc = scratch_on(c->device, &sp);
+    if (rc) return rc;
+    double* partial = sp;
+    double* Ehist = sp + RED_BLOCKS + 8;
+    if (want_e) {
+        rc = run_energy(c, u, ud, Ehist, partial);
+        if (rc) return rc;
+    }
+    rc = run_prologue(c, u, ud);
+    if (rc) return rc;
+    for (int st = 0; st < nsteps; ++st) {
+        rc = rk4_step_launches(c, u, ud, dt);
+        if (rc) return rc;
+        if (want_e && (st + 1) % energy_every == 0) {
+            rc = run_energy(c, u, ud, Ehist + (st + 1) / energy_every, partial);
+            if (rc) return rc;
+        }
+    }
+    if (want_e) HIPCHK(hipMemcpyAsync(energy_out, Ehist, size_t(ne) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    rc = sync(c);
+    if (rc) return rc;
+    for (int64_t q = 0; q < ne; ++q)
+        if (!std::isfinite(energy_out[q])) {
+            set_error("wt2_rk4_steps_noalloc: non-finite energy (the evolution blew up)");
+            return WT2_ERR_NONFINITE;
+        }
     return WT2_OK;
 }
 

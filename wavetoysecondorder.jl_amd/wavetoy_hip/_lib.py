@@ -29,7 +29,7 @@ WT2_NO_GRAPH = 4
 #: every entry point include/wt2.h declares
 EXPORTS = (
     "wt2_gll", "wt2_ctx_create", "wt2_ctx_create_mesh", "wt2_ctx_footprint", "wt2_ctx_destroy", "wt2_ctx_info", "wt2_ctx_set_background",
-    "wt2_coords", "wt2_rhs3d", "wt2_rhs_full", "wt2_rk4_steps", "wt2_energy", "wt2_set_stream",
+    "wt2_coords", "wt2_rhs3d", "wt2_rhs_full", "wt2_rk4_steps", "wt2_rk4_steps_noalloc", "wt2_energy", "wt2_set_stream",
     "wt2_batch_create", "wt2_batch_destroy", "wt2_batch_add", "wt2_batch_remove", "wt2_batch_step",
     "wt2_batch_step_async", "wt2_batch_sync",
     "wt2_batch_slot", "wt2_ctx_autotune", "wt2_ctx_tiles", "wt2_ctx_set_tiles", "wt2_launch_pass", "wt2_last_error", "wt2_version",
@@ -120,6 +120,7 @@ def lib():
         L.wt2_rhs_full.argtypes = [vp, dp, dp, dp, dp, ctypes.c_int32]
         L.wt2_rk4_steps.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
                                     dp, ctypes.c_int32]
+        L.wt2_rk4_steps_noalloc.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, dp]
         L.wt2_energy.argtypes = [vp, dp, dp, dp, ctypes.c_int32]
         L.wt2_set_stream.argtypes = [vp, vp]
         L.wt2_launch_pass.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, dp, dp, ctypes.c_double]
