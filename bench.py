@@ -28,17 +28,23 @@ TAU_LITERAL = 20.0
 PEAK_HBM_GBS = 8000.0
 
 # Compulsory HBM bytes per grid point of each pencil pass of one RK4 step
-# (reads + writes of the fields the pass must touch), unfolded schedule:
+# (reads + writes of the fields the pass must touch), unfolded schedule, RK4
+# in Horner form (stage s writes S = Y + c_s dt k, stage 4 writes Y):
 #   fp64: the first pass reads S(u,u_t) 16 + writes P 8; the middle one adds a
-#   P read 8; the final (RK) pass at stage 1 reads Y 16 + P 8, writes S 16 +
-#   A 16; stages 2,3 read S 16 + P 8 + Y 16 + A 16 and write S 16 + A 16;
-#   stage 4 reads S 16 + P 8 + A 16, writes Y 16.
+#   P read 8; the final (RK) pass at stage 1 reads Y 16 + P 8, writes S 16;
+#   stages 2,3 read S 16 + P 8 + Y 16 and write S 16; stage 4 reads S 16 +
+#   P 8 + Y 16, writes Y 16.
 PASS_BYTES = {
-    "fp64": {"first": [24, 24, 24, 24], "mid": [32, 32, 32, 32], "final": [56, 88, 88, 56]},
-    #   mixed: S, P and the RK increment in FP32; Y in FP64 (stage 1 reads Y).
-    "mixed": {"first": [20, 12, 12, 12], "mid": [24, 16, 16, 16], "final": [36, 52, 52, 52]},
+    "fp64": {"first": [24, 24, 24, 24], "mid": [32, 32, 32, 32], "final": [40, 56, 56, 56]},
+    #   mixed: S and P in FP32; Y in FP64 (stage 1 reads Y).
+    "mixed": {"first": [20, 12, 12, 12], "mid": [24, 16, 16, 16], "final": [28, 36, 36, 44]},
 }
-FUSED_MIN_BYTES = {"fp64": 256, "mixed": 208}   # whole-step minimum (3-register RK4)
+# Whole-step minimum of the classical 3-register stage form (Y, S, A: 16
+# field-pair accesses per point): the algorithmic bytes the roofline uses.
+FUSED_MIN_BYTES = {"fp64": 256, "mixed": 208}
+# Whole-step minimum of the Horner form the kernels run (Y, S: 11 field-pair
+# accesses): stage 1 Y -> S, stages 2, 3 S, Y -> S, stage 4 S, Y -> Y.
+HORNER_MIN_BYTES = {"fp64": 176, "mixed": 128}
 
 
 def step_bytes(prec, launches=12):

@@ -220,7 +220,6 @@ struct wt2_ctx {
     // in a mixed context, where a block is also one FP64 field of scratch)
     void* Sb;       // stage input
     void* Pb;       // partial sums
-    void* Ab;       // RK accumulator (FP64) / increment (FP32)
     double* Y[2];   // device staging for host-pointer calls (lazy)
     double* O[2];   // device staging for host-pointer outputs (lazy)
     double* Lu;     // energy scratch
@@ -426,7 +425,6 @@ static void stage_items(wt2_ctx* c, int stage, const double* u, const double* v,
         e.epi = EPI_RK1 + (stage - 1);
         e.So_u = fld(c, c->Sb, 0); e.So_v = fld(c, c->Sb, 1);
         e.Yu = Yu; e.Yv = Yv;
-        e.Au = fld(c, c->Ab, 0); e.Av = fld(c, c->Ab, 1);
         e.dt = dt;
         it.pass[1] = e;
         it.has_pass[1] = true;
@@ -468,7 +466,6 @@ static void stage_items(wt2_ctx* c, int stage, const double* u, const double* v,
         p.epi = last ? final_epi : (have_p ? EPI_MID : EPI_FIRST);
         p.So_u = fld(c, c->Sb, 0); p.So_v = fld(c, c->Sb, 1);
         p.Yu = Yu; p.Yv = Yv;
-        p.Au = fld(c, c->Ab, 0); p.Av = fld(c, c->Ab, 1);
         p.Ou = Ou; p.Ov = Ov;
         p.dt = dt;
         it.pass[q] = p;
@@ -642,9 +639,9 @@ int wt2_ctx_footprint(const wt2_mesh_desc* desc, int32_t with_background, int64_
     const int64_t n = Ne * N * N * N;
     const int64_t es = desc->precision == WT2_MIXED ? int64_t(sizeof(float)) : int64_t(sizeof(double));
     const int64_t ts = op_table_size(N);
-    // the same allocations as wt2_ctx_create: S, P, A register blocks, Lu,
+    // the same allocations as wt2_ctx_create: S, P register blocks, Lu,
     // reduction partials, operator tables; plus the curved coefficients
-    int64_t b = 3 * (2 * n * es) + n * int64_t(sizeof(double)) + (RED_BLOCKS + 8) * int64_t(sizeof(double)) +
+    int64_t b = 2 * (2 * n * es) + n * int64_t(sizeof(double)) + (RED_BLOCKS + 8) * int64_t(sizeof(double)) +
                 ts * int64_t(sizeof(double) + 2 * sizeof(float));
     if (desc->warp_A != 0.0 || with_background)
         b += 13 * n * int64_t(sizeof(double) + sizeof(float)) + n * int64_t(sizeof(double)) +
@@ -994,7 +991,6 @@ static int ctx_create_impl(const wt2_mesh_desc* desc, const MeshIn* mi, wt2_ctx*
     const size_t blk = 2 * n * elem_size(c);
     int rc = alloc_dev(c, &c->Sb, blk);
     if (rc == WT2_OK) rc = alloc_dev(c, &c->Pb, blk);
-    if (rc == WT2_OK) rc = alloc_dev(c, &c->Ab, blk);
     if (rc == WT2_OK) rc = alloc_dev(c, reinterpret_cast<void**>(&c->Lu), n * sizeof(double));
     if (rc == WT2_OK) rc = alloc_dev(c, reinterpret_cast<void**>(&c->red), (RED_BLOCKS + 8) * sizeof(double));
     if (rc == WT2_OK) {
@@ -1034,7 +1030,7 @@ void wt2_ctx_destroy(wt2_ctx* c)
 {
     if (!c) return;
     if (c->gvalid) hipGraphExecDestroy(c->gexec);
-    hipFree(c->Sb); hipFree(c->Pb); hipFree(c->Ab);
+    hipFree(c->Sb); hipFree(c->Pb);
     for (int f = 0; f < 2; ++f) { hipFree(c->Y[f]); hipFree(c->O[f]); }
     hipFree(c->Lu); hipFree(c->red); hipFree(c->Edev);
     hipFree(c->op64); hipFree(c->op32); hipFree(c->op32c);

@@ -21,7 +21,7 @@ enum Epi : int {
     EPI_MID = 1,      // P += c
     EPI_OUT_RHS = 2,  // (du, dv) = (u_t, P + c)
     EPI_OUT_LAP = 3,  // out = P + c
-    EPI_RK1 = 4,      // classical RK4 stage updates with k = (u_t, P + c)
+    EPI_RK1 = 4,      // RK4 stage updates (Horner form) with k = (u_t, P + c)
     EPI_RK2 = 5,
     EPI_RK3 = 6,
     EPI_RK4 = 7,
@@ -68,7 +68,6 @@ struct PassDesc {
     void* P;                          // partial sum of the u_t equation (variant partial type)
     void* So_u; void* So_v;           // next stage input (partial type)
     double* Yu; double* Yv;           // RK state
-    void* Au; void* Av;               // RK accumulator (FP64) / increment (FP32, mixed)
     double* Ou; double* Ov;           // outputs
     double dt;
     AxisOps ops;
