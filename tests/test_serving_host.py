@@ -78,8 +78,9 @@ def test_footprint_scales_with_points_and_precision(prec):
     curved = wh.ctx_footprint(m, e, prec, with_background=True)
     n = 6 ** 3 * 4 ** 3
     es = 8 if prec == "fp64" else 4
-    # three two-field register blocks + the FP64 Lu field, plus small tables
-    assert 6 * n * es + 8 * n <= flat <= 6 * n * es + 8 * n + 16384
+    # two two-field register blocks (stage input S, partial sums P) + the FP64
+    # Lu field, plus small tables
+    assert 4 * n * es + 8 * n <= flat <= 4 * n * es + 8 * n + 16384
     # curved adds 13 FP64 + 13 FP32 coefficients, the energy weight and the face buffer
     assert curved - flat == 13 * n * 12 + 8 * n + 64 * 6 * 36 * 8
     r = _req(0, 6, 4, curved=True, T=1.0)
