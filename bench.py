@@ -189,6 +189,7 @@ def curved_run(wh, torch, args):
     res = {}
     stream = torch.cuda.Stream()
     for prec in ("fp64", "mixed"):
+        print(f"curved: {prec}", file=sys.stderr, flush=True)
         ws = wh.Workspace(mesh, elem, tau=tau, eps_ko=EPS_KO, precision=prec, bg=bg)
         ws.set_stream(stream.cuda_stream)
         dt = wh.recommended_dt_rk4(ws, 0.2)
@@ -288,6 +289,10 @@ def batch_specs(n, seed, t_final=1.0):
 
 
 def run_batch(wh, torch, specs, precision, chunk=64):
+    """All instances in one wt2_batch, stepped to their ends; returns the live
+    instances (workspaces still open), the point-updates, the wall time of the
+    stepping, the tuned shapes and the launches of one RK4 step with every
+    instance live."""
     insts = [make_instance(wh, torch, sp, precision) for sp in specs]
     shapes = {}
     for (ws, u, v, dt, ns), sp in zip(insts, specs):
@@ -303,42 +308,60 @@ def run_batch(wh, torch, specs, precision, chunk=64):
     work = sum(ws.npoints * ns for ws, u, v, dt, ns in insts)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    fin = 0
+    fin = b.step(1)
+    launches = b.launches_per_step()
     while fin < len(insts):
         fin += b.step(chunk)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     b.close()
-    return insts, work, wall, shapes
+    return insts, work, wall, shapes, launches
 
 
 def batch_run(wh, torch, args):
     """The batched workload: args.instances independent evolutions, per instance
     M ~ U{2..12} elements per side and N ~ U{4,6,8,10}, half flat, half warped
     with the ADM bump background, Gaussian pulse (centre U[0.3,0.7]^3, width
-    U[0.05,0.2], amplitude 1), RK4 with dt = 0.2 h_min to t = 1 (or --t-final)."""
+    U[0.05,0.2], amplitude 1), RK4 with dt = 0.2 h_min to t = 1 (or --t-final).
+    FP64 first, then mixed; every instance's mixed result against its FP64 one
+    (relative error in the energy norm of the u_t term, which is the H norm,
+    and in the max norm)."""
     T = 1.0 if args.t_final is None else args.t_final
     specs = batch_specs(args.instances, args.seed, T)
     out = {"config": {"workload": f"{len(specs)} instances, M~U{{2..12}}, N~U{{4,6,8,10}}, half warped+ADM, "
-                                  f"Gaussian pulses, RK4 dt=0.2 h_min to t={T}", "seed": args.seed}}
-    res = {}
-    for prec in ("mixed", "fp64"):
-        insts, work, wall, shapes = run_batch(wh, torch, specs, prec)
-        res[prec] = [i[1].cpu().numpy() for i in insts[:16]]
+                                  f"Gaussian pulses, RK4 dt=0.2 h_min to t={T}, one wt2_batch per precision "
+                                  f"(hipGraph per step)", "seed": args.seed}}
+    u64 = []
+    for prec in ("fp64", "mixed"):
+        print(f"batch: {len(specs)} instances, {prec}", file=sys.stderr, flush=True)
+        insts, work, wall, shapes, launches = run_batch(wh, torch, specs, prec)
         out[prec] = {"point_updates": work, "wall_s": wall, "value": work / wall,
                      "total_points": sum(i[0].npoints for i in insts),
-                     "distinct_shapes_tuned": len(shapes)}
-        if prec == "mixed":
+                     "distinct_shapes_tuned": len(shapes), "launches_per_step_all_live": launches}
+        print(f"batch: {prec} {work / wall:.3e} point-updates/s, {wall:.1f} s, "
+              f"{launches} launches per step", file=sys.stderr, flush=True)
+        if prec == "fp64":
+            u64 = [i[1].cpu().numpy() for i in insts]
+        else:
             out["tiles_example"] = {f"N{k[0]}M{k[1]}{'c' if k[2] else 'f'}": v for k, v in list(shapes.items())[:6]}
+            hn, mx = [], []
+            for (ws, u, v, dt, ns), ref in zip(insts, u64):
+                r = torch.from_numpy(ref).cuda()
+                d = (u - r).contiguous()
+                z = torch.zeros_like(d)
+                hn.append(math.sqrt(ws.energy(z, d) / ws.energy(z, r)))
+                mx.append(float((d.abs().max() / r.abs().max()).item()))
+            w = int(np.argmax(hn))
+            out["mixed_vs_fp64_per_instance"] = {
+                "hnorm_rel_err_max": max(hn), "hnorm_rel_err_median": float(np.median(hn)),
+                "max_rel_err_max": max(mx), "max_rel_err_median": float(np.median(mx)),
+                "worst_instance": {"N": specs[w][0], "M": specs[w][1], "warped": bool(specs[w][2]),
+                                   "width": specs[w][4]}}
         for i in insts:
             i[0].close()
         del insts
         torch.cuda.empty_cache()
-    errs = []
-    for a, b, sp in zip(res["mixed"], res["fp64"], specs[:16]):
-        errs.append(float(np.sqrt(np.sum((a - b) ** 2) / max(np.sum(b ** 2), 1e-300))))
     out["mixed_speedup"] = out["mixed"]["value"] / out["fp64"]["value"]
-    out["mixed_rel_l2_err_vs_fp64_first16"] = {"max": max(errs), "median": float(np.median(errs))}
     return out
 
 
@@ -349,9 +372,9 @@ def serving_run(wh, torch, args, rates=None):
     requests one at a time through the plain FP64 single-instance path, whose
     latencies follow from a FIFO queue on the same arrival times."""
     from wavetoy_hip import serving as sv
-    rates = rates or [float(r) for r in args.rates.split(",")]
     reqs = sv.draw_requests(args.requests, None, args.seed)
     work = sum(r.npoints for r in reqs)
+    mean_work = float(np.mean([sv.request_work_estimate(r) for r in reqs]))
     srv = sv.Server(budget_bytes=200e9, precision="mixed", energy_every=20)
     t0 = time.perf_counter()
     srv.prewarm(reqs, verbose=True)
@@ -359,11 +382,23 @@ def serving_run(wh, torch, args, rates=None):
     print(f"serving: {len(reqs)} requests, {work} points, tuned in {prewarm_s:.1f} s", file=sys.stderr, flush=True)
     out = {"config": {"workload": f"{len(reqs)} requests drawn like the batched workload, T~U[0.5,2], "
                                   "Poisson arrivals; continuous batching, mixed precision, 200 GB budget",
-                      "seed": args.seed, "requests": len(reqs), "tile_tuning_s_before_start": prewarm_s}}
+                      "seed": args.seed, "requests": len(reqs), "tile_tuning_s_before_start": prewarm_s,
+                      "mean_request_point_updates": mean_work}}
     fp64_resp, service, work64 = sv.run_one_at_a_time(reqs, "fp64", energy_every=20)
     out["one_at_a_time_fp64"] = {"point_updates": work64, "busy_s": float(sum(service)),
-                                 "throughput_point_updates_per_s": work64 / sum(service)}
+                                 "throughput_point_updates_per_s": work64 / sum(service),
+                                 "saturation_rate_per_s": len(reqs) / sum(service)}
     print(f"serving: one-at-a-time FP64 {sum(service):.1f} s", file=sys.stderr, flush=True)
+    if rates is None and args.rates == "auto":
+        # saturation of the continuous batch: one closed-loop run with every
+        # request present at t = 0 gives the batch's own service rate
+        st = srv.run(sv.with_arrivals(reqs, 1e9, args.seed))
+        sat = len(reqs) / st.wall
+        out["saturation"] = {"rate_per_s": sat, "closed_run_wall_s": st.wall,
+                             "throughput_point_updates_per_s": st.summary()["throughput_point_updates_per_s"]}
+        print(f"serving: saturation {sat:.2f} requests/s", file=sys.stderr, flush=True)
+        rates = [round(f * sat, 2) for f in (0.25, 0.5, 0.8)] + [2.0 * round(sat, 2)]
+    rates = rates or [float(r) for r in args.rates.split(",")]
     by_rid = {r.rid: r for r in fp64_resp}
     out["rates"] = {}
     for lam in rates:
@@ -388,7 +423,8 @@ def serving_run(wh, torch, args, rates=None):
         summ["mixed_vs_fp64_rel_l2_err_max"] = float(np.max(errs))
         summ["mixed_vs_fp64_rel_l2_err_median"] = float(np.median(errs))
         summ["mixed_vs_fp64_nonfinite"] = int(np.sum(~np.isfinite(errs)))
-        out["rates"][str(int(lam))] = summ
+        summ["rate_per_s"] = lam
+        out["rates"][f"{lam:g}"] = summ
         print(f"serving: rate {lam}: p50 {summ['p50_latency_s']:.3f} s p99 {summ['p99_latency_s']:.3f} s "
               f"{summ['throughput_point_updates_per_s']:.3e} pt-upd/s", file=sys.stderr, flush=True)
     return out
@@ -634,6 +670,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="plain config: skip the curved, batch and serving legs")
+    ap.add_argument("--extra-requests", type=int, default=48, help="plain config: requests of the serving leg")
     ap.add_argument("--repeats", type=int, default=5,
                     help="plain config: further runs of --steps steps after the timed one (median, minimum)")
     ap.add_argument("--no-tune", action="store_true", help="default tile shapes instead of the autotuner's")
@@ -646,7 +685,9 @@ def main():
     ap.add_argument("--requests", type=int, default=64, help="serving config: requests per rate")
     ap.add_argument("--no-sweep", action="store_true",
                     help="passes config: only the default tiles, each pass launched 3 times (for counter runs)")
-    ap.add_argument("--rates", default="50,100,200,300,500", help="serving config: arrival rates (1/s)")
+    ap.add_argument("--rates", default="auto",
+                    help="serving config: arrival rates (1/s), comma-separated, or 'auto' (fractions of the "
+                         "measured saturation rate)")
     ap.add_argument("--t-final", type=float, default=None,
                     help="curved config: evolve to this time (default: --steps steps)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -771,6 +812,29 @@ def main():
         del rmx
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(u0, v0, r64["dt"])
+    del r64
+    torch.cuda.empty_cache()
+    if not args.no_extra:
+        # the other workloads of the metric, each in its own key: the curved
+        # ADM step to t = 2 (FP64 and mixed, error at t = 2), the 512-instance
+        # batch (FP64 and mixed, per-instance error), the serving loop
+        # (saturation and p50 / p99 at rates below and above it)
+        ns = argparse.Namespace(**vars(args))
+        ns.t_final = 2.0
+        t0 = time.perf_counter()
+        print("extra: curved 24^3 x 10^3 to t = 2", file=sys.stderr, flush=True)
+        out["curved_t2"] = curved_run(wh, torch, ns)
+        out["curved_t2"]["leg_wall_s"] = time.perf_counter() - t0
+        ns.t_final = 1.0
+        ns.instances = 512
+        t0 = time.perf_counter()
+        out["batch_512"] = batch_run(wh, torch, ns)
+        out["batch_512"]["leg_wall_s"] = time.perf_counter() - t0
+        ns.requests = args.extra_requests
+        ns.rates = "auto"
+        t0 = time.perf_counter()
+        out["serving"] = serving_run(wh, torch, ns)
+        out["serving"]["leg_wall_s"] = time.perf_counter() - t0
     print(json.dumps(out), flush=True)
 
 

@@ -266,6 +266,11 @@ int wt2_batch_sync(wt2_batch* b);
 int wt2_batch_slot(const wt2_batch* b, int32_t slot, int64_t* steps_done, double* energies,
                    int32_t cap, int32_t* n_energies);
 
+/* Kernel launches of one RK4 step of the batch as last planned by a step call
+ * (0 before the first step and after the set of unfinished instances changed),
+ * and the number of unfinished instances.  No counterpart in the reference. */
+int wt2_batch_info(const wt2_batch* b, int32_t* launches_per_step, int32_t* active);
+
 /* Autotuner (no counterpart in the reference): times every valid pass-kernel
  * kind and tile size per axis on this context with state (u, ud) (device
  * pointers, only read) and keeps the fastest; the choice is cached per

@@ -1958,4 +1958,20 @@ int wt2_batch_slot(const wt2_batch* b, int32_t slot, int64_t* steps_done, double
     return WT2_OK;
 }
 
+int wt2_batch_info(const wt2_batch* b, int32_t* launches_per_step, int32_t* active)
+{
+    if (!b || !launches_per_step || !active) { set_error("wt2_batch_info: null pointer"); return WT2_ERR_ARG; }
+    int32_t n = 0;
+    if (!b->dirty)
+        for (const auto& sp : b->plans) {
+            n += int32_t(sp.curved.size());
+            for (int q = 0; q < 3; ++q) n += int32_t(sp.flat[q].size());
+        }
+    *launches_per_step = n;
+    int32_t a = 0;
+    for (const auto& s : b->slots) a += batch_active(s) ? 1 : 0;
+    *active = a;
+    return WT2_OK;
+}
+
 }  // extern "C"

@@ -32,7 +32,7 @@ EXPORTS = (
     "wt2_coords", "wt2_rhs3d", "wt2_rhs_full", "wt2_rk4_steps", "wt2_rk4_steps_noalloc", "wt2_energy", "wt2_set_stream",
     "wt2_batch_create", "wt2_batch_destroy", "wt2_batch_add", "wt2_batch_remove", "wt2_batch_step",
     "wt2_batch_step_async", "wt2_batch_sync",
-    "wt2_batch_slot", "wt2_ctx_autotune", "wt2_ctx_tiles", "wt2_ctx_set_tiles", "wt2_launch_pass", "wt2_last_error", "wt2_version",
+    "wt2_batch_slot", "wt2_batch_info", "wt2_ctx_autotune", "wt2_ctx_tiles", "wt2_ctx_set_tiles", "wt2_launch_pass", "wt2_last_error", "wt2_version",
     "wt2_fo_coef", "wt2_fo_rhs", "wt2_fo_energy",
 )
 
@@ -135,6 +135,7 @@ def lib():
         L.wt2_batch_sync.argtypes = [vp]
         L.wt2_batch_slot.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), dp, ctypes.c_int32,
                                      ctypes.POINTER(ctypes.c_int32)]
+        L.wt2_batch_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         L.wt2_ctx_autotune.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_int32]
         L.wt2_ctx_tiles.argtypes = [vp, dp, dp]
         L.wt2_ctx_set_tiles.argtypes = [vp, dp, dp]

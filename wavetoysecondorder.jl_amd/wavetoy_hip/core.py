@@ -884,6 +884,14 @@ class Batch:
     def sync(self):
         check(lib().wt2_batch_sync(self.ptr))
 
+    def launches_per_step(self):
+        """Kernel launches of one RK4 step as last planned (0 before the first
+        step or after the set of unfinished instances changed)."""
+        n = ctypes.c_int32()
+        a = ctypes.c_int32()
+        check(lib().wt2_batch_info(self.ptr, ctypes.byref(n), ctypes.byref(a)))
+        return n.value
+
     def steps_done(self, slot):
         """Steps queued so far for a slot (no device synchronisation)."""
         done = ctypes.c_int64()

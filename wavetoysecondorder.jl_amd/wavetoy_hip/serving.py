@@ -120,6 +120,15 @@ def build_instance(req, precision):
     return ws, u, v, dt, nsteps
 
 
+def request_work_estimate(req):
+    """Point-updates a request needs: N^3 M^3 points times its RK4 steps, the
+    step count from h_min estimated as the smallest GLL gap / M (times 0.82, the
+    warp's smallest stretch, on warped meshes)."""
+    xs = np.sort(np.asarray(make_element(req.N).xs, dtype=np.float64))
+    h = float(np.min(np.diff(xs))) / req.M * (0.82 if req.curved else 1.0)
+    return req.npoints * int(math.ceil(req.T / (req.cfl * h)))
+
+
 def request_footprint(req, precision, energy_every):
     """Device bytes a live request holds: the context's allocations, the FP64
     state u and u_t, and the energy-sample buffer.  The step count behind the
@@ -180,6 +189,9 @@ class ServeStats:
     max_live: int = 0
     peak_bytes: int = 0
     chunks: int = 0
+    host_admit_s: float = 0.0     # building contexts and state of admitted requests
+    host_retire_s: float = 0.0    # copying results out, freeing contexts
+    wait_s: float = 0.0           # host waiting for queued chunks
 
     def summary(self):
         lat = np.array([r.latency for r in self.responses if not r.error])
@@ -194,6 +206,9 @@ class ServeStats:
             "max_live_requests": self.max_live,
             "peak_footprint_bytes": self.peak_bytes,
             "chunks": self.chunks,
+            "host_admit_s": self.host_admit_s,
+            "host_retire_s": self.host_retire_s,
+            "host_wait_s": self.wait_s,
         }
 
 
@@ -255,6 +270,7 @@ class Server:
 
         def admit():
             nonlocal used, live_pts
+            ta = clk()
             t = now()
             while pending and pending[0].arrival <= t:
                 queue.append(pending.popleft())
@@ -270,9 +286,11 @@ class Server:
                 stats.point_updates += r.npoints * ns
             stats.max_live = max(stats.max_live, len(live))
             stats.peak_bytes = max(stats.peak_bytes, used)
+            stats.host_admit_s += clk() - ta
 
         def retire(slots):
             nonlocal used, live_pts
+            tr = clk()
             for slot in slots:
                 L = live.pop(slot)
                 done, E = batch.slot_info(slot)
@@ -282,6 +300,7 @@ class Server:
                 L.ws.close()
                 used -= L.need
                 live_pts -= L.req.npoints
+            stats.host_retire_s += clk() - tr
 
         while pending or queue or live:
             admit()
@@ -295,7 +314,9 @@ class Server:
             batch.step_async(k)
             stats.chunks += 1
             admit()                    # host work for new arrivals overlaps the queued chunk
+            tw = clk()
             batch.sync()
+            stats.wait_s += clk() - tw
             adv = max(batch.steps_done(s) - before[s] for s in before)
             if adv > 0:
                 dt_meas = (clk() - t_issue) / adv
