@@ -19,7 +19,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
-#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -55,163 +54,210 @@ void mesh_coords(const wt2_mesh_desc& d, const double* xi_n, double* xyz)
     }
 }
 
-static int upload(CurvedData& cd, const std::vector<double>& coef, const std::vector<double>& wv,
-                  hipStream_t s)
+// Per-node coefficients (SoA, 13 x npts, FP64 and an FP32 copy) and the
+// energy weight H3/wgt, computed on the device once per context, from the
+// element map and the background (bg10 == nullptr: Minkowski):
+//   J = dx/dxi (the analytic warp map, or G applied along each reference axis
+//   to the node coordinates xyz of a general mesh), sqrt(det gamma) and the
+//   cofactor inverse gamma^ij (src/wave3d_curved.jl:85-94), then
+//   K^ab = |J| Jinv_ai K^ij Jinv_bj, c^a = |J| Jinv_ai c^i, wgt = alpha/(sqrt(g)|J|),
+//   kappa_a = K^aa + sum_{b != a} |K^ab|.
+// emin[e]: the smallest distance between reference-axis-adjacent nodes of
+// element e (src/evolve.jl:60-69).  One workgroup per element.
+struct CoefArgs {
+    int N, M0, M1, M2;
+    double x0[3], L[3];
+    double warpA;
+    const double* xi;      // GLL nodes, weights and G (device copies)
+    const double* w;
+    const double* G;
+    const double* xyz;     // general mesh: node coordinates (3,N,N,N,Ne); nullptr: warp map
+    const double* bg10;    // node-major background, nullptr: Minkowski
+    double* coef64;
+    float* coef32;
+    double* wv;
+    double* emin;
+    int64_t npts;
+};
+
+__global__ void __launch_bounds__(256) curved_coef_kernel(CoefArgs a)
 {
-    const size_t n = size_t(cd.npts);
-    std::vector<float> c32(coef.size());
-    for (size_t q = 0; q < coef.size(); ++q) c32[q] = float(coef[q]);
-    if (hipMemcpyAsync(cd.coef64, coef.data(), 13 * n * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(cd.coef32, c32.data(), 13 * n * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(cd.wv, wv.data(), n * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-        set_error("curved set-up: host-to-device copy failed");
-        return WT2_ERR_HIP;
+    __shared__ double sG[256], sxi[16], sw[16];
+    __shared__ double X[3 * 1000];
+    __shared__ double red[256];
+    const int N = a.N, N3 = N * N * N, tid = threadIdx.x;
+    const int64_t e = blockIdx.x;
+    const int ec[3] = {int(e % a.M0), int((e / a.M0) % a.M1), int(e / (int64_t(a.M0) * a.M1))};
+    const int Ms[3] = {a.M0, a.M1, a.M2};
+    for (int q = tid; q < N * N; q += blockDim.x) sG[q] = a.G[q];
+    for (int q = tid; q < N; q += blockDim.x) { sxi[q] = a.xi[q]; sw[q] = a.w[q]; }
+    __syncthreads();
+    const double tp = 2.0 * M_PI;
+    for (int q = tid; q < N3; q += blockDim.x) {
+        const int ii[3] = {q % N, (q / N) % N, q / (N * N)};
+        if (a.xyz) {
+            for (int c = 0; c < 3; ++c) X[3 * q + c] = a.xyz[3 * (e * N3 + q) + c];
+        } else {
+            double xr[3];
+            for (int c = 0; c < 3; ++c) xr[c] = (ec[c] + sxi[ii[c]]) / Ms[c];
+            const double sv = a.warpA * sin(tp * xr[0]) * sin(tp * xr[1]) * sin(tp * xr[2]);
+            for (int c = 0; c < 3; ++c) X[3 * q + c] = a.x0[c] + a.L[c] * (xr[c] + sv);
+        }
     }
-    return WT2_OK;
+    __syncthreads();
+    double hmin = 1e300;
+    for (int q = tid; q < N3; q += blockDim.x) {
+        const int i = q % N, j = (q / N) % N, k = q / (N * N);
+        const int ii[3] = {i, j, k};
+        double J[3][3];
+        if (a.xyz) {
+            for (int c = 0; c < 3; ++c)
+                for (int b = 0; b < 3; ++b) {
+                    double acc = 0.0;
+                    for (int p = 0; p < N; ++p) {
+                        int jj[3] = {i, j, k};
+                        jj[b] = p;
+                        acc += sG[ii[b] * N + p] * X[3 * (jj[0] + N * (jj[1] + N * jj[2])) + c];
+                    }
+                    J[c][b] = acc;
+                }
+        } else {
+            double xr[3], sn[3], cs[3];
+            for (int c = 0; c < 3; ++c) {
+                xr[c] = (ec[c] + sxi[ii[c]]) / Ms[c];
+                sn[c] = sin(tp * xr[c]);
+                cs[c] = cos(tp * xr[c]);
+            }
+            const double A = a.warpA;
+            const double ds[3] = {A * tp * cs[0] * sn[1] * sn[2], A * tp * sn[0] * cs[1] * sn[2],
+                                  A * tp * sn[0] * sn[1] * cs[2]};
+            for (int c = 0; c < 3; ++c)
+                for (int b = 0; b < 3; ++b) J[c][b] = a.L[c] * ((c == b ? 1.0 : 0.0) + ds[b]) / Ms[b];
+        }
+        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        double Ji[3][3];
+        Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+        Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+        Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+        Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+        Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+        const double aJ = fabs(det);
+        const int64_t n = e * N3 + q;
+        double al = 1.0, be[3] = {0, 0, 0}, g[6] = {1, 0, 0, 1, 0, 1};
+        if (a.bg10) {
+            const double* b = a.bg10 + 10 * n;
+            al = b[0];
+            for (int c = 0; c < 3; ++c) be[c] = b[1 + c];
+            for (int c = 0; c < 6; ++c) g[c] = b[4 + c];
+        }
+        const double g11 = g[0], g12 = g[1], g13 = g[2], g22 = g[3], g23 = g[4], g33 = g[5];
+        const double dg = g11 * (g22 * g33 - g23 * g23) - g12 * (g12 * g33 - g23 * g13) + g13 * (g12 * g23 - g22 * g13);
+        const double sg = sqrt(dg);
+        double gu[3][3];
+        gu[0][0] = (g22 * g33 - g23 * g23) / dg;
+        gu[0][1] = gu[1][0] = (g13 * g23 - g12 * g33) / dg;
+        gu[0][2] = gu[2][0] = (g12 * g23 - g13 * g22) / dg;
+        gu[1][1] = (g11 * g33 - g13 * g13) / dg;
+        gu[1][2] = gu[2][1] = (g12 * g13 - g11 * g23) / dg;
+        gu[2][2] = (g11 * g22 - g12 * g12) / dg;
+        double K[3][3], cv[3];
+        for (int p = 0; p < 3; ++p) {
+            cv[p] = sg * be[p] / al;
+            for (int r = 0; r < 3; ++r) K[p][r] = sg * (al * gu[p][r] - be[p] * be[r] / al);
+        }
+        double Kh[3][3], ch[3];
+        for (int c = 0; c < 3; ++c) {
+            ch[c] = 0.0;
+            for (int p = 0; p < 3; ++p) ch[c] += aJ * Ji[c][p] * cv[p];
+            for (int b = 0; b < 3; ++b) {
+                double acc = 0.0;
+                for (int p = 0; p < 3; ++p)
+                    for (int r = 0; r < 3; ++r) acc += Ji[c][p] * K[p][r] * Ji[b][r];
+                Kh[c][b] = aJ * acc;
+            }
+        }
+        const double vals[13] = {Kh[0][0], Kh[0][1], Kh[0][2], Kh[1][1], Kh[1][2], Kh[2][2],
+                                 ch[0], ch[1], ch[2], al / (sg * aJ),
+                                 Kh[0][0] + fabs(Kh[0][1]) + fabs(Kh[0][2]),
+                                 Kh[1][1] + fabs(Kh[0][1]) + fabs(Kh[1][2]),
+                                 Kh[2][2] + fabs(Kh[0][2]) + fabs(Kh[1][2])};
+        for (int c = 0; c < 13; ++c) {
+            a.coef64[c * a.npts + n] = vals[c];
+            a.coef32[c * a.npts + n] = float(vals[c]);
+        }
+        a.wv[n] = sw[i] * sw[j] * sw[k] / vals[9];
+        for (int c = 0; c < 3; ++c) {
+            if (ii[c] == 0) continue;
+            int jj[3] = {i, j, k};
+            jj[c] -= 1;
+            const double* P = X + 3 * q;
+            const double* Q = X + 3 * (jj[0] + N * (jj[1] + N * jj[2]));
+            const double dd = sqrt((P[0] - Q[0]) * (P[0] - Q[0]) + (P[1] - Q[1]) * (P[1] - Q[1]) +
+                                   (P[2] - Q[2]) * (P[2] - Q[2]));
+            hmin = fmin(hmin, dd);
+        }
+    }
+    red[tid] = hmin;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (tid < st) red[tid] = fmin(red[tid], red[tid + st]);
+        __syncthreads();
+    }
+    if (tid == 0) a.emin[e] = red[0];
 }
 
-// Per-node coefficients (SoA, 13 x npts) and the energy weight H3/wgt, from the
-// analytic warp Jacobian and the background (nullptr = Minkowski).
-// xyz != nullptr: node coordinates (3,N,N,N,Ne) of a general mesh, with
-// J = dx/dxi from G applied along each reference axis; otherwise the warp map.
-static void compute_coefs(const wt2_mesh_desc& d, const double* xi_n, const double* w,
-                          const double* bg10, std::vector<double>& coef, std::vector<double>& wv,
-                          double* h_min, const double* xyz = nullptr, const double* Gm = nullptr)
+// Compute cd's coefficients on the device (set-up path: the background and
+// general-mesh coordinates are staged through temporary device buffers).
+static int compute_coefs_device(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const double* w,
+                                const double* G, const double* bg10, hipStream_t s)
 {
     const int N = d.N;
-    const int* M = d.M;
-    const int64_t Ne = int64_t(M[0]) * M[1] * M[2];
-    const int64_t npts = int64_t(N) * N * N * Ne;
-    coef.assign(size_t(13 * npts), 0.0);
-    wv.assign(size_t(npts), 0.0);
-    double L[3];
-    for (int a = 0; a < 3; ++a) L[a] = d.x1[a] - d.x0[a];
-    const double tp = 2.0 * M_PI;
-    unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
-    if (const char* env = std::getenv("OMP_NUM_THREADS")) nt = std::max(1, std::min(64, std::atoi(env)));
-    std::vector<double> hmins(nt, 1e300);
-    auto work = [&](unsigned tid) {
-    double hmin = 1e300;
-    for (int64_t e = tid; e < Ne; e += nt) {
-        const int ec[3] = {int(e % M[0]), int((e / M[0]) % M[1]), int(e / (int64_t(M[0]) * M[1]))};
-        double xl[16][16][16][3];
-        if (xyz)
-            for (int k = 0; k < N; ++k)
-                for (int j = 0; j < N; ++j)
-                    for (int i = 0; i < N; ++i)
-                        for (int a = 0; a < 3; ++a)
-                            xl[k][j][i][a] = xyz[size_t(3 * (i + int64_t(N) * (j + int64_t(N) * (k + int64_t(N) * e))) + a)];
-        for (int k = 0; k < N; ++k)
-            for (int j = 0; j < N; ++j)
-                for (int i = 0; i < N; ++i) {
-                    const int ii[3] = {i, j, k};
-                    double xi[3];
-                    for (int a = 0; a < 3; ++a) xi[a] = (ec[a] + xi_n[ii[a]]) / M[a];
-                    const double s1 = std::sin(tp * xi[0]), s2 = std::sin(tp * xi[1]), s3 = std::sin(tp * xi[2]);
-                    const double c1 = std::cos(tp * xi[0]), c2 = std::cos(tp * xi[1]), c3 = std::cos(tp * xi[2]);
-                    const double A = d.warp_A;
-                    const double sv = A * s1 * s2 * s3;
-                    const double ds[3] = {A * tp * c1 * s2 * s3, A * tp * s1 * c2 * s3, A * tp * s1 * s2 * c3};
-                    double J[3][3];
-                    if (xyz) {
-                        for (int a = 0; a < 3; ++a)
-                            for (int b = 0; b < 3; ++b) {
-                                double s = 0.0;
-                                for (int p = 0; p < N; ++p) {
-                                    int jj[3] = {i, j, k};
-                                    jj[b] = p;
-                                    s += Gm[ii[b] * N + p] * xl[jj[2]][jj[1]][jj[0]][a];
-                                }
-                                J[a][b] = s;
-                            }
-                    } else {
-                        for (int a = 0; a < 3; ++a) xl[k][j][i][a] = d.x0[a] + L[a] * (xi[a] + sv);
-                        for (int a = 0; a < 3; ++a)
-                            for (int b = 0; b < 3; ++b) J[a][b] = L[a] * ((a == b ? 1.0 : 0.0) + ds[b]) / M[b];
-                    }
-                    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
-                                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-                    double Ji[3][3];
-                    Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
-                    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
-                    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
-                    Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
-                    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
-                    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
-                    Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
-                    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
-                    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
-                    const double aJ = std::fabs(det);
-                    const int64_t n = i + int64_t(N) * (j + int64_t(N) * (k + int64_t(N) * e));
-                    double al = 1.0, be[3] = {0, 0, 0}, g[6] = {1, 0, 0, 1, 0, 1};
-                    if (bg10) {
-                        const double* b = bg10 + 10 * n;
-                        al = b[0];
-                        for (int q = 0; q < 3; ++q) be[q] = b[1 + q];
-                        for (int q = 0; q < 6; ++q) g[q] = b[4 + q];
-                    }
-                    // sqrt(det gamma) and the cofactor inverse (src/wave3d_curved.jl:85-94)
-                    const double g11 = g[0], g12 = g[1], g13 = g[2], g22 = g[3], g23 = g[4], g33 = g[5];
-                    const double dg = g11 * (g22 * g33 - g23 * g23) - g12 * (g12 * g33 - g23 * g13) +
-                                      g13 * (g12 * g23 - g22 * g13);
-                    const double sg = std::sqrt(dg);
-                    double gu[3][3];
-                    gu[0][0] = (g22 * g33 - g23 * g23) / dg;
-                    gu[0][1] = gu[1][0] = (g13 * g23 - g12 * g33) / dg;
-                    gu[0][2] = gu[2][0] = (g12 * g23 - g13 * g22) / dg;
-                    gu[1][1] = (g11 * g33 - g13 * g13) / dg;
-                    gu[1][2] = gu[2][1] = (g12 * g13 - g11 * g23) / dg;
-                    gu[2][2] = (g11 * g22 - g12 * g12) / dg;
-                    double K[3][3], cv[3];
-                    for (int p = 0; p < 3; ++p) {
-                        cv[p] = sg * be[p] / al;
-                        for (int q = 0; q < 3; ++q) K[p][q] = sg * (al * gu[p][q] - be[p] * be[q] / al);
-                    }
-                    double Kh[3][3], ch[3];
-                    for (int a = 0; a < 3; ++a) {
-                        ch[a] = 0.0;
-                        for (int p = 0; p < 3; ++p) ch[a] += aJ * Ji[a][p] * cv[p];
-                        for (int b = 0; b < 3; ++b) {
-                            double acc = 0.0;
-                            for (int p = 0; p < 3; ++p)
-                                for (int q = 0; q < 3; ++q) acc += Ji[a][p] * K[p][q] * Ji[b][q];
-                            Kh[a][b] = aJ * acc;
-                        }
-                    }
-                    const double vals[13] = {Kh[0][0], Kh[0][1], Kh[0][2], Kh[1][1], Kh[1][2], Kh[2][2],
-                                             ch[0], ch[1], ch[2], al / (sg * aJ),
-                                             Kh[0][0] + std::fabs(Kh[0][1]) + std::fabs(Kh[0][2]),
-                                             Kh[1][1] + std::fabs(Kh[0][1]) + std::fabs(Kh[1][2]),
-                                             Kh[2][2] + std::fabs(Kh[0][2]) + std::fabs(Kh[1][2])};
-                    for (int q = 0; q < 13; ++q) coef[size_t(q * npts + n)] = vals[q];
-                    wv[size_t(n)] = w[i] * w[j] * w[k] / vals[9];
-                }
-        // smallest spacing between reference-axis-adjacent nodes (src/evolve.jl:60-69)
-        for (int k = 0; k < N; ++k)
-            for (int j = 0; j < N; ++j)
-                for (int i = 0; i < N; ++i) {
-                    const int ii[3] = {i, j, k};
-                    for (int a = 0; a < 3; ++a) {
-                        if (ii[a] == 0) continue;
-                        int jj[3] = {i, j, k};
-                        jj[a] -= 1;
-                        const double* p = xl[k][j][i];
-                        const double* q = xl[jj[2]][jj[1]][jj[0]];
-                        const double dd = std::sqrt((p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) +
-                                                    (p[2] - q[2]) * (p[2] - q[2]));
-                        hmin = std::min(hmin, dd);
-                    }
-                }
+    const size_t n = size_t(cd.npts), Ne = size_t(cd.Ne);
+    double* tmp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // ops, bg, xyz, emin
+    auto cleanup = [&]() { for (double* p : tmp) if (p) (void)hipFree(p); };
+    const size_t ops_n = 16 + 16 + 256;
+    if (hipMalloc(&tmp[0], ops_n * sizeof(double)) != hipSuccess ||
+        (bg10 && hipMalloc(&tmp[1], 10 * n * sizeof(double)) != hipSuccess) ||
+        (!cd.xyz.empty() && hipMalloc(&tmp[2], 3 * n * sizeof(double)) != hipSuccess) ||
+        hipMalloc(&tmp[3], Ne * sizeof(double)) != hipSuccess) {
+        cleanup();
+        set_error("curved set-up: device allocation failed");
+        return WT2_ERR_OOM;
     }
-    hmins[tid] = hmin;
-    };
-    std::vector<std::thread> th;
-    for (unsigned t = 1; t < nt; ++t) th.emplace_back(work, t);
-    work(0);
-    for (auto& t : th) t.join();
-    *h_min = *std::min_element(hmins.begin(), hmins.end());
+    std::vector<double> ops(ops_n, 0.0);
+    for (int i = 0; i < N; ++i) { ops[i] = xi[i]; ops[16 + i] = w[i]; }
+    for (int i = 0; i < N * N; ++i) ops[32 + i] = G[i];
+    CoefArgs a;
+    a.N = N; a.M0 = d.M[0]; a.M1 = d.M[1]; a.M2 = d.M[2];
+    for (int c = 0; c < 3; ++c) { a.x0[c] = d.x0[c]; a.L[c] = d.x1[c] - d.x0[c]; }
+    a.warpA = d.warp_A;
+    a.xi = tmp[0]; a.w = tmp[0] + 16; a.G = tmp[0] + 32;
+    a.xyz = tmp[2]; a.bg10 = tmp[1];
+    a.coef64 = cd.coef64; a.coef32 = cd.coef32; a.wv = cd.wv; a.emin = tmp[3];
+    a.npts = cd.npts;
+    std::vector<double> emin(Ne);
+    hipError_t err = hipMemcpyAsync(tmp[0], ops.data(), ops_n * sizeof(double), hipMemcpyHostToDevice, s);
+    if (err == hipSuccess && bg10) err = hipMemcpyAsync(tmp[1], bg10, 10 * n * sizeof(double), hipMemcpyHostToDevice, s);
+    if (err == hipSuccess && tmp[2]) err = hipMemcpyAsync(tmp[2], cd.xyz.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, s);
+    if (err == hipSuccess) {
+        hipLaunchKernelGGL(curved_coef_kernel, dim3(unsigned(Ne)), dim3(256), 0, s, a);
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess) err = hipMemcpyAsync(emin.data(), tmp[3], Ne * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
+    cleanup();
+    if (err != hipSuccess) {
+        set_error(std::string("curved set-up (coefficients): ") + hipGetErrorString(err));
+        return WT2_ERR_HIP;
+    }
+    cd.h_min = *std::min_element(emin.begin(), emin.end());
+    return WT2_OK;
 }
 
 int curved_init(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const double* w,
@@ -245,17 +291,13 @@ int curved_init(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const 
         cd.pending = true;
         return WT2_OK;
     }
-    std::vector<double> coef, wv;
-    compute_coefs(d, xi, w, bg10, coef, wv, &cd.h_min);
-    return upload(cd, coef, wv, s);
+    return compute_coefs_device(cd, d, xi, w, G, bg10, s);
 }
 
 int curved_set_background(CurvedData& cd, const wt2_mesh_desc& d, const double* xi,
                           const double* w, const double* G, const double* bg10, hipStream_t s)
 {
-    std::vector<double> coef, wv;
-    compute_coefs(d, xi, w, bg10, coef, wv, &cd.h_min, cd.xyz.empty() ? nullptr : cd.xyz.data(), G);
-    const int rc = upload(cd, coef, wv, s);
+    const int rc = compute_coefs_device(cd, d, xi, w, G, bg10, s);
     if (rc == WT2_OK) cd.pending = false;
     return rc;
 }
