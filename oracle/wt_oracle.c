@@ -27,6 +27,13 @@ struct wto_ctx {
     double* hphys;    /* H3*|J| per node */
     double mu, inv_mu5;
     int* fconn;       /* general mesh: per (e, f) neighbour (-1 outer), its face, D4 code, tag */
+    /* general mesh, first-order curvilinear path: metric terms |J| dxi^a/dx_n
+     * (conservative-curl form, 9 per node, [a*3+n]), |J| per node, element
+     * extent along each reference axis, mu of the curvilinear apply_D! */
+    double* JAp;
+    double* Jp;
+    double* ell;
+    double mu_c, inv_mu5_c;
 };
 
 /* ------------------------------------------------------------------ GLL */
@@ -174,6 +181,7 @@ static void apply_D_line1d(const wto_ctx* c, int M, int periodic, double h,
                            const double* in, double* out);
 
 static wto_ctx* create_impl(const wto_desc* d, const double* bg, const double* mesh_xyz);
+static void curv_setup(wto_ctx* c);
 
 /* Planted mistake in the ADM coefficients (0 = none), for the pin tests that
  * must show they catch it; read when a context is created. */
@@ -217,7 +225,7 @@ wto_ctx* wto_create_mesh(const wto_desc* d, long Ne, const double* xyz, const in
             int nb = nbr[6 * e + f];
             if (nb == 0) {
                 int tag = bdry[6 * e + f];
-                if (tag < 1 || tag > 7) { free(fc); return NULL; }
+                if (tag < 1 || tag > 8) { free(fc); return NULL; }
                 o[0] = -1; o[1] = 0; o[2] = 0; o[3] = tag;
                 continue;
             }
@@ -247,6 +255,7 @@ wto_ctx* wto_create_mesh(const wto_desc* d, long Ne, const double* xyz, const in
     wto_ctx* c = create_impl(&dm, NULL, xyz);
     if (!c) { free(fc); return NULL; }
     c->fconn = fc;
+    curv_setup(c);
     return c;
 }
 
@@ -402,7 +411,8 @@ static wto_ctx* create_impl(const wto_desc* d, const double* bg, const double* m
 void wto_destroy(wto_ctx* c)
 {
     if (!c) return;
-    free(c->coef); free(c->coords); free(c->hphys); free(c->fconn); free(c);
+    free(c->coef); free(c->coords); free(c->hphys); free(c->fconn);
+    free(c->JAp); free(c->Jp); free(c->ell); free(c);
 }
 
 long wto_npoints(const wto_ctx* c) { return c->npts; }
@@ -746,6 +756,8 @@ static void op_curved(const wto_ctx* c, const double* u, const double* v,
                 const double* vn = (en >= 0 && with_adv) ? v + en * npe : NULL;
                 const double* con = en >= 0 ? c->coef + 13 * en * (long)npe : NULL;
                 int som = en < 0 && (c->fconn ? tag == 7 : c->d.face_kind[2 * a + sd] == 7);
+                /* excision face (tag 8, src/wave_strong_rhs.jl:188-213): no SAT */
+                if (en < 0 && c->fconn && tag == 8) continue;
                 for (int q = 0; q < N; ++q)
                 for (int p = 0; p < N; ++p) {
                     int n = fidx(N, a, r, p, q);
@@ -1088,6 +1100,387 @@ double wto_fo_energy(const wto_ctx* c, const double* Phi, const double* Pi, cons
         E += (Pi[q] * Pi[q] / sg[q] +
               sg[q] * (gu[q] * d1 * d1 + gu[3 * n + q] * d2 * d2 + gu[5 * n + q] * d3 * d3 + 2 * gu[n + q] * d1 * d2 +
                        2 * gu[2 * n + q] * d1 * d3 + 2 * gu[4 * n + q] * d2 * d3)) * c->hphys[q] / 2;
+    }
+    free(D);
+    return E;
+}
+
+/* ------------------------------ first-order (Phi, Pi) system, curvilinear */
+
+/* The curvilinear path of wave3d_curved_rhs! (metric given,
+ * src/wave3d_curved.jl:218-238, METHODS.md "Curvilinear path"): metric terms
+ * in the conservative-curl form computed from the node coordinates with the
+ * element's own differentiation matrix,
+ *   J a^i_n = (grad_xi x C^n)_i,  C^n = 1/2 (X_l grad_xi X_m - X_m grad_xi X_l),
+ * (n, l, m) and (i, j, k) cyclic, so that sum_i D_i(J a^i_n) = 0 discretely
+ * (free stream); the split form 1/2 (conservative + advective) of the
+ * gradient and the divergence (a skew-adjoint pair in the mass
+ * Hd = w_i w_j w_k |J|) with the centred-flux SAT across element faces
+ * (neighbour values through the D4 face map), one-sided at outer faces.
+ * The metric is stored with the element's orientation divided out
+ * (JAp = sign(J) J a, |J|), so the formulas hold on left-handed elements. */
+
+static void elem_dref(const wto_ctx* c, const double* f, int b, int i, int j, int k, double* out)
+{
+    int N = c->N;
+    int ii[3] = {i, j, k};
+    double s = 0.0;
+    for (int p = 0; p < N; ++p) {
+        int jj[3] = {i, j, k};
+        jj[b] = p;
+        s += c->G[ii[b] * N + p] * f[jj[0] + N * (jj[1] + N * jj[2])];
+    }
+    *out = s;
+}
+
+static void curv_setup(wto_ctx* c)
+{
+    int N = c->N, npe = c->npe;
+    long Ne = c->Ne, n = c->npts;
+    c->JAp = (double*)malloc(sizeof(double) * 9 * n);
+    c->Jp = (double*)malloc(sizeof(double) * n);
+    c->ell = (double*)malloc(sizeof(double) * 3 * Ne);
+    #pragma omp parallel for schedule(static)
+    for (long e = 0; e < Ne; ++e) {
+        double* X = (double*)malloc(sizeof(double) * 3 * npe);
+        double* C = (double*)malloc(sizeof(double) * 9 * npe);   /* C^n_k at [(n*3+k)*npe + node] */
+        for (int q = 0; q < npe; ++q)
+            for (int a = 0; a < 3; ++a) X[a * npe + q] = c->coords[3 * (e * npe + q) + a];
+        for (int q = 0; q < npe; ++q) {
+            int i = q % N, j = (q / N) % N, k = q / (N * N);
+            double dX[3][3];   /* dX[m][kk] = D_kk X_m */
+            for (int m = 0; m < 3; ++m)
+                for (int kk = 0; kk < 3; ++kk) elem_dref(c, X + m * npe, kk, i, j, k, &dX[m][kk]);
+            for (int nn = 0; nn < 3; ++nn) {
+                int l = (nn + 1) % 3, m = (nn + 2) % 3;
+                for (int kk = 0; kk < 3; ++kk)
+                    C[(nn * 3 + kk) * npe + q] = 0.5 * (X[l * npe + q] * dX[m][kk] - X[m * npe + q] * dX[l][kk]);
+            }
+            double det = dX[0][0] * (dX[1][1] * dX[2][2] - dX[1][2] * dX[2][1]) -
+                         dX[0][1] * (dX[1][0] * dX[2][2] - dX[1][2] * dX[2][0]) +
+                         dX[0][2] * (dX[1][0] * dX[2][1] - dX[1][1] * dX[2][0]);
+            c->Jp[e * npe + q] = det;   /* signed for now */
+        }
+        for (int q = 0; q < npe; ++q) {
+            int i = q % N, j = (q / N) % N, k = q / (N * N);
+            double sg = c->Jp[e * npe + q] < 0 ? -1.0 : 1.0;
+            for (int ia = 0; ia < 3; ++ia) {
+                int ja = (ia + 1) % 3, ka = (ia + 2) % 3;
+                for (int nn = 0; nn < 3; ++nn) {
+                    double d1, d2;
+                    elem_dref(c, C + (nn * 3 + ka) * npe, ja, i, j, k, &d1);
+                    elem_dref(c, C + (nn * 3 + ja) * npe, ka, i, j, k, &d2);
+                    c->JAp[9 * (e * npe + q) + ia * 3 + nn] = sg * (d1 - d2);
+                }
+            }
+        }
+        for (int q = 0; q < npe; ++q) c->Jp[e * npe + q] = fabs(c->Jp[e * npe + q]);
+        for (int a = 0; a < 3; ++a) {
+            double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+            for (int qq = 0; qq < N; ++qq)
+                for (int p = 0; p < N; ++p) {
+                    int n0 = fidx(N, a, 0, p, qq), n1 = fidx(N, a, N - 1, p, qq);
+                    for (int m = 0; m < 3; ++m) { lo[m] += X[m * npe + n0]; hi[m] += X[m * npe + n1]; }
+                }
+            double d2 = 0.0;
+            for (int m = 0; m < 3; ++m) d2 += (hi[m] - lo[m]) * (hi[m] - lo[m]);
+            c->ell[3 * e + a] = sqrt(d2) / (N * N);
+        }
+        free(X); free(C);
+    }
+    c->mu_c = -1.0;
+}
+
+int wto_curv_metric(const wto_ctx* c, double* JA9, double* Jp)
+{
+    if (!c->JAp) return -1;
+    memcpy(JA9, c->JAp, sizeof(double) * 9 * c->npts);
+    memcpy(Jp, c->Jp, sizeof(double) * c->npts);
+    return 0;
+}
+
+/* the face (axis a, side sd) of element e: neighbour, its face, D4 code, tag */
+static void curv_face(const wto_ctx* c, long e, int f, long* en, int* fn, int* ori, int* tag)
+{
+    const int* fc = c->fconn + 4 * (6 * e + f);
+    *en = fc[0]; *fn = fc[1]; *ori = fc[2]; *tag = fc[3];
+}
+
+/* D = grad Phi (3 fields of npts): split form + centred SAT. */
+void wto_curv_grad(const wto_ctx* c, const double* Phi, double* D3)
+{
+    int N = c->N, npe = c->npe;
+    long n = c->npts;
+    #pragma omp parallel for schedule(static)
+    for (long e = 0; e < c->Ne; ++e) {
+        const double* fe = Phi + e * npe;
+        const double* JA = c->JAp + 9 * e * npe;
+        double* W = (double*)malloc(sizeof(double) * 3 * npe);
+        double out[3][WTO_NMAX * WTO_NMAX * WTO_NMAX];
+        for (int q = 0; q < npe; ++q) for (int m = 0; m < 3; ++m) out[m][q] = 0.0;
+        for (int nn = 0; nn < 3; ++nn) {
+            for (int q = 0; q < npe; ++q)
+                for (int a = 0; a < 3; ++a) W[a * npe + q] = JA[9 * q + a * 3 + nn] * fe[q];
+            for (int q = 0; q < npe; ++q) {
+                int i = q % N, j = (q / N) % N, k = q / (N * N);
+                double cons = 0.0, adv = 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    double d1, d2;
+                    elem_dref(c, W + a * npe, a, i, j, k, &d1);
+                    elem_dref(c, fe, a, i, j, k, &d2);
+                    cons += d1;
+                    adv += JA[9 * q + a * 3 + nn] * d2;
+                }
+                out[nn][q] = 0.5 * (cons + adv) / c->Jp[e * npe + q];
+            }
+        }
+        for (int f = 0; f < 6; ++f) {
+            long en; int fn, ori, tag;
+            curv_face(c, e, f, &en, &fn, &ori, &tag);
+            if (en < 0) continue;
+            int a = f / 2, r = (f & 1) ? N - 1 : 0, an = fn / 2, rn = (fn & 1) ? N - 1 : 0;
+            double s = (f & 1) ? 1.0 : -1.0;
+            for (int qq = 0; qq < N; ++qq)
+                for (int p = 0; p < N; ++p) {
+                    int pn, qn;
+                    orient_map(ori, N, p, qq, &pn, &qn);
+                    int q = fidx(N, a, r, p, qq), qnb = fidx(N, an, rn, pn, qn);
+                    double jump = Phi[en * npe + qnb] - fe[q];
+                    double sc = 0.5 * s * jump / (c->w[r] * c->Jp[e * npe + q]);
+                    for (int nn = 0; nn < 3; ++nn) out[nn][q] += sc * JA[9 * q + a * 3 + nn];
+                }
+        }
+        for (int nn = 0; nn < 3; ++nn)
+            for (int q = 0; q < npe; ++q) D3[nn * n + e * npe + q] = out[nn][q];
+        free(W);
+    }
+}
+
+/* out = div F (F: 3 fields of npts): split form + centred SAT. */
+void wto_curv_div(const wto_ctx* c, const double* F3, double* out)
+{
+    int N = c->N, npe = c->npe;
+    long n = c->npts;
+    #pragma omp parallel for schedule(static)
+    for (long e = 0; e < c->Ne; ++e) {
+        const double* JA = c->JAp + 9 * e * npe;
+        double* W = (double*)malloc(sizeof(double) * 3 * npe);
+        double Fe[3][WTO_NMAX * WTO_NMAX * WTO_NMAX];
+        for (int q = 0; q < npe; ++q)
+            for (int m = 0; m < 3; ++m) Fe[m][q] = F3[m * n + e * npe + q];
+        for (int q = 0; q < npe; ++q)
+            for (int a = 0; a < 3; ++a)
+                W[a * npe + q] = JA[9 * q + a * 3 + 0] * Fe[0][q] + JA[9 * q + a * 3 + 1] * Fe[1][q] +
+                                 JA[9 * q + a * 3 + 2] * Fe[2][q];
+        for (int q = 0; q < npe; ++q) {
+            int i = q % N, j = (q / N) % N, k = q / (N * N);
+            double cons = 0.0, adv = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                double d1;
+                elem_dref(c, W + a * npe, a, i, j, k, &d1);
+                cons += d1;
+                for (int m = 0; m < 3; ++m) {
+                    double d2;
+                    elem_dref(c, Fe[m], a, i, j, k, &d2);
+                    adv += JA[9 * q + a * 3 + m] * d2;
+                }
+            }
+            out[e * npe + q] = 0.5 * (cons + adv) / c->Jp[e * npe + q];
+        }
+        for (int f = 0; f < 6; ++f) {
+            long en; int fn, ori, tag;
+            curv_face(c, e, f, &en, &fn, &ori, &tag);
+            if (en < 0) continue;
+            int a = f / 2, r = (f & 1) ? N - 1 : 0, an = fn / 2, rn = (fn & 1) ? N - 1 : 0;
+            double s = (f & 1) ? 1.0 : -1.0, sn = (fn & 1) ? 1.0 : -1.0;
+            for (int qq = 0; qq < N; ++qq)
+                for (int p = 0; p < N; ++p) {
+                    int pn, qn;
+                    orient_map(ori, N, p, qq, &pn, &qn);
+                    int q = fidx(N, a, r, p, qq), qnb = fidx(N, an, rn, pn, qn);
+                    long gn = en * npe + qnb;
+                    const double* JAn = c->JAp + 9 * gn;
+                    double Wn = JAn[an * 3 + 0] * F3[gn] + JAn[an * 3 + 1] * F3[n + gn] + JAn[an * 3 + 2] * F3[2 * n + gn];
+                    double qo = s * W[a * npe + q], qb = sn * Wn;
+                    out[e * npe + q] += -0.5 * (qo + qb) / (c->w[r] * c->Jp[e * npe + q]);
+                }
+        }
+        free(W);
+    }
+}
+
+/* The KO derivative on a general mesh, along each element's reference axis
+ * `axis`: (1/ell) [G u + s/(2 w_r) (u_nbr - u)] at faces with a neighbour,
+ * one-sided at outer faces (apply_D!'s form with the element's extent ell
+ * along that axis in place of h). */
+void wto_curv_D(const wto_ctx* c, int axis, const double* in, double* out)
+{
+    int N = c->N, npe = c->npe;
+    #pragma omp parallel for schedule(static)
+    for (long e = 0; e < c->Ne; ++e) {
+        const double* fe = in + e * npe;
+        double il = 1.0 / c->ell[3 * e + axis];
+        for (int q = 0; q < npe; ++q) {
+            int i = q % N, j = (q / N) % N, k = q / (N * N);
+            double d;
+            elem_dref(c, fe, axis, i, j, k, &d);
+            out[e * npe + q] = il * d;
+        }
+        for (int sd = 0; sd < 2; ++sd) {
+            int f = 2 * axis + sd;
+            long en; int fn, ori, tag;
+            curv_face(c, e, f, &en, &fn, &ori, &tag);
+            if (en < 0) continue;
+            int r = sd ? N - 1 : 0, an = fn / 2, rn = (fn & 1) ? N - 1 : 0;
+            double s = sd ? 1.0 : -1.0;
+            for (int qq = 0; qq < N; ++qq)
+                for (int p = 0; p < N; ++p) {
+                    int pn, qn;
+                    orient_map(ori, N, p, qq, &pn, &qn);
+                    int q = fidx(N, axis, r, p, qq), qnb = fidx(N, an, rn, pn, qn);
+                    out[e * npe + q] += il * s * (in[en * npe + qnb] - fe[q]) / (2.0 * c->w[r]);
+                }
+        }
+    }
+}
+
+/* mu of the curvilinear KO derivative: 30 power iterations of D_1^2 from the
+ * alternating start vector isodd(i+j+k+m) (src/wave3d_curved.jl:158-178). */
+double wto_curv_mu(const wto_ctx* c)
+{
+    wto_ctx* cc = (wto_ctx*)c;
+    if (cc->mu_c >= 0) return cc->mu_c;
+    long n = c->npts;
+    int N = c->N, npe = c->npe;
+    double* v = (double*)malloc(sizeof(double) * n);
+    double* w = (double*)malloc(sizeof(double) * n);
+    for (long q = 0; q < n; ++q) {
+        long e = q / npe;
+        int r = (int)(q % npe);
+        int i = r % N, j = (r / N) % N, k = r / (N * N);
+        v[q] = ((i + 1 + j + 1 + k + 1 + e + 1) % 2 == 1) ? 1.0 : -1.0;
+    }
+    double mu2 = 0.0;
+    for (int it = 0; it < 30; ++it) {
+        wto_curv_D(c, 0, v, w);
+        wto_curv_D(c, 0, w, v);
+        double s = 0.0;
+        for (long q = 0; q < n; ++q) s += v[q] * v[q];
+        mu2 = sqrt(s);
+        if (!(mu2 > 0)) break;
+        for (long q = 0; q < n; ++q) v[q] /= mu2;
+    }
+    cc->mu_c = sqrt(mu2);
+    cc->inv_mu5_c = cc->mu_c > 0 ? pow(1.0 / cc->mu_c, 5) : 0.0;
+    free(v); free(w);
+    return cc->mu_c;
+}
+
+/* wave3d_curved_rhs! with metric (src/wave3d_curved.jl:218-256): the
+ * gradient and divergence above, KO eps mu^-5 D_d^6 per reference axis on both
+ * fields, then the curvilinear boundary pass _apply_bc3d_curv!
+ * (src/boundaries3d.jl:131-236): one kind bc->kinds[0] on every outer face
+ * whose tag is not bc->excision_tag, with the physical outward unit normal
+ * n = s JAp^a / |JAp^a| and the lift wt = |JAp^a| / (w_r |J|). */
+void wto_fo_rhs_curv(const wto_ctx* c, const double* Phi, const double* Pi, const double* coef11,
+                     double eps_ko, const wto_bc3d* bc, double* dPhi, double* dPi)
+{
+    long n = c->npts;
+    int N = c->N, npe = c->npe;
+    const double *al = coef11, *sg = coef11 + n, *b[3] = {coef11 + 2 * n, coef11 + 3 * n, coef11 + 4 * n};
+    const double* gu = coef11 + 5 * n;
+    static const int gmap[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    double* D = (double*)malloc(sizeof(double) * 3 * n);
+    double* F = (double*)malloc(sizeof(double) * 3 * n);
+    wto_curv_grad(c, Phi, D);
+    #pragma omp parallel for schedule(static)
+    for (long q = 0; q < n; ++q) {
+        for (int i = 0; i < 3; ++i)
+            F[i * n + q] = b[i][q] * Pi[q] + al[q] * sg[q] * (gu[gmap[i][0] * n + q] * D[q] + gu[gmap[i][1] * n + q] * D[n + q] +
+                                                            gu[gmap[i][2] * n + q] * D[2 * n + q]);
+        dPhi[q] = b[0][q] * D[q] + b[1][q] * D[n + q] + b[2][q] * D[2 * n + q] + (al[q] / sg[q]) * Pi[q];
+    }
+    wto_curv_div(c, F, dPi);
+    if (eps_ko != 0.0) {
+        double koT = eps_ko * pow(1.0 / wto_curv_mu(c), 5);
+        double* s1 = (double*)malloc(sizeof(double) * n);
+        double* s2 = (double*)malloc(sizeof(double) * n);
+        for (int fld = 0; fld < 2; ++fld) {
+            const double* x = fld == 0 ? Phi : Pi;
+            double* o = fld == 0 ? dPhi : dPi;
+            for (int d = 0; d < 3; ++d) {
+                wto_curv_D(c, d, x, s1);
+                wto_curv_D(c, d, s1, s2);
+                wto_curv_D(c, d, s2, s1);
+                wto_curv_D(c, d, s1, s2);
+                wto_curv_D(c, d, s2, s1);
+                wto_curv_D(c, d, s1, s2);
+                for (long q = 0; q < n; ++q) o[q] += koT * s2[q];
+            }
+        }
+        free(s1); free(s2);
+    }
+    if (bc) {
+        int kind = bc->kinds[0];
+        for (long e = 0; e < c->Ne; ++e)
+            for (int f = 0; f < 6; ++f) {
+                long en; int fn, ori, tag;
+                curv_face(c, e, f, &en, &fn, &ori, &tag);
+                if (en >= 0 || tag == bc->excision_tag || kind == WTO_BC_EXCISION) continue;
+                int a = f / 2, r = (f & 1) ? N - 1 : 0;
+                double s = (f & 1) ? 1.0 : -1.0;
+                for (int qq = 0; qq < N; ++qq)
+                    for (int p = 0; p < N; ++p) {
+                        long q = e * npe + fidx(N, a, r, p, qq);
+                        const double* JA = c->JAp + 9 * q;
+                        double nf[3] = {s * JA[a * 3 + 0], s * JA[a * 3 + 1], s * JA[a * 3 + 2]};
+                        double JF = sqrt(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]);
+                        double nx = nf[0] / JF, ny = nf[1] / JF, nz = nf[2] / JF;
+                        double av = al[q] / sg[q];
+                        double a_n = al[q] * sqrt(gu[q] * nx * nx + gu[3 * n + q] * ny * ny + gu[5 * n + q] * nz * nz +
+                                                  2 * gu[n + q] * nx * ny + 2 * gu[2 * n + q] * nx * nz +
+                                                  2 * gu[4 * n + q] * ny * nz);
+                        double bn = b[0][q] * nx + b[1][q] * ny + b[2][q] * nz;
+                        double wt = JF / (c->w[r] * c->Jp[q]);
+                        if (kind == WTO_BC_FULL_DIRICHLET) {
+                            double tau = bc->sigma * (fabs(a_n - bn) + fabs(a_n + bn)) * wt;
+                            double gF = bc->gPhi ? bc->gPhi[q] : 0.0, gP = bc->gPi ? bc->gPi[q] : 0.0;
+                            dPhi[q] += -tau * (Phi[q] - gF);
+                            dPi[q] += -tau * (Pi[q] - gP);
+                        } else {
+                            double qn = nx * D[q] + ny * D[n + q] + nz * D[2 * n + q];
+                            double rr = Pi[q] + ((bn + a_n) / av) * qn;
+                            double g = 0.0;
+                            if (kind == WTO_BC_DIRICHLET) {
+                                double gP = bc->gPi ? bc->gPi[q] : 0.0;
+                                double gd = bc->gD ? nx * bc->gD[q] + ny * bc->gD[n + q] + nz * bc->gD[2 * n + q] : 0.0;
+                                g = gP + ((bn + a_n) / av) * gd;
+                            }
+                            dPi[q] += -bc->sigma * fabs(a_n + bn) * wt * (rr - g);
+                        }
+                    }
+            }
+    }
+    free(D); free(F);
+}
+
+/* wave3d_energy with metric: E = sum 1/2 [Pi^2/sqrtg + sqrtg g^{ij} D_i D_j] Hd,
+ * Hd = w_i w_j w_k |J|. */
+double wto_fo_energy_curv(const wto_ctx* c, const double* Phi, const double* Pi, const double* coef11)
+{
+    long n = c->npts;
+    int N = c->N, npe = c->npe;
+    const double *sg = coef11 + n, *gu = coef11 + 5 * n;
+    double* D = (double*)malloc(sizeof(double) * 3 * n);
+    wto_curv_grad(c, Phi, D);
+    double E = 0.0;
+    for (long q = 0; q < n; ++q) {
+        int r = (int)(q % npe);
+        double Hd = c->w[r % N] * c->w[(r / N) % N] * c->w[r / (N * N)] * c->Jp[q];
+        double d1 = D[q], d2 = D[n + q], d3 = D[2 * n + q];
+        E += (Pi[q] * Pi[q] / sg[q] +
+              sg[q] * (gu[q] * d1 * d1 + gu[3 * n + q] * d2 * d2 + gu[5 * n + q] * d3 * d3 + 2 * gu[n + q] * d1 * d2 +
+                       2 * gu[2 * n + q] * d1 * d3 + 2 * gu[4 * n + q] * d2 * d3)) * Hd / 2;
     }
     free(D);
     return E;

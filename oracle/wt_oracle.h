@@ -93,15 +93,33 @@ double wto_energy(const wto_ctx* c, const double* u, const double* v);
 #define WTO_BC_EXCISION       3
 #define WTO_BC_FULL_DIRICHLET 4
 typedef struct wto_bc3d {
-    int kinds[6];          /* per face -x,+x,-y,+y,-z,+z                     */
+    int kinds[6];          /* per face -x,+x,-y,+y,-z,+z (curvilinear: kinds[0] on every outer face) */
+    int excision_tag;      /* curvilinear: outer faces with this tag get no SAT (0: none) */
     double sigma;
     const double* gPhi;    /* nullable targets (per node)                    */
     const double* gPi;
+    const double* gD;      /* curvilinear Dirichlet: grad Phi target, 3 fields (nullable) */
 } wto_bc3d;
 void wto_fo_coef(long npts, const double* bg10, double* coef11);
 void wto_fo_rhs(const wto_ctx* c, const double* Phi, const double* Pi, const double* coef11,
                 double eps_ko, const wto_bc3d* bc, double* dPhi, double* dPi);
 double wto_fo_energy(const wto_ctx* c, const double* Phi, const double* Pi, const double* coef11);
+
+/* The first-order system on a general mesh (wto_create_mesh): the
+ * curvilinear path of wave3d_curved_rhs! (metric given).  Metric terms
+ * JAp[9 n + 3 a + m] = sign(J) J dxi^a/dx_m in the conservative-curl form and
+ * |J| per node; split-form gradient (D3: 3 fields) and divergence (F3: 3
+ * fields) with the centred-flux SAT; the KO derivative along each element's
+ * reference axis; mu of that derivative; the RHS and the energy
+ * (mass w_i w_j w_k |J|). */
+int wto_curv_metric(const wto_ctx* c, double* JA9, double* Jp);
+void wto_curv_grad(const wto_ctx* c, const double* Phi, double* D3);
+void wto_curv_div(const wto_ctx* c, const double* F3, double* out);
+void wto_curv_D(const wto_ctx* c, int axis, const double* in, double* out);
+double wto_curv_mu(const wto_ctx* c);
+void wto_fo_rhs_curv(const wto_ctx* c, const double* Phi, const double* Pi, const double* coef11,
+                     double eps_ko, const wto_bc3d* bc, double* dPhi, double* dPi);
+double wto_fo_energy_curv(const wto_ctx* c, const double* Phi, const double* Pi, const double* coef11);
 
 /* Plant a known mistake in the ADM coefficients of contexts created after the
  * call (0 restores the correct formulas).  Only for pin tests, which must show

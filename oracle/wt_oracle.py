@@ -42,23 +42,32 @@ _BC_NAMES = {"dirichlet": 1, "sommerfeld": 2, "excision": 3, "full_dirichlet": 4
 class BC3d(ctypes.Structure):
     _fields_ = [
         ("kinds", ctypes.c_int * 6),
+        ("excision_tag", ctypes.c_int),
         ("sigma", ctypes.c_double),
         ("gPhi", ctypes.POINTER(ctypes.c_double)),
         ("gPi", ctypes.POINTER(ctypes.c_double)),
+        ("gD", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
-def make_bc3d(kinds, sigma=1.0, gPhi=None, gPi=None):
+def make_bc3d(kinds, sigma=1.0, gPhi=None, gPi=None, gD=None, excision_tag=0):
     """make_bc3d (src/boundaries3d.jl:44-54): kinds per face (-x,+x,-y,+y,-z,+z),
-    names or BC_* codes; gPhi / gPi: optional per-node targets."""
+    names or BC_* codes (a single name: every face); gPhi / gPi: optional
+    per-node targets; gD: the grad Phi target (3 fields) of the curvilinear
+    Dirichlet kind; excision_tag: curvilinear outer faces with this mesh tag
+    get no SAT."""
+    if isinstance(kinds, (str, int)):
+        kinds = [kinds] * 6
     b = BC3d()
     for f in range(6):
         k = kinds[f]
         b.kinds[f] = _BC_NAMES[k] if isinstance(k, str) else int(k)
+    b.excision_tag = int(excision_tag)
     b.sigma = float(sigma)
-    b._keep = [None if g is None else np.ascontiguousarray(g, dtype=np.float64) for g in (gPhi, gPi)]
+    b._keep = [None if g is None else np.ascontiguousarray(g, dtype=np.float64).reshape(-1) for g in (gPhi, gPi, gD)]
     b.gPhi = _p(b._keep[0])
     b.gPi = _p(b._keep[1])
+    b.gD = _p(b._keep[2])
     return b
 
 
@@ -114,6 +123,15 @@ def lib():
         L.wto_fo_energy.argtypes = [vp, dp, dp, dp]
         L.wto_fo_energy.restype = ctypes.c_double
         L.wto_plant_mistake.argtypes = [ctypes.c_int]
+        L.wto_curv_metric.argtypes = [vp, dp, dp]
+        L.wto_curv_grad.argtypes = [vp, dp, dp]
+        L.wto_curv_div.argtypes = [vp, dp, dp]
+        L.wto_curv_D.argtypes = [vp, ctypes.c_int, dp, dp]
+        L.wto_curv_mu.argtypes = [vp]
+        L.wto_curv_mu.restype = ctypes.c_double
+        L.wto_fo_rhs_curv.argtypes = [vp, dp, dp, dp, ctypes.c_double, ctypes.POINTER(BC3d), dp, dp]
+        L.wto_fo_energy_curv.argtypes = [vp, dp, dp, dp]
+        L.wto_fo_energy_curv.restype = ctypes.c_double
         L.wto_assemble_1d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, dp]
         _lib = L
@@ -311,3 +329,67 @@ class Oracle:
             Phi += dt / 6 * (k1[0] + 2 * k2[0] + 2 * k3[0] + k4[0])
             Pi += dt / 6 * (k1[1] + 2 * k2[1] + 2 * k3[1] + k4[1])
         return Phi, Pi
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+
+
+def _curv_methods():
+    """The curvilinear first-order path on general meshes (Oracle.from_mesh)."""
+
+    def curv_metric(self):
+        """(JAp (npts, 3, 3) [a, m] = sign(J) J dxi^a/dx_m, |J| (npts,))."""
+        JA = np.zeros(9 * self.n); J = np.zeros(self.n)
+        if lib().wto_curv_metric(self.ctx, _p(JA), _p(J)) != 0:
+            raise ValueError("not a general mesh")
+        return JA.reshape(self.n, 3, 3), J
+
+    def curv_grad(self, Phi):
+        D = np.zeros(3 * self.n)
+        lib().wto_curv_grad(self.ctx, _p(_c(Phi)), _p(D))
+        return D.reshape(3, self.n)
+
+    def curv_div(self, F3):
+        out = np.zeros(self.n)
+        lib().wto_curv_div(self.ctx, _p(_c(F3)), _p(out))
+        return out
+
+    def curv_D(self, axis, x):
+        out = np.zeros(self.n)
+        lib().wto_curv_D(self.ctx, int(axis), _p(_c(x)), _p(out))
+        return out
+
+    def curv_mu(self):
+        return lib().wto_curv_mu(self.ctx)
+
+    def fo_rhs_curv(self, Phi, Pi, coef, eps_ko=0.1, bc=None):
+        coef = _c(coef)
+        assert coef.size == 11 * self.n
+        dPhi = self._new(); dPi = self._new()
+        lib().wto_fo_rhs_curv(self.ctx, _p(_c(Phi)), _p(_c(Pi)), _p(coef), float(eps_ko),
+                              None if bc is None else ctypes.byref(bc), _p(dPhi), _p(dPi))
+        return dPhi, dPi
+
+    def fo_energy_curv(self, Phi, Pi, coef):
+        return lib().wto_fo_energy_curv(self.ctx, _p(_c(Phi)), _p(_c(Pi)), _p(_c(coef)))
+
+    def fo_rk4_curv(self, Phi, Pi, coef, dt, nsteps, eps_ko=0.1, bc=None):
+        """Classical RK4 of the curvilinear first-order system (stage order of
+        test/wave3d_tests.jl:32-44)."""
+        Phi = np.array(Phi, dtype=np.float64, copy=True)
+        Pi = np.array(Pi, dtype=np.float64, copy=True)
+        for _ in range(nsteps):
+            k1 = self.fo_rhs_curv(Phi, Pi, coef, eps_ko, bc)
+            k2 = self.fo_rhs_curv(Phi + dt / 2 * k1[0], Pi + dt / 2 * k1[1], coef, eps_ko, bc)
+            k3 = self.fo_rhs_curv(Phi + dt / 2 * k2[0], Pi + dt / 2 * k2[1], coef, eps_ko, bc)
+            k4 = self.fo_rhs_curv(Phi + dt * k3[0], Pi + dt * k3[1], coef, eps_ko, bc)
+            Phi += dt / 6 * (k1[0] + 2 * k2[0] + 2 * k3[0] + k4[0])
+            Pi += dt / 6 * (k1[1] + 2 * k2[1] + 2 * k3[1] + k4[1])
+        return Phi, Pi
+
+    for f in (curv_metric, curv_grad, curv_div, curv_D, curv_mu, fo_rhs_curv, fo_energy_curv, fo_rk4_curv):
+        setattr(Oracle, f.__name__, f)
+
+
+_curv_methods()

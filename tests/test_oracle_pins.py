@@ -1002,3 +1002,193 @@ def test_oracle_cubed_cube_rayleigh_and_symmetry():
     a, b = np.dot(v * H, o.lap(u)), np.dot(u * H, o.lap(v))
     assert abs(a - b) < 1e-11 * math.sqrt(abs(np.dot(u * H, o.lap(u)) * np.dot(v * H, o.lap(v))))
     assert np.dot(u * H, o.lap(u)) < 0
+
+
+# ------------------------------------------------------------------ first-order system, curvilinear meshes
+
+def _flat_bg10(n, shift=(0.0, 0.0, 0.0)):
+    bg = np.zeros((n, 10))
+    bg[:, 0] = 1.0
+    bg[:, 1:4] = shift
+    bg[:, 4] = bg[:, 7] = bg[:, 9] = 1.0
+    return bg
+
+
+def _outer_nodes(mesh):
+    N = mesh.N
+    out = np.zeros((mesh.Ne, N, N, N), bool)
+    for e, f in zip(*np.nonzero(mesh.neighbour == 0)):
+        sl = [e, slice(None), slice(None), slice(None)]
+        sl[3 - f // 2] = N - 1 if f % 2 else 0
+        out[tuple(sl)] = True
+    return out.reshape(-1)
+
+
+def test_curv_metric_free_stream_and_skew_pair():
+    """Curvilinear first-order path (src/wave3d_curved.jl:218-238, METHODS.md
+    "Curvilinear path"): with the conservative-curl metric terms a constant
+    has a zero gradient and a constant vector a zero divergence to rounding
+    (the discrete metric identities) on the cubed cube, the inflated cube and
+    the radial shell; the gradient of a linear function is exact where the
+    element maps are polynomials of degree < N (cubed cube); and the split-form
+    gradient and divergence with the centred SAT are a skew-adjoint pair in
+    Hd = w_i w_j w_k |J|: <v, div F> + <grad v, F> = 0 for fields that vanish at
+    the outer face nodes (the interface terms cancel, across unlike and
+    turned faces and left-handed elements)."""
+    import wavetoy_hip as wh
+    meshes = [wh.make_cubed_cube_mesh(4, 2, 0.3), wh.make_inflated_cube_mesh(4, 0.1, 0.3, 1.0, 1),
+              wh.make_radial_shell_mesh(5, 0.5, 1.0, 1)]
+    rng = np.random.default_rng(11)
+    for q, mesh in enumerate(meshes):
+        o = _mesh_oracle(mesh)
+        JA, J = o.curv_metric()
+        scale = np.max(np.abs(o.curv_grad(rng.standard_normal(o.n))))
+        assert np.max(np.abs(o.curv_grad(np.full(o.n, 2.5)))) < 1e-13 * scale
+        assert np.max(np.abs(o.curv_div(np.stack([np.ones(o.n), -0.5 * np.ones(o.n), 0.3 * np.ones(o.n)])))) < 1e-13 * scale
+        if q == 0:
+            x = o.coords()
+            g = o.curv_grad(x[:, 0] + 2 * x[:, 1] - x[:, 2])
+            assert np.max(np.abs(g - np.array([1.0, 2.0, -1.0])[:, None])) < 1e-12
+        out = _outer_nodes(mesh)
+        v = rng.standard_normal(o.n)
+        F = rng.standard_normal((3, o.n))
+        v[out] = 0.0
+        F[:, out] = 0.0
+        _, w, _ = wo.gll(mesh.N)
+        Hd = np.tile(np.einsum("k,j,i->kji", w, w, w).reshape(-1), mesh.Ne) * J
+        a = np.dot(v * Hd, o.curv_div(F))
+        b = np.sum(o.curv_grad(v) * F * Hd)
+        assert abs(a + b) < 1e-12 * abs(a), (q, a, b)
+
+
+def test_curv_grad_div_converge():
+    """Gradient and divergence of smooth closed-form fields on the inflated
+    cube (13 patches, spherical shells): max-norm error falls with M, by more
+    than 4x per halving at N = 4 and 12x at N = 6."""
+    import wavetoy_hip as wh
+    for N, Ms, rate in ((4, (1, 2, 4), 4.0), (6, (1, 2), 12.0)):
+        errs = []
+        for M in Ms:
+            o = _mesh_oracle(wh.make_inflated_cube_mesh(N, 0.1, 0.3, 1.0, M))
+            x = o.coords()
+            phi = np.sin(2 * x[:, 0] + 1) * np.cos(1.5 * x[:, 1]) + x[:, 2] ** 3
+            gex = np.stack([2 * np.cos(2 * x[:, 0] + 1) * np.cos(1.5 * x[:, 1]),
+                            -1.5 * np.sin(2 * x[:, 0] + 1) * np.sin(1.5 * x[:, 1]), 3 * x[:, 2] ** 2])
+            F = np.stack([np.sin(x[:, 1]) + x[:, 0] ** 2, x[:, 0] * x[:, 2], np.cos(x[:, 0] + x[:, 2])])
+            dex = 2 * x[:, 0] - np.sin(x[:, 0] + x[:, 2])
+            eg = np.max(np.abs(o.curv_grad(phi) - gex)) / np.max(np.abs(gex))
+            ed = np.max(np.abs(o.curv_div(F) - dex)) / np.max(np.abs(dex))
+            errs.append(max(eg, ed))
+        for a, b in zip(errs, errs[1:]):
+            assert a / b > rate, (N, errs)
+
+
+def test_curv_fo_reduces_to_the_affine_path():
+    """An axis-aligned uniform mesh handed over as a general mesh: the
+    curvilinear right-hand side (metric terms, split form, centred SAT, KO
+    along each element axis with the element extent as h, the physical-normal
+    boundary pass with Sommerfeld or full-Dirichlet data) equals the affine
+    path's wave3d_curved_rhs!, itself pinned to the PDE and the reference's
+    tests, to 1e-12; mu and the energy agree too."""
+    import wavetoy_hip as wh
+    N, M = 5, (3, 2, 2)
+    ref = wo.Oracle(N, M, x1=(1.0, 0.7, 0.8))
+    X = ref.coords().reshape(-1, N, N, N, 3)
+    nbr, nface, bdry = wh.connect_by_coords(N, X, 7)
+    om = wo.Oracle.from_mesh(N, X, nbr, nface, bdry)
+    x = ref.coords()
+    rng = np.random.default_rng(3)
+    bg = _flat_bg10(ref.n, 0.05 * rng.standard_normal(3))
+    bg[:, 0] = 1 + 0.1 * np.sin(3 * x[:, 0])
+    bg[:, 4:10] = (1.1, 0.05, 0.02, 0.9, -0.03, 1.2)
+    coef = wo.fo_coef(bg.reshape(-1))
+    Phi, Pi = rng.standard_normal(ref.n), rng.standard_normal(ref.n)
+    assert abs(om.curv_mu() / ref.mu - 1) < 1e-12
+    for eps in (0.0, 0.1):
+        for kind, kw in (("sommerfeld", {}), ("full_dirichlet", dict(gPhi=0.3 * Phi, gPi=-0.2 * Pi))):
+            a = ref.fo_rhs(Phi, Pi, coef, eps, wo.make_bc3d([kind] * 6, **kw))
+            b = om.fo_rhs_curv(Phi, Pi, coef, eps, wo.make_bc3d(kind, **kw))
+            for i in range(2):
+                assert np.max(np.abs(a[i] - b[i])) < 1e-12 * np.max(np.abs(a[i])), (eps, kind, i)
+    assert abs(om.fo_energy_curv(Phi, Pi, coef) / ref.fo_energy(Phi, Pi, coef) - 1) < 1e-13
+
+
+def test_curv_fo_sommerfeld_spectrum_and_excision_drain():
+    """On the 7-patch ball (N = 3) with the Sommerfeld outer sphere and no KO
+    the first-order operator's spectrum has max Re lambda <= rounding (the
+    reference's curved-Sommerfeld stability check); on the radial shell with
+    the outward radial shift (superluminal at the excised inner sphere, tag 8,
+    no SAT there) a noisy evolution drains out with non-increasing energy
+    (METHODS.md "3D BH-excision")."""
+    import wavetoy_hip as wh
+    mesh = wh.make_cubed_ball_mesh(3, 1, 1.0, outer_tag=7)
+    o = _mesh_oracle(mesh)
+    n = o.n
+    coef = wo.fo_coef(_flat_bg10(n).reshape(-1))
+    bc = wo.make_bc3d("sommerfeld")
+    A = np.zeros((2 * n, 2 * n))
+    for j in range(2 * n):
+        e = np.zeros(2 * n)
+        e[j] = 1.0
+        a, b = o.fo_rhs_curv(e[:n], e[n:], coef, 0.0, bc)
+        A[:n, j], A[n:, j] = a, b
+    lam = np.linalg.eigvals(A)
+    assert lam.real.max() < 1e-12 * np.abs(lam).max()
+    R1, R2, V = 0.5, 1.0, 1.5
+    mesh = wh.make_radial_shell_mesh(4, R1, R2, 2)
+    o = _mesh_oracle(mesh)
+    x = o.coords()
+    r = np.linalg.norm(x, axis=1)
+    br = V + (0.05 - V) * (r - R1) / (R2 - R1)
+    bg = _flat_bg10(o.n)
+    bg[:, 1:4] = (br / r)[:, None] * x
+    coef = wo.fo_coef(bg.reshape(-1))
+    bc = wo.make_bc3d("sommerfeld", excision_tag=8)
+    rng = np.random.default_rng(0)
+    Phi, Pi = 1e-3 * rng.standard_normal(o.n), 1e-3 * rng.standard_normal(o.n)
+    dt = 0.1 * o.hmin / (V + 1)
+    E = [o.fo_energy_curv(Phi, Pi, coef)]
+    for _ in range(6):
+        Phi, Pi = o.fo_rk4_curv(Phi, Pi, coef, dt, 20, eps_ko=0.1, bc=bc)
+        E.append(o.fo_energy_curv(Phi, Pi, coef))
+    assert np.all(np.diff(E) < 0) and E[-1] < 0.2 * E[0], E
+
+
+def test_curv_fo_plane_wave_converges_with_dirichlet_data():
+    """The curvilinear field-radiation Dirichlet pass with exact boundary data
+    (Pi and grad Phi of the plane wave, src/boundaries3d.jl:131-236): RK4 of the
+    plane wave sin(k (x + y + z) - k sqrt(3) t) on the cubed cube to t = 0.05;
+    the Hd-norm error falls with M."""
+    import wavetoy_hip as wh
+    k = 2 * math.pi / 2
+    w = k * math.sqrt(3)
+    errs = []
+    for M in (1, 2):
+        mesh = wh.make_cubed_cube_mesh(5, M, 0.3)
+        o = _mesh_oracle(mesh)
+        x = o.coords()
+        ph = lambda t: np.sin(k * x.sum(axis=1) - w * t)
+        pi = lambda t: -w * np.cos(k * x.sum(axis=1) - w * t)
+        coef = wo.fo_coef(_flat_bg10(o.n).reshape(-1))
+        JA, J = o.curv_metric()
+        _, wq, _ = wo.gll(5)
+        Hd = np.tile(np.einsum("k,j,i->kji", wq, wq, wq).reshape(-1), mesh.Ne) * J
+        dt, T = 0.1 * o.hmin / math.sqrt(3), 0.05
+        ns = int(math.ceil(T / dt))
+        dt = T / ns
+        Phi, Pi = ph(0.0), pi(0.0)
+        t = 0.0
+        for _ in range(ns):
+            def rhs(P, Q, tt):
+                g = k * np.cos(k * x.sum(axis=1) - w * tt)
+                bc = wo.make_bc3d("dirichlet", gPi=pi(tt), gD=np.stack([g, g, g]))
+                return o.fo_rhs_curv(P, Q, coef, 0.0, bc)
+            k1 = rhs(Phi, Pi, t)
+            k2 = rhs(Phi + dt / 2 * k1[0], Pi + dt / 2 * k1[1], t + dt / 2)
+            k3 = rhs(Phi + dt / 2 * k2[0], Pi + dt / 2 * k2[1], t + dt / 2)
+            k4 = rhs(Phi + dt * k3[0], Pi + dt * k3[1], t + dt)
+            Phi = Phi + dt / 6 * (k1[0] + 2 * k2[0] + 2 * k3[0] + k4[0])
+            Pi = Pi + dt / 6 * (k1[1] + 2 * k2[1] + 2 * k3[1] + k4[1])
+            t += dt
+        errs.append(math.sqrt(np.sum(Hd * (Phi - ph(t)) ** 2) / np.sum(Hd * ph(t) ** 2)))
+    assert errs[1] < errs[0] / 4 and errs[1] < 5e-3, errs

@@ -11,6 +11,6 @@ from .core import (  # noqa: F401
     BC3d, make_bc3d, wave3d_curved_rhs, wave3d_energy, eigenmode_radial, outgoing_pulse,
     FACE_SUBLUMINAL, FACE_OUTFLOW, FACE_INFLOW, FACE_SONIC, classify_face1d, classify_face3d,
     validate_bc1d, HexMesh, connect_by_coords, make_cubed_ball_mesh,
-    make_cubed_cube_mesh, make_inflated_cube_mesh,
+    make_cubed_cube_mesh, make_inflated_cube_mesh, make_radial_shell_mesh, EXCISION_BDRY_TAG,
 )
 from .evolve import evolve3d  # noqa: F401

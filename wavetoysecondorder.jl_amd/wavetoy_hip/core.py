@@ -200,17 +200,18 @@ def _sphere_surface(r):
     return f
 
 
-def _patch_mesh(N, M, a, layers):
+def _patch_mesh(N, M, a, layers, core=True):
     """Coordinates (Ne, N, N, N, 3) of a patch mesh: the cube [-a, a]^3 of M^3
-    affine elements (x fastest), then for each layer (inner, outer surface)
-    six patches of M^3 elements, one per cube face (axis d, sign -, +), with
-    local axes (s, t, rho): s, t in [-1, 1] along axes d+1, d+2 (cyclic), rho
-    in [0, 1] outward, x = (1 - rho) inner(s, t) + rho outer(s, t)."""
+    affine elements (x fastest; left out with core=False), then for each layer
+    (inner, outer surface) six patches of M^3 elements, one per cube face
+    (axis d, sign -, +), with local axes (s, t, rho): s, t in [-1, 1] along
+    axes d+1, d+2 (cyclic), rho in [0, 1] outward,
+    x = (1 - rho) inner(s, t) + rho outer(s, t)."""
     xs = make_element(N).xs
     g = (np.arange(M)[:, None] + xs[None, :]) / M          # (element, node) -> [0, 1]
     els = []
     c = -a + 2.0 * a * g
-    for ez in range(M):
+    for ez in range(M if core else 0):
         for ey in range(M):
             for ex in range(M):
                 Z, Y, Xc = np.meshgrid(c[ez], c[ey], c[ex], indexing="ij")
@@ -270,6 +271,24 @@ def make_cubed_ball_mesh(N, M, R=1.0, a=None, outer_tag=1):
     a = 0.5 * R if a is None else float(a)
     X = _patch_mesh(N, M, a, [(_cube_surface(a), _sphere_surface(R))])
     return HexMesh(N, X, *connect_by_coords(N, X, outer_tag))
+
+
+EXCISION_BDRY_TAG = 8    # excision / pure-outflow outer faces: no SAT (src/wave_strong_rhs.jl:188-213)
+
+
+def make_radial_shell_mesh(N, R1=0.5, R2=1.0, M=2, inner_tag=EXCISION_BDRY_TAG, outer_tag=7):
+    """The spherical shell R1 <= |x| <= R2 as 6 patches of M^3 elements
+    (make_radial_shell_mesh, src/evolve.jl:922-930): one per cube face,
+    equiangular in the angles, linear in the radius.  The inner sphere carries
+    inner_tag (8: excision, no SAT) and the outer one outer_tag (7:
+    Sommerfeld, 1: Dirichlet)."""
+    X = _patch_mesh(N, M, R1, [(_sphere_surface(R1), _sphere_surface(R2))], core=False)
+    nbr, nface, bdry = connect_by_coords(N, X, outer_tag)
+    mesh = HexMesh(N, X, nbr, nface, bdry)
+    for e, f in zip(*np.nonzero(mesh.neighbour == 0)):
+        r = np.linalg.norm(_face_nodes(mesh.coords[e:e + 1], f).reshape(-1, 3), axis=1).mean()
+        mesh.bdry[e, f] = inner_tag if abs(r - R1) < abs(r - R2) else outer_tag
+    return mesh
 
 
 # -------------------------------------------------------- device workspace
