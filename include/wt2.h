@@ -110,7 +110,8 @@ int wt2_ctx_create(const wt2_mesh_desc* desc, wt2_ctx** out);
  * Otherwise orientation[f,e] = s + 2 rp + 4 rq maps this face's node (p,q)
  * to the neighbour face's (p',q') = (q,p) if s else (p,q), then p' -> N-1-p'
  * if rp, q' -> N-1-q' if rq.  Outer faces carry bdry tag 1..6 (Dirichlet
- * value desc->bdry_values[tag-1]) or 7 (Sommerfeld).  desc supplies N, tau,
+ * value desc->bdry_values[tag-1]), 7 (Sommerfeld) or 8 (excision: no SAT,
+ * src/wave_strong_rhs.jl:188-213).  desc supplies N, tau,
  * precision, flags, bdry_values and sommerfeld_inv_R; eps_ko must be 0.  The
  * geometry J = dx/dxi is G applied to coords.  Ne must factor as
  * M0*M1*M2 with each factor within wt2_ctx_create's per-axis limits. */
@@ -201,13 +202,18 @@ int wt2_energy(wt2_ctx* ctx, const double* u, const double* ud, double* E, int32
 #define WT2_BC_EXCISION       3
 #define WT2_BC_FULL_DIRICHLET 4
 
-/* make_bc3d (src/boundaries3d.jl:44-54) for the affine pass. */
+/* make_bc3d (src/boundaries3d.jl:44-54).  Affine meshes: one kind per side.
+ * General meshes (wt2_ctx_create_mesh): kinds[0] on every outer face except
+ * those whose bdry tag equals excision_tag, which get no SAT
+ * (_apply_bc3d_curv!, src/boundaries3d.jl:131-165). */
 typedef struct wt2_bc3d {
     int32_t kinds[6];        /* WT2_BC_* per face -x,+x,-y,+y,-z,+z          */
-    int32_t reserved;
+    int32_t excision_tag;    /* general meshes: outer faces with this tag get no SAT (0: none) */
     double  sigma;           /* penalty scale (reference default 1)           */
     const double* gPhi;      /* field-radiation / full-state targets, per node */
     const double* gPi;       /* (nullable: zero)                               */
+    const double* gD;        /* general meshes, WT2_BC_DIRICHLET: the grad Phi
+                                target, 3 fields of npoints (nullable: zero)  */
 } wt2_bc3d;
 
 /* sample_background3d! (src/wave3d_curved.jl:75-115) on the device: coef11
@@ -219,7 +225,14 @@ int wt2_fo_coef(wt2_ctx* ctx, const double* bg10, double* coef11, int32_t flags)
  * DPhi_d = apply_D!(Phi, d); F^i = b^i Pi + alpha sqrtg g^{ij} DPhi_j;
  * Pidot = sum_i apply_D!(F^i, i); Phidot = b . DPhi + (alpha/sqrtg) Pi; then
  * eps mu^-5 D_d^6 on both fields; then the affine boundary pass
- * (src/boundaries3d.jl:76-126) when bc3d is non-null.  FP64. */
+ * (src/boundaries3d.jl:76-126) when bc3d is non-null.  FP64.
+ * On a context from wt2_ctx_create_mesh, the curvilinear path (metric given,
+ * :218-256): the gradient and divergence in the split form with metric terms
+ * in the conservative-curl form (free stream to rounding) and the centred
+ * SAT across element faces, KO along each element's reference axes (the
+ * element's extent in place of h; mu from 30 power iterations of that
+ * derivative), and the physical-normal boundary pass _apply_bc3d_curv!
+ * (src/boundaries3d.jl:131-280). */
 int wt2_fo_rhs(wt2_ctx* ctx, double* dPhi, double* dPi, const double* Phi, const double* Pi,
                const double* coef11, double eps_ko, const wt2_bc3d* bc3d, int32_t flags);
 

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "wavetoy_hip", "libwt2.so")
-SOURCES = ["wt2_flat.hip", "wt2_curved.hip", "wt2_fo.hip", "wt2_capi.hip"]
+SOURCES = ["wt2_flat.hip", "wt2_curved.hip", "wt2_fo.hip", "wt2_focurv.hip", "wt2_capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

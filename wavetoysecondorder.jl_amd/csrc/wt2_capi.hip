@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "wt2_curved.hpp"
+#include "wt2_focurv.hpp"
 #include "wt2_internal.hpp"
 
 namespace wt2 {
@@ -231,6 +232,11 @@ struct wt2_ctx {
     int Ecap;
     CurvedData cd;
     double* fo_D;   // first-order path: DPhi_1..3 (lazy)
+    // first-order path on general meshes (lazy): metric terms, |J|, element
+    // extents, flux, KO channels, G and w on the device, mu of the KO derivative
+    double* fc_JAp; double* fc_Jp; double* fc_ell; double* fc_F; double* fc_ko;
+    double* fc_ops;
+    double fc_mu;
     int64_t bytes;
     // captured RK4 step
     hipGraphExec_t gexec;
@@ -741,9 +747,10 @@ int wt2_ctx_create_mesh(const wt2_mesh_desc* desc, int64_t Ne, const double* coo
             int2 cf;
             if (nb == 0) {
                 const int tag = bdry[k];
-                if (tag < 1 || tag > SOMMERFELD_TAG) {
+                if (tag < 1 || tag > EXCISION_TAG) {
                     std::snprintf(buf, sizeof(buf),
-                                  "wt2_ctx_create_mesh: outer face %d of element %lld has tag %d (1..6 Dirichlet, 7 Sommerfeld)",
+                                  "wt2_ctx_create_mesh: outer face %d of element %lld has tag %d (1..6 Dirichlet, 7 Sommerfeld, "
+                                  "8 excision)",
                                   f + 1, (long long)(e + 1), tag);
                     set_error(buf);
                     return WT2_ERR_ARG;
@@ -1035,6 +1042,7 @@ void wt2_ctx_destroy(wt2_ctx* c)
     hipFree(c->Lu); hipFree(c->red); hipFree(c->Edev);
     hipFree(c->op64); hipFree(c->op32); hipFree(c->op32c);
     hipFree(c->fo_D);
+    hipFree(c->fc_JAp); hipFree(c->fc_Jp); hipFree(c->fc_ell); hipFree(c->fc_F); hipFree(c->fc_ko); hipFree(c->fc_ops);
     curved_free(c->cd);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
     delete c;
@@ -1483,15 +1491,82 @@ int wt2_energy(wt2_ctx* c, const double* u, const double* ud, double* E, int32_t
 
 namespace {
 
+bool fo_general(const wt2_ctx* c) { return c->cd.conn != nullptr; }
+
+void fc_desc(const wt2_ctx* c, FcDesc& d)
+{
+    std::memset(&d, 0, sizeof(d));
+    d.N = c->N;
+    d.npts = c->npts;
+    d.G = c->fc_ops;
+    d.w = c->fc_ops + 256;
+    d.conn = c->cd.conn;
+    d.JAp = c->fc_JAp; d.Jp = c->fc_Jp; d.ell = c->fc_ell;
+    d.D = c->fo_D; d.F = c->fc_F; d.ko = c->fc_ko;
+    d.partial = c->red;
+}
+
+// The first-order path: affine uniform meshes (per-axis apply_D!), or general
+// meshes given by coordinates and connectivity (metric terms set up here on
+// first use).  Warped uniform meshes and context backgrounds are refused (the
+// coefficients come in coef11; a warped mesh can be handed over as a general
+// mesh).
 int fo_ready(wt2_ctx* c, const char* who)
 {
-    if (c->curved || c->d.warp_A != 0.0) {
-        set_error(std::string(who) + ": the first-order path is built for affine meshes without a context background "
-                                     "(the coefficients come in coef11)");
+    if (!fo_general(c) && (c->curved || c->d.warp_A != 0.0)) {
+        set_error(std::string(who) + ": the first-order path is built for affine meshes and for meshes given by "
+                                     "coordinates and connectivity, without a context background (the "
+                                     "coefficients come in coef11)");
         return WT2_ERR_UNSUPPORTED;
     }
-    if (!c->fo_D) return alloc_dev(c, reinterpret_cast<void**>(&c->fo_D), size_t(3) * size_t(c->npts) * sizeof(double));
-    return WT2_OK;
+    const size_t n = size_t(c->npts);
+    if (!c->fo_D) {
+        const int rc = alloc_dev(c, reinterpret_cast<void**>(&c->fo_D), 3 * n * sizeof(double));
+        if (rc) return rc;
+    }
+    if (!fo_general(c) || c->fc_JAp) return WT2_OK;
+    int rc = alloc_dev(c, reinterpret_cast<void**>(&c->fc_JAp), 9 * n * sizeof(double));
+    if (!rc) rc = alloc_dev(c, reinterpret_cast<void**>(&c->fc_Jp), n * sizeof(double));
+    if (!rc) rc = alloc_dev(c, reinterpret_cast<void**>(&c->fc_ell), 3 * size_t(c->Ne) * sizeof(double));
+    if (!rc) rc = alloc_dev(c, reinterpret_cast<void**>(&c->fc_F), 3 * n * sizeof(double));
+    if (!rc) rc = alloc_dev(c, reinterpret_cast<void**>(&c->fc_ko), 12 * n * sizeof(double));
+    if (!rc) rc = alloc_dev(c, reinterpret_cast<void**>(&c->fc_ops), (256 + 16) * sizeof(double));
+    double* xyz = nullptr;
+    if (!rc && hipMalloc(&xyz, 3 * n * sizeof(double)) != hipSuccess) {
+        set_error(std::string(who) + ": device staging allocation failed");
+        rc = WT2_ERR_OOM;
+    }
+    if (!rc) {
+        double ops[256 + 16] = {0};
+        for (int q = 0; q < c->N * c->N; ++q) ops[q] = c->G[q];
+        for (int q = 0; q < c->N; ++q) ops[256 + q] = c->w[q];
+        rc = pass_rc(hipMemcpyAsync(c->fc_ops, ops, sizeof(ops), hipMemcpyHostToDevice, c->stream), "metric set-up copy");
+        if (!rc) rc = pass_rc(hipMemcpyAsync(xyz, c->cd.xyz.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice,
+                                             c->stream), "metric set-up copy");
+        FcDesc d;
+        fc_desc(c, d);
+        d.xyz = xyz;
+        if (!rc) rc = pass_rc(launch_fc(d, 0, c->stream), "metric terms");
+        if (!rc) rc = sync(c);
+    }
+    if (xyz) (void)hipFree(xyz);
+    c->fc_mu = -1.0;
+    if (rc) {
+        hipFree(c->fc_JAp); c->fc_JAp = nullptr;
+    }
+    return rc;
+}
+
+// mu of the general-mesh KO derivative, computed once
+int fc_mu_ready(wt2_ctx* c)
+{
+    if (c->fc_mu >= 0) return WT2_OK;
+    FcDesc d;
+    fc_desc(c, d);
+    double mu = 0.0;
+    const int rc = pass_rc(fc_mu(d, c->fc_ko, c->fc_ko + c->npts, c->stream, &mu), "KO spectral radius");
+    if (!rc) c->fc_mu = mu;
+    return rc;
 }
 
 void fo_desc(const wt2_ctx* c, FoDesc& fd)
@@ -1599,15 +1674,36 @@ int wt2_fo_rhs(wt2_ctx* c, double* dPhi, double* dPi, const double* Phi, const d
         set_error("device field pointers must be 16-byte aligned");
         return WT2_ERR_ARG;
     }
-    fd.koT = eps_ko * c->inv_mu5;
-    if (bc3d) {
-        for (int f = 0; f < 6; ++f) fd.kinds[f] = bc3d->kinds[f];
-        fd.sigma = bc3d->sigma;
+    if (fo_general(c)) {
+        // curvilinear path: one kind on every outer face but the excision tag
+        const double* gD = bc3d ? bc3d->gD : nullptr;
+        if (host && gD) rc = tmp.in(c, gD, 3 * n, &gD);
+        else if (!host && gD && misaligned(gD)) { set_error("device field pointers must be 16-byte aligned"); return WT2_ERR_ARG; }
+        if (!rc && eps_ko > 0) rc = fc_mu_ready(c);
+        if (rc) return rc;
+        FcDesc d;
+        fc_desc(c, d);
+        d.Phi = fd.Phi; d.Pi = fd.Pi; d.coef = fd.coef; d.dPhi = fd.dPhi; d.dPi = fd.dPi;
+        d.koT = (eps_ko > 0 && c->fc_mu > 0) ? eps_ko * std::pow(1.0 / c->fc_mu, 5) : 0.0;
+        d.has_bc = bc3d != nullptr;
+        if (bc3d) {
+            d.kind = bc3d->kinds[0];
+            d.excision_tag = bc3d->excision_tag;
+            d.sigma = bc3d->sigma;
+            d.gPhi = fd.gPhi; d.gPi = fd.gPi; d.gD = gD;
+        }
+        rc = pass_rc(launch_fc(d, 1, c->stream), "first-order RHS (general mesh)");
     } else {
-        for (int f = 0; f < 6; ++f) fd.kinds[f] = WT2_BC_EXCISION;   // no boundary terms
-        fd.sigma = 0.0;
+        fd.koT = eps_ko * c->inv_mu5;
+        if (bc3d) {
+            for (int f = 0; f < 6; ++f) fd.kinds[f] = bc3d->kinds[f];
+            fd.sigma = bc3d->sigma;
+        } else {
+            for (int f = 0; f < 6; ++f) fd.kinds[f] = WT2_BC_EXCISION;   // no boundary terms
+            fd.sigma = 0.0;
+        }
+        rc = pass_rc(launch_fo_rhs(fd, c->stream), "first-order RHS");
     }
-    rc = pass_rc(launch_fo_rhs(fd, c->stream), "first-order RHS");
     if (!rc && host) {
         rc = pass_rc(hipMemcpyAsync(dPhi, fd.dPhi, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy out");
         if (!rc) rc = pass_rc(hipMemcpyAsync(dPi, fd.dPi, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy out");
@@ -1635,6 +1731,22 @@ int wt2_fo_energy(wt2_ctx* c, const double* Phi, const double* Pi, const double*
     } else if (!fo_aligned({Phi, Pi, coef11})) {
         set_error("device field pointers must be 16-byte aligned");
         return WT2_ERR_ARG;
+    }
+    if (fo_general(c)) {
+        FcDesc d;
+        fc_desc(c, d);
+        d.Phi = fd.Phi; d.Pi = fd.Pi; d.coef = fd.coef;
+        d.grad_only = 1;
+        rc = pass_rc(launch_fc(d, 2, c->stream), "first-order energy (general mesh)");
+        std::vector<double> part(RED_BLOCKS);
+        if (!rc) rc = pass_rc(hipMemcpyAsync(part.data(), c->red, RED_BLOCKS * sizeof(double), hipMemcpyDeviceToHost,
+                                             c->stream), "copy out");
+        if (!rc) rc = sync(c);
+        if (rc) return rc;
+        double s = 0.0;
+        for (double p : part) s += p;
+        *E = s;
+        return WT2_OK;
     }
     fd.grad_only = 1;
     for (int f = 0; f < 6; ++f) fd.kinds[f] = WT2_BC_EXCISION;

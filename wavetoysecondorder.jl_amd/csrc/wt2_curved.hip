@@ -572,7 +572,7 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
             // this outer face is a Sommerfeld face; bvf: its Dirichlet value
             int64_t en;
             int fn, pn = p, qn = qq;
-            bool som;
+            bool som, exc = false;
             TC bvf;
             if (GEN && a.conn) {
                 const int2 cf = a.conn[e * 6 + f];
@@ -583,12 +583,20 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
                 if (ori & 2) pn = N - 1 - pn;
                 if (ori & 4) qn = N - 1 - qn;
                 som = tag == 7;
+                exc = tag == 8;
                 bvf = (tag >= 1 && tag <= 6) ? a.k.bv[tag - 1] : TC(0);
             } else {
                 en = nbr_elem(e, ax, side, a.M0, a.M1, a.M2, a.per0, a.per1, a.per2);
                 fn = (ax << 1) | (1 - side);
                 som = (a.k.som >> f) & 1;
                 bvf = a.k.bv[f];
+            }
+            if (en < 0 && exc) {
+                // excision face (src/wave_strong_rhs.jl:188-213): no SAT at all
+#pragma unroll
+                for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = TC(0);
+                Pt[q] = TC(0);
+                continue;
             }
             if (en < 0 && som) {
                 // Sommerfeld face (src/wave.jl:444-518): no SIPG terms; the drag

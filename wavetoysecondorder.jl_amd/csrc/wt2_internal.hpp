@@ -14,6 +14,7 @@ constexpr int NMIN = 3;
 constexpr int NMAX = 10;
 constexpr int MMAX_AXIS = 64;   // one element line per axis fits one wavefront
 constexpr int SOMMERFELD_TAG = 7;   // face_kind of a Sommerfeld face (SOMMERFELD_BDRY_TAG, src/wave.jl:23)
+constexpr int EXCISION_TAG = 8;     // excision / pure-outflow outer face: no SAT (src/wave_strong_rhs.jl:188-213)
 
 // Epilogue of a pencil pass: what is done with the axis contribution c.
 enum Epi : int {

@@ -64,10 +64,11 @@ class MeshDesc(ctypes.Structure):
 class BC3d(ctypes.Structure):
     _fields_ = [
         ("kinds", ctypes.c_int32 * 6),
-        ("reserved", ctypes.c_int32),
+        ("excision_tag", ctypes.c_int32),
         ("sigma", ctypes.c_double),
         ("gPhi", ctypes.c_void_p),
         ("gPi", ctypes.c_void_p),
+        ("gD", ctypes.c_void_p),
     ]
 
 

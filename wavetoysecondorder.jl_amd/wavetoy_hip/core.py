@@ -528,17 +528,19 @@ class Workspace:
     def fo_rhs(self, dPhi, dPi, Phi, Pi, coef, eps_ko=0.1, bc3d=None):
         """wave3d_curved_rhs!(Phidot, Pidot, Phi, Pi, coef; eps_KO, bc3d) on the
         affine path (src/wave3d_curved.jl:205-258)."""
-        g = (None, None) if bc3d is None else (bc3d.gPhi_arr, bc3d.gPi_arr)
-        (p1, p2, p3, p4, pc, pg1, pg2), fl = self._ptrs(dPhi, dPi, Phi, Pi, coef, g[0], g[1],
-                                                        writable=(0, 1), mult={4: 11})
+        g = (None, None, None) if bc3d is None else (bc3d.gPhi_arr, bc3d.gPi_arr, bc3d.gD_arr)
+        (p1, p2, p3, p4, pc, pg1, pg2, pg3), fl = self._ptrs(dPhi, dPi, Phi, Pi, coef, g[0], g[1], g[2],
+                                                             writable=(0, 1), mult={4: 11, 7: 3})
         bc = None
         if bc3d is not None:
             bc = _lib.BC3d()
             for f in range(6):
                 bc.kinds[f] = bc3d.kinds[f]
+            bc.excision_tag = bc3d.excision_tag
             bc.sigma = bc3d.sigma
             bc.gPhi = pg1
             bc.gPi = pg2
+            bc.gD = pg3
         self._sync_torch()
         check(lib().wt2_fo_rhs(self.ctx, p1, p2, p3, p4, pc, float(eps_ko),
                                None if bc is None else ctypes.byref(bc), fl))
@@ -646,17 +648,24 @@ class BC3d:
     _NAMES = {"dirichlet": _lib.WT2_BC_DIRICHLET, "sommerfeld": _lib.WT2_BC_SOMMERFELD,
               "excision": _lib.WT2_BC_EXCISION, "full_dirichlet": _lib.WT2_BC_FULL_DIRICHLET}
 
-    def __init__(self, kinds, sigma=1.0, gPhi=None, gPi=None):
+    def __init__(self, kinds, sigma=1.0, gPhi=None, gPi=None, gD=None, excision_tag=0):
+        if isinstance(kinds, (str, int)):
+            kinds = (kinds,) * 6
         if len(kinds) != 6:
             raise ValueError("make_bc3d needs six face kinds")
         self.kinds = tuple(self._NAMES[k] if isinstance(k, str) else int(k) for k in kinds)
         self.sigma = float(sigma)
         self.gPhi_arr = gPhi
         self.gPi_arr = gPi
+        self.gD_arr = gD                 # general meshes, Dirichlet kind: grad Phi target (3 fields)
+        self.excision_tag = int(excision_tag)
 
 
-def make_bc3d(kinds, sigma=1.0, gPhi=None, gPi=None):
-    return BC3d(kinds, sigma, gPhi, gPi)
+def make_bc3d(kinds, sigma=1.0, gPhi=None, gPi=None, gD=None, excision_tag=0):
+    """make_bc3d; on general meshes one kind (a single name) for every outer
+    face but those tagged excision_tag (make_bc3d(kinds; excision_tag),
+    src/boundaries3d.jl:44-54, 131-165)."""
+    return BC3d(kinds, sigma, gPhi, gPi, gD, excision_tag)
 
 
 # characteristic classes of a boundary face (src/boundaries1d.jl:79-83)
