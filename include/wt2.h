@@ -236,6 +236,17 @@ int wt2_fo_coef(wt2_ctx* ctx, const double* bg10, double* coef11, int32_t flags)
 int wt2_fo_rhs(wt2_ctx* ctx, double* dPhi, double* dPi, const double* Phi, const double* Pi,
                const double* coef11, double eps_ko, const wt2_bc3d* bc3d, int32_t flags);
 
+/* One classical RK4 stage of the first-order system in place
+ * (OrdinaryDiffEq RK4() stages of src/evolve.jl:1309-1313, stage order of
+ * test/wave3d_tests.jl:32-44): the stage's right-hand side as wt2_fo_rhs, on
+ * the state (stage 1) or the context's stage input, followed by one update
+ * pass over both fields (stages 1-3 write the next stage input and the
+ * accumulator, stage 4 the new state).  Phi and Pi are device pointers
+ * (16-byte aligned).  The caller may change the boundary data of bc3d between
+ * stages (time-dependent targets). */
+int wt2_fo_rk4_stage(wt2_ctx* ctx, int32_t stage, double* Phi, double* Pi, const double* coef11, double dt,
+                     double eps_ko, const wt2_bc3d* bc3d);
+
 /* wave3d_energy (src/wave3d_curved.jl:268-287):
  * E = sum 1/2 [Pi^2/sqrtg + sqrtg g^{ij} DPhi_i DPhi_j] Hphys, FP64,
  * deterministic reduction order. */

@@ -92,12 +92,13 @@ def test_face_kinds_of_the_cube():
     BC = wh.BC3d._NAMES
     N, M = 4, (3, 2, 2)
     n = N ** 3 * 12
-    row, _, _, speed = ev._background3d("minkowski")
-    bg = np.broadcast_to(row, (n, 10)).copy()
+    x = np.zeros((n, 3))
+    bgf, _, _, _, speed = ev._background3d("minkowski")
+    bg = bgf(x)
     assert ev._face_kinds("auto", bg, M, N) == (BC["sommerfeld"],) * 6
     assert speed == math.sqrt(3.0)
-    row, _, _, speed = ev._background3d("constant_shift", (2.0, 0.1, 0.0))
-    bg = np.broadcast_to(row, (n, 10)).copy()
+    bgf, _, _, _, speed = ev._background3d("constant_shift", (2.0, 0.1, 0.0))
+    bg = bgf(x)
     k = ev._face_kinds("auto", bg, M, N)
     assert k == (BC["excision"], BC["full_dirichlet"]) + (BC["sommerfeld"],) * 4
     assert abs(speed - (2.1 + math.sqrt(3.0))) < 1e-15
@@ -105,6 +106,18 @@ def test_face_kinds_of_the_cube():
     assert ev._face_kinds(ok, bg, M, N)[2] == BC["dirichlet"]
     with pytest.raises(ValueError):
         ev._face_kinds(("sommerfeld",) * 6, bg, M, N)
+
+
+def test_radial_shift_background():
+    """The excision shell's background (src/evolve.jl:1129-1145): flat alpha and
+    gamma, an outward shift ramping linearly from V at R1 to 1/20 at R2."""
+    bgf, phi, pi, d, speed = ev._background3d("radial_shift", (1.5, 0.0, 0.0), 0.5, 1.0)
+    x = np.array([[0.5, 0.0, 0.0], [0.0, 0.0, -1.0], [0.0, 0.75, 0.0]])
+    bg = bgf(x)
+    assert np.allclose(bg[:, 0], 1.0) and np.allclose(bg[:, [4, 7, 9]], 1.0) and np.allclose(bg[:, [5, 6, 8]], 0.0)
+    assert np.allclose(bg[0, 1:4], [1.5, 0, 0]) and np.allclose(bg[1, 1:4], [0, 0, -0.05])
+    assert np.allclose(bg[2, 1:4], [0, 0.5 * (1.5 + 0.05), 0])
+    assert speed == 2.5
 
 
 def test_side_nodes():
@@ -130,7 +143,11 @@ def test_driver_argument_errors():
     with pytest.raises(ValueError):
         wh.evolve3d(formulation="bogus", N=4, M=2)
     with pytest.raises(ValueError):
-        wh.evolve3d(formulation="conservative", N=4, M=2, mesh_kind="cubed_cube")
+        wh.evolve3d(formulation="conservative", N=4, M=2, mesh_kind="bogus")
+    with pytest.raises(ValueError, match="exact"):
+        wh.evolve3d(formulation="conservative", N=4, M=2, mesh_kind="cubed_cube", bc="dirichlet", ic="gaussian")
+    with pytest.raises(ValueError, match="sommerfeld"):
+        wh.evolve3d(formulation="conservative", N=4, M=2, mesh_kind="cubed_cube", bc="auto")
     with pytest.raises(ValueError):
         wh.evolve3d(formulation="conservative", N=4, M=2, bc="sommerfeld")
     with pytest.raises(ValueError):
@@ -145,7 +162,7 @@ def test_driver_argument_errors():
 
 def test_evolve3d_mesh_kind_arguments():
     with pytest.raises(ValueError, match="mesh_kind"):
-        wh.evolve3d(mesh_kind="radial_shell", device=False)
+        wh.evolve3d(mesh_kind="bogus_shell", device=False)
     with pytest.raises(ValueError, match="spherical"):
         wh.evolve3d(mesh_kind="cubed_cube", outer_bc="sommerfeld", device=False)
     with pytest.raises(ValueError, match="outer_bc"):

@@ -237,6 +237,7 @@ struct wt2_ctx {
     double* fc_JAp; double* fc_Jp; double* fc_ell; double* fc_F; double* fc_ko;
     double* fc_ops;
     double fc_mu;
+    double* fo_rk;  // first-order RK4 registers: S, A, k (2 fields each, lazy)
     int64_t bytes;
     // captured RK4 step
     hipGraphExec_t gexec;
@@ -506,6 +507,11 @@ static int run_stage(wt2_ctx* c, int stage, const double* u, const double* v, do
 }
 
 // partial: the reduction's RED_BLOCKS partial sums (nullptr: the context's own).
+namespace {
+bool fo_general(const wt2_ctx* c);
+int fc_mu_ready(wt2_ctx* c);
+}  // namespace
+
 static int run_energy(wt2_ctx* c, const double* u, const double* v, double* Edst, double* partial = nullptr)
 {
     static const double zero6[6] = {0, 0, 0, 0, 0, 0};
@@ -1042,6 +1048,7 @@ void wt2_ctx_destroy(wt2_ctx* c)
     hipFree(c->Lu); hipFree(c->red); hipFree(c->Edev);
     hipFree(c->op64); hipFree(c->op32); hipFree(c->op32c);
     hipFree(c->fo_D);
+    hipFree(c->fo_rk);
     hipFree(c->fc_JAp); hipFree(c->fc_Jp); hipFree(c->fc_ell); hipFree(c->fc_F); hipFree(c->fc_ko); hipFree(c->fc_ops);
     curved_free(c->cd);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
@@ -1059,6 +1066,12 @@ int wt2_ctx_info(const wt2_ctx* c, wt2_info* info)
     info->Ne = c->Ne;
     info->curved = c->curved;
     info->mu = c->mu;
+    if (fo_general(c) && c->fc_JAp) {
+        // general mesh with the first-order path set up: mu of its KO derivative
+        const int mrc = fc_mu_ready(const_cast<wt2_ctx*>(c));
+        if (mrc) return mrc;
+        info->mu = c->fc_mu;
+    }
     info->h_min = c->curved ? c->cd.h_min : c->h_min;
     info->tau = c->d.tau;
     info->max_sym_eig = c->max_sym_eig;
@@ -1708,6 +1721,30 @@ int wt2_fo_rhs(wt2_ctx* c, double* dPhi, double* dPi, const double* Phi, const d
         rc = pass_rc(hipMemcpyAsync(dPhi, fd.dPhi, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy out");
         if (!rc) rc = pass_rc(hipMemcpyAsync(dPi, fd.dPi, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy out");
     }
+    if (!rc) rc = sync(c);
+    return rc;
+}
+
+int wt2_fo_rk4_stage(wt2_ctx* c, int32_t stage, double* Phi, double* Pi, const double* coef11, double dt,
+                     double eps_ko, const wt2_bc3d* bc3d)
+{
+    if (!c || !Phi || !Pi || !coef11) { set_error("wt2_fo_rk4_stage: null pointer"); return WT2_ERR_ARG; }
+    if (stage < 1 || stage > 4 || !(dt > 0) || !std::isfinite(dt)) { set_error("wt2_fo_rk4_stage: bad stage / dt"); return WT2_ERR_ARG; }
+    if (misaligned(Phi) || misaligned(Pi)) { set_error("wt2_fo_rk4_stage: device pointers must be 16-byte aligned"); return WT2_ERR_ARG; }
+    int rc = fo_ready(c, "wt2_fo_rk4_stage");
+    if (rc) return rc;
+    const size_t n = size_t(c->npts);
+    const size_t st = (n + 1) & ~size_t(1);    // field stride of the registers, 16-byte aligned
+    if (!c->fo_rk) {
+        rc = alloc_dev(c, reinterpret_cast<void**>(&c->fo_rk), 6 * st * sizeof(double));
+        if (rc) return rc;
+    }
+    double* R = c->fo_rk;                      // S (2 fields), A (2), k (2)
+    const double* in0 = stage == 1 ? Phi : R;
+    const double* in1 = stage == 1 ? Pi : R + st;
+    rc = wt2_fo_rhs(c, R + 4 * st, R + 5 * st, in0, in1, coef11, eps_ko, bc3d, 0);
+    if (rc) return rc;
+    rc = pass_rc(launch_fo_rk_update(Phi, Pi, R, int64_t(st), int64_t(n), stage, dt, c->stream), "first-order RK4 update");
     if (!rc) rc = sync(c);
     return rc;
 }

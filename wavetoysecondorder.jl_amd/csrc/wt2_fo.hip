@@ -490,6 +490,44 @@ hipError_t launch_fo_energy(const FoDesc& fd, hipStream_t s)
     }
 }
 
+// Classical RK4 stage update of the first-order system, both fields in one
+// pass, k = (kPhi, kPi) the stage's right-hand side (test/wave3d_tests.jl:37-42):
+//   stage 1: A = Y + dt/6 k, S = Y + dt/2 k;  stage 2: A += dt/3 k, S = Y + dt/2 k;
+//   stage 3: A += dt/3 k, S = Y + dt k;       stage 4: Y = A + dt/6 k.
+// Y: the caller's (Phi, Pi); S, A, k: the context's registers, field f of
+// each at base + f * stride.
+__global__ void fo_rk_update_kernel(double* __restrict__ Y0, double* __restrict__ Y1, double* __restrict__ R,
+                                    int64_t stride, int64_t n, int stage, double dt)
+{
+    for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < 2 * n; g += int64_t(gridDim.x) * blockDim.x) {
+        const int f = g >= n;
+        const int64_t q = g - f * n;
+        double* Y = f ? Y1 : Y0;
+        double* S = R + f * stride;
+        double* A = R + (2 + f) * stride;
+        const double kq = R[(4 + f) * stride + q];
+        if (stage == 1) {
+            const double y = Y[q];
+            A[q] = y + dt / 6.0 * kq;
+            S[q] = y + 0.5 * dt * kq;
+        } else if (stage == 4) {
+            Y[q] = A[q] + dt / 6.0 * kq;
+        } else {
+            A[q] += dt / 3.0 * kq;
+            S[q] = Y[q] + (stage == 3 ? dt : 0.5 * dt) * kq;
+        }
+    }
+}
+
+hipError_t launch_fo_rk_update(double* Y0, double* Y1, double* R, int64_t stride, int64_t n, int stage, double dt,
+                               hipStream_t s)
+{
+    const int64_t blocks = std::min<int64_t>((2 * n + 255) / 256, 65536);
+    hipLaunchKernelGGL(fo_rk_update_kernel, dim3(unsigned(std::max<int64_t>(blocks, 1))), dim3(256), 0, s, Y0, Y1, R,
+                       stride, n, stage, dt);
+    return hipGetLastError();
+}
+
 hipError_t launch_fo_coef(const double* bg10, double* coef11, int64_t npts, hipStream_t s)
 {
     const int64_t blocks = std::min<int64_t>((npts + 255) / 256, 4096);

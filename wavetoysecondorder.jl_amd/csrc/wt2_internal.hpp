@@ -116,6 +116,10 @@ struct FoDesc {
 hipError_t launch_fo_rhs(const FoDesc& d, hipStream_t s);
 hipError_t launch_fo_energy(const FoDesc& d, hipStream_t s);
 hipError_t launch_fo_coef(const double* bg10, double* coef11, int64_t npts, hipStream_t s);
+// Classical RK4 stage update of (Phi, Pi) = (Y0, Y1); registers S, A, k (two
+// fields each) at R + f * stride, f = 0..5.
+hipError_t launch_fo_rk_update(double* Y0, double* Y1, double* R, int64_t stride, int64_t n, int stage, double dt,
+                               hipStream_t s);
 
 // A batch of pencil passes: one launch over every instance of a group (same N,
 // precision variant, axis, kernel kind and epilogue), driven by a device table

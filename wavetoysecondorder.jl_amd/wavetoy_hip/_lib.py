@@ -33,7 +33,7 @@ EXPORTS = (
     "wt2_batch_create", "wt2_batch_destroy", "wt2_batch_add", "wt2_batch_remove", "wt2_batch_step",
     "wt2_batch_step_async", "wt2_batch_sync",
     "wt2_batch_slot", "wt2_batch_info", "wt2_ctx_autotune", "wt2_ctx_tiles", "wt2_ctx_set_tiles", "wt2_launch_pass", "wt2_last_error", "wt2_version",
-    "wt2_fo_coef", "wt2_fo_rhs", "wt2_fo_energy",
+    "wt2_fo_coef", "wt2_fo_rhs", "wt2_fo_rk4_stage", "wt2_fo_energy",
 )
 
 # first-order boundary kinds (src/boundaries1d.jl:59-62)
@@ -142,6 +142,8 @@ def lib():
         L.wt2_ctx_set_tiles.argtypes = [vp, dp, dp]
         L.wt2_fo_coef.argtypes = [vp, dp, dp, ctypes.c_int32]
         L.wt2_fo_rhs.argtypes = [vp, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.POINTER(BC3d), ctypes.c_int32]
+        L.wt2_fo_rk4_stage.argtypes = [vp, ctypes.c_int32, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                                       ctypes.POINTER(BC3d)]
         L.wt2_fo_energy.argtypes = [vp, dp, dp, dp, dp, ctypes.c_int32]
         L.wt2_last_error.restype = ctypes.c_char_p
         L.wt2_version.restype = ctypes.c_char_p

@@ -546,6 +546,26 @@ class Workspace:
                                None if bc is None else ctypes.byref(bc), fl))
         return dPhi, dPi
 
+    def fo_rk4_stage(self, stage, Phi, Pi, coef, dt, eps_ko=0.1, bc3d=None):
+        """One fused RK4 stage of the first-order system in place on (Phi, Pi),
+        CUDA float64 tensors (wt2_fo_rk4_stage)."""
+        g = (None, None, None) if bc3d is None else (bc3d.gPhi_arr, bc3d.gPi_arr, bc3d.gD_arr)
+        (p0, p1, pc, pg1, pg2, pg3), fl = self._ptrs(Phi, Pi, coef, g[0], g[1], g[2], writable=(0, 1),
+                                                     mult={2: 11, 5: 3})
+        if fl:
+            raise ValueError("fo_rk4_stage needs device tensors")
+        bc = None
+        if bc3d is not None:
+            bc = _lib.BC3d()
+            for f in range(6):
+                bc.kinds[f] = bc3d.kinds[f]
+            bc.excision_tag = bc3d.excision_tag
+            bc.sigma = bc3d.sigma
+            bc.gPhi, bc.gPi, bc.gD = pg1, pg2, pg3
+        self._sync_torch()
+        check(lib().wt2_fo_rk4_stage(self.ctx, int(stage), p0, p1, pc,
+                                     float(dt), float(eps_ko), None if bc is None else ctypes.byref(bc)))
+
     def fo_energy(self, Phi, Pi, coef):
         (p1, p2, pc), fl = self._ptrs(Phi, Pi, coef, mult={2: 11})
         E = ctypes.c_double()

@@ -90,16 +90,18 @@ def _evolve3d_strong(N=5, M=8, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="
 
     mesh_kind (src/evolve.jl:913-928): "cubical" (above), "cubed_cube" (the
     cube [-1, 1]^3 as 7 patches around the inner cube [-R, R]^3, outer faces
-    Dirichlet) or "inflated_cube" (the ball of radius R2 as 13 patches: cube
+    Dirichlet), "inflated_cube" (the ball of radius R2 as 13 patches: cube
     [-L, L]^3, inflation to the sphere R1, shells to R2; outer_bc "dirichlet"
-    or "sommerfeld" with sommerfeld_R = R2).  On those the initial data are
+    or "sommerfeld" with sommerfeld_R = R2) or "radial_shell" (the shell
+    R1 <= r <= R2 as 6 patches, the inner sphere an excision surface, tag 8,
+    with no SAT; outer_bc as for the inflated cube).  On those the initial data are
     centred at the origin, the sampled line is y = z = 0 and tau defaults to
     8 (N-1)^2 (the reference's tau_mult, src/evolve.jl:962-968)."""
     if outer_bc not in ("dirichlet", "sommerfeld"):
         raise ValueError(f"evolve3d: unknown outer_bc {outer_bc!r}")
-    if outer_bc == "sommerfeld" and mesh_kind != "inflated_cube":
+    if outer_bc == "sommerfeld" and mesh_kind not in ("inflated_cube", "radial_shell"):
         raise ValueError(f"evolve3d: outer_bc = {outer_bc!r} is only supported on the spherical meshes "
-                         "(inflated cube); the cube takes 'dirichlet'")
+                         "(inflated cube, radial shell); the cube takes 'dirichlet'")
     elem = core.make_element(N)
     som_R = math.inf
     if mesh_kind == "cubical":
@@ -107,18 +109,24 @@ def _evolve3d_strong(N=5, M=8, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="
                 else core.make_uniform_hex(M, x0, x1))
         tau_used = core.tau_effective(tau, N)
         coords = core.host_coords(mesh, elem)
-    elif mesh_kind in ("cubed_cube", "inflated_cube"):
+    elif mesh_kind in ("cubed_cube", "inflated_cube", "radial_shell"):
         if mesh_kind == "cubed_cube":
             x0, x1 = -1.0, 1.0
             mesh = core.make_cubed_cube_mesh(N, M, R)
         else:
             x0, x1 = -float(R2), float(R2)
-            mesh = core.make_inflated_cube_mesh(N, L, R1, R2, M, outer_tag=7 if outer_bc == "sommerfeld" else 1)
+            tag = 7 if outer_bc == "sommerfeld" else 1
+            if mesh_kind == "inflated_cube":
+                mesh = core.make_inflated_cube_mesh(N, L, R1, R2, M, outer_tag=tag)
+            else:
+                # the inner sphere R1 is an excision surface (tag 8, no SAT)
+                mesh = core.make_radial_shell_mesh(N, R1, R2, M, outer_tag=tag)
             som_R = float(R2) if outer_bc == "sommerfeld" else math.inf
         tau_used = 8.0 * (N - 1) ** 2 if tau is None else float(tau)
         coords = mesh.coords.reshape(-1, 3)
     else:
-        raise ValueError(f"evolve3d: unknown mesh_kind {mesh_kind!r} (cubical, cubed_cube, inflated_cube)")
+        raise ValueError(f"evolve3d: unknown mesh_kind {mesh_kind!r} (cubical, cubed_cube, inflated_cube, "
+                         "radial_shell)")
     patch = mesh_kind != "cubical"
     bg = None
     if background == "adm_bump":
@@ -223,34 +231,46 @@ def _evolve3d_strong(N=5, M=8, x0=0.0, x1=1.0, warp_A=0.0, background=None, ic="
                 ud_final=vf, dt=dt, nsteps=nsteps, dx=work.h_min, tau=tau_used, mu=work.mu,
                 y_target=y_target, z_target=z_target, mesh_kind=mesh_kind, outer_bc=outer_bc,
                 sommerfeld_R=som_R, x0=x0, x1=x1,
-                integrator_name="RK4", work=work, coords=coords)
+                integrator_name="RK4", work=work, coords=coords, mesh=mesh)
 
 
 # ------------------------------------------------ first-order (Phi, Pi) driver
 
-def _background3d(kind, shift=(0.0, 0.0, 0.0)):
-    """The built-in static backgrounds of the cube (_background3d,
-    src/evolve.jl:1109-1128): Minkowski or a constant shift, alpha = 1,
-    gamma = I.  Returns the 10 background values of every node, the exact
-    plane wave Phi = sin(k (x + y + z) - omega t) with k = 2 pi,
-    omega = -k (b1 + b2 + b3) + k sqrt(3), its Pi = -k sqrt(3) cos(...), and
-    the largest characteristic speed |b1| + |b2| + |b3| + sqrt(3)."""
-    if kind == "minkowski":
-        b = (0.0, 0.0, 0.0)
-    elif kind == "constant_shift":
-        b = tuple(float(v) for v in shift)
-    elif kind == "radial_shift":
-        raise ValueError("evolve3d (conservative): background 'radial_shift' needs the radial-shell mesh, "
-                         "which this package does not build")
-    else:
-        raise ValueError(f"evolve3d (conservative): unknown background {kind!r} (minkowski, constant_shift)")
-    k = 2.0 * math.pi
-    s = b[0] + b[1] + b[2]
-    omega = -k * s + k * math.sqrt(3.0)
-    row = np.array([1.0, b[0], b[1], b[2], 1.0, 0.0, 0.0, 1.0, 0.0, 1.0])
-    phi_e = lambda t, x, y, z, lib=np: lib.sin(k * (x + y + z) - omega * t)
-    pi_e = lambda t, x, y, z, lib=np: (-omega - k * s) * lib.cos(k * (x + y + z) - omega * t)
-    return row, phi_e, pi_e, abs(b[0]) + abs(b[1]) + abs(b[2]) + math.sqrt(3.0)
+def _background3d(kind, shift=(0.0, 0.0, 0.0), R1=0.5, R2=1.0):
+    """The built-in static backgrounds (_background3d, src/evolve.jl:1109-1145):
+    Minkowski or a constant shift (alpha = 1, gamma = I, with the exact plane
+    wave Phi = sin(k (x + y + z) - omega t), k = 2 pi,
+    omega = -k (b1 + b2 + b3) + k sqrt(3)), or the radial shift of the
+    excision shell (flat alpha, gamma; beta = b_r x / r with b_r ramping
+    linearly from V = shift[0] (> 1) at R1 to 1/20 at R2; no exact solution).
+    Returns a function of the node coordinates (n, 3) giving the 10 background
+    values per node, Phi_e, Pi_e, grad Phi_e (3 functions) and the largest
+    characteristic speed."""
+    if kind in ("minkowski", "constant_shift"):
+        b = (0.0, 0.0, 0.0) if kind == "minkowski" else tuple(float(v) for v in shift)
+        k = 2.0 * math.pi
+        s = b[0] + b[1] + b[2]
+        omega = -k * s + k * math.sqrt(3.0)
+        row = np.array([1.0, b[0], b[1], b[2], 1.0, 0.0, 0.0, 1.0, 0.0, 1.0])
+        bgf = lambda x: np.broadcast_to(row, (x.shape[0], 10)).copy()
+        phi_e = lambda t, x, y, z, lib=np: lib.sin(k * (x + y + z) - omega * t)
+        pi_e = lambda t, x, y, z, lib=np: (-omega - k * s) * lib.cos(k * (x + y + z) - omega * t)
+        d_e = lambda t, x, y, z, lib=np: k * lib.cos(k * (x + y + z) - omega * t)
+        return bgf, phi_e, pi_e, (d_e, d_e, d_e), abs(b[0]) + abs(b[1]) + abs(b[2]) + math.sqrt(3.0)
+    if kind == "radial_shift":
+        V, Vout = float(shift[0]), 1.0 / 20.0
+
+        def bgf(x):
+            r = np.linalg.norm(x, axis=1)
+            br = V + (Vout - V) * (r - R1) / (R2 - R1)
+            out = np.zeros((x.shape[0], 10))
+            out[:, 0] = 1.0
+            out[:, 1:4] = (br / r)[:, None] * x
+            out[:, 4] = out[:, 7] = out[:, 9] = 1.0
+            return out
+        z = lambda t, x, y, zz, lib=np: 0.0 * x
+        return bgf, z, z, (z, z, z), V + 1.0
+    raise ValueError(f"evolve3d (conservative): unknown background {kind!r} (minkowski, constant_shift, radial_shift)")
 
 
 def _side_node(M, N, f):
@@ -299,44 +319,71 @@ def _face_kinds(bc, bg10, M, N):
     return tuple(kinds)
 
 
-def _evolve3d_conservative(N=4, M=8, x0=0.0, x1=1.0, mesh_kind="cubical", background="minkowski",
-                           shift=(0.0, 0.0, 0.0), ic="exact", bc="periodic", ic_width=0.15,
+def _evolve3d_conservative(N=4, M=8, x0=0.0, x1=1.0, mesh_kind="cubical", R=0.3, L=0.2, R1=0.5, R2=1.0,
+                           background="minkowski", shift=(0.0, 0.0, 0.0), ic="exact", bc="periodic", ic_width=0.15,
                            noise_amp=math.sqrt(np.finfo(np.float64).eps), eps_ko=0.0, t0=0.0, t1=1.0,
                            Nt=200, cfl=0.1, slice_y=None, slice_z=None, seed=0):
     """The first-order (Phi, Pi) ADM driver (_evolve3d_conservative,
-    src/evolve.jl:1149-1366) on the cube mesh: wave3d_curved_rhs! on the
-    device (wt2_fo_rhs) under classical RK4 with fixed dt = cfl h_min /
-    max_speed (capped by 1.4 / (eps_ko mu)), the last step cut to end at t1.
-    The reference picks RK4 for N <= 4 and higher-order adaptive-free
-    Runge-Kutta methods above; this driver runs RK4 for every N.
+    src/evolve.jl:1149-1366): wave3d_curved_rhs! on the device under classical
+    RK4 with fixed dt = cfl h_min / max_speed (capped by 1.4 / (eps_ko mu)),
+    the last step cut to end at t1; each stage is one wt2_fo_rk4_stage call
+    (the right-hand side and one fused update pass).  The reference picks RK4
+    for N <= 4 and higher-order Runge-Kutta methods above; this driver runs
+    RK4 for every N.
 
-    bc: "periodic", "auto" (classify each face) or a 6-tuple of kinds.  ic:
-    "exact" (plane wave; l2_err against it, full-Dirichlet faces driven by
-    it), "gaussian" (width ic_width about the cube centre) or "noise"
-    (noise_amp times normal deviates from numpy's generator with `seed`).
-    Returns the reference's outputs: ts, ts_actual, xs_line, perm, Phis, Pis
-    (line samples), l2_err, energy, Phi_final, Pi_final, dt, dx, kinds."""
+    mesh_kind: "cubical" (the affine path; bc "periodic", "auto" or a 6-tuple
+    of kinds, classified per face) or "cubed_cube", "inflated_cube",
+    "radial_shell" (the curvilinear path; bc "sommerfeld" or "dirichlet" on
+    every outer face, "periodic" meaning Sommerfeld; the radial shell's inner
+    sphere is an excision surface, tag 8).  ic: "exact" (plane wave; l2_err
+    against it, Dirichlet data from it), "gaussian" (width ic_width about the
+    domain centre) or "noise".  Returns the reference's outputs: ts,
+    ts_actual, xs_line, perm, Phis, Pis (line samples), l2_err, energy,
+    Phi_final, Pi_final, dt, dx, kinds."""
     import torch
-    if mesh_kind != "cubical":
-        raise ValueError(f"evolve3d (conservative): mesh_kind {mesh_kind!r} is not built here (cubical only)")
-    periodic = bc == "periodic"
-    if not (periodic or bc == "auto" or (isinstance(bc, (tuple, list)) and len(bc) == 6)):
+    curv = mesh_kind != "cubical"
+    if mesh_kind not in ("cubical", "cubed_cube", "inflated_cube", "radial_shell"):
+        raise ValueError(f"evolve3d conservative: unknown mesh_kind {mesh_kind!r} "
+                         "(cubical, cubed_cube, inflated_cube, radial_shell)")
+    if background == "radial_shift" and mesh_kind != "radial_shell":
+        raise ValueError("evolve3d (conservative): background 'radial_shift' belongs to the radial shell "
+                         "(mesh_kind='radial_shell')")
+    if not curv and not (bc in ("periodic", "auto") or (isinstance(bc, (tuple, list)) and len(bc) == 6)):
         raise ValueError(f"evolve3d: bc must be 'periodic', 'auto' or a 6-tuple; got {bc!r}")
     if ic not in ("exact", "gaussian", "noise"):
         raise ValueError(f"evolve3d: unknown ic {ic!r} (exact, gaussian, noise)")
-    mesh = core.make_uniform_hex(M, x0, x1, periodic)
+    periodic = bc == "periodic" and not curv
+    ck = None
+    if curv:
+        ck = "sommerfeld" if bc == "periodic" else bc
+        if ck not in ("sommerfeld", "dirichlet"):
+            raise ValueError(f"evolve3d: curvilinear ({mesh_kind}) bc must be 'sommerfeld' or 'dirichlet'; got {bc!r}")
+        if ck == "dirichlet" and ic != "exact":
+            raise ValueError("evolve3d: curvilinear bc='dirichlet' needs ic='exact' (it injects the exact solution)")
+    outer_tag = 1 if ck == "dirichlet" else core.SOMMERFELD_BDRY_TAG
+    if mesh_kind == "cubical":
+        mesh = core.make_uniform_hex(M, x0, x1, periodic)
+    elif mesh_kind == "cubed_cube":
+        mesh = core.make_cubed_cube_mesh(N, M, R)
+        x0, x1 = -1.0, 1.0
+    elif mesh_kind == "inflated_cube":
+        mesh = core.make_inflated_cube_mesh(N, L, R1, R2, M, outer_tag=outer_tag)
+        x0, x1 = -R2, R2
+    else:
+        mesh = core.make_radial_shell_mesh(N, R1, R2, M, outer_tag=outer_tag)
+        x0, x1 = -R2, R2
     elem = core.make_element(N)
-    coords = core.host_coords(mesh, elem)
+    coords = mesh.coords.reshape(-1, 3) if curv else core.host_coords(mesh, elem)
     n = coords.shape[0]
-    row, phi_e, pi_e, max_speed = _background3d(background, shift)
-    bg10 = np.broadcast_to(row, (n, 10)).copy()
-    kinds = None if periodic else _face_kinds(bc, bg10, mesh.M, N)
+    bgf, phi_e, pi_e, d_e, max_speed = _background3d(background, shift, R1, R2)
+    bg10 = bgf(coords)
+    kinds = None
+    if not curv and not periodic:
+        kinds = _face_kinds(bc, bg10, mesh.M, N)
     work = core.Workspace(mesh, elem)
-    coef = work.fo_coef(torch.from_numpy(bg10.reshape(-1)).cuda())
+    coef = work.fo_coef(torch.from_numpy(bg10.reshape(-1).copy()).cuda())
     dx = work.h_min
     dt = cfl * dx / max_speed
-    if eps_ko != 0:
-        dt = min(dt, 1.4 / (eps_ko * work.mu))
 
     xg, yg, zg = (torch.from_numpy(coords[:, a].copy()).cuda() for a in range(3))
     if ic == "exact":
@@ -347,46 +394,55 @@ def _evolve3d_conservative(N=4, M=8, x0=0.0, x1=1.0, mesh_kind="cubical", backgr
         r2 = np.sum((coords - c0) ** 2, axis=1)
         Phi0 = np.exp(-r2 / (2 * ic_width * ic_width))
         Pi0 = np.zeros(n)
-    elif ic == "noise":
+    else:
         rng = np.random.default_rng(seed)
         Phi0 = noise_amp * rng.standard_normal(n)
         Pi0 = noise_amp * rng.standard_normal(n)
-    Phi = torch.from_numpy(np.ascontiguousarray(Phi0)).cuda()
-    Pi = torch.from_numpy(np.ascontiguousarray(Pi0)).cuda()
+    Phi = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(Phi0, (n,)))).cuda()
+    Pi = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(Pi0, (n,)))).cuda()
 
     withdata = ic == "exact"
-    needdata = kinds is not None and core.BC3d._NAMES["full_dirichlet"] in kinds
-    gPhi = torch.zeros_like(Phi) if needdata else None
+    curv_dir = curv and ck == "dirichlet"
+    needdata = (kinds is not None and core.BC3d._NAMES["full_dirichlet"] in kinds) or curv_dir
+    gPhi = torch.zeros_like(Phi) if needdata and not curv else None
     gPi = torch.zeros_like(Phi) if needdata else None
-    bc3d = None if kinds is None else core.make_bc3d(kinds, gPhi=gPhi, gPi=gPi)
+    gD = torch.zeros(3 * n, dtype=torch.float64, device="cuda") if curv_dir else None
+    if curv:
+        bc3d = core.make_bc3d(ck, gPi=gPi, gD=gD,
+                              excision_tag=core.EXCISION_BDRY_TAG if mesh_kind == "radial_shell" else 0)
+    else:
+        bc3d = None if kinds is None else core.make_bc3d(kinds, gPhi=gPhi, gPi=gPi)
+    energy0 = work.fo_energy(Phi, Pi, coef)        # sets up a general mesh's metric terms
+    if eps_ko != 0:
+        dt = min(dt, 1.4 / (eps_ko * work.info()["mu"]))
 
-    def rhs(dP, dQ, P, Q, t):
-        if needdata and withdata:
+    def set_data(t):
+        if not (needdata and withdata):
+            return
+        gPi.copy_(pi_e(t, xg, yg, zg, lib=torch))
+        if curv_dir:
+            for a in range(3):
+                gD[a * n:(a + 1) * n].copy_(d_e[a](t, xg, yg, zg, lib=torch))
+        else:
             gPhi.copy_(phi_e(t, xg, yg, zg, lib=torch))
-            gPi.copy_(pi_e(t, xg, yg, zg, lib=torch))
-        work.fo_rhs(dP, dQ, P, Q, coef, eps_ko, bc3d)
-
-    k = [torch.empty_like(Phi) for _ in range(8)]
-    sP, sQ = torch.empty_like(Phi), torch.empty_like(Phi)
 
     def rk4_step(t, h):
-        rhs(k[0], k[1], Phi, Pi, t)
-        torch.add(Phi, k[0], alpha=h / 2, out=sP); torch.add(Pi, k[1], alpha=h / 2, out=sQ)
-        rhs(k[2], k[3], sP, sQ, t + h / 2)
-        torch.add(Phi, k[2], alpha=h / 2, out=sP); torch.add(Pi, k[3], alpha=h / 2, out=sQ)
-        rhs(k[4], k[5], sP, sQ, t + h / 2)
-        torch.add(Phi, k[4], alpha=h, out=sP); torch.add(Pi, k[5], alpha=h, out=sQ)
-        rhs(k[6], k[7], sP, sQ, t + h)
-        for y, a, b, c, d in ((Phi, k[0], k[2], k[4], k[6]), (Pi, k[1], k[3], k[5], k[7])):
-            y.add_(a + 2 * b + 2 * c + d, alpha=h / 6)
+        for stage, ts in ((1, t), (2, t + h / 2), (3, t + h / 2), (4, t + h)):
+            set_data(ts)
+            work.fo_rk4_stage(stage, Phi, Pi, coef, h, eps_ko, bc3d)
 
     c_mid = (x0 + x1) / 2
     y_target = c_mid if slice_y is None else float(slice_y)
     z_target = c_mid if slice_z is None else float(slice_z)
     sidx, xs_line = build_slice(coords, y_target, z_target)
+    if sidx.size == 0 and curv and slice_y is None and slice_z is None:
+        # no node on the centre line (even N on the inner cube): the nearest node line
+        q = int(np.argmin((coords[:, 1] - y_target) ** 2 + (coords[:, 2] - z_target) ** 2))
+        y_target, z_target = float(coords[q, 1]), float(coords[q, 2])
+        sidx, xs_line = build_slice(coords, y_target, z_target)
     if sidx.size == 0:
         raise ValueError(f"evolve3d: slice y={y_target} z={z_target} hit no nodes")
-    H = core.Geometry(mesh, elem).Hphys
+    H = _hphys(N, mesh.coords) if curv else core.Geometry(mesh, elem).Hphys
     ts = np.linspace(t0, t1, Nt)
     Phis = np.empty((len(sidx), Nt)); Pis = np.empty_like(Phis)
     ts_actual = np.empty(Nt); l2_err = np.empty(Nt); energy = np.empty(Nt)
@@ -406,9 +462,9 @@ def _evolve3d_conservative(N=4, M=8, x0=0.0, x1=1.0, mesh_kind="cubical", backgr
         Phis[:, q] = Ph[sidx]; Pis[:, q] = Qh[sidx]
         ref = phi_e(t, coords[:, 0], coords[:, 1], coords[:, 2]) if withdata else 0.0
         l2_err[q] = math.sqrt(float(np.sum((Ph - ref) ** 2 * H)))
-        energy[q] = work.fo_energy(Phi, Pi, coef)
+        energy[q] = energy0 if (q == 0 and t == t0) else work.fo_energy(Phi, Pi, coef)
     return dict(ts=ts, ts_actual=ts_actual, xs_line=xs_line, perm=np.arange(len(sidx)), Phis=Phis, Pis=Pis,
                 l2_err=l2_err, energy=energy, Phi_final=Phi.cpu().numpy(), Pi_final=Pi.cpu().numpy(),
-                dt=dt, dx=dx, nsteps=nsteps, kinds=kinds, background=background, ic=ic, bc=bc,
+                dt=dt, dx=dx, nsteps=nsteps, kinds=kinds if not curv else (ck,), background=background, ic=ic, bc=bc,
                 mesh_kind=mesh_kind, x0=x0, x1=x1, y_target=y_target, z_target=z_target,
-                integrator_name="RK4", work=work, coords=coords, coef=coef)
+                integrator_name="RK4", work=work, coords=coords, coef=coef, mesh=mesh)
