@@ -485,6 +485,12 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
     for (int r = 0; r < N; ++r) { u[r] = TC(0); v[r] = TS(0); cv[r] = TC(0); }
     if (a.lap) load_seg_s<N, TS, TC>(static_cast<const TS*>(a.Su) + g0, gs, u);
     if (a.needv) load_seg_s<N, TS, TS>(static_cast<const TS*>(a.Sv) + g0, gs, v);
+    // the partial sum is read with the stage input, so that both loads are in
+    // flight together instead of one after the compute
+    TP* P = static_cast<TP*>(a.P);
+    const int epi = a.epi;
+    TC pv[N];
+    if (epi != EPI_FIRST) load_seg_s<N, TP, TC>(P + g0, gs, pv);
     if (a.ko) {
         TC x[N];
 #pragma unroll
@@ -494,16 +500,11 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
         for (int r = 0; r < N; ++r) cv[r] = x[r];
     }
     if (a.lap) sipg_add<N, TC>(u, cv, op, a.k, xch, hasL, hasR, TC(v[0]), TC(v[N - 1]));
-    const int epi0 = a.epi;
-    const bool fold = a.fold && epi0 >= EPI_RK1;
+    const bool fold = a.fold && epi >= EPI_RK1;
     // a folding pass with the LDS exchange needs every work-item at its barriers
     if (!active && !(XCH && fold)) return;
 
-    TP* P = static_cast<TP*>(a.P);
-    const int epi = a.epi;
     if (epi == EPI_FIRST) { if (active) store_seg_s<N, TP, TC>(P + g0, gs, cv); return; }
-    TC pv[N];
-    load_seg_s<N, TP, TC>(P + g0, gs, pv);
 #pragma unroll
     for (int r = 0; r < N; ++r) cv[r] += pv[r];          // u_t' = P + c
     if (epi == EPI_MID) { if (active) store_seg_s<N, TP, TC>(P + g0, gs, cv); return; }
@@ -888,12 +889,37 @@ __device__ __forceinline__ void lap1_load(const double* __restrict__ p, TC (&x)[
     }
 }
 
+// The two neighbours' lines of this work-item's line (la, lb) along axis AX
+// (element e, coordinate c along the axis), read from global memory; absent
+// neighbours (outer faces) leave them unset.
+template <int N, typename TC>
+struct Lap1Nbr {
+    TC xl[N], xr[N];
+    bool hasL, hasR;
+};
+
+template <int N, typename TC, int AX>
+__device__ __forceinline__ void lap1_load_nbrs(const Lap1Args<N, TC>& a, int e, int c, int la, int lb,
+                                               Lap1Nbr<N, TC>& nb)
+{
+    constexpr int N3 = N * N * N;
+    constexpr int GS = AX == 0 ? 1 : (AX == 1 ? N : N * N);        // memory stride along the line
+    const int gbase = AX == 0 ? N * (la + N * lb) : (AX == 1 ? la + N * N * lb : la + N * lb);
+    const int Ma = AX == 0 ? a.M0 : (AX == 1 ? a.M1 : a.M2);
+    const int estr = AX == 0 ? 1 : (AX == 1 ? a.M0 : a.M0 * a.M1);
+    const bool per = (AX == 0 ? a.per0 : (AX == 1 ? a.per1 : a.per2)) != 0;
+    nb.hasL = c > 0 || per;
+    nb.hasR = c < Ma - 1 || per;
+    if (nb.hasL) lap1_load<N, GS, TC>(a.u + int64_t(e + (c > 0 ? -1 : Ma - 1) * estr) * N3 + gbase, nb.xl);
+    if (nb.hasR) lap1_load<N, GS, TC>(a.u + int64_t(e + (c < Ma - 1 ? 1 : 1 - Ma) * estr) * N3 + gbase, nb.xr);
+}
+
 // One axis of the single-pass Laplacian for this work-item's line (la, lb)
-// of element e (coordinate c along the axis): c_AX from the element's own line
-// (LDS) and the two neighbours' lines (global memory), added to R.
+// of element e: c_AX from the element's own line (LDS) and the two
+// neighbours' lines (loaded by lap1_load_nbrs), added to R.
 template <int N, typename TC, int AX>
 __device__ __forceinline__ void lap1_axis(const Lap1Args<N, TC>& a, const TC* __restrict__ op, const TC* Ue,
-                                          TC* Re, int e, int c, int la, int lb)
+                                          TC* Re, int e, int la, int lb, const Lap1Nbr<N, TC>& nb)
 {
     constexpr int RS = Lap1Shape<N>::RS;
     constexpr int N3 = N * N * N;
@@ -901,25 +927,20 @@ __device__ __forceinline__ void lap1_axis(const Lap1Args<N, TC>& a, const TC* __
     constexpr int GS = AX == 0 ? 1 : (AX == 1 ? N : N * N);        // memory stride along the line
     const int lbase = AX == 0 ? RS * (la + N * lb) : (AX == 1 ? la + RS * N * lb : la + RS * lb);
     const int gbase = AX == 0 ? N * (la + N * lb) : (AX == 1 ? la + N * N * lb : la + N * lb);
-    const int Ma = AX == 0 ? a.M0 : (AX == 1 ? a.M1 : a.M2);
-    const int estr = AX == 0 ? 1 : (AX == 1 ? a.M0 : a.M0 * a.M1);
-    const bool per = (AX == 0 ? a.per0 : (AX == 1 ? a.per1 : a.per2)) != 0;
-    const bool hasL = c > 0 || per, hasR = c < Ma - 1 || per;
-    TC xl[N], xr[N], u[N];
-    if (hasL) lap1_load<N, GS, TC>(a.u + int64_t(e + (c > 0 ? -1 : Ma - 1) * estr) * N3 + gbase, xl);
-    if (hasR) lap1_load<N, GS, TC>(a.u + int64_t(e + (c < Ma - 1 ? 1 : 1 - Ma) * estr) * N3 + gbase, xr);
+    const bool hasL = nb.hasL, hasR = nb.hasR;
+    TC u[N];
 #pragma unroll
     for (int m = 0; m < N; ++m) u[m] = Ue[lbase + m * LS];
     TC uL = TC(0), gnL = TC(0), uR = TC(0), gnR = TC(0);
     if (hasL) {
         TC y0;
-        apply_G_ends<N, TC>(op, xl, y0, gnL);
-        uL = xl[N - 1];
+        apply_G_ends<N, TC>(op, nb.xl, y0, gnL);
+        uL = nb.xl[N - 1];
     }
     if (hasR) {
         TC yN;
-        apply_G_ends<N, TC>(op, xr, gnR, yN);
-        uR = xr[0];
+        apply_G_ends<N, TC>(op, nb.xr, gnR, yN);
+        uR = nb.xr[0];
     }
     TC v0 = TC(0), vN = TC(0);
     if (a.v) {
@@ -962,6 +983,14 @@ __global__ void __launch_bounds__(256) flat_lap1_kernel(Lap1Args<N, TC> a)
     const int nel = min(S::EW, a.Ne - t0);
     __shared__ int elist[S::EW];
     if (tid < nel) elist[tid] = lap1_elem(a, t0 + tid);
+
+    const int slot = tid / S::L, line = tid - slot * S::L;
+    const bool act = slot < S::EW && slot < nel;
+    const int e = act ? lap1_elem(a, t0 + slot) : 0;
+    const int la = line % N, lb = line / N;
+    int ec[3] = {0, 0, 0};
+    if (act) { ec[0] = e % a.M0; ec[1] = (e / a.M0) % a.M1; ec[2] = e / (a.M0 * a.M1); }
+    Lap1Nbr<N, TC> nb;
     __syncthreads();
 
     // stage the elements' u into LDS (each element contiguous in memory)
@@ -972,22 +1001,18 @@ __global__ void __launch_bounds__(256) flat_lap1_kernel(Lap1Args<N, TC> a)
     }
     __syncthreads();
 
-    const int slot = tid / S::L, line = tid - slot * S::L;
-    const bool act = slot < S::EW && slot < nel;
-    const int e = act ? elist[slot] : 0;
-    const int la = line % N, lb = line / N;
-    int ec[3] = {0, 0, 0};
-    if (act) { ec[0] = e % a.M0; ec[1] = (e / a.M0) % a.M1; ec[2] = e / (a.M0 * a.M1); }
     TC* Ue = U + (act ? slot : 0) * S::ES;
     TC* Re = R + (act ? slot : 0) * S::ES;
 
     // axes z, y, x: the running sum R = c_z, R += c_y, R += c_x (the order of
     // the pencil passes, so both paths round alike)
-    if (act) lap1_axis<N, TC, 2>(a, op, Ue, Re, e, ec[2], la, lb);
+    // (loading the next axis's neighbour lines one phase early raises the
+    // register count and measured 25 % slower at 32^3 x 8^3)
+    if (act) { lap1_load_nbrs<N, TC, 2>(a, e, ec[2], la, lb, nb); lap1_axis<N, TC, 2>(a, op, Ue, Re, e, la, lb, nb); }
     __syncthreads();
-    if (act) lap1_axis<N, TC, 1>(a, op, Ue, Re, e, ec[1], la, lb);
+    if (act) { lap1_load_nbrs<N, TC, 1>(a, e, ec[1], la, lb, nb); lap1_axis<N, TC, 1>(a, op, Ue, Re, e, la, lb, nb); }
     __syncthreads();
-    if (act) lap1_axis<N, TC, 0>(a, op, Ue, Re, e, ec[0], la, lb);
+    if (act) { lap1_load_nbrs<N, TC, 0>(a, e, ec[0], la, lb, nb); lap1_axis<N, TC, 0>(a, op, Ue, Re, e, la, lb, nb); }
     __syncthreads();
     // coalesced write-out
     for (int q = tid; q < nel * N3; q += blockDim.x) {
