@@ -167,3 +167,26 @@ def test_evolve3d_mesh_kind_arguments():
         wh.evolve3d(mesh_kind="cubed_cube", outer_bc="sommerfeld", device=False)
     with pytest.raises(ValueError, match="outer_bc"):
         wh.evolve3d(mesh_kind="inflated_cube", outer_bc="absorbing", device=False)
+
+
+def test_spectral_radius_estimate_against_dense_eigenvalues():
+    """The patch-mesh time step's spectral-radius estimate (power iteration in
+    the H |J| inner product with a Rayleigh-quotient stop) against a dense
+    eigenvalue solve of the operator on a small cubed ball (the oracle's
+    operator, the same SIPG form as the device's)."""
+    import wt_oracle as wo
+    mesh = wh.make_cubed_ball_mesh(3, 1, 1.0)
+    o = wo.Oracle.from_mesh(3, mesh.coords, mesh.neighbour, mesh.neighbour_face, mesh.bdry, tau=8.0 * 4)
+    H = ev._hphys(3, mesh.coords)
+    n = o.n
+    z = np.zeros(n)
+    A = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        A[:, j] = o.lap(e, z)
+    rho = np.max(np.abs(np.linalg.eigvals(A)))
+    est = ev._spectral_radius(lambda v: o.lap(np.ascontiguousarray(v), z), H, n, tol=1e-4)
+    assert est <= rho * (1 + 1e-9) and est > 0.99 * rho, (est, rho)
+    with pytest.raises(RuntimeError):
+        ev._spectral_radius(lambda v: o.lap(np.ascontiguousarray(v), z), H, n, tol=1e-15, max_iters=3)

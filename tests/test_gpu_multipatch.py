@@ -2,8 +2,10 @@
 multi-patch cubed ball, whose patches meet with orientation != 0, and
 relabelled copies of uniform meshes.
 
-The oracle has no restatement of general connectivity, so these tests pin the
-path by what the mathematics fixes:
+The oracle restates the general-connectivity path too (wto_create_mesh: the
+same D4 encoding, the same matching of face nodes by coordinates), so the
+1e-12 parity test below shows the two agree; the mathematics pins both
+independently:
   * on affine elements the operator is the uniform-mesh one (oracle-pinned)
     whatever the element numbering and the orientation of each element's
     local axes;

@@ -725,8 +725,8 @@ def validate_bc1d(cls, kind, face):
     Dirichlet or Sommerfeld, outflow faces excision, inflow faces the full
     state; sonic faces none.  Raises ValueError."""
     if cls == FACE_SONIC:
-        raise ValueError(f"{face} boundary face is at a sonic point (|beta| ~ alpha/sqrt(gamma)): a characteristic "
-                         "speed vanishes there and no boundary condition is well-posed")
+        raise ValueError(f"{face}: the normal shift equals the light speed there (|beta_n| = alpha sqrt(gamma^nn)), "
+                         "so one characteristic speed is zero and none of the boundary kinds applies")
     ok = ((cls == FACE_SUBLUMINAL and kind in (_lib.WT2_BC_DIRICHLET, _lib.WT2_BC_SOMMERFELD))
           or (cls == FACE_OUTFLOW and kind == _lib.WT2_BC_EXCISION)
           or (cls == FACE_INFLOW and kind == _lib.WT2_BC_FULL_DIRICHLET))
