@@ -30,6 +30,39 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.EXPORTS) == declared
 
 
+def _c_param_counts():
+    text = open(os.path.join(ROOT, "include", "wt2.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for name, params in re.findall(r"\b(wt2_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", text):
+        params = params.strip()
+        out[name] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_julia_binding_matches_the_header():
+    """Every ccall of the Julia binding names a declared entry with the
+    declared number of arguments, and its BC3d mirrors wt2_bc3d field by
+    field (no Julia toolchain here, so the binding is checked as text)."""
+    jl = open(os.path.join(ROOT, "wavetoysecondorder.jl_amd", "julia", "WaveToyHIP.jl")).read()
+    counts = _c_param_counts()
+    calls = re.findall(r"ccall\(\(:(wt2_[a-z0-9_]+), libwt2\), \w+,\s*\(([^()]*(?:\{[^}]*\}[^()]*)*)\)", jl)
+    assert len(calls) == jl.count("ccall(") and len(calls) >= 15
+    for name, types in calls:
+        assert name in counts, name
+        n = len([t for t in re.split(r",(?![^{]*\})", types) if t.strip()])
+        assert n == counts[name], (name, n, counts[name])
+    body = re.search(r"struct BC3d\n(.*?)\nend", jl, flags=re.S).group(1)
+    jl_types = re.findall(r"::\s*([A-Za-z{}0-9,]+)", body)
+    c_types = [t for _, t in _lib.BC3d._fields_]
+    want = {"NTuple{6,Int32}": ctypes.c_int32 * 6, "Int32": ctypes.c_int32, "Float64": ctypes.c_double,
+            "Ptr{Float64}": ctypes.c_void_p}
+    assert len(jl_types) == len(c_types)
+    for jt, ct in zip(jl_types, c_types):
+        w = want[jt]
+        assert ctypes.sizeof(w) == ctypes.sizeof(ct) and ctypes.alignment(w) == ctypes.alignment(ct), (jt, ct)
+
+
 def test_version_string():
     assert b"gfx950" in _lib.lib().wt2_version()
 

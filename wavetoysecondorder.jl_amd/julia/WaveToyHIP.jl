@@ -69,7 +69,8 @@ make_inflated_cube_mesh + make_geometry + to_device, src/evolve.jl:916-936):
 `geom.coords` (3,N,N,N,Ne) and `conn` = mesh.conn (`neighbour`,
 `neighbour_face`, `bdry`; (6,Ne)).  The D4 orientation of every shared face is
 found by the library from the face nodes, so HexMeshes' `orientation` codes
-are not passed.  Outer faces with bdry 7 are Sommerfeld faces.
+are not passed.  Outer faces with bdry 7 are Sommerfeld faces, those with
+bdry 8 (EXCISION_BDRY_TAG) excision faces that carry no SAT.
 """
 function Context(geom, conn; τ, bdry_values = ntuple(_ -> 0.0, 6), precision = :fp64,
                  sommerfeld_R = Inf)
@@ -130,21 +131,30 @@ const _BC = Dict(:dirichlet => Int32(1), :sommerfeld => Int32(2), :excision => I
 
 struct BC3d
     kinds::NTuple{6,Int32}
-    reserved::Int32
+    excision_tag::Int32      # general meshes: outer faces with this bdry tag get no SAT
     σ::Float64
     gΦ::Ptr{Float64}
     gΠ::Ptr{Float64}
+    gD::Ptr{Float64}         # general meshes, :dirichlet: the ∇Φ target (3 fields)
 end
 
-"""
-    make_bc3d(kinds; σ = 1, gΦ = nothing, gΠ = nothing)
+const EXCISION_BDRY_TAG = Int32(8)
 
-make_bc3d of src/boundaries3d.jl:44-54 for the affine pass (host targets must
-stay alive for the call).
 """
-make_bc3d(kinds; σ = 1.0, gΦ = nothing, gΠ = nothing) =
-    BC3d(ntuple(i -> kinds[i] isa Symbol ? _BC[kinds[i]] : Int32(kinds[i]), 6), Int32(0), Float64(σ),
-         gΦ === nothing ? Ptr{Float64}(C_NULL) : pointer(gΦ), gΠ === nothing ? Ptr{Float64}(C_NULL) : pointer(gΠ))
+    make_bc3d(kinds; σ = 1, gΦ = nothing, gΠ = nothing, gD = nothing, excision_tag = 0)
+
+make_bc3d of src/boundaries3d.jl:44-54.  `kinds` is one kind per face on an
+affine mesh, or a single kind for every outer face of a general mesh
+(_apply_bc3d_curv!, src/boundaries3d.jl:131-165), whose faces tagged
+`excision_tag` get no SAT.  Targets are host arrays (kept alive across the
+call) or device pointers (`Ptr{Float64}`), to match the call's flags.
+"""
+function make_bc3d(kinds; σ = 1.0, gΦ = nothing, gΠ = nothing, gD = nothing, excision_tag = 0)
+    ks = kinds isa Symbol || kinds isa Integer ? ntuple(_ -> kinds, 6) : kinds
+    ptr(a) = a === nothing ? Ptr{Float64}(C_NULL) : a isa Ptr ? Ptr{Float64}(a) : pointer(a)
+    BC3d(ntuple(i -> ks[i] isa Symbol ? _BC[ks[i]] : Int32(ks[i]), 6), Int32(excision_tag), Float64(σ),
+         ptr(gΦ), ptr(gΠ), ptr(gD))
+end
 
 """
     fo_coef(bg10; ctx) -> coef (11 fields of N^3 Ne values)
@@ -186,5 +196,95 @@ function wave3d_energy(Φ::Array{Float64,4}, Π::Array{Float64,4}, coef::Vector{
                 ctx.ptr, Φ, Π, coef, E, WT2_HOST_PTRS))
     return E[]
 end
+
+"""
+    fo_rk4_stage!(stage, Φ, Π, coef, dt; ctx, ε_KO = 0.1, bc3d = nothing)
+
+One classical RK4 stage (1..4) of the first-order system in place on device
+arrays (`Ptr{Float64}` to 16-byte aligned ROCArray storage): the stages of
+OrdinaryDiffEq RK4() in src/evolve.jl:1309-1313, each the right-hand side
+fused with its update.  Boundary targets may change between stages.
+"""
+function fo_rk4_stage!(stage::Integer, Φ::Ptr{Float64}, Π::Ptr{Float64}, coef::Ptr{Float64}, dt::Real;
+                       ctx::Context, ε_KO::Real = 0.1, bc3d = nothing)
+    bc = bc3d === nothing ? Ptr{BC3d}(C_NULL) : Ref(bc3d)
+    check(ccall((:wt2_fo_rk4_stage, libwt2), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Ptr{BC3d}),
+                ctx.ptr, Int32(stage), Φ, Π, coef, Float64(dt), Float64(ε_KO), bc))
+    return nothing
+end
+
+"""
+    rk4_steps_noalloc!(u, u̇, dt, nsteps; ctx, energy_every=0) -> energies or nothing
+
+rk4_steps! on device arrays (`Ptr{Float64}`) for callers that cannot allocate
+device memory; no counterpart in the reference.  At most 65536 energy
+samples; calls that overlap in time on one GPU race.
+"""
+function rk4_steps_noalloc!(u::Ptr{Float64}, u̇::Ptr{Float64}, dt::Float64, nsteps::Integer;
+                            ctx::Context, energy_every::Integer = 0)
+    ne = energy_every > 0 ? 1 + nsteps ÷ energy_every : 0
+    E = zeros(max(ne, 1))
+    check(ccall((:wt2_rk4_steps_noalloc, libwt2), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cint, Cint, Ptr{Float64}),
+                ctx.ptr, u, u̇, dt, nsteps, energy_every, ne > 0 ? E : C_NULL))
+    return ne > 0 ? E[1:ne] : nothing
+end
+
+# ---- batches: many independent evolutions (an ensemble of step! loops,
+# src/evolve.jl:1028-1031) advanced together on one device
+
+mutable struct Batch
+    ptr::Ptr{Cvoid}
+    members::Vector{Any}     # keeps contexts and arrays alive while they are in the batch
+end
+
+function Batch()
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:wt2_batch_create, libwt2), Cint, (Ref{Ptr{Cvoid}},), out))
+    b = Batch(out[], Any[])
+    finalizer(x -> ccall((:wt2_batch_destroy, libwt2), Cvoid, (Ptr{Cvoid},), x.ptr), b)
+    return b
+end
+
+"""
+    add!(b, ctx, u, u̇, dt, nsteps; energy_every = 0) -> slot
+
+Adds one evolution on device arrays (`Ptr{Float64}`); returns its slot.
+"""
+function add!(b::Batch, ctx::Context, u::Ptr{Float64}, u̇::Ptr{Float64}, dt::Real, nsteps::Integer;
+              energy_every::Integer = 0)
+    slot = Ref{Int32}(0)
+    check(ccall((:wt2_batch_add, libwt2), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cdouble, Int64, Cint, Ref{Int32}),
+                b.ptr, ctx.ptr, u, u̇, Float64(dt), Int64(nsteps), Int32(energy_every), slot))
+    push!(b.members, ctx)
+    return Int(slot[])
+end
+
+"""
+    step!(b, max_steps) -> number of evolutions that finished
+"""
+function step!(b::Batch, max_steps::Integer)
+    fin = Ref{Int32}(0)
+    check(ccall((:wt2_batch_step, libwt2), Cint, (Ptr{Cvoid}, Cint, Ref{Int32}), b.ptr, Int32(max_steps), fin))
+    return Int(fin[])
+end
+
+"""
+    slot_state(b, slot, max_energies) -> (steps_done, energies)
+"""
+function slot_state(b::Batch, slot::Integer, max_energies::Integer = 0)
+    done = Ref{Int64}(0)
+    nE = Ref{Int32}(0)
+    E = zeros(max(max_energies, 1))
+    check(ccall((:wt2_batch_slot, libwt2), Cint,
+                (Ptr{Cvoid}, Cint, Ref{Int64}, Ptr{Float64}, Cint, Ref{Int32}),
+                b.ptr, Int32(slot), done, max_energies > 0 ? E : C_NULL, Int32(max_energies), nE))
+    return Int(done[]), E[1:min(Int(nE[]), max_energies)]
+end
+
+remove!(b::Batch, slot::Integer) =
+    check(ccall((:wt2_batch_remove, libwt2), Cint, (Ptr{Cvoid}, Cint), b.ptr, Int32(slot)))
 
 end # module

@@ -418,8 +418,8 @@ def _evolve3d_conservative(N=4, M=8, x0=0.0, x1=1.0, mesh_kind="cubical", R=0.3,
         rng = np.random.default_rng(seed)
         Phi0 = noise_amp * rng.standard_normal(n)
         Pi0 = noise_amp * rng.standard_normal(n)
-    Phi = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(Phi0, (n,)))).cuda()
-    Pi = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(Pi0, (n,)))).cuda()
+    Phi = torch.from_numpy(np.array(np.broadcast_to(Phi0, (n,)), dtype=np.float64)).cuda()
+    Pi = torch.from_numpy(np.array(np.broadcast_to(Pi0, (n,)), dtype=np.float64)).cuda()
 
     withdata = ic == "exact"
     curv_dir = curv and ck == "dirichlet"
