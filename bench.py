@@ -560,6 +560,99 @@ def fo_run(wh, torch, args):
     return out
 
 
+FO_CURV_ALG_BYTES = 200   # read Phi, Pi, 11 coefficients, 9 metric terms J a^i_n and |J|; write Phidot, Pidot
+
+
+def fo_curv_run(wh, torch, args):
+    """The first-order (Phi, Pi) system on a curvilinear multi-patch mesh (the
+    metric path of wave3d_curved_rhs!): the inflated cube, 13 patches of 8^3
+    elements, N = 8 (3,407,872 points), ADM bump background, Sommerfeld outer
+    sphere, no KO (the reference's default): RHS evaluations (wt2_fo_rhs) and
+    RK4 steps of fused stages (wt2_fo_rk4_stage) per second, the CPU oracle
+    timed on the same mesh family with 4^3 elements per patch."""
+    mesh = wh.make_inflated_cube_mesh(N_NODES, 0.1, 0.3, 1.0, 8, outer_tag=7)
+    ws = wh.Workspace(mesh, wh.make_element(N_NODES))
+    x = mesh.coords.reshape(-1, 3)
+    bg = wh.sample_background3d(wh.adm_bump_background(), 0.0, x[:, 0], x[:, 1], x[:, 2])
+    coef = ws.fo_coef(torch.from_numpy(bg.reshape(-1).copy()).cuda())
+    Phi0, _ = wh.gaussian_pulse(x, (0.1, -0.05, 0.05), width=0.1)
+    Phi = torch.from_numpy(Phi0).cuda()
+    Pi = torch.zeros_like(Phi)
+    dP = torch.empty_like(Phi); dQ = torch.empty_like(Phi)
+    bc = wh.make_bc3d("sommerfeld")
+    stream = torch.cuda.Stream()
+    ws.set_stream(stream.cuda_stream)
+    K = max(1, min(args.steps, 200))
+    dt = 0.2 * ws.h_min
+    npts = ws.npoints
+    with torch.cuda.stream(stream):
+        for _ in range(max(args.warmup, 2)):
+            ws.fo_rhs(dP, dQ, Phi, Pi, coef, 0.0, bc)
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True); b = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        a.record(stream)
+        for _ in range(K):
+            ws.fo_rhs(dP, dQ, Phi, Pi, coef, 0.0, bc)
+        b.record(stream)
+        torch.cuda.synchronize()
+        wall_rhs = time.perf_counter() - t0
+        ev_rhs = a.elapsed_time(b) / K
+        for _ in range(max(args.warmup, 1)):
+            for st in range(1, 5):
+                ws.fo_rk4_stage(st, Phi, Pi, coef, dt, 0.0, bc)
+        torch.cuda.synchronize()
+        c = torch.cuda.Event(enable_timing=True); d = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        c.record(stream)
+        for _ in range(K):
+            for st in range(1, 5):
+                ws.fo_rk4_stage(st, Phi, Pi, coef, dt, 0.0, bc)
+        d.record(stream)
+        torch.cuda.synchronize()
+        wall_step = time.perf_counter() - t0
+        ev_step = c.elapsed_time(d) / K
+    assert bool(torch.isfinite(Phi).all()) and bool(torch.isfinite(Pi).all())
+    ws.set_stream(None)
+    gbs = FO_CURV_ALG_BYTES * npts / (ev_rhs * 1e-3) / 1e9
+    out = {"metric": "first-order (Phi, Pi) RK4 steps on a curvilinear mesh: grid-point updates/s",
+           "value": npts * K / wall_step, "unit": "grid-point updates/s", "n_gpus": 1, "steps": K,
+           "warmup": args.warmup, "ms_per_step": wall_step * 1e3 / K, "event_ms_per_step": ev_step,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (Gaussian Phi, Pi = 0, ADM bump background)",
+           "config": {"workload": "wave3d_curved_rhs! metric path, inflated cube 13 patches of 8^3 elements, "
+                                  "N = 8 (3,407,872 points), ADM bump background, Sommerfeld outer sphere, "
+                                  "KO off, dt = 0.2 h_min", "points": npts, "dt": dt},
+           "rhs": {"ms": wall_rhs * 1e3 / K, "event_ms": ev_rhs, "points_per_s": npts * K / wall_rhs},
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": gbs / PEAK_HBM_GBS, "traffic": None,
+                        "kernel": "fc_grad_kernel<8> + fc_div_kernel<8> (one RHS)",
+                        "algorithmic_bytes_per_point": FO_CURV_ALG_BYTES}}
+    ws.close()
+    if not args.no_cpu:
+        import wt_oracle as wo
+        small = wh.make_inflated_cube_mesh(N_NODES, 0.1, 0.3, 1.0, 4, outer_tag=7)
+        o = wo.Oracle.from_mesh(small.N, small.coords, small.neighbour, small.neighbour_face, small.bdry)
+        xc = o.coords()
+        bgc = wh.sample_background3d(wh.adm_bump_background(), 0.0, xc[:, 0], xc[:, 1], xc[:, 2])
+        cc = wo.fo_coef(bgc.reshape(-1))
+        Pc = np.exp(-np.sum(xc ** 2, axis=1) / 0.02)
+        Qc = np.zeros_like(Pc)
+        bcc = wo.make_bc3d("sommerfeld")
+        o.fo_rhs_curv(Pc, Qc, cc, 0.0, bcc)
+        reps, t0 = 0, time.perf_counter()
+        while reps < 3 or time.perf_counter() - t0 < 5.0:
+            o.fo_rhs_curv(Pc, Qc, cc, 0.0, bcc)
+            reps += 1
+        tc = (time.perf_counter() - t0) / reps
+        out["cpu_baseline"] = {"value": o.n / tc, "unit": "grid points/s (RHS evaluations)",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                               "kind": "port", "sample": f"CPU oracle (C, FP64), {reps} curvilinear first-order "
+                                                         f"RHS evaluations on the inflated cube with 4^3 elements "
+                                                         f"per patch ({o.n} points), {tc:.3f} s each"}
+    return out
+
+
 def patch_run(wh, torch, args):
     """The multi-patch inflated cube (13 patches of 8^3 elements, N = 8:
     3,407,872 points; patches meet with orientation != 0) with a Sommerfeld
@@ -676,7 +769,8 @@ def main():
     ap.add_argument("--repeats", type=int, default=5,
                     help="plain config: further runs of --steps steps after the timed one (median, minimum)")
     ap.add_argument("--no-tune", action="store_true", help="default tile shapes instead of the autotuner's")
-    ap.add_argument("--config", choices=["plain", "curved", "batch", "serving", "passes", "fo", "mixed_error", "patch"],
+    ap.add_argument("--config", choices=["plain", "curved", "batch", "serving", "passes", "fo", "fo_curv",
+                                         "mixed_error", "patch"],
                     default="plain",
                     help="plain: the FP64 flat workload (headline); curved: the warped-mesh "
                          "ADM workload, FP64 vs mixed, error at --t-final")
@@ -727,6 +821,9 @@ def main():
         return
     if args.config == "patch":
         print(json.dumps(patch_run(wh, torch, args)), flush=True)
+        return
+    if args.config == "fo_curv":
+        print(json.dumps(fo_curv_run(wh, torch, args)), flush=True)
         return
     if args.config == "mixed_error":
         print(json.dumps(mixed_error_run(wh, torch, args)), flush=True)

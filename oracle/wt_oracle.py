@@ -240,7 +240,8 @@ class Oracle:
         return self
 
     def __del__(self):
-        if getattr(self, "ctx", None):
+        # at interpreter exit the module globals may already be gone
+        if getattr(self, "ctx", None) and lib is not None:
             lib().wto_destroy(self.ctx)
             self.ctx = None
 

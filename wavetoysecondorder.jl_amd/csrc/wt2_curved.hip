@@ -365,20 +365,21 @@ __device__ __forceinline__ int fidx(int a, int r, int p, int q)
     return p + N * (q + N * r);
 }
 
-// neighbour element across face (a, side), -1 at a boundary
-__device__ __forceinline__ int64_t nbr_elem(int64_t e, int a, int side, int M0, int M1, int M2,
+// neighbour element across face (a, side) of the element with grid
+// coordinates ec, -1 at a boundary (element counts fit an int)
+__device__ __forceinline__ int64_t nbr_elem(const int (&ec)[3], int a, int side, int M0, int M1, int M2,
                                             int p0, int p1, int p2)
 {
-    int ec[3] = {int(e % M0), int((e / M0) % M1), int(e / (int64_t(M0) * M1))};
     const int Ms[3] = {M0, M1, M2};
     const int ps[3] = {p0, p1, p2};
-    int t = ec[a] + (side ? 1 : -1);
+    int c[3] = {ec[0], ec[1], ec[2]};
+    int t = c[a] + (side ? 1 : -1);
     if (t < 0 || t >= Ms[a]) {
         if (!ps[a]) return -1;
         t = (t + Ms[a]) % Ms[a];
     }
-    ec[a] = t;
-    return ec[0] + int64_t(M0) * (ec[1] + int64_t(M1) * ec[2]);
+    c[a] = t;
+    return c[0] + int64_t(M0) * (c[1] + int64_t(M1) * c[2]);
 }
 
 // symmetric K^ab stored as (11, 12, 13, 22, 23, 33)
@@ -507,6 +508,12 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
     TC* Pt = Sf + 18 * N2;              // 6 N^2: point (+ tangential) term per face node
     const int i = lane % N, j = lane / N;
     const int64_t np = a.npts;
+    // this element's grid coordinates (uniform meshes), once per work-item
+    int ec[3];
+    {
+        const int ei = int(e), exy = ei % (a.M0 * a.M1);
+        ec[0] = exy % a.M0; ec[1] = exy / a.M0; ec[2] = ei / (a.M0 * a.M1);
+    }
 
     TC W0[N], W1[N], W2[N], cgv[N];
     if (act) {
@@ -586,7 +593,7 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
                 exc = tag == 8;
                 bvf = (tag >= 1 && tag <= 6) ? a.k.bv[tag - 1] : TC(0);
             } else {
-                en = nbr_elem(e, ax, side, a.M0, a.M1, a.M2, a.per0, a.per1, a.per2);
+                en = nbr_elem(ec, ax, side, a.M0, a.M1, a.M2, a.per0, a.per1, a.per2);
                 fn = (ax << 1) | (1 - side);
                 som = (a.k.som >> f) & 1;
                 bvf = a.k.bv[f];

@@ -118,8 +118,9 @@ __device__ __forceinline__ void fo_store(double* __restrict__ p, bool on, const 
 // The pointwise step at node q (element e, node (i, j, k)): F^i = b^i Pi +
 // alpha sqrtg g^{ij} DPhi_j, dF += b . DPhi + (alpha / sqrtg) Pi, and the
 // affine outer-boundary terms (src/boundaries3d.jl:76-126) into dF and dP.
+// ec: the element's coordinates (ex, ey, ez) in the element grid.
 template <int N>
-__device__ __forceinline__ void fo_point(const FoArgs& a, int64_t q, int64_t e, int i, int j, int k,
+__device__ __forceinline__ void fo_point(const FoArgs& a, int64_t q, const int (&ec)[3], int i, int j, int k,
                                          double D1, double D2, double D3, double Pq,
                                          double& dF, double& dP, double& F1, double& F2, double& F3)
 {
@@ -136,7 +137,6 @@ __device__ __forceinline__ void fo_point(const FoArgs& a, int64_t q, int64_t e, 
     dF = dF + (b1 * D1 + b2 * D2 + b3 * D3 + (al / sg) * Pq);
     if (!a.anybc) return;
     const int nd[3] = {i, j, k};
-    const int ec[3] = {int(e % a.M0), int((e / a.M0) % a.M1), int(e / (int64_t(a.M0) * a.M1))};
     const int Mv[3] = {a.M0, a.M1, a.M2};
     const int pv[3] = {a.per0, a.per1, a.per2};
     const double Dv[3] = {D1, D2, D3};
@@ -204,36 +204,42 @@ __global__ void __launch_bounds__(FO_MAXT) fo_pass_kernel(FoArgs a)
 
     if constexpr (PHASE == 0 && AX == 2) {
         if (a.fuse) {
-            // DPhi_3 stays in registers; KO sums are kept for the pointwise step
-            double x[N], d3[N], pi[N], acc[2][N];
+            // DPhi_3 stays in registers; KO sums are kept for the pointwise
+            // step.  Pi is read twice (for its KO chain and per node in the
+            // pointwise step) rather than held, and the pointwise step runs
+            // two nodes at a time: together they hold the pass to three
+            // wavefronts per SIMD (161 VGPRs at N = 8, against 219 and two).
+            // Forcing four (128 VGPRs) spills and measured 14 % slower.
+            double x[N], d3[N], acc[2][N];
             fo_load<N, GS>(a.Phi + g0, active, x);
             fo_D<N>(op, a, xch, hasL, hasR, x, d3);
-            fo_load<N, GS>(a.Pi + g0, active, pi);
             for (int fld = 0; fld < 2; ++fld) {
-                double* out = (fld == 0 ? a.dPhi : a.dPi) + g0;
-                fo_load<N, GS>(out, active, acc[fld]);
                 if (a.koT != 0.0) {
                     double y[N];
-                    if (fld == 1) {
-#pragma unroll
-                        for (int r = 0; r < N; ++r) x[r] = pi[r];
-                    }
+                    if (fld == 1) fo_load<N, GS>(a.Pi + g0, active, x);
 #pragma unroll 1
                     for (int rep = 0; rep < 3; ++rep) {
                         fo_D<N>(op, a, xch, hasL, hasR, x, y);
                         fo_D<N>(op, a, xch, hasL, hasR, y, x);
                     }
+                    fo_load<N, GS>((fld == 0 ? a.dPhi : a.dPi) + g0, active, acc[fld]);
 #pragma unroll
                     for (int r = 0; r < N; ++r) acc[fld][r] = fma(a.koT, x[r], acc[fld][r]);
+                } else {
+                    fo_load<N, GS>((fld == 0 ? a.dPhi : a.dPi) + g0, active, acc[fld]);
                 }
             }
             if (!active) return;
             const int i = ll % N, j = ll / N;
-#pragma unroll
+            // the element's grid coordinates, once per work-item (32-bit: the
+            // element count fits an int)
+            const int exy = e % (a.M0 * a.M1);
+            const int ec[3] = {exy % a.M0, exy / a.M0, e / (a.M0 * a.M1)};
+#pragma unroll 2
             for (int r = 0; r < N; ++r) {
                 const int64_t q = g0 + int64_t(r) * GS;
                 double dP = 0.0, F1, F2, F3;
-                fo_point<N>(a, q, e, i, j, r, a.D[q], a.D[n + q], d3[r], pi[r], acc[0][r], dP, F1, F2, F3);
+                fo_point<N>(a, q, ec, i, j, r, a.D[q], a.D[n + q], d3[r], a.Pi[q], acc[0][r], dP, F1, F2, F3);
                 a.D[q] = F1; a.D[n + q] = F2; a.D[2 * n + q] = F3;
                 if (dP != 0.0) acc[1][r] += dP;
             }
@@ -295,8 +301,10 @@ __global__ void __launch_bounds__(256) fo_flux_kernel(FoArgs a)
     for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
         const int64_t e = q / N3;
         const int r = int(q - e * N3);
+        const int ei = int(e), exy = ei % (a.M0 * a.M1);
+        const int ec[3] = {exy % a.M0, exy / a.M0, ei / (a.M0 * a.M1)};
         double dF = a.dPhi[q], dP = 0.0, F1, F2, F3;
-        fo_point<N>(a, q, e, r % N, (r / N) % N, r / (N * N), a.D[q], a.D[n + q], a.D[2 * n + q], a.Pi[q],
+        fo_point<N>(a, q, ec, r % N, (r / N) % N, r / (N * N), a.D[q], a.D[n + q], a.D[2 * n + q], a.Pi[q],
                     dF, dP, F1, F2, F3);
         a.D[q] = F1; a.D[n + q] = F2; a.D[2 * n + q] = F3;
         a.dPhi[q] = dF;
