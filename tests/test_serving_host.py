@@ -81,8 +81,9 @@ def test_footprint_scales_with_points_and_precision(prec):
     # two two-field register blocks (stage input S, partial sums P) + the FP64
     # Lu field, plus small tables
     assert 4 * n * es + 8 * n <= flat <= 4 * n * es + 8 * n + 16384
-    # curved adds 13 FP64 + 13 FP32 coefficients, the energy weight and the face buffer
-    assert curved - flat == 13 * n * 12 + 8 * n + 64 * 6 * 36 * 8
+    # curved adds 13 FP64 + 13 FP32 coefficients, the energy weight, the face
+    # flux buffer and the 5 FP64 + 5 FP32 face-node coefficients
+    assert curved - flat == 13 * n * 12 + 8 * n + 64 * 6 * 36 * 8 + 5 * 64 * 6 * 36 * 12
     r = _req(0, 6, 4, curved=True, T=1.0)
     assert sv.request_footprint(r, prec, 20) >= curved + 16 * n
     assert sv.request_footprint(r, prec, 0) == curved + 16 * n

@@ -657,7 +657,7 @@ int wt2_ctx_footprint(const wt2_mesh_desc* desc, int32_t with_background, int64_
                 ts * int64_t(sizeof(double) + 2 * sizeof(float));
     if (desc->warp_A != 0.0 || with_background)
         b += 13 * n * int64_t(sizeof(double) + sizeof(float)) + n * int64_t(sizeof(double)) +
-             Ne * 6 * N * N * int64_t(sizeof(double));
+             Ne * 6 * N * N * int64_t(sizeof(double)) + 5 * Ne * 6 * N * N * int64_t(sizeof(double) + sizeof(float));
     *bytes = b;
     return WT2_OK;
 }

@@ -212,6 +212,104 @@ __global__ void __launch_bounds__(256) curved_coef_kernel(CoefArgs a)
     if (tid == 0) a.emin[e] = red[0];
 }
 
+// neighbour element across face (a, side) of the element with grid
+// coordinates ec, -1 at a boundary (element counts fit an int)
+__device__ __forceinline__ int64_t nbr_elem(const int (&ec)[3], int a, int side, int M0, int M1, int M2,
+                                            int p0, int p1, int p2)
+{
+    const int Ms[3] = {M0, M1, M2};
+    const int ps[3] = {p0, p1, p2};
+    int c[3] = {ec[0], ec[1], ec[2]};
+    int t = c[a] + (side ? 1 : -1);
+    if (t < 0 || t >= Ms[a]) {
+        if (!ps[a]) return -1;
+        t = (t + Ms[a]) % Ms[a];
+    }
+    c[a] = t;
+    return c[0] + int64_t(M0) * (c[1] + int64_t(M1) * c[2]);
+}
+
+// symmetric K^ab stored as (11, 12, 13, 22, 23, 33)
+__device__ __forceinline__ int kidx(int a, int b)
+{
+    if (a == b) return a == 0 ? 0 : (a == 1 ? 3 : 5);
+    const int s = a + b;
+    return s == 1 ? 1 : (s == 2 ? 2 : 4);
+}
+
+// Face-node coefficients of the face terms (static, like the node
+// coefficients): for face node q = f N^2 + p + N qq of element e (face f on
+// axis a = f / 2), K^{a0}, K^{a1}, K^{a2} and c^a at the node, and kappa_a --
+// the average of the two sides' values where the face has a neighbour, the
+// own value at an outer face.  Stored per element in the order of the face
+// loops, so the operator reads them with adjacent work-items at adjacent
+// addresses instead of gathering them at x-face nodes (node stride N) from
+// the node fields, which the L2 no longer holds by then.  The FP32 copy
+// averages the FP32 node values in FP32, as the FP32 operator did.
+struct FaceCoefArgs {
+    int N, M0, M1, M2, per0, per1, per2;
+    const double* coef64;
+    const float* coef32;
+    const int2* conn;          // general connectivity, or nullptr (uniform mesh)
+    double* fc64;
+    float* fc32;
+    int64_t npts, Ne;
+};
+
+__device__ __forceinline__ int fidx_n(int N, int a, int r, int p, int q)
+{
+    if (a == 0) return r + N * (p + N * q);
+    if (a == 1) return p + N * (r + N * q);
+    return p + N * (q + N * r);
+}
+
+__global__ void __launch_bounds__(256) curved_facecoef_kernel(FaceCoefArgs a)
+{
+    const int N = a.N, N2 = N * N, N3 = N2 * N, F2 = 6 * N2;
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= a.Ne * F2) return;
+    const int64_t e = t / F2;
+    const int q = int(t - e * F2);
+    const int f = q / N2, pq = q - f * N2, p = pq % N, qq = pq / N;
+    const int ax = f >> 1, side = f & 1;
+    const int64_t gn = e * N3 + fidx_n(N, ax, side ? N - 1 : 0, p, qq);
+    int64_t en;
+    int fn, pn = p, qn = qq;
+    if (a.conn) {
+        const int2 cf = a.conn[e * 6 + f];
+        en = cf.x;
+        fn = cf.y & 7;
+        const int ori = (cf.y >> 3) & 7;
+        if (ori & 1) { pn = qq; qn = p; }
+        if (ori & 2) pn = N - 1 - pn;
+        if (ori & 4) qn = N - 1 - qn;
+    } else {
+        const int ei = int(e), exy = ei % (a.M0 * a.M1);
+        const int ec[3] = {exy % a.M0, exy / a.M0, ei / (a.M0 * a.M1)};
+        en = nbr_elem(ec, ax, side, a.M0, a.M1, a.M2, a.per0, a.per1, a.per2);
+        fn = (ax << 1) | (1 - side);
+    }
+    const int64_t np = a.npts;
+    double* o64 = a.fc64 + e * 5 * F2 + q;
+    float* o32 = a.fc32 + e * 5 * F2 + q;
+    for (int b = 0; b < 3; ++b) {
+        o64[b * F2] = a.coef64[kidx(ax, b) * np + gn];
+        o32[b * F2] = a.coef32[kidx(ax, b) * np + gn];
+    }
+    double k64 = a.coef64[(10 + ax) * np + gn];
+    float k32 = a.coef32[(10 + ax) * np + gn];
+    if (en >= 0) {
+        const int axn = fn >> 1;
+        const int64_t gnn = en * N3 + fidx_n(N, axn, (fn & 1) ? N - 1 : 0, pn, qn);
+        k64 = 0.5 * (k64 + a.coef64[(10 + axn) * np + gnn]);
+        k32 = 0.5f * (k32 + a.coef32[(10 + axn) * np + gnn]);
+    }
+    o64[3 * F2] = k64;
+    o32[3 * F2] = k32;
+    o64[4 * F2] = a.coef64[(6 + ax) * np + gn];
+    o32[4 * F2] = a.coef32[(6 + ax) * np + gn];
+}
+
 // Compute cd's coefficients on the device (set-up path: the background and
 // general-mesh coordinates are staged through temporary device buffers).
 static int compute_coefs_device(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const double* w,
@@ -249,6 +347,17 @@ static int compute_coefs_device(CurvedData& cd, const wt2_mesh_desc& d, const do
         hipLaunchKernelGGL(curved_coef_kernel, dim3(unsigned(Ne)), dim3(256), 0, s, a);
         err = hipGetLastError();
     }
+    if (err == hipSuccess) {
+        FaceCoefArgs fa;
+        fa.N = N; fa.M0 = d.M[0]; fa.M1 = d.M[1]; fa.M2 = d.M[2];
+        fa.per0 = cd.periodic[0]; fa.per1 = cd.periodic[1]; fa.per2 = cd.periodic[2];
+        fa.coef64 = cd.coef64; fa.coef32 = cd.coef32; fa.conn = cd.conn;
+        fa.fc64 = cd.fc64; fa.fc32 = cd.fc32;
+        fa.npts = cd.npts; fa.Ne = cd.Ne;
+        const int64_t nt = cd.Ne * 6 * N * N;
+        hipLaunchKernelGGL(curved_facecoef_kernel, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, s, fa);
+        err = hipGetLastError();
+    }
     if (err == hipSuccess) err = hipMemcpyAsync(emin.data(), tmp[3], Ne * sizeof(double), hipMemcpyDeviceToHost, s);
     if (err == hipSuccess) err = hipStreamSynchronize(s);
     cleanup();
@@ -279,12 +388,15 @@ int curved_init(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, const 
     if (hipMalloc(&cd.coef64, 13 * n * sizeof(double)) != hipSuccess ||
         hipMalloc(&cd.coef32, 13 * n * sizeof(float)) != hipSuccess ||
         hipMalloc(&cd.wv, n * sizeof(double)) != hipSuccess ||
-        hipMalloc(&cd.fbuf, fb * sizeof(double)) != hipSuccess) {
+        hipMalloc(&cd.fbuf, fb * sizeof(double)) != hipSuccess ||
+        hipMalloc(&cd.fc64, 5 * fb * sizeof(double)) != hipSuccess ||
+        hipMalloc(&cd.fc32, 5 * fb * sizeof(float)) != hipSuccess) {
         curved_free(cd);
         set_error("curved set-up: device allocation failed");
         return WT2_ERR_OOM;
     }
-    *bytes += int64_t(13 * n * (sizeof(double) + sizeof(float)) + n * sizeof(double) + fb * sizeof(double));
+    *bytes += int64_t(13 * n * (sizeof(double) + sizeof(float)) + n * sizeof(double) + fb * sizeof(double) +
+                      5 * fb * (sizeof(double) + sizeof(float)));
     if (!bg10) {
         // Minkowski coefficients are computed on first use, so that a
         // background set right after creation is computed only once
@@ -326,7 +438,8 @@ int curved_init_mesh(CurvedData& cd, const wt2_mesh_desc& d, const double* xi, c
 
 void curved_free(CurvedData& cd)
 {
-    hipFree(cd.coef64); hipFree(cd.coef32); hipFree(cd.wu); hipFree(cd.wv); hipFree(cd.fbuf); hipFree(cd.conn);
+    (void)hipFree(cd.coef64); (void)hipFree(cd.coef32); (void)hipFree(cd.wu); (void)hipFree(cd.wv);
+    (void)hipFree(cd.fbuf); (void)hipFree(cd.conn); (void)hipFree(cd.fc64); (void)hipFree(cd.fc32);
     cd = CurvedData();
 }
 
@@ -349,6 +462,7 @@ struct CArgs {
     const void* u; const void* v;   // stage fields (TS)
     const TC* coef;                 // 13 x npts SoA
     TC* fbuf;                       // 6 N^2 per element
+    const TC* fcoef;                // 5 x 6 N^2 per element: face-node coefficients
     const TC* op;                   // even/odd operator table (OpLayout<N>)
     void* out;                      // output (TO): wgt (...) or unweighted
     int64_t npts;
@@ -363,31 +477,6 @@ __device__ __forceinline__ int fidx(int a, int r, int p, int q)
     if (a == 0) return r + N * (p + N * q);
     if (a == 1) return p + N * (r + N * q);
     return p + N * (q + N * r);
-}
-
-// neighbour element across face (a, side) of the element with grid
-// coordinates ec, -1 at a boundary (element counts fit an int)
-__device__ __forceinline__ int64_t nbr_elem(const int (&ec)[3], int a, int side, int M0, int M1, int M2,
-                                            int p0, int p1, int p2)
-{
-    const int Ms[3] = {M0, M1, M2};
-    const int ps[3] = {p0, p1, p2};
-    int c[3] = {ec[0], ec[1], ec[2]};
-    int t = c[a] + (side ? 1 : -1);
-    if (t < 0 || t >= Ms[a]) {
-        if (!ps[a]) return -1;
-        t = (t + Ms[a]) % Ms[a];
-    }
-    c[a] = t;
-    return c[0] + int64_t(M0) * (c[1] + int64_t(M1) * c[2]);
-}
-
-// symmetric K^ab stored as (11, 12, 13, 22, 23, 33)
-__device__ __forceinline__ int kidx(int a, int b)
-{
-    if (a == b) return a == 0 ? 0 : (a == 1 ? 3 : 5);
-    const int s = a + b;
-    return s == 1 ? 1 : (s == 2 ? 2 : 4);
 }
 
 template <int N, typename TC>
@@ -428,10 +517,10 @@ __device__ __forceinline__ void faceflux_body(const CArgs<N, TC>& a, const int64
         const int i = n % N, j = (n / N) % N, l = n / (N * N);
         TC g[3];
         grad_lds<N, TC>(su, kl, i, j, l, g);
-        const int64_t gn = e * N * N * N + n;
-        TC F = kab<N, TC>(a.coef, a.npts, gn, ax, 0) * g[0];
-        F = fma(kab<N, TC>(a.coef, a.npts, gn, ax, 1), g[1], F);
-        F = fma(kab<N, TC>(a.coef, a.npts, gn, ax, 2), g[2], F);
+        const TC* fc = a.fcoef + e * 5 * 6 * N * N + q;
+        TC F = fc[0] * g[0];
+        F = fma(fc[6 * N * N], g[1], F);
+        F = fma(fc[2 * 6 * N * N], g[2], F);
         a.fbuf[e * 6 * N * N + q] = F;
     }
 }
@@ -574,6 +663,7 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
             const TC nrm = side ? TC(1) : TC(-1);
             const int n = fidx<N>(ax, r, p, qq);
             const int64_t gn = eb + n;
+            const TC* fc = a.fcoef + e * 5 * S::F2 + q;   // K^{a0}, K^{a1}, K^{a2}, kappa, c^a
             // neighbour element en (-1: outer face), its face fn, the neighbour
             // face's node (pn, qn) that meets this face's (p, qq), and whether
             // this outer face is a Sommerfeld face; bvf: its Dirichlet value
@@ -613,12 +703,13 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
                 // wF/Hphys = |n_u| / (w_r |J|).  B holds u_t (zero without it).
 #pragma unroll
                 for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = TC(0);
-                const TC nu = sqrt(kab<N, TC>(a.coef, np, gn, ax, ax) / a.coef[9 * np + gn]);
+                const TC nu = sqrt(fc[ax * S::F2] / a.coef[9 * np + gn]);
                 const TC drag = fma(nrm, a.fbuf[e * S::F2 + q], nu * fma(a.k.iR, A[n], B[n]));
                 Pt[q] = -drag * T.winv[r];
                 continue;
             }
-            TC du, dF = TC(0), kap = a.coef[(10 + ax) * np + gn], dv = TC(0), alpha;
+            TC du, dF = TC(0), dv = TC(0), alpha;
+            const TC kap = fc[3 * S::F2];
             if (en >= 0) {
                 // outward normal fluxes of the two sides: nrm F (own) and
                 // nrm_n F_n (neighbour); dF = F + (nrm nrm_n) F_n
@@ -628,7 +719,6 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
                 const TC Fn = a.fbuf[en * S::F2 + fn * N2 + pn + N * qn];
                 const TC Fo = a.fbuf[e * S::F2 + q];
                 dF = ((fn & 1) == side) ? Fo + Fn : Fo - Fn;
-                kap = TC(0.5) * (kap + a.coef[(10 + axn) * np + gnn]);
                 if (adv) dv = B[n] - TC(v[gnn]);
                 alpha = TC(0.5);
             } else {
@@ -638,9 +728,9 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
             }
             const TC an = alpha * nrm;
 #pragma unroll
-            for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = an * kab<N, TC>(a.coef, np, gn, ax, b) * du;
+            for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = an * fc[b * S::F2] * du;
             TC pt = TC(-0.5) * nrm * dF - a.k.tau * kap * du;
-            if (adv) pt -= nrm * a.coef[(6 + ax) * np + gn] * dv;
+            if (adv) pt -= nrm * fc[4 * S::F2] * dv;
             Pt[q] = pt * T.winv[r];
         }
     }
@@ -761,6 +851,8 @@ static void make_cargs(const CurvedData& cd, const CurvedCall& cc, CArgs<N, TC>&
     a.coef = reinterpret_cast<const TC*>(std::is_same<TC, float>::value ? static_cast<const void*>(cd.coef32)
                                                                           : static_cast<const void*>(cd.coef64));
     a.fbuf = reinterpret_cast<TC*>(cd.fbuf);
+    a.fcoef = reinterpret_cast<const TC*>(std::is_same<TC, float>::value ? static_cast<const void*>(cd.fc32)
+                                                                           : static_cast<const void*>(cd.fc64));
     a.op = static_cast<const TC*>(std::is_same<TC, float>::value ? cd.op32 : cd.op64);
     a.out = cc.out ? static_cast<void*>(cc.out) : cc.Pv;
     a.npts = cd.npts;

@@ -30,6 +30,11 @@ struct CurvedData {
     double* wu = nullptr;       // energy weight of u*Lu: H3 (per node)
     double* wv = nullptr;       // energy weight of v^2: H3 / wgt
     double* fbuf = nullptr;     // face flux buffer: 6 N^2 per element
+    // face-node coefficients of the face terms, 5 x 6 N^2 per element in the
+    // order of the face loop (K^{a0}, K^{a1}, K^{a2}, kappa, c^a; a = the
+    // face's axis), FP64 and FP32
+    double* fc64 = nullptr;
+    float* fc32 = nullptr;
     bool pending = false;       // allocated; coefficients not computed yet
     const void* op64 = nullptr; // even/odd operator tables (owned by the context)
     const void* op32 = nullptr;
