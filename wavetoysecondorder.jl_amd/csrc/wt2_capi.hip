@@ -234,7 +234,7 @@ struct wt2_ctx {
     double* fo_D;   // first-order path: DPhi_1..3 (lazy)
     // first-order path on general meshes (lazy): metric terms, |J|, element
     // extents, flux, KO channels, G and w on the device, mu of the KO derivative
-    double* fc_JAp; double* fc_Jp; double* fc_ell; double* fc_F; double* fc_ko;
+    double* fc_JAp; double* fc_Jp; double* fc_ell; double* fc_F; double* fc_ko; double* fc_fb;
     double* fc_ops;
     double fc_mu;
     double* fo_rk;  // first-order RK4 registers: S, A, k (2 fields each, lazy)
@@ -1050,6 +1050,7 @@ void wt2_ctx_destroy(wt2_ctx* c)
     hipFree(c->fo_D);
     hipFree(c->fo_rk);
     hipFree(c->fc_JAp); hipFree(c->fc_Jp); hipFree(c->fc_ell); hipFree(c->fc_F); hipFree(c->fc_ko); hipFree(c->fc_ops);
+    hipFree(c->fc_fb);
     curved_free(c->cd);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
     delete c;
@@ -1515,7 +1516,7 @@ void fc_desc(const wt2_ctx* c, FcDesc& d)
     d.w = c->fc_ops + 256;
     d.conn = c->cd.conn;
     d.JAp = c->fc_JAp; d.Jp = c->fc_Jp; d.ell = c->fc_ell;
-    d.D = c->fo_D; d.F = c->fc_F; d.ko = c->fc_ko;
+    d.D = c->fo_D; d.F = c->fc_F; d.ko = c->fc_ko; d.fb = c->fc_fb;
     d.partial = c->red;
 }
 
@@ -1542,6 +1543,7 @@ int fo_ready(wt2_ctx* c, const char* who)
     if (!rc) rc = alloc_dev(c, reinterpret_cast<void**>(&c->fc_Jp), n * sizeof(double));
     if (!rc) rc = alloc_dev(c, reinterpret_cast<void**>(&c->fc_ell), 3 * size_t(c->Ne) * sizeof(double));
     if (!rc) rc = alloc_dev(c, reinterpret_cast<void**>(&c->fc_F), 3 * n * sizeof(double));
+    if (!rc) rc = alloc_dev(c, reinterpret_cast<void**>(&c->fc_fb), 6 * size_t(c->Ne) * c->N * c->N * sizeof(double));
     if (!rc) rc = alloc_dev(c, reinterpret_cast<void**>(&c->fc_ko), 12 * n * sizeof(double));
     if (!rc) rc = alloc_dev(c, reinterpret_cast<void**>(&c->fc_ops), (256 + 16) * sizeof(double));
     double* xyz = nullptr;

@@ -174,9 +174,18 @@ __global__ void __launch_bounds__(FC_THREADS) fc_grad_kernel(FcDesc d)
     for (int q = tid; q < N3; q += FC_THREADS) Ph[q] = d.Phi[e * N3 + q];
     double out[S::PER][3];
     for (int n = 0; n < 3; ++n) {
+        // the work-item's nodes' metric terms of this component, read once
+        // for the flux form and the advective form
+        double ja[S::PER][3];
         __syncthreads();
-        for (int q = tid; q < N3; q += FC_THREADS)
-            for (int a = 0; a < 3; ++a) W[a * N3 + q] = d.JAp[int64_t(a * 3 + n) * np + e * N3 + q] * Ph[q];
+        for (int r = 0; r < S::PER; ++r) {
+            const int q = tid + r * FC_THREADS;
+            if (q >= N3) continue;
+            for (int a = 0; a < 3; ++a) {
+                ja[r][a] = d.JAp[int64_t(a * 3 + n) * np + e * N3 + q];
+                W[a * N3 + q] = ja[r][a] * Ph[q];
+            }
+        }
         __syncthreads();
         for (int r = 0; r < S::PER; ++r) {
             const int q = tid + r * FC_THREADS;
@@ -185,7 +194,7 @@ __global__ void __launch_bounds__(FC_THREADS) fc_grad_kernel(FcDesc d)
             double cons = 0.0, adv = 0.0;
             for (int a = 0; a < 3; ++a) {
                 cons += fc_dref<N>(G, W + a * N3, a, i, j, k);
-                adv = fma(d.JAp[int64_t(a * 3 + n) * np + e * N3 + q], fc_dref<N>(G, Ph, a, i, j, k), adv);
+                adv = fma(ja[r][a], fc_dref<N>(G, Ph, a, i, j, k), adv);
             }
             out[r][n] = 0.5 * (cons + adv) / d.Jp[e * N3 + q];
         }
@@ -196,11 +205,12 @@ __global__ void __launch_bounds__(FC_THREADS) fc_grad_kernel(FcDesc d)
         const int ijk[3] = {q % N, (q / N) % N, q / (N * N)};
         const int64_t gq = e * N3 + q;
         // centred SAT: 1/2 s JAp^a (Phi_nbr - Phi) / (w_r |J|) on each face the node lies on
+        bool outer = false;
         for (int f = 0; f < 6; ++f) {
             const int a = f >> 1, sd = f & 1, rr = sd ? N - 1 : 0;
             if (ijk[a] != rr) continue;
             const int2 cf = d.conn[e * 6 + f];
-            if (cf.x < 0) continue;
+            if (cf.x < 0) { outer = true; continue; }
             const int fn = cf.y & 7, ori = (cf.y >> 3) & 7;
             const int p = a == 0 ? ijk[1] : ijk[0], qq = a == 2 ? ijk[1] : ijk[2];
             int pn, qn;
@@ -209,7 +219,10 @@ __global__ void __launch_bounds__(FC_THREADS) fc_grad_kernel(FcDesc d)
             const double sc = 0.5 * (sd ? 1.0 : -1.0) * (d.Phi[gn] - Ph[q]) / (d.w[rr] * d.Jp[gq]);
             for (int n = 0; n < 3; ++n) out[r][n] = fma(sc, d.JAp[int64_t(a * 3 + n) * np + gq], out[r][n]);
         }
-        for (int n = 0; n < 3; ++n) d.D[n * np + gq] = out[r][n];
+        // grad Phi is read again only by the energy (grad_only) and by the
+        // boundary pass at outer-face nodes
+        if (d.grad_only || outer)
+            for (int n = 0; n < 3; ++n) d.D[n * np + gq] = out[r][n];
         if (d.grad_only) continue;
         const double* cf = d.coef;
         const double al = cf[gq], sgm = cf[np + gq];
@@ -219,10 +232,22 @@ __global__ void __launch_bounds__(FC_THREADS) fc_grad_kernel(FcDesc d)
         const double Pi = d.Pi[gq];
         const double D0 = out[r][0], D1 = out[r][1], D2 = out[r][2];
         const double as = al * sgm;
-        d.F[gq] = b0 * Pi + as * (g11 * D0 + g12 * D1 + g13 * D2);
-        d.F[np + gq] = b1 * Pi + as * (g12 * D0 + g22 * D1 + g23 * D2);
-        d.F[2 * np + gq] = b2 * Pi + as * (g13 * D0 + g23 * D1 + g33 * D2);
+        const double Fv[3] = {b0 * Pi + as * (g11 * D0 + g12 * D1 + g13 * D2),
+                              b1 * Pi + as * (g12 * D0 + g22 * D1 + g23 * D2),
+                              b2 * Pi + as * (g13 * D0 + g23 * D1 + g33 * D2)};
+        for (int m = 0; m < 3; ++m) d.F[m * np + gq] = Fv[m];
         d.dPhi[gq] = b0 * D0 + b1 * D1 + b2 * D2 + (al / sgm) * Pi;
+        // the face-normal flux of each face this node lies on, for the
+        // neighbour's SAT (a contiguous face buffer instead of six gathers
+        // of its metric terms and flux per face node)
+        for (int f = 0; f < 6; ++f) {
+            const int a = f >> 1, rr = (f & 1) ? N - 1 : 0;
+            if (ijk[a] != rr) continue;
+            const int p = a == 0 ? ijk[1] : ijk[0], qq = a == 2 ? ijk[1] : ijk[2];
+            double Wf = 0.0;
+            for (int m = 0; m < 3; ++m) Wf = fma(d.JAp[int64_t(a * 3 + m) * np + gq], Fv[m], Wf);
+            d.fb[(e * 6 + f) * N * N + p + N * qq] = Wf;
+        }
     }
 }
 
@@ -241,12 +266,18 @@ __global__ void __launch_bounds__(FC_THREADS) fc_div_kernel(FcDesc d)
     for (int q = tid; q < N * N; q += FC_THREADS) G[q] = d.G[q];
     for (int q = tid; q < 3 * N3; q += FC_THREADS) Fl[q] = d.F[int64_t(q / N3) * np + e * N3 + q % N3];
     __syncthreads();
-    for (int q = tid; q < N3; q += FC_THREADS)
+    // the work-item's nodes' metric terms, read once for both forms
+    double ja[S::PER][9];
+    for (int r = 0; r < S::PER; ++r) {
+        const int q = tid + r * FC_THREADS;
+        if (q >= N3) continue;
+        for (int c = 0; c < 9; ++c) ja[r][c] = d.JAp[int64_t(c) * np + e * N3 + q];
         for (int a = 0; a < 3; ++a) {
             double s = 0.0;
-            for (int m = 0; m < 3; ++m) s = fma(d.JAp[int64_t(a * 3 + m) * np + e * N3 + q], Fl[m * N3 + q], s);
+            for (int m = 0; m < 3; ++m) s = fma(ja[r][a * 3 + m], Fl[m * N3 + q], s);
             W[a * N3 + q] = s;
         }
+    }
     __syncthreads();
     for (int r = 0; r < S::PER; ++r) {
         const int q = tid + r * FC_THREADS;
@@ -257,8 +288,7 @@ __global__ void __launch_bounds__(FC_THREADS) fc_div_kernel(FcDesc d)
         double cons = 0.0, adv = 0.0;
         for (int a = 0; a < 3; ++a) {
             cons += fc_dref<N>(G, W + a * N3, a, i, j, k);
-            for (int m = 0; m < 3; ++m)
-                adv = fma(d.JAp[int64_t(a * 3 + m) * np + gq], fc_dref<N>(G, Fl + m * N3, a, i, j, k), adv);
+            for (int m = 0; m < 3; ++m) adv = fma(ja[r][a * 3 + m], fc_dref<N>(G, Fl + m * N3, a, i, j, k), adv);
         }
         const double Jq = d.Jp[gq];
         double out = 0.5 * (cons + adv) / Jq;
@@ -273,9 +303,7 @@ __global__ void __launch_bounds__(FC_THREADS) fc_div_kernel(FcDesc d)
                 const int p = a == 0 ? j : i, qq = a == 2 ? j : k;
                 int pn, qn;
                 fc_orient(ori, N, p, qq, pn, qn);
-                const int64_t gn = int64_t(cf.x) * N3 + fc_fidx(N, an, (fn & 1) ? N - 1 : 0, pn, qn);
-                double Wn = 0.0;
-                for (int m = 0; m < 3; ++m) Wn = fma(d.JAp[int64_t(an * 3 + m) * np + gn], d.F[m * np + gn], Wn);
+                const double Wn = d.fb[(int64_t(cf.x) * 6 + fn) * N * N + pn + N * qn];
                 const double qo = s * W[a * N3 + q], qb = ((fn & 1) ? 1.0 : -1.0) * Wn;
                 out -= 0.5 * (qo + qb) / (d.w[rr] * Jq);
                 continue;

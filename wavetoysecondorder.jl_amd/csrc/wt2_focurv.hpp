@@ -22,6 +22,8 @@ struct FcDesc {
     const double* coef;      // 11 fields (make_coef3d order)
     double* D;               // 3 fields: grad Phi
     double* F;               // 3 fields: flux
+    double* fb;              // per face node (6 N^2 per element): the face-normal
+                             // flux sum_m J a^f_m F^m, read by the neighbour
     double* dPhi; double* dPi;
     double* ko;              // 12 fields: the KO channels, ping-pong
     double koT;              // eps mu^-5 (0: no KO)
