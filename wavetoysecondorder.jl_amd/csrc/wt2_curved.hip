@@ -606,55 +606,9 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
 
     TC W0[N], W1[N], W2[N], cgv[N];
     if (act) {
-        // ---- P2: gradients and fluxes on this z-line
-        TC Gi[N], Gj[N];
-#pragma unroll
-        for (int p = 0; p < N; ++p) { Gi[p] = T.G[i * N + p]; Gj[p] = T.G[j * N + p]; }
-        TC g2[N], gv2[N];
-        {
-            TC x[N];
-#pragma unroll
-            for (int k = 0; k < N; ++k) x[k] = A[i + N * j + N2 * k];
-            apply_G<N, TC>(T.op, x, g2);
-#pragma unroll
-            for (int k = 0; k < N; ++k) x[k] = B[i + N * j + N2 * k];
-            if (adv) apply_G<N, TC>(T.op, x, gv2);
-            else {
-#pragma unroll
-                for (int k = 0; k < N; ++k) gv2[k] = TC(0);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-            const int n = i + N * j + N2 * k;
-            TC g0 = TC(0), g1 = TC(0);
-#pragma unroll
-            for (int p = 0; p < N; ++p) {
-                g0 = fma(Gi[p], A[p + N * j + N2 * k], g0);
-                g1 = fma(Gj[p], A[i + N * p + N2 * k], g1);
-            }
-            const int64_t gn = eb + n;
-            const TC K00 = a.coef[0 * np + gn], K01 = a.coef[1 * np + gn], K02 = a.coef[2 * np + gn];
-            const TC K11 = a.coef[3 * np + gn], K12 = a.coef[4 * np + gn], K22 = a.coef[5 * np + gn];
-            TC F0 = K00 * g0, F1 = K01 * g0, F2 = K02 * g0;
-            F0 = fma(K01, g1, F0); F1 = fma(K11, g1, F1); F2 = fma(K12, g1, F2);
-            F0 = fma(K02, g2[k], F0); F1 = fma(K12, g2[k], F1); F2 = fma(K22, g2[k], F2);
-            TC cg = TC(0);
-            if (adv) {
-                const TC c0 = a.coef[6 * np + gn], c1 = a.coef[7 * np + gn], c2 = a.coef[8 * np + gn];
-                const TC vn = B[n];
-                F0 = fma(c0, vn, F0); F1 = fma(c1, vn, F1); F2 = fma(c2, vn, F2);
-                TC gv0 = TC(0), gv1 = TC(0);
-#pragma unroll
-                for (int p = 0; p < N; ++p) {
-                    gv0 = fma(Gi[p], B[p + N * j + N2 * k], gv0);
-                    gv1 = fma(Gj[p], B[i + N * p + N2 * k], gv1);
-                }
-                cg = c0 * gv0 + c1 * gv1 + c2 * gv2[k];
-            }
-            W0[k] = F0; W1[k] = F1; W2[k] = F2; cgv[k] = cg;
-        }
-        // ---- P2 (face-parallel): face-node terms of this element
+        // ---- P2 (face-parallel): face-node terms of this element, first, while
+        // the fluxes of the z-line are not yet live in registers (more of the
+        // face loop's gathers in flight)
         for (int q = lane; q < S::F2; q += S::L) {
             const int f = q / N2, pq = q - f * N2;
             const int p = pq % N, qq = pq / N;
@@ -732,6 +686,54 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
             TC pt = TC(-0.5) * nrm * dF - a.k.tau * kap * du;
             if (adv) pt -= nrm * fc[4 * S::F2] * dv;
             Pt[q] = pt * T.winv[r];
+        }
+        // ---- P2: gradients and fluxes on this z-line
+        TC Gi[N], Gj[N];
+#pragma unroll
+        for (int p = 0; p < N; ++p) { Gi[p] = T.G[i * N + p]; Gj[p] = T.G[j * N + p]; }
+        TC g2[N], gv2[N];
+        {
+            TC x[N];
+#pragma unroll
+            for (int k = 0; k < N; ++k) x[k] = A[i + N * j + N2 * k];
+            apply_G<N, TC>(T.op, x, g2);
+#pragma unroll
+            for (int k = 0; k < N; ++k) x[k] = B[i + N * j + N2 * k];
+            if (adv) apply_G<N, TC>(T.op, x, gv2);
+            else {
+#pragma unroll
+                for (int k = 0; k < N; ++k) gv2[k] = TC(0);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const int n = i + N * j + N2 * k;
+            TC g0 = TC(0), g1 = TC(0);
+#pragma unroll
+            for (int p = 0; p < N; ++p) {
+                g0 = fma(Gi[p], A[p + N * j + N2 * k], g0);
+                g1 = fma(Gj[p], A[i + N * p + N2 * k], g1);
+            }
+            const int64_t gn = eb + n;
+            const TC K00 = a.coef[0 * np + gn], K01 = a.coef[1 * np + gn], K02 = a.coef[2 * np + gn];
+            const TC K11 = a.coef[3 * np + gn], K12 = a.coef[4 * np + gn], K22 = a.coef[5 * np + gn];
+            TC F0 = K00 * g0, F1 = K01 * g0, F2 = K02 * g0;
+            F0 = fma(K01, g1, F0); F1 = fma(K11, g1, F1); F2 = fma(K12, g1, F2);
+            F0 = fma(K02, g2[k], F0); F1 = fma(K12, g2[k], F1); F2 = fma(K22, g2[k], F2);
+            TC cg = TC(0);
+            if (adv) {
+                const TC c0 = a.coef[6 * np + gn], c1 = a.coef[7 * np + gn], c2 = a.coef[8 * np + gn];
+                const TC vn = B[n];
+                F0 = fma(c0, vn, F0); F1 = fma(c1, vn, F1); F2 = fma(c2, vn, F2);
+                TC gv0 = TC(0), gv1 = TC(0);
+#pragma unroll
+                for (int p = 0; p < N; ++p) {
+                    gv0 = fma(Gi[p], B[p + N * j + N2 * k], gv0);
+                    gv1 = fma(Gj[p], B[i + N * p + N2 * k], gv1);
+                }
+                cg = c0 * gv0 + c1 * gv1 + c2 * gv2[k];
+            }
+            W0[k] = F0; W1[k] = F1; W2[k] = F2; cgv[k] = cg;
         }
     }
     __syncthreads();
