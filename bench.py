@@ -653,6 +653,25 @@ def fo_curv_run(wh, torch, args):
     return out
 
 
+def mfma_ab_run(args):
+    """The per-axis contractions of the drop-in RHS at N = 8, FP64, on VALU
+    (the even/odd split the kernels use) and on MFMA (v_mfma_f64_16x16x4f64
+    with [G G; G] as one operand), 32^3 elements: the standalone program
+    wt2_mfma_ab built next to libwt2.so, run as a child process."""
+    import subprocess
+    exe = os.path.join(ROOT, "wavetoysecondorder.jl_amd", "wavetoy_hip", "wt2_mfma_ab")
+    if not os.path.exists(exe):
+        raise SystemExit("bench.py: wt2_mfma_ab is not built (run __graft_entry__.build())")
+    r = subprocess.run([exe, str(M_ELEMS)], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise SystemExit(f"bench.py: wt2_mfma_ab failed ({r.returncode}): {r.stderr.strip()} {r.stdout.strip()}")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"metric": "per-axis G u and G G u of every element line, three axes (us per launch)",
+            "config": {"workload": f"{M_ELEMS}^3 elements x 8^3 nodes, FP64, contractions of the drop-in RHS "
+                                   "without its face terms, element staged in LDS, running sum in LDS"},
+            **res}
+
+
 def patch_run(wh, torch, args):
     """The multi-patch inflated cube (13 patches of 8^3 elements, N = 8:
     3,407,872 points; patches meet with orientation != 0) with a Sommerfeld
@@ -770,7 +789,7 @@ def main():
                     help="plain config: further runs of --steps steps after the timed one (median, minimum)")
     ap.add_argument("--no-tune", action="store_true", help="default tile shapes instead of the autotuner's")
     ap.add_argument("--config", choices=["plain", "curved", "batch", "serving", "passes", "fo", "fo_curv",
-                                         "mixed_error", "patch"],
+                                         "mixed_error", "patch", "mfma_ab"],
                     default="plain",
                     help="plain: the FP64 flat workload (headline); curved: the warped-mesh "
                          "ADM workload, FP64 vs mixed, error at --t-final")
@@ -795,6 +814,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes_read": src.numel() * 8, "bytes_written": dst.numel() * 8}))
+        return
+    if args.config == "mfma_ab":
+        print(json.dumps(mfma_ab_run(args)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only (asked for {args.gpus}); instances or element slabs "

@@ -20,7 +20,18 @@
  *   - the H-norm energy (test/kernels3d_tests.jl:32-47),
  *   - the first-order (Phi, Pi) system of wave3d_curved_rhs! with its KO,
  *     affine boundary pass and energy (src/wave3d_curved.jl:205-287,
- *     src/boundaries3d.jl:56-126).
+ *     src/boundaries3d.jl:56-126),
+ *   - general connectivity (multi-patch meshes whose faces meet with a D4
+ *     orientation, src/wave_strong_rhs.jl:41-73; tags 1..6 Dirichlet, 7
+ *     Sommerfeld, 8 excision) for the second-order operator, and the
+ *     curvilinear first-order path on such meshes (metric given:
+ *     src/wave3d_curved.jl:218-256, src/boundaries3d.jl:131-280).
+ * Every function is checked against known answers, closed forms or
+ * invariants in tests/test_oracle_pins.py.  Parity unpinned: on elements that
+ * are not affine, the curvilinear KO derivative takes the element's extent
+ * along each reference axis as its h (HexSBPSAT's apply_D!, which fixes that
+ * choice in the reference, is not in the reference tree); on affine elements
+ * it reduces to the affine path's KO, which is pinned.
  * Layout: fields are (N,N,N,Ne) column-major (i fastest), elements x-fastest
  * e = ex + Mx*(ey + My*ez); faces 1..6 = (-x,+x,-y,+y,-z,+z).
  */

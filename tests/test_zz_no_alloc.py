@@ -83,3 +83,18 @@ def test_noalloc_refuses_more_energy_samples_than_its_array():
     torch.cuda.synchronize()
     assert torch.equal(u, before)          # nothing ran
     ws.close()
+
+
+def test_host_mirror_equals_rk4_steps():
+    """Workspace.rk4_steps_noalloc (the Python mirror of the entry point)
+    gives rk4_steps's bits."""
+    import torch
+    ws, u0, v0 = _workspace(5, 3, "fp64", False)
+    dt = 0.2 * ws.h_min
+    ua, va = torch.from_numpy(u0).cuda(), torch.from_numpy(v0).cuda()
+    ub, vb = ua.clone(), va.clone()
+    Ea = ws.rk4_steps(ua, va, dt, 6, energy_every=2)
+    Eb = ws.rk4_steps_noalloc(ub, vb, dt, 6, energy_every=2)
+    torch.cuda.synchronize()
+    assert torch.equal(ua, ub) and torch.equal(va, vb) and np.array_equal(Ea, Eb)
+    ws.close()

@@ -1,4 +1,6 @@
-"""Build libwt2.so (HIP for gfx950) in-tree with hipcc.
+"""Build libwt2.so (HIP for gfx950) in-tree with hipcc, and next to it the
+standalone A/B program wt2_mfma_ab (VALU against MFMA contractions; run by
+bench.py --config mfma_ab).
 
 Usage: python wavetoysecondorder.jl_amd/build.py   (or __graft_entry__.build()).
 """
@@ -9,6 +11,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "wavetoy_hip", "libwt2.so")
+PROBE_SRC = "wt2_mfma_ab.hip"
+PROBE_OUT = os.path.join(HERE, "wavetoy_hip", "wt2_mfma_ab")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 SOURCES = ["wt2_flat.hip", "wt2_curved.hip", "wt2_fo.hip", "wt2_focurv.hip", "wt2_capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -48,6 +53,13 @@ def build(verbose=False, jobs=None):
         print(" ".join(link), flush=True)
     subprocess.run(link, check=True)
     os.replace(OUT + ".tmp", OUT)
+    probe = [HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+             "-packed-fp32-ops", "-I", INCLUDE, "-I", CSRC, "-o", PROBE_OUT + ".tmp", os.path.join(CSRC, PROBE_SRC),
+             "-L", os.path.dirname(OUT), "-lwt2", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(probe), flush=True)
+    subprocess.run(probe, check=True)
+    os.replace(PROBE_OUT + ".tmp", PROBE_OUT)
     return OUT
 
 

@@ -464,6 +464,22 @@ class Workspace:
                                   E.ctypes.data if ne else None, flags))
         return E[:ne] if ne else None
 
+    def rk4_steps_noalloc(self, u, ud, dt, nsteps, energy_every=0):
+        """rk4_steps for callers that cannot allocate device memory
+        (wt2_rk4_steps_noalloc): device tensors only, kernels launched one by
+        one, the energy history in the library's fixed device array (at most
+        WT2_NOALLOC_MAX_ENERGIES samples).  Calls that overlap in time on one
+        GPU race."""
+        (pu, pud), fl = self._ptrs(u, ud, writable=(0, 1))
+        if fl:
+            raise ValueError("rk4_steps_noalloc needs device tensors")
+        ne = 1 + nsteps // energy_every if energy_every > 0 else 0
+        E = np.zeros(max(ne, 1))
+        self._sync_torch()
+        check(lib().wt2_rk4_steps_noalloc(self.ctx, pu, pud, float(dt), int(nsteps), int(energy_every),
+                                          E.ctypes.data if ne else None))
+        return E[:ne] if ne else None
+
     def set_stream(self, stream_handle):
         """Run this context's kernels on an external HIP stream (int handle, e.g.
         torch.cuda.current_stream().cuda_stream); None restores its own."""
