@@ -285,14 +285,16 @@ def test_evolve3d_inflated_cube_sommerfeld_lets_the_pulse_out():
 
 
 def test_batch_of_patch_meshes_matches_single_runs():
-    # batches (one launch per kernel group per stage) give the bits of rk4_steps
+    # batches (one launch per kernel group per stage) give the bits of rk4_steps,
+    # also with a uniform warped context of the same N and precision in the mix
     import torch
     specs = [(wh.make_inflated_cube_mesh(5, 0.1, 0.3, 1.0, 2, outer_tag=7), "fp64"),
              (wh.make_cubed_cube_mesh(5, 2, 0.3), "fp64"),
+             (wh.make_warped_uniform_hex(3, 3, 3, -1.0, 1.0, 0.05), "fp64"),
              (wh.make_cubed_ball_mesh(5, 2, outer_tag=7), "mixed")]
     singles, runs = [], []
     for mesh, prec in specs:
-        x = mesh.coords.reshape(-1, 3)
+        x = mesh.coords.reshape(-1, 3) if hasattr(mesh, "coords") else wh.host_coords(mesh, wh.make_element(5))
         u0 = np.exp(-np.sum((x - 0.1) ** 2, axis=1) / 0.05)
         v0 = 0.1 * np.cos(math.pi * x[:, 2])
         for rep in range(2):

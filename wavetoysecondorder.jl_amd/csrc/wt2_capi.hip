@@ -1878,8 +1878,11 @@ static int batch_build(wt2_batch* b)
             StageItems it;
             stage_items(s.ctx, stage, nullptr, nullptr, nullptr, nullptr, s.u, s.v, s.dt, it);
             if (it.has_curved) {
+                // uniform and general-connectivity contexts in separate groups,
+                // so each runs the element-kernel instantiation of its single run
                 char key[64];
-                std::snprintf(key, sizeof(key), "%d/%d/%d", s.ctx->N, it.cc.mixed, it.cc.in32);
+                std::snprintf(key, sizeof(key), "%d/%d/%d/%d", s.ctx->N, it.cc.mixed, it.cc.in32,
+                              s.ctx->cd.conn != nullptr ? 1 : 0);
                 auto& g = cgroups[key];
                 g.first.push_back(&s.ctx->cd);
                 g.second.push_back(it.cc);
