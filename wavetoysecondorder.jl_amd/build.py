@@ -14,7 +14,7 @@ OUT = os.path.join(HERE, "wavetoy_hip", "libwt2.so")
 PROBE_SRC = "wt2_mfma_ab.hip"
 PROBE_OUT = os.path.join(HERE, "wavetoy_hip", "wt2_mfma_ab")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
-SOURCES = ["wt2_flat.hip", "wt2_curved.hip", "wt2_fo.hip", "wt2_focurv.hip", "wt2_capi.hip"]
+SOURCES = ["wt2_flat.hip", "wt2_flat_lean.hip", "wt2_curved.hip", "wt2_fo.hip", "wt2_focurv.hip", "wt2_capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

@@ -21,8 +21,7 @@
 //      coalesced epilogue adds it to the partial sum of the u_t equation, or
 //      (last axis) finishes the RHS (u_t' = P + c, u' = u_t) and applies the
 //      RK4 stage update in the same pass.
-#include "wt2_internal.hpp"
-#include "wt2_ops.hpp"
+#include "wt2_pass.hpp"
 
 #include <algorithm>
 #include <cstdlib>
@@ -30,198 +29,6 @@
 #include <vector>
 
 namespace wt2 {
-
-// Per-axis scalars (kernel arguments).  The differentiation matrix itself is
-// not a kernel argument: it lives in a small device table that each workgroup
-// copies to LDS (as kernel arguments the matrices took more SGPRs than exist
-// and were spilled).
-template <int N, typename TC>
-struct AxisConst {
-    TC kh;                   // KO scale kos / h
-    TC koL, koR, tau, ih2, gL, gR;
-    TC winv0, winvN;         // 1 / w at the two face nodes
-    TC hv, hiR;              // Sommerfeld faces: h and h / R
-    int somL, somR;          // 1: that outer face is a Sommerfeld face
-};
-
-template <int N, typename TC>
-struct KArgs {
-    AxisConst<N, TC> k;
-    const TC* op;      // operator table (OpLayout<N>)
-    const void* Su; const void* Sv;
-    void* P;
-    void* So_u; void* So_v;
-    double* Yu; double* Yv;
-    double* Ou; double* Ov;
-    double dt;
-    // per-axis geometry, precomputed on the host (the kernel has no branch on
-    // the axis; see launch_t for the meaning of each field)
-    int Ma;            // elements along the line
-    int T0, slo, shi;  // tile -> (t % T0) * slo + (t / T0) * shi = first element
-    int estride;       // element-index stride along the line
-    int nbk;           // line blocks per element column
-    int CPE, CO, clen; // chunks per element, chunk offset per block, chunk length
-    int ES, CS;        // LDS strides: per element, per chunk within an element
-    int SR, LA, LB;    // segment: node stride, line -> (line/N)*LA + (line%N)*LB
-    int LT, lpw;       // lines per tile, lines per wavefront
-    int GS;            // node stride of a segment in memory (LDS-free kernel)
-    int periodic, epi;
-    int lap, ko, needv, keepv, fold;
-    int xcd;           // 1: each XCD takes a contiguous range of tiles
-};
-
-// Workgroups are dispatched round-robin over the 8 XCDs; with xcd set, XCD x
-// runs the contiguous tile range [x T/8, (x+1) T/8) instead of every 8th tile,
-// so the line blocks of one element column share that XCD's L2.
-__device__ __forceinline__ int pass_tile(int xcd)
-{
-    int wg = blockIdx.x;
-    const int nwg = gridDim.x, full = (nwg / 8) * 8;
-    if (xcd && wg < full) wg = (wg % 8) * (nwg / 8) + wg / 8;
-    return wg;
-}
-
-// Classical RK4 on the linear autonomous system y' = A y + b (static
-// background, constant boundary data) is the polynomial
-// y + dt f + dt^2/2 A f + dt^3/6 A^2 f + dt^4/24 A^3 f, f = A y + b, which the
-// stages evaluate in Horner form with one register besides y:
-//   S1 = y + dt/4 f(y), S2 = y + dt/3 f(S1), S3 = y + dt/2 f(S2),
-//   y <- y + dt f(S3).
-// The stage-4 coefficient is dt; the others come from here.
-__device__ __forceinline__ double rk_horner_c(int epi, double dt)
-{
-    return epi == EPI_RK1 ? 0.25 * dt : (epi == EPI_RK2 ? dt / 3.0 : (epi == EPI_RK3 ? 0.5 * dt : dt));
-}
-
-// Face values of the neighbouring elements on the same line: the low
-// neighbour's last value and the high neighbour's first value.
-// ShflX: the line's elements sit in lanes of this wavefront (shuffles).
-template <typename TC>
-struct ShflX {
-    int srcL, srcR;
-    __device__ __forceinline__ void operator()(TC last, TC first, TC& fromL, TC& fromR)
-    {
-        fromL = shfl64(last, srcL);
-        fromR = shfl64(first, srcR);
-    }
-};
-
-// LdsX: the line's elements are spread over the workgroup; values go through
-// LDS, one barrier per exchange (two alternating buffers, so a buffer is only
-// rewritten after every work-item has passed the barrier that follows its
-// reads).  Every work-item of the workgroup must make the same calls.
-template <typename TC>
-struct LdsX {
-    TC* buf;           // 2 buffers x (last, first) x n
-    int n, me, idxL, idxR, k;
-    __device__ __forceinline__ void operator()(TC last, TC first, TC& fromL, TC& fromR)
-    {
-        TC* b = buf + (k & 1) * 2 * n;
-        b[me] = last;
-        b[n + me] = first;
-        __syncthreads();
-        fromL = b[idxL];
-        fromR = b[n + idxR];
-        ++k;
-    }
-};
-
-// x <- (kos D)^6 x on this lane's element segment; D = (1/h) G plus the
-// centred-flux SAT (apply_D!, METHODS.md:256-264); an outer face takes the same
-// SAT against a zero ghost value, which keeps H D skew on a bounded line.
-template <int N, typename TC, typename X>
-__device__ __forceinline__ void ko_chain(TC (&x)[N], const TC* __restrict__ op,
-                                         const AxisConst<N, TC>& k, X& xch,
-                                         bool hasL, bool hasR)
-{
-#pragma unroll 1
-    for (int rep = 0; rep < 6; ++rep) {
-        TC xl, xr;
-        xch(x[N - 1], x[0], xl, xr);
-        if (!hasL) xl = TC(0);
-        if (!hasR) xr = TC(0);
-        TC y[N];
-        apply_G<N, TC>(op, x, y);
-        const TC x0 = x[0], xN = x[N - 1];
-#pragma unroll
-        for (int i = 0; i < N; ++i) x[i] = k.kh * y[i];
-        x[0] = fma(k.koL, xl - x0, x[0]);
-        x[N - 1] = fma(k.koR, xr - xN, x[N - 1]);
-    }
-}
-
-// The two end values y_0, y_{N-1} of y = G x, with the same arithmetic as
-// apply_G (a neighbour's face derivative).
-template <int N, typename TC>
-__device__ __forceinline__ void apply_G_ends(const TC* __restrict__ op, const TC (&x)[N], TC& y0, TC& yN)
-{
-    using L = OpLayout<N>;
-    constexpr int H = L::H, E = L::E;
-    TC e[E], o[H];
-#pragma unroll
-    for (int p = 0; p < H; ++p) {
-        e[p] = x[p] + x[N - 1 - p];
-        o[p] = x[p] - x[N - 1 - p];
-    }
-    if (E > H) e[E - 1] = x[H];
-    TC b = TC(0), av = TC(0);
-#pragma unroll
-    for (int p = 0; p < H; ++p) b = fma(op[L::AO + p], o[p], b);
-#pragma unroll
-    for (int p = 0; p < E; ++p) av = fma(op[L::AE + p], e[p], av);
-    y0 = b + av;
-    yN = b - av;
-}
-
-// out += 1D SIPG Laplacian of u along this axis (README.md:57-81):
-//   (1/h^2){ G G u + H^-1 [ alpha G^T B du - 1/2 B d(Gu) - tau |B| du ] },
-// given g = G u and the neighbours' face values (uL, gnL: value and G u at the
-// last node of the low neighbour; uR, gnR: at the first node of the high one).
-// An outer Sommerfeld face (src/wave.jl:444-518) has no SIPG terms and instead
-// the drag -(u_t + d_n u + u/R) / (h w) on its node, d_n u = -+(1/h) G u at the
-// low / high face; v0, vN: u_t at the segment's two end nodes.  The drag rides
-// in the dg slot of the face term, (1/h^2)(1/w)(+-1/2 dg - tau du) with du = 0
-// and dg = 2 (+-G u - h u_t - (h/R) u) (the factor 2 and 1/2 are exact).
-template <int N, typename TC>
-__device__ __forceinline__ void sipg_seg(const TC (&u)[N], const TC (&g)[N], TC (&out)[N],
-                                         const TC* __restrict__ op, const AxisConst<N, TC>& k,
-                                         TC uL, TC gnL, TC uR, TC gnR, bool hasL, bool hasR,
-                                         TC v0, TC vN)
-{
-    using L = OpLayout<N>;
-    TC duL, dgL, aL, duR, dgR, aR;
-    if (hasL)        { duL = u[0] - uL; dgL = g[0] - gnL; aL = TC(0.5); }
-    else if (k.somL) { duL = TC(0); dgL = TC(2) * (g[0] - fma(k.hv, v0, k.hiR * u[0])); aL = TC(0); }
-    else             { duL = u[0] - k.gL; dgL = TC(0); aL = TC(1); }
-    if (hasR)        { duR = u[N - 1] - uR; dgR = g[N - 1] - gnR; aR = TC(0.5); }
-    else if (k.somR) { duR = TC(0); dgR = TC(2) * (g[N - 1] + fma(k.hv, vN, k.hiR * u[N - 1])); aR = TC(0); }
-    else             { duR = u[N - 1] - k.gR; dgR = TC(0); aR = TC(1); }
-    const TC tL = -aL * duL, tR = aR * duR;
-    TC gg[N];
-    apply_G<N, TC>(op, g, gg);
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        TC s = fma(op[L::C0 + i], tL, gg[i]);
-        s = fma(op[L::CN + i], tR, s);
-        if (i == 0) s = fma(k.winv0, TC(0.5) * dgL - k.tau * duL, s);
-        if (i == N - 1) s = fma(k.winvN, TC(-0.5) * dgR - k.tau * duR, s);
-        out[i] = fma(s, k.ih2, out[i]);
-    }
-}
-
-// The same with the neighbour values exchanged along the line.
-template <int N, typename TC, typename X>
-__device__ __forceinline__ void sipg_add(const TC (&u)[N], TC (&out)[N], const TC* __restrict__ op,
-                                         const AxisConst<N, TC>& k, X& xch,
-                                         bool hasL, bool hasR, TC v0, TC vN)
-{
-    TC g[N];
-    apply_G<N, TC>(op, u, g);
-    TC uL, gnL, uR, gnR;
-    xch(u[N - 1], u[0], uL, uR);
-    xch(g[N - 1], g[0], gnL, gnR);
-    sipg_seg<N, TC>(u, g, out, op, k, uL, gnL, uR, gnR, hasL, hasR, v0, vN);
-}
 
 template <int N, typename TC, typename TS, typename TP>
 __device__ __forceinline__ void flat_pass_body(const KArgs<N, TC>& a, const int tile)
@@ -252,8 +59,7 @@ __device__ __forceinline__ void flat_pass_body(const KArgs<N, TC>& a, const int 
 
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
-    __shared__ TC op[OpLayout<N>::SIZE];
-    for (int q = tid; q < OpLayout<N>::SIZE; q += nthr) op[q] = a.op[q];
+    const TC* __restrict__ op = a.op;   // wave-uniform reads (SGPRs)
     // copy-loop mapping: (chunk, offset) pairs; chunks of clen contiguous values
     int o, c0, cstep, ostep;
     if (clen <= nthr) { o = tid % clen; c0 = tid / clen; cstep = nthr / clen; ostep = clen; }
@@ -354,70 +160,6 @@ __device__ __forceinline__ void flat_pass_body(const KArgs<N, TC>& a, const int 
             static_cast<TP*>(a.So_v)[g] = TP(a.Yv[g] + cs * kv);
         }
     }
-}
-
-// Segment load / store: N values at stride gs (gs == 1: widest aligned vector).
-template <int N, typename TS, typename TC>
-__device__ __forceinline__ void load_seg_s(const TS* __restrict__ p, int gs, TC (&x)[N]);
-template <int N, typename TD, typename TC>
-__device__ __forceinline__ void store_seg_s(TD* __restrict__ p, int gs, const TC (&x)[N]);
-
-template <int N, typename TS, typename TC>
-__device__ __forceinline__ void load_seg(const TS* __restrict__ p, TC (&x)[N])
-{
-    if constexpr (std::is_same<TS, double>::value && N % 2 == 0) {
-        const double2* q = reinterpret_cast<const double2*>(p);
-#pragma unroll
-        for (int r = 0; r < N / 2; ++r) {
-            const double2 t = q[r];
-            x[2 * r] = TC(t.x);
-            x[2 * r + 1] = TC(t.y);
-        }
-    } else if constexpr (std::is_same<TS, float>::value && N % 4 == 0) {
-        const float4* q = reinterpret_cast<const float4*>(p);
-#pragma unroll
-        for (int r = 0; r < N / 4; ++r) {
-            const float4 t = q[r];
-            x[4 * r] = TC(t.x); x[4 * r + 1] = TC(t.y); x[4 * r + 2] = TC(t.z); x[4 * r + 3] = TC(t.w);
-        }
-    } else {
-#pragma unroll
-        for (int r = 0; r < N; ++r) x[r] = TC(p[r]);
-    }
-}
-
-template <int N, typename TD, typename TC>
-__device__ __forceinline__ void store_seg(TD* __restrict__ p, const TC (&x)[N])
-{
-    if constexpr (std::is_same<TD, double>::value && N % 2 == 0) {
-        double2* q = reinterpret_cast<double2*>(p);
-#pragma unroll
-        for (int r = 0; r < N / 2; ++r) q[r] = make_double2(double(x[2 * r]), double(x[2 * r + 1]));
-    } else if constexpr (std::is_same<TD, float>::value && N % 4 == 0) {
-        float4* q = reinterpret_cast<float4*>(p);
-#pragma unroll
-        for (int r = 0; r < N / 4; ++r)
-            q[r] = make_float4(float(x[4 * r]), float(x[4 * r + 1]), float(x[4 * r + 2]), float(x[4 * r + 3]));
-    } else {
-#pragma unroll
-        for (int r = 0; r < N; ++r) p[r] = TD(x[r]);
-    }
-}
-
-template <int N, typename TS, typename TC>
-__device__ __forceinline__ void load_seg_s(const TS* __restrict__ p, int gs, TC (&x)[N])
-{
-    if (gs == 1) { load_seg<N, TS, TC>(p, x); return; }
-#pragma unroll
-    for (int r = 0; r < N; ++r) x[r] = TC(p[r * gs]);
-}
-
-template <int N, typename TD, typename TC>
-__device__ __forceinline__ void store_seg_s(TD* __restrict__ p, int gs, const TC (&x)[N])
-{
-    if (gs == 1) { store_seg<N, TD, TC>(p, x); return; }
-#pragma unroll
-    for (int r = 0; r < N; ++r) p[r * gs] = TD(x[r]);
 }
 
 // The x pass: along x a lane's element segment is N contiguous values in
@@ -767,6 +509,9 @@ static hipError_t launch_t(const PassDesc& p, hipStream_t s)
     const size_t static_lds = static_lds_bound<N, TC>();
     if (p.direct) {
         // LDS-free pass (neighbour values by shuffles, or through LDS)
+        // partial-sum and output epilogues: the form with the operator table
+        // in SGPRs (wt2_flat_lean.hip)
+        if (p.epi < EPI_RK1) return launch_xpass_lean<N, TC, TS, TP>(a, tiles, nthr, p.direct == 2, s);
         (void)hipGetLastError();
         if (p.direct == 2)
             hipLaunchKernelGGL((flat_xpass_kernel<N, TC, TS, TP, true>), dim3(tiles), dim3(nthr), 0, s, a);
@@ -970,9 +715,12 @@ __global__ void __launch_bounds__(256) flat_lap1_kernel(Lap1Args<N, TC> a)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     TC* U = reinterpret_cast<TC*>(smem_raw);     // EW x ES
     TC* R = U + S::EW * S::ES;                   // EW x ES
-    __shared__ TC op[OpLayout<N>::SIZE];
+    // the operator table is read with wave-uniform addresses from its device
+    // table (scalar loads into SGPRs): an LDS copy's values were held in VGPRs
+    // (N = 8 FP64: 152 VGPRs, 3 wavefronts per SIMD, against 86 and 5 this way;
+    // 106 -> 90 us at 32^3 x 8^3)
+    const TC* __restrict__ op = a.op;
     const int tid = threadIdx.x;
-    for (int q = tid; q < OpLayout<N>::SIZE; q += blockDim.x) op[q] = a.op[q];
 
     // workgroups of one XCD (dispatch round-robin over 8) take a contiguous
     // range of element blocks, so that face neighbours share that XCD's L2
@@ -1139,6 +887,7 @@ static hipError_t batch_build_t(const PassDesc* d, int n, FlatBatchPlan& plan)
     plan.N = N;
     plan.variant = d[0].variant;
     plan.direct = d[0].direct;
+    plan.lean = d[0].epi < EPI_RK1;
     plan.ntiles = int(work.size());
     plan.nthr = nthr;
     plan.shmem = plan.direct ? 0 : shmem;
@@ -1151,6 +900,8 @@ static hipError_t batch_launch_t(const FlatBatchPlan& plan, hipStream_t s)
     if (plan.ntiles == 0) return hipSuccess;
     const KArgs<N, TC>* tab = static_cast<const KArgs<N, TC>*>(plan.tab);
     if (plan.direct) {
+        if (plan.lean)
+            return launch_xpass_lean_batch<N, TC, TS, TP>(tab, plan.work, plan.ntiles, plan.nthr, plan.direct == 2, s);
         (void)hipGetLastError();
         if (plan.direct == 2)
             hipLaunchKernelGGL((flat_xpass_batch_kernel<N, TC, TS, TP, true>), dim3(plan.ntiles), dim3(plan.nthr),

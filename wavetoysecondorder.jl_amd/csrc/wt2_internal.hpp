@@ -125,7 +125,7 @@ hipError_t launch_fo_rk_update(double* Y0, double* Y1, double* R, int64_t stride
 // precision variant, axis, kernel kind and epilogue), driven by a device table
 // of per-instance descriptors and a packed (instance, tile) work list.
 struct FlatBatchPlan {
-    int N = 0, variant = 0, direct = 0, ntiles = 0, nthr = 0;
+    int N = 0, variant = 0, direct = 0, lean = 0, ntiles = 0, nthr = 0;
     size_t shmem = 0;
     void* tab = nullptr;     // device descriptors
     int2* work = nullptr;    // device work list
