@@ -213,6 +213,8 @@ struct wt2_ctx {
     int curved;
     bool mesh_som;  // general mesh with Sommerfeld-tagged outer faces
     int fold;       // RK4 schedule with the first pass of each stage folded into the previous stage
+    int fmid;       // folded schedule: the axis of each stage's middle pass
+    int fe[4];      // folded schedule: the axis of stage s's last (RK + fold) pass
     int nb[3];
     int direct[3];  // per axis: LDS-free pass kernel
     int device;
@@ -395,16 +397,36 @@ struct StageItems {
     PassDesc pass[3];
 };
 
-// The folded RK4 schedule (flat contexts whose x and y passes are LDS-free
-// kernels): stage s runs z (adding to P, which already holds the folded
-// contribution of axis f_s) and then e_s with the RK update and the fold:
-// e_s's contribution of stage s+1, computed on the new stage input in
-// registers.  e = (y, x, y, x), f_s = e_(s-1), f_1 = x (the previous step's
-// stage 4, or the prologue at the start of a run).  8 passes per step
-// instead of 12.
+// The folded RK4 schedule (flat contexts whose e-axis passes are LDS-free
+// kernels): stage s runs the middle axis m (adding to P, which already holds
+// the folded contribution of axis f_s) and then e_s with the RK update and
+// the fold: e_s's contribution of stage s+1, computed on the new stage input
+// in registers.  f_s = e_(s-1), f_1 = e_4 (the previous step's stage 4, or
+// the prologue at the start of a run).  8 passes per step instead of 12.
+// The default m = x, e = (y, z, y, z): at 32^3 x 8^3 the x pass is the
+// cheapest middle pass (106 us against 125 for z, FP64) and the folding
+// passes cost about the same along z as along x, so x never carries the
+// fold: 1.306 -> 1.276 ms per FP64 step, 0.900 -> 0.833 ms mixed.
+// WT2_FOLD_ORDER = "zyx" selects m = z, e = (y, x, y, x).
+static void fold_order(wt2_ctx* c)
+{
+    static const int order = [] {
+        const char* ev = std::getenv("WT2_FOLD_ORDER");
+        return ev && std::strcmp(ev, "zyx") == 0 ? 1 : 0;
+    }();
+    static const int tab[2][5] = {{0, 1, 2, 1, 2}, {2, 1, 0, 1, 0}};
+    c->fmid = tab[order][0];
+    for (int s = 0; s < 4; ++s) c->fe[s] = tab[order][1 + s];
+}
+
 static bool fold_on(const wt2_ctx* c)
 {
-    return c->fold && !c->curved && c->direct[0] != 0 && c->direct[1] != 0;
+    return c->fold && !c->curved && c->direct[c->fe[0]] != 0 && c->direct[c->fe[1]] != 0;
+}
+
+static bool fold_axis(const wt2_ctx* c, int a)
+{
+    return a == c->fe[0] || a == c->fe[1];
 }
 
 static void stage_items(wt2_ctx* c, int stage, const double* u, const double* v, double* Ou, double* Ov,
@@ -418,14 +440,14 @@ static void stage_items(wt2_ctx* c, int stage, const double* u, const double* v,
         const void* Sv = stage == 1 ? static_cast<const void*>(Yv) : fld(c, c->Sb, 1);
         const int variant = !mixed ? VAR_F64 : (stage == 1 ? VAR_MIX_IN64 : VAR_MIX_IN32);
         it = StageItems();
-        PassDesc z = base_pass(c, 2, bv);
+        PassDesc z = base_pass(c, c->fmid, bv);
         z.variant = variant;
-        z.lap = 1; z.ko = ko; z.needv = ko || som_axis(c, 2); z.keepv = 0;
+        z.lap = 1; z.ko = ko; z.needv = ko || som_axis(c, c->fmid); z.keepv = 0;
         z.Su = Su; z.Sv = Sv; z.P = c->Pb;
         z.epi = EPI_MID;
         it.pass[0] = z;
         it.has_pass[0] = true;
-        PassDesc e = base_pass(c, (stage & 1) ? 1 : 0, bv);
+        PassDesc e = base_pass(c, c->fe[stage - 1], bv);
         e.variant = variant;
         e.lap = 1; e.ko = ko; e.needv = 1; e.keepv = 1; e.fold = 1;
         e.Su = Su; e.Sv = Sv; e.P = c->Pb;
@@ -612,7 +634,8 @@ TuneKey tune_key(const wt2_ctx* c)
     k.v[0] = c->N; k.v[1] = c->d.M[0]; k.v[2] = c->d.M[1]; k.v[3] = c->d.M[2];
     k.v[4] = c->d.precision; k.v[5] = c->curved; k.v[6] = c->d.eps_ko > 0;
     k.v[7] = c->d.periodic[0] + 2 * c->d.periodic[1] + 4 * c->d.periodic[2];
-    k.v[8] = int(som_axis(c, 0)) + 2 * int(som_axis(c, 1)) + 4 * int(som_axis(c, 2)) + 8 * (c->fold != 0);
+    k.v[8] = int(som_axis(c, 0)) + 2 * int(som_axis(c, 1)) + 4 * int(som_axis(c, 2)) + 8 * (c->fold != 0) +
+             16 * c->fmid;
     return k;
 }
 }  // namespace
@@ -986,6 +1009,7 @@ static int ctx_create_impl(const wt2_mesh_desc* desc, const MeshIn* mi, wt2_ctx*
         // with the XCD-ordered z passes); WT2_FOLD=0 selects the 12-pass one
         const char* ev = std::getenv("WT2_FOLD");
         c->fold = ev ? std::atoi(ev) != 0 : 1;
+        fold_order(c);
     }
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1174,14 +1198,15 @@ int wt2_rhs_full(wt2_ctx* c, double* du, double* dud, const double* u, const dou
     return rc;
 }
 
-// Start of a folded run: P = the x contribution of stage 1 on Y.
+// Start of a folded run: P = the stage-1 contribution of axis e_4 on Y.
 static int run_prologue(wt2_ctx* c, const double* Yu, const double* Yv)
 {
     if (!fold_on(c)) return WT2_OK;
-    PassDesc p = base_pass(c, 0, c->d.bdry_values);
+    const int ax = c->fe[3];
+    PassDesc p = base_pass(c, ax, c->d.bdry_values);
     p.variant = is_mixed(c) ? VAR_MIX_IN64 : VAR_F64;
     const bool ko = c->d.eps_ko > 0 && c->kos > 0;
-    p.lap = 1; p.ko = ko; p.needv = ko || som_axis(c, 0);
+    p.lap = 1; p.ko = ko; p.needv = ko || som_axis(c, ax);
     p.Su = Yu; p.Sv = Yv; p.P = c->Pb;
     p.epi = EPI_FIRST;
     return pass_rc(launch_flat_pass(p, c->stream), "flat pass (prologue)");
@@ -1382,9 +1407,15 @@ extern "C" {
 
 // Autotuner (no counterpart in the reference): picks, per axis, the pass kernel
 // (LDS-free or LDS-staged) and the element planes per tile by timing every
-// valid candidate of that axis's stage-2 pass on the context's stream with the
-// given state (device pointers; the state is only read).  Results are cached
-// per (N, M, precision, curved, KO, periodicity) for later contexts.
+// valid candidate on the context's stream with the given state (device
+// pointers; the state is only read), the other axes at their best so far
+// (z, then y, then x; twice for a folded context).  A folded context times stages 1-3 of the step with the
+// candidate (six passes: every pass kernel the step runs, stage 4 being stage
+// 2's with Y written); the passes timed in their step order, not one pass
+// launched back to back, which favoured tiles that lost in the step (32^3 x
+// 8^3 FP64: 1.287 against 1.266 ms).  Otherwise that axis's stage-2 pass is
+// timed alone.  Results are cached per (N, M, precision, curved, KO,
+// periodicity, schedule) for later contexts.
 int wt2_ctx_autotune(wt2_ctx* c, double* u, double* ud, double dt, int32_t flags)
 {
     if (!c || !u || !ud || !(dt > 0)) { set_error("wt2_ctx_autotune: bad arguments"); return WT2_ERR_ARG; }
@@ -1411,28 +1442,41 @@ int wt2_ctx_autotune(wt2_ctx* c, double* u, double* ud, double dt, int32_t flags
     HIPCHK(hipEventCreate(&e1));
     TuneVal best;
     const int N = c->N;
-    for (int a = 2; a >= 0; --a) {
+    // folded contexts: two sweeps, since an axis's best tile depends on the
+    // tiles of the passes around it in the step
+    const int sweeps = fold_on(c) ? 2 : 1;
+    for (int it = 0; it < 3 * sweeps; ++it) {
+        const int a = 2 - it % 3;
         float tbest = 1e30f;
         int bnb = c->nb[a], bdir = c->direct[a];
         const int lpw = 64 / c->d.M[a];
         for (int dir = 2; dir >= 0; --dir) {
-            // the folded schedule needs LDS-free x and y passes: an LDS-staged
-            // candidate there would switch it off (8 -> 12 launches per step)
-            if (dir == 0 && a <= 1 && c->fold && !c->curved) continue;
+            // the folded schedule needs LDS-free passes along its e axes: an
+            // LDS-staged candidate there would switch it off (8 -> 12 launches
+            // per step)
+            if (dir == 0 && fold_axis(c, a) && c->fold && !c->curved) continue;
             for (int nb = 1; nb <= N; ++nb) {
                 if (!tiles_ok(c, a, nb, dir)) continue;
                 c->direct[a] = dir;
                 c->nb[a] = nb;
-                rc = run_stage(c, 2, nullptr, nullptr, nullptr, nullptr, u, ud, dt, a, false);
-                if (rc) { set_error(""); continue; }     // configuration not launchable: skip it
-                // best of three trials of 6 launches each (single trials vary
-                // by several percent from launch to launch)
-                const int reps = 6;
+                const bool seq = fold_on(c);
+                auto once = [&]() {
+                    if (!seq) return run_stage(c, 2, nullptr, nullptr, nullptr, nullptr, u, ud, dt, a, false);
+                    for (int st = 1; st <= 3; ++st) {
+                        const int r = run_stage(c, st, nullptr, nullptr, nullptr, nullptr, u, ud, dt);
+                        if (r) return r;
+                    }
+                    return int(WT2_OK);
+                };
+                rc = once();
+                if (rc) { set_error(""); rc = 0; continue; }     // configuration not launchable: skip it
+                // best of three trials (single trials vary by several percent
+                // from launch to launch)
+                const int reps = seq ? 2 : 6;
                 float ms = 1e30f;
                 for (int trial = 0; trial < 3 && !rc; ++trial) {
                     (void)hipEventRecord(e0, c->stream);
-                    for (int r = 0; r < reps && !rc; ++r)
-                        rc = run_stage(c, 2, nullptr, nullptr, nullptr, nullptr, u, ud, dt, a, false);
+                    for (int r = 0; r < reps && !rc; ++r) rc = once();
                     (void)hipEventRecord(e1, c->stream);
                     (void)hipEventSynchronize(e1);
                     float t = 0.f;
