@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -556,6 +557,117 @@ struct VolTables {
     TC w[N], winv[N];
 };
 
+// Face terms of face node q (face f = q / N^2) of element e: S^b = alpha n K^ab du
+// into Sf and the point term into Pt (both in LDS, per element); A, B: the
+// element's u, u_t in LDS; ec: the element's grid coordinates (uniform meshes).
+template <int N, typename TC, typename TS, bool GEN>
+__device__ __forceinline__ void curved_face_node(const CArgs<N, TC>& a, const int64_t e, const int (&ec)[3],
+                                                 const int q, const TC* A, const TC* B, TC* Sf, TC* Pt,
+                                                 const TC* __restrict__ winv)
+{
+    constexpr int N2 = N * N, N3 = N * N * N, F2 = 6 * N * N;
+    const TS* u = static_cast<const TS*>(a.u);
+    const TS* v = static_cast<const TS*>(a.v);
+    const bool adv = a.adv != 0;
+    const int64_t np = a.npts;
+    const int64_t eb = e * N3;
+    const int f = q / N2, pq = q - f * N2;
+    const int p = pq % N, qq = pq / N;
+    const int ax = f >> 1, side = f & 1;
+    const int r = side ? N - 1 : 0;
+    const TC nrm = side ? TC(1) : TC(-1);
+    const int n = fidx<N>(ax, r, p, qq);
+    const int64_t gn = eb + n;
+    const TC* fc = a.fcoef + e * 5 * F2 + q;   // K^{a0}, K^{a1}, K^{a2}, kappa, c^a
+    // neighbour element en (-1: outer face), its face fn, the neighbour
+    // face's node (pn, qn) that meets this face's (p, qq), and whether
+    // this outer face is a Sommerfeld face; bvf: its Dirichlet value
+    int64_t en;
+    int fn, pn = p, qn = qq;
+    bool som, exc = false;
+    TC bvf;
+    if (GEN && a.conn) {
+        const int2 cf = a.conn[e * 6 + f];
+        en = cf.x;
+        fn = cf.y & 7;
+        const int ori = (cf.y >> 3) & 7, tag = cf.y >> 6;
+        if (ori & 1) { pn = qq; qn = p; }
+        if (ori & 2) pn = N - 1 - pn;
+        if (ori & 4) qn = N - 1 - qn;
+        som = tag == 7;
+        exc = tag == 8;
+        bvf = (tag >= 1 && tag <= 6) ? a.k.bv[tag - 1] : TC(0);
+    } else {
+        en = nbr_elem(ec, ax, side, a.M0, a.M1, a.M2, a.per0, a.per1, a.per2);
+        fn = (ax << 1) | (1 - side);
+        som = (a.k.som >> f) & 1;
+        bvf = a.k.bv[f];
+    }
+    if (en < 0 && exc) {
+        // excision face (src/wave_strong_rhs.jl:188-213): no SAT at all
+#pragma unroll
+        for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = TC(0);
+        Pt[q] = TC(0);
+        return;
+    }
+    if (en < 0 && som) {
+        // Sommerfeld face (src/wave.jl:444-518): no SIPG terms; the drag
+        // -(wF/Hphys)(u_t + d_n u + u/R), here unweighted (the output
+        // is multiplied by wgt = 1/|J|): |n_u| d_n u = n F^a with this
+        // element's own normal flux, |n_u| = sqrt(K^aa / wgt), and
+        // wF/Hphys = |n_u| / (w_r |J|).  B holds u_t (zero without it).
+#pragma unroll
+        for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = TC(0);
+        const TC nu = sqrt(fc[ax * F2] / a.coef[9 * np + gn]);
+        const TC drag = fma(nrm, a.fbuf[e * F2 + q], nu * fma(a.k.iR, A[n], B[n]));
+        Pt[q] = -drag * winv[r];
+        return;
+    }
+    TC du, dF = TC(0), dv = TC(0), alpha;
+    const TC kap = fc[3 * F2];
+    if (en >= 0) {
+        // outward normal fluxes of the two sides: nrm F (own) and
+        // nrm_n F_n (neighbour); dF = F + (nrm nrm_n) F_n
+        const int axn = fn >> 1;
+        const int64_t gnn = en * N3 + fidx<N>(axn, (fn & 1) ? N - 1 : 0, pn, qn);
+        du = A[n] - TC(u[gnn]);
+        const TC Fn = a.fbuf[en * F2 + fn * N2 + pn + N * qn];
+        const TC Fo = a.fbuf[e * F2 + q];
+        dF = ((fn & 1) == side) ? Fo + Fn : Fo - Fn;
+        if (adv) dv = B[n] - TC(v[gnn]);
+        alpha = TC(0.5);
+    } else {
+        du = A[n] - bvf;
+        if (adv) dv = B[n];
+        alpha = TC(1);
+    }
+    const TC an = alpha * nrm;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = an * fc[b * F2] * du;
+    TC pt = TC(-0.5) * nrm * dF - a.k.tau * kap * du;
+    if (adv) pt -= nrm * fc[4 * F2] * dv;
+    Pt[q] = pt * winv[r];
+}
+
+// Tangential lifts of S at face node q of an element: added to its point term.
+template <int N, typename TC>
+__device__ __forceinline__ void curved_face_lift(const int q, const TC* Sf, TC* Pt, const TC* __restrict__ GW,
+                                                 const TC* __restrict__ winv)
+{
+    constexpr int N2 = N * N;
+    const int f = q / N2, pq = q - f * N2;
+    const int tp = pq % N, tq = pq / N;
+    const int ax = f >> 1, r = (f & 1) ? N - 1 : 0;
+    const int tb0 = ax == 0 ? 1 : 0, tb1 = ax == 2 ? 1 : 2;
+    TC t0 = TC(0), t1 = TC(0);
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+        t0 = fma(GW[l * N + tp], Sf[(tb0 * 6 + f) * N2 + l + N * tq], t0);
+        t1 = fma(GW[l * N + tq], Sf[(tb1 * 6 + f) * N2 + tp + N * l], t1);
+    }
+    Pt[q] = fma(winv[r], fma(winv[tp], t0, winv[tq] * t1), Pt[q]);
+}
+
 // GEN: the mesh may carry a connectivity table (a.conn); false compiles the
 // uniform mesh's neighbour arithmetic alone
 template <int N, typename TC, typename TS, typename TO, bool GEN>
@@ -609,84 +721,8 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
         // ---- P2 (face-parallel): face-node terms of this element, first, while
         // the fluxes of the z-line are not yet live in registers (more of the
         // face loop's gathers in flight)
-        for (int q = lane; q < S::F2; q += S::L) {
-            const int f = q / N2, pq = q - f * N2;
-            const int p = pq % N, qq = pq / N;
-            const int ax = f >> 1, side = f & 1;
-            const int r = side ? N - 1 : 0;
-            const TC nrm = side ? TC(1) : TC(-1);
-            const int n = fidx<N>(ax, r, p, qq);
-            const int64_t gn = eb + n;
-            const TC* fc = a.fcoef + e * 5 * S::F2 + q;   // K^{a0}, K^{a1}, K^{a2}, kappa, c^a
-            // neighbour element en (-1: outer face), its face fn, the neighbour
-            // face's node (pn, qn) that meets this face's (p, qq), and whether
-            // this outer face is a Sommerfeld face; bvf: its Dirichlet value
-            int64_t en;
-            int fn, pn = p, qn = qq;
-            bool som, exc = false;
-            TC bvf;
-            if (GEN && a.conn) {
-                const int2 cf = a.conn[e * 6 + f];
-                en = cf.x;
-                fn = cf.y & 7;
-                const int ori = (cf.y >> 3) & 7, tag = cf.y >> 6;
-                if (ori & 1) { pn = qq; qn = p; }
-                if (ori & 2) pn = N - 1 - pn;
-                if (ori & 4) qn = N - 1 - qn;
-                som = tag == 7;
-                exc = tag == 8;
-                bvf = (tag >= 1 && tag <= 6) ? a.k.bv[tag - 1] : TC(0);
-            } else {
-                en = nbr_elem(ec, ax, side, a.M0, a.M1, a.M2, a.per0, a.per1, a.per2);
-                fn = (ax << 1) | (1 - side);
-                som = (a.k.som >> f) & 1;
-                bvf = a.k.bv[f];
-            }
-            if (en < 0 && exc) {
-                // excision face (src/wave_strong_rhs.jl:188-213): no SAT at all
-#pragma unroll
-                for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = TC(0);
-                Pt[q] = TC(0);
-                continue;
-            }
-            if (en < 0 && som) {
-                // Sommerfeld face (src/wave.jl:444-518): no SIPG terms; the drag
-                // -(wF/Hphys)(u_t + d_n u + u/R), here unweighted (the output
-                // is multiplied by wgt = 1/|J|): |n_u| d_n u = n F^a with this
-                // element's own normal flux, |n_u| = sqrt(K^aa / wgt), and
-                // wF/Hphys = |n_u| / (w_r |J|).  B holds u_t (zero without it).
-#pragma unroll
-                for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = TC(0);
-                const TC nu = sqrt(fc[ax * S::F2] / a.coef[9 * np + gn]);
-                const TC drag = fma(nrm, a.fbuf[e * S::F2 + q], nu * fma(a.k.iR, A[n], B[n]));
-                Pt[q] = -drag * T.winv[r];
-                continue;
-            }
-            TC du, dF = TC(0), dv = TC(0), alpha;
-            const TC kap = fc[3 * S::F2];
-            if (en >= 0) {
-                // outward normal fluxes of the two sides: nrm F (own) and
-                // nrm_n F_n (neighbour); dF = F + (nrm nrm_n) F_n
-                const int axn = fn >> 1;
-                const int64_t gnn = en * N3 + fidx<N>(axn, (fn & 1) ? N - 1 : 0, pn, qn);
-                du = A[n] - TC(u[gnn]);
-                const TC Fn = a.fbuf[en * S::F2 + fn * N2 + pn + N * qn];
-                const TC Fo = a.fbuf[e * S::F2 + q];
-                dF = ((fn & 1) == side) ? Fo + Fn : Fo - Fn;
-                if (adv) dv = B[n] - TC(v[gnn]);
-                alpha = TC(0.5);
-            } else {
-                du = A[n] - bvf;
-                if (adv) dv = B[n];
-                alpha = TC(1);
-            }
-            const TC an = alpha * nrm;
-#pragma unroll
-            for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = an * fc[b * S::F2] * du;
-            TC pt = TC(-0.5) * nrm * dF - a.k.tau * kap * du;
-            if (adv) pt -= nrm * fc[4 * S::F2] * dv;
-            Pt[q] = pt * T.winv[r];
-        }
+        for (int q = lane; q < S::F2; q += S::L)
+            curved_face_node<N, TC, TS, GEN>(a, e, ec, q, A, B, Sf, Pt, T.winv);
         // ---- P2: gradients and fluxes on this z-line
         TC Gi[N], Gj[N];
 #pragma unroll
@@ -744,19 +780,7 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
             A[i + N * j + N2 * k] = W0[k];
             B[i + N * j + N2 * k] = W1[k];
         }
-        for (int q = lane; q < S::F2; q += S::L) {
-            const int f = q / N2, pq = q - f * N2;
-            const int tp = pq % N, tq = pq / N;
-            const int ax = f >> 1, r = (f & 1) ? N - 1 : 0;
-            const int tb0 = ax == 0 ? 1 : 0, tb1 = ax == 2 ? 1 : 2;
-            TC t0 = TC(0), t1 = TC(0);
-#pragma unroll
-            for (int l = 0; l < N; ++l) {
-                t0 = fma(T.GW[l * N + tp], Sf[(tb0 * 6 + f) * N2 + l + N * tq], t0);
-                t1 = fma(T.GW[l * N + tq], Sf[(tb1 * 6 + f) * N2 + tp + N * l], t1);
-            }
-            Pt[q] = fma(T.winv[r], fma(T.winv[tp], t0, T.winv[tq] * t1), Pt[q]);
-        }
+        for (int q = lane; q < S::F2; q += S::L) curved_face_lift<N, TC>(q, Sf, Pt, T.GW, T.winv);
     }
     __syncthreads();
     if (!act) return;
@@ -800,6 +824,142 @@ __device__ __forceinline__ void volume_body(const CArgs<N, TC>& a, const int64_t
     }
 }
 
+// Element kernel, one work-item per element node (the form every launch
+// uses; volume_body above is the one-work-item-per-z-line form it replaced).
+// Each work-item issues all of its node's coefficient loads at the start, so
+// a workgroup keeps 10 N^3 loads in flight (the z-line form issued them node
+// by node, two round trips per node, at two wavefronts per SIMD), and holds
+// few registers (N = 10 FP64: one element of 1024 work-items).
+//   P1  u, u_t -> LDS (A, B); the node's K^ab, c^a (and wgt) -> registers;
+//   P2  face terms (face-parallel over the 6 N^2 face nodes, into Sf, Pt);
+//       the gradients of u and u_t at the node from LDS; the fluxes
+//       W^a = K^ab g_b + c^a u_t and c^a d_a u_t;
+//   P3  W^0, W^1, W^2 -> LDS (A, B, C); tangential lifts at the face nodes;
+//   P4  sum_a G_a W^a + c^a d_a u_t + normal lifts + face terms, times wgt.
+template <int N>
+struct VolNodeShape {
+    static constexpr int N2 = N * N, N3 = N * N * N, F2 = 6 * N * N;
+    static constexpr int EW = N3 >= 256 ? 1 : 256 / N3;      // elements per workgroup
+    static constexpr int NT = ((EW * N3 + 63) / 64) * 64;    // work-items per workgroup
+    static constexpr int PER = 3 * N3 + 24 * N2;             // LDS values per element
+};
+
+template <int N, typename TC, typename TS, typename TO, bool GEN>
+__device__ __forceinline__ void volume_node_body(const CArgs<N, TC>& a, const int64_t e0, const int nel)
+{
+    using S = VolNodeShape<N>;
+    constexpr int N3 = S::N3, N2 = S::N2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ VolTables<N, TC> T;
+    const int tid = threadIdx.x;
+    for (int q = tid; q < N2; q += blockDim.x) {
+        T.G[q] = a.k.G[q];
+        T.GW[q] = a.k.G[q] * a.k.w[q / N];
+    }
+    for (int q = tid; q < N; q += blockDim.x) { T.w[q] = a.k.w[q]; T.winv[q] = a.k.winv[q]; }
+
+    const TS* u = static_cast<const TS*>(a.u);
+    const TS* v = static_cast<const TS*>(a.v);
+    const bool adv = a.adv != 0;
+    const int64_t np = a.npts;
+    const int slot = tid / N3, n = tid - slot * N3;
+    const bool act = slot < nel;
+    const int sl = act ? slot : 0;
+    const int64_t e = e0 + sl;
+    const int64_t gn = e * N3 + n;
+    TC* lds = reinterpret_cast<TC*>(smem_raw);
+    TC* A = lds + sl * S::PER;          // u, then W^0
+    TC* B = A + N3;                     // u_t, then W^1
+    TC* C = B + N3;                     // W^2
+    TC* Sf = C + N3;                    // 3 x 6 N^2: S^b per face node
+    TC* Pt = Sf + 18 * N2;              // 6 N^2: point (+ tangential) term per face node
+    const int i = n % N, j = (n / N) % N, k = n / N2;
+
+    // P1
+    TC K00 = TC(0), K01 = TC(0), K02 = TC(0), K11 = TC(0), K12 = TC(0), K22 = TC(0);
+    TC c0 = TC(0), c1 = TC(0), c2 = TC(0), wg = TC(1), vn = TC(0);
+    if (act) {
+        K00 = a.coef[0 * np + gn]; K01 = a.coef[1 * np + gn]; K02 = a.coef[2 * np + gn];
+        K11 = a.coef[3 * np + gn]; K12 = a.coef[4 * np + gn]; K22 = a.coef[5 * np + gn];
+        if (adv) { c0 = a.coef[6 * np + gn]; c1 = a.coef[7 * np + gn]; c2 = a.coef[8 * np + gn]; }
+        if (a.weighted) wg = a.coef[9 * np + gn];
+        A[n] = TC(u[gn]);
+        vn = adv ? TC(v[gn]) : TC(0);
+        B[n] = vn;
+    }
+    int ec[3];
+    {
+        const int ei = int(e), exy = ei % (a.M0 * a.M1);
+        ec[0] = exy % a.M0; ec[1] = exy / a.M0; ec[2] = ei / (a.M0 * a.M1);
+    }
+    __syncthreads();
+
+    // P2
+    TC W0 = TC(0), W1 = TC(0), W2 = TC(0), cg = TC(0);
+    if (act) {
+        for (int q = n; q < S::F2; q += N3) curved_face_node<N, TC, TS, GEN>(a, e, ec, q, A, B, Sf, Pt, T.winv);
+        // gradients of u (g) and u_t (h) in one loop, not unrolled: each G
+        // value is used for both as soon as it is read (unrolled, or as two
+        // loops, the compiler kept the loop's 3N G and 6N field values live:
+        // N = 10 FP64 118 VGPRs, one workgroup per CU, against 64 and two)
+        TC g0 = TC(0), g1 = TC(0), g2 = TC(0), h0 = TC(0), h1 = TC(0), h2 = TC(0);
+#pragma unroll 1
+        for (int p = 0; p < N; ++p) {
+            const TC gi = T.G[i * N + p], gj = T.G[j * N + p], gk = T.G[k * N + p];
+            g0 = fma(gi, A[p + N * j + N2 * k], g0);
+            g1 = fma(gj, A[i + N * p + N2 * k], g1);
+            g2 = fma(gk, A[i + N * j + N2 * p], g2);
+            if (adv) {
+                h0 = fma(gi, B[p + N * j + N2 * k], h0);
+                h1 = fma(gj, B[i + N * p + N2 * k], h1);
+                h2 = fma(gk, B[i + N * j + N2 * p], h2);
+            }
+        }
+        W0 = K00 * g0; W1 = K01 * g0; W2 = K02 * g0;
+        W0 = fma(K01, g1, W0); W1 = fma(K11, g1, W1); W2 = fma(K12, g1, W2);
+        W0 = fma(K02, g2, W0); W1 = fma(K12, g2, W1); W2 = fma(K22, g2, W2);
+        if (adv) {
+            W0 = fma(c0, vn, W0); W1 = fma(c1, vn, W1); W2 = fma(c2, vn, W2);
+            cg = c0 * h0 + c1 * h1 + c2 * h2;
+        }
+    }
+    __syncthreads();
+    // P3
+    if (act) {
+        A[n] = W0; B[n] = W1; C[n] = W2;
+        for (int q = n; q < S::F2; q += N3) curved_face_lift<N, TC>(q, Sf, Pt, T.GW, T.winv);
+    }
+    __syncthreads();
+    if (!act) return;
+    // P4
+    TC d2 = TC(0);
+#pragma unroll
+    for (int p = 0; p < N; ++p) d2 = fma(T.G[k * N + p], C[i + N * j + N2 * p], d2);
+    TC s = cg + d2;
+#pragma unroll
+    for (int p = 0; p < N; ++p) {
+        s = fma(T.G[i * N + p], A[p + N * j + N2 * k], s);
+        s = fma(T.G[j * N + p], B[i + N * p + N2 * k], s);
+    }
+    // lifts along the normal lines: x faces (0, 1) at (j, k), y faces (2, 3)
+    // at (i, k), z faces (4, 5) at (i, j)
+    s = fma(T.G[i] * T.winv[i], Sf[(0 * 6 + 0) * N2 + j + N * k], s);
+    s = fma(T.G[(N - 1) * N + i] * T.winv[i], Sf[(0 * 6 + 1) * N2 + j + N * k], s);
+    s = fma(T.G[j] * T.winv[j], Sf[(1 * 6 + 2) * N2 + i + N * k], s);
+    s = fma(T.G[(N - 1) * N + j] * T.winv[j], Sf[(1 * 6 + 3) * N2 + i + N * k], s);
+    s = fma(T.G[k] * T.winv[k], Sf[(2 * 6 + 4) * N2 + i + N * j], s);
+    s = fma(T.G[(N - 1) * N + k] * T.winv[k], Sf[(2 * 6 + 5) * N2 + i + N * j], s);
+    // point and tangential terms at the face nodes
+    if (i == 0) s += Pt[0 * N2 + j + N * k];
+    if (i == N - 1) s += Pt[1 * N2 + j + N * k];
+    if (j == 0) s += Pt[2 * N2 + i + N * k];
+    if (j == N - 1) s += Pt[3 * N2 + i + N * k];
+    if (k == 0) s += Pt[4 * N2 + i + N * j];
+    if (k == N - 1) s += Pt[5 * N2 + i + N * j];
+    if (a.weighted) s *= wg;
+    static_cast<TO*>(a.out)[gn] = TO(s);
+}
+
 template <int N, typename TC, typename TS>
 __global__ void __launch_bounds__(256) curved_faceflux_kernel(CArgs<N, TC> a)
 {
@@ -819,6 +979,32 @@ curved_volume_kernel(CArgs<N, TC> a, int64_t Ne)
     volume_body<N, TC, TS, TO, GEN>(a, e0, int(min<int64_t>(VolShape<N>::EW, Ne - e0)));
 }
 
+// Two workgroups per CU at N = 8 and 10: NT / 128 wavefronts per SIMD (N = 10:
+// 8, so at most 64 VGPRs, reached without spills).  Other N need no bound
+// (N = 9 would spill at its 80).
+template <int N>
+struct VolNodeWaves {
+    static constexpr int value = (VolNodeShape<N>::NT == 512 || VolNodeShape<N>::NT == 1024) ? VolNodeShape<N>::NT / 128 : 1;
+};
+
+template <int N, typename TC, typename TS, typename TO, bool GEN>
+__global__ void __launch_bounds__(VolNodeShape<N>::NT) __attribute__((amdgpu_waves_per_eu(VolNodeWaves<N>::value)))
+curved_volnode_kernel(CArgs<N, TC> a, int64_t Ne)
+{
+    const int64_t e0 = int64_t(blockIdx.x) * VolNodeShape<N>::EW;
+    volume_node_body<N, TC, TS, TO, GEN>(a, e0, int(min<int64_t>(VolNodeShape<N>::EW, Ne - e0)));
+}
+
+template <int N, typename TC, typename TS, typename TO, bool GEN>
+__global__ void __launch_bounds__(VolNodeShape<N>::NT) __attribute__((amdgpu_waves_per_eu(VolNodeWaves<N>::value)))
+curved_volnode_batch_kernel(const CArgs<N, TC>* __restrict__ tab, const int2* __restrict__ work)
+{
+    const int2 w = work[blockIdx.x];
+    const CArgs<N, TC>& a = tab[w.x];
+    const int64_t Ne = a.npts / (N * N * N);
+    volume_node_body<N, TC, TS, TO, GEN>(a, w.y, int(min<int64_t>(VolNodeShape<N>::EW, Ne - w.y)));
+}
+
 template <int N, typename TC, typename TS>
 __global__ void __launch_bounds__(256) curved_faceflux_batch_kernel(const CArgs<N, TC>* __restrict__ tab,
                                                                     const int2* __restrict__ work)
@@ -836,6 +1022,65 @@ curved_volume_batch_kernel(const CArgs<N, TC>* __restrict__ tab, const int2* __r
     const CArgs<N, TC>& a = tab[w.x];
     const int64_t Ne = a.npts / (N * N * N);
     volume_body<N, TC, TS, TO, GEN>(a, w.y, int(min<int64_t>(VolShape<N>::EW, Ne - w.y)));
+}
+
+// Element-kernel form: one work-item per node (default) or per z-line
+// (WT2_CURVED_VOL=line, the earlier form, kept for comparison).
+static bool vol_node()
+{
+    static const bool node = [] {
+        const char* ev = std::getenv("WT2_CURVED_VOL");
+        return !(ev && std::strcmp(ev, "line") == 0);
+    }();
+    return node;
+}
+
+template <int N>
+static int vol_ew()
+{
+    return vol_node() ? VolNodeShape<N>::EW : VolShape<N>::EW;
+}
+
+template <int N, typename TC, typename TS, typename TO, bool GEN>
+static hipError_t launch_vol(const CArgs<N, TC>* a, const CArgs<N, TC>* tab, const int2* work, int64_t nblk,
+                             int64_t Ne, hipStream_t st)
+{
+    const bool node = vol_node();
+    const size_t shmem = node ? size_t(VolNodeShape<N>::EW) * VolNodeShape<N>::PER * sizeof(TC)
+                              : size_t(VolShape<N>::EW) * VolShape<N>::PER * sizeof(TC);
+    const int nt = node ? VolNodeShape<N>::NT : VolShape<N>::NT;
+    const void* kern;
+    if (node)
+        kern = a ? reinterpret_cast<const void*>(curved_volnode_kernel<N, TC, TS, TO, GEN>)
+                 : reinterpret_cast<const void*>(curved_volnode_batch_kernel<N, TC, TS, TO, GEN>);
+    else
+        kern = a ? reinterpret_cast<const void*>(curved_volume_kernel<N, TC, TS, TO, GEN>)
+                 : reinterpret_cast<const void*>(curved_volume_batch_kernel<N, TC, TS, TO, GEN>);
+    // raise the dynamic-LDS limit once per kernel (the first launch of a
+    // configuration is always eager, outside any capture)
+    static bool raised[4] = {false, false, false, false};
+    const int slot = (node ? 2 : 0) + (a ? 1 : 0);
+    hipError_t err = hipSuccess;
+    if (shmem + sizeof(VolTables<N, TC>) + 256 > 64 * 1024 && !raised[slot]) {
+        err = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
+        if (err != hipSuccess) return err;
+        raised[slot] = true;
+    }
+    (void)hipGetLastError();
+    if (a) {
+        if (node)
+            hipLaunchKernelGGL((curved_volnode_kernel<N, TC, TS, TO, GEN>), dim3(unsigned(nblk)), dim3(nt), shmem, st, *a, Ne);
+        else
+            hipLaunchKernelGGL((curved_volume_kernel<N, TC, TS, TO, GEN>), dim3(unsigned(nblk)), dim3(nt), shmem, st, *a, Ne);
+    } else {
+        if (node)
+            hipLaunchKernelGGL((curved_volnode_batch_kernel<N, TC, TS, TO, GEN>), dim3(unsigned(nblk)), dim3(nt), shmem, st,
+                               tab, work);
+        else
+            hipLaunchKernelGGL((curved_volume_batch_kernel<N, TC, TS, TO, GEN>), dim3(unsigned(nblk)), dim3(nt), shmem, st,
+                               tab, work);
+    }
+    return hipGetLastError();
 }
 
 template <int N, typename TC>
@@ -874,20 +1119,10 @@ static hipError_t launch_c(const CurvedData& cd, const CurvedCall& cc, hipStream
     hipLaunchKernelGGL((curved_faceflux_kernel<N, TC, TS>), dim3(unsigned(cd.Ne)), dim3(256), 0, st, a);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
-    using VS = VolShape<N>;
-    const size_t shmem = size_t(VS::EW) * VS::PER * sizeof(TC);
-    const int gen = cd.conn != nullptr;
-    auto kern = gen ? curved_volume_kernel<N, TC, TS, TO, true> : curved_volume_kernel<N, TC, TS, TO, false>;
-    static bool raised[2] = {false, false};
-    if (shmem + sizeof(VolTables<N, TC>) + 256 > 64 * 1024 && !raised[gen]) {
-        err = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
-        if (err != hipSuccess) return err;
-        raised[gen] = true;
-    }
-    const int64_t nblk = (cd.Ne + VS::EW - 1) / VS::EW;
-    hipLaunchKernelGGL(kern, dim3(unsigned(nblk)), dim3(VS::NT), shmem, st, a, int64_t(cd.Ne));
-    return hipGetLastError();
+    const int ew = vol_ew<N>();
+    const int64_t nblk = (cd.Ne + ew - 1) / ew;
+    if (cd.conn != nullptr) return launch_vol<N, TC, TS, TO, true>(&a, nullptr, nullptr, nblk, cd.Ne, st);
+    return launch_vol<N, TC, TS, TO, false>(&a, nullptr, nullptr, nblk, cd.Ne, st);
 }
 
 template <int N>
@@ -915,7 +1150,7 @@ static hipError_t cbatch_build_t(const CurvedData* const* cds, const CurvedCall*
         make_cargs<N, TC>(*cds[i], ccs[i], tab[i]);
         plan.gen |= cds[i]->conn != nullptr;
         for (int64_t e = 0; e < cds[i]->Ne; ++e) work.push_back(make_int2(i, int(e)));
-        for (int64_t e = 0; e < cds[i]->Ne; e += VolShape<N>::EW) work2.push_back(make_int2(i, int(e)));
+        for (int64_t e = 0; e < cds[i]->Ne; e += vol_ew<N>()) work2.push_back(make_int2(i, int(e)));
     }
     const size_t wb2 = sizeof(int2) * work2.size();
     if (wb2 > plan.work2_cap) {
@@ -958,19 +1193,8 @@ static hipError_t cbatch_launch_t(const CurvedBatchPlan& plan, hipStream_t st)
     hipLaunchKernelGGL((curved_faceflux_batch_kernel<N, TC, TS>), dim3(plan.nwork), dim3(256), 0, st, tab, plan.work);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
-    using VS = VolShape<N>;
-    const size_t shmem = size_t(VS::EW) * VS::PER * sizeof(TC);
-    const int gen = plan.gen != 0;
-    auto kern = gen ? curved_volume_batch_kernel<N, TC, TS, TO, true> : curved_volume_batch_kernel<N, TC, TS, TO, false>;
-    static bool raised[2] = {false, false};
-    if (shmem + sizeof(VolTables<N, TC>) + 256 > 64 * 1024 && !raised[gen]) {
-        err = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
-        if (err != hipSuccess) return err;
-        raised[gen] = true;
-    }
-    hipLaunchKernelGGL(kern, dim3(plan.nwork2), dim3(VS::NT), shmem, st, tab, plan.work2);
-    return hipGetLastError();
+    if (plan.gen) return launch_vol<N, TC, TS, TO, true>(nullptr, tab, plan.work2, plan.nwork2, 0, st);
+    return launch_vol<N, TC, TS, TO, false>(nullptr, tab, plan.work2, plan.nwork2, 0, st);
 }
 
 template <int N>
