@@ -900,7 +900,7 @@ def main():
                    "seed": args.seed, "pulse_centre": [float(c) for c in x0]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel": f"flat_xpass_kernel<8,double,...> (all {lps} pencil passes of a step)",
+                     "kernel": f"flat_lean_kernel + flat_xpass_kernel <8,double,...> (all {lps} pencil passes of a step)",
                      "launches_per_step": lps,
                      "algorithmic_bytes_per_point_step": FUSED_MIN_BYTES["fp64"],
                      "pass_design_bytes_per_point_step": step_bytes("fp64", lps),

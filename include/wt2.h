@@ -297,9 +297,10 @@ int wt2_batch_info(const wt2_batch* b, int32_t* launches_per_step, int32_t* acti
 
 /* Autotuner (no counterpart in the reference): times every valid pass-kernel
  * kind and tile size per axis on this context with state (u, ud) (device
- * pointers, only read) and keeps the fastest; the choice is cached per
- * (N, M, precision, curved, KO, periodicity) and applied to later contexts of
- * the same shape.  flags & 1 forces a re-measurement. */
+ * pointers, only read) and keeps the fastest (a flat context with the folded
+ * RK4 schedule times stages 1-3 of the step with each candidate); the choice is
+ * cached per (N, M, precision, curved, KO, periodicity, schedule) and applied to
+ * later contexts of the same shape.  flags & 1 forces a re-measurement. */
 int wt2_ctx_autotune(wt2_ctx* ctx, double* u, double* ud, double dt, int32_t flags);
 /* Current tile choice: element planes per tile and LDS-free flag per axis. */
 int wt2_ctx_tiles(const wt2_ctx* ctx, int32_t* nb3, int32_t* kind3);
