@@ -162,7 +162,11 @@ int wt2_rhs_full(wt2_ctx* ctx, double* du, double* dud, const double* u,
  * boundary data), so the RK4 polynomial is evaluated in Horner form --
  * S1 = y + dt/4 f(y), S2 = y + dt/3 f(S1), S3 = y + dt/2 f(S2),
  * y <- y + dt f(S3) -- equal to the classical stage sums up to rounding, with
- * no accumulator register.  If energy_out is non-null and energy_every > 0 it receives
+ * no accumulator register.  WT2_MIXED: the stage inputs and partial sums are
+ * FP32; stages 1-3 read an FP32 mirror of (u, ud) made at the start of the
+ * call (they round their result to FP32, so Y is needed only to FP32
+ * precision), stage 4 reads and writes (u, ud) in FP64 and renews the mirror.
+ * If energy_out is non-null and energy_every > 0 it receives
  * 1 + nsteps/energy_every energies (host array): before the first step and
  * after every energy_every-th step. */
 int wt2_rk4_steps(wt2_ctx* ctx, double* u, double* ud, double dt, int32_t nsteps,
@@ -321,7 +325,7 @@ int wt2_ctx_set_tiles(wt2_ctx* ctx, const int32_t* nb3, const int32_t* kind3);
  * axis == 3, the drop-in right-hand side of wt2_rhs3d (the context's own
  * bdry_values) into an internal buffer.  For axes -1..2 the context registers
  * must have been initialised by a wt2_rk4_steps call with the same state
- * pointers. */
+ * pointers (in mixed contexts, together with the FP32 mirror of the state). */
 int wt2_launch_pass(wt2_ctx* ctx, int32_t axis, int32_t stage, double* u, double* ud, double dt);
 
 const char* wt2_last_error(void);

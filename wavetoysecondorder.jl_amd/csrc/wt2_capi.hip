@@ -223,6 +223,7 @@ struct wt2_ctx {
     // in a mixed context, where a block is also one FP64 field of scratch)
     void* Sb;       // stage input
     void* Pb;       // partial sums
+    float* Yf;      // mixed: FP32 mirror of the RK state (2 fields), else null
     double* Y[2];   // device staging for host-pointer calls (lazy)
     double* O[2];   // device staging for host-pointer outputs (lazy)
     double* Lu;     // energy scratch
@@ -436,9 +437,10 @@ static void stage_items(wt2_ctx* c, int stage, const double* u, const double* v,
         const bool mixed = is_mixed(c);
         const bool ko = c->d.eps_ko > 0 && c->kos > 0;
         const double* bv = c->d.bdry_values;
-        const void* Su = stage == 1 ? static_cast<const void*>(Yu) : fld(c, c->Sb, 0);
-        const void* Sv = stage == 1 ? static_cast<const void*>(Yv) : fld(c, c->Sb, 1);
-        const int variant = !mixed ? VAR_F64 : (stage == 1 ? VAR_MIX_IN64 : VAR_MIX_IN32);
+        // stage 1 reads Y (mixed: its FP32 mirror)
+        const void* Su = stage != 1 ? fld(c, c->Sb, 0) : mixed ? static_cast<const void*>(c->Yf) : Yu;
+        const void* Sv = stage != 1 ? fld(c, c->Sb, 1) : mixed ? static_cast<const void*>(c->Yf + c->npts) : Yv;
+        const int variant = !mixed ? VAR_F64 : VAR_MIX_IN32;
         it = StageItems();
         PassDesc z = base_pass(c, c->fmid, bv);
         z.variant = variant;
@@ -454,6 +456,7 @@ static void stage_items(wt2_ctx* c, int stage, const double* u, const double* v,
         e.epi = EPI_RK1 + (stage - 1);
         e.So_u = fld(c, c->Sb, 0); e.So_v = fld(c, c->Sb, 1);
         e.Yu = Yu; e.Yv = Yv;
+        e.Yfu = mixed ? c->Yf : nullptr; e.Yfv = mixed ? c->Yf + c->npts : nullptr;
         e.dt = dt;
         it.pass[1] = e;
         it.has_pass[1] = true;
@@ -462,9 +465,13 @@ static void stage_items(wt2_ctx* c, int stage, const double* u, const double* v,
     const bool mixed = is_mixed(c);
     const bool ko = c->d.eps_ko > 0 && c->kos > 0;
     const double* bv = c->d.bdry_values;
-    const void* Su = stage <= 1 ? static_cast<const void*>(stage == 0 ? u : Yu) : fld(c, c->Sb, 0);
-    const void* Sv = stage <= 1 ? static_cast<const void*>(stage == 0 ? v : Yv) : fld(c, c->Sb, 1);
-    const int variant = !mixed ? VAR_F64 : (stage <= 1 ? VAR_MIX_IN64 : VAR_MIX_IN32);
+    // stage 0 reads (u, v); stage 1 reads Y (mixed: its FP32 mirror); later stages S
+    const bool mirror = mixed && stage == 1;
+    const void* Su = stage == 0 ? static_cast<const void*>(u) : stage > 1 ? fld(c, c->Sb, 0)
+                   : mirror ? static_cast<const void*>(c->Yf) : static_cast<const void*>(Yu);
+    const void* Sv = stage == 0 ? static_cast<const void*>(v) : stage > 1 ? fld(c, c->Sb, 1)
+                   : mirror ? static_cast<const void*>(c->Yf + c->npts) : static_cast<const void*>(Yv);
+    const int variant = !mixed ? VAR_F64 : (stage == 0 ? VAR_MIX_IN64 : VAR_MIX_IN32);
     const int final_epi = stage == 0 ? EPI_OUT_RHS : EPI_RK1 + (stage - 1);
     void* P = c->Pb;
     bool have_p = false;
@@ -495,6 +502,7 @@ static void stage_items(wt2_ctx* c, int stage, const double* u, const double* v,
         p.epi = last ? final_epi : (have_p ? EPI_MID : EPI_FIRST);
         p.So_u = fld(c, c->Sb, 0); p.So_v = fld(c, c->Sb, 1);
         p.Yu = Yu; p.Yv = Yv;
+        p.Yfu = mixed ? c->Yf : nullptr; p.Yfv = mixed ? c->Yf + c->npts : nullptr;
         p.Ou = Ou; p.Ov = Ov;
         p.dt = dt;
         it.pass[q] = p;
@@ -678,6 +686,7 @@ int wt2_ctx_footprint(const wt2_mesh_desc* desc, int32_t with_background, int64_
     // reduction partials, operator tables; plus the curved coefficients
     int64_t b = 2 * (2 * n * es) + n * int64_t(sizeof(double)) + (RED_BLOCKS + 8) * int64_t(sizeof(double)) +
                 ts * int64_t(sizeof(double) + 2 * sizeof(float));
+    if (desc->precision == WT2_MIXED) b += 2 * n * int64_t(sizeof(float));   // FP32 mirror of the state
     if (desc->warp_A != 0.0 || with_background)
         b += 13 * n * int64_t(sizeof(double) + sizeof(float)) + n * int64_t(sizeof(double)) +
              Ne * 6 * N * N * int64_t(sizeof(double)) + 5 * Ne * 6 * N * N * int64_t(sizeof(double) + sizeof(float));
@@ -1028,6 +1037,7 @@ static int ctx_create_impl(const wt2_mesh_desc* desc, const MeshIn* mi, wt2_ctx*
     const size_t blk = 2 * n * elem_size(c);
     int rc = alloc_dev(c, &c->Sb, blk);
     if (rc == WT2_OK) rc = alloc_dev(c, &c->Pb, blk);
+    if (rc == WT2_OK && is_mixed(c)) rc = alloc_dev(c, reinterpret_cast<void**>(&c->Yf), 2 * n * sizeof(float));
     if (rc == WT2_OK) rc = alloc_dev(c, reinterpret_cast<void**>(&c->Lu), n * sizeof(double));
     if (rc == WT2_OK) rc = alloc_dev(c, reinterpret_cast<void**>(&c->red), (RED_BLOCKS + 8) * sizeof(double));
     if (rc == WT2_OK) {
@@ -1067,7 +1077,7 @@ void wt2_ctx_destroy(wt2_ctx* c)
 {
     if (!c) return;
     if (c->gvalid) hipGraphExecDestroy(c->gexec);
-    hipFree(c->Sb); hipFree(c->Pb);
+    hipFree(c->Sb); hipFree(c->Pb); hipFree(c->Yf);
     for (int f = 0; f < 2; ++f) { hipFree(c->Y[f]); hipFree(c->O[f]); }
     hipFree(c->Lu); hipFree(c->red); hipFree(c->Edev);
     hipFree(c->op64); hipFree(c->op32); hipFree(c->op32c);
@@ -1198,16 +1208,24 @@ int wt2_rhs_full(wt2_ctx* c, double* du, double* dud, const double* u, const dou
     return rc;
 }
 
-// Start of a folded run: P = the stage-1 contribution of axis e_4 on Y.
+// Start of a run: in mixed contexts the FP32 mirror of Y; in a folded
+// schedule P = the stage-1 contribution of axis e_4 on Y (or its mirror).
 static int run_prologue(wt2_ctx* c, const double* Yu, const double* Yv)
 {
+    const bool mixed = is_mixed(c);
+    if (mixed) {
+        const int rc = pass_rc(launch_mirror32(Yu, Yv, c->Yf, c->Yf + c->npts, c->npts, c->stream), "state mirror");
+        if (rc) return rc;
+    }
     if (!fold_on(c)) return WT2_OK;
     const int ax = c->fe[3];
     PassDesc p = base_pass(c, ax, c->d.bdry_values);
-    p.variant = is_mixed(c) ? VAR_MIX_IN64 : VAR_F64;
+    p.variant = mixed ? VAR_MIX_IN32 : VAR_F64;
     const bool ko = c->d.eps_ko > 0 && c->kos > 0;
     p.lap = 1; p.ko = ko; p.needv = ko || som_axis(c, ax);
-    p.Su = Yu; p.Sv = Yv; p.P = c->Pb;
+    p.Su = mixed ? static_cast<const void*>(c->Yf) : Yu;
+    p.Sv = mixed ? static_cast<const void*>(c->Yf + c->npts) : Yv;
+    p.P = c->Pb;
     p.epi = EPI_FIRST;
     return pass_rc(launch_flat_pass(p, c->stream), "flat pass (prologue)");
 }
@@ -1435,7 +1453,10 @@ int wt2_ctx_autotune(wt2_ctx* c, double* u, double* ud, double dt, int32_t flags
         }
     }
     if (c->gvalid) { (void)hipGraphExecDestroy(c->gexec); c->gvalid = false; }
-    int rc = run_stage(c, 1, nullptr, nullptr, nullptr, nullptr, u, ud, dt, -2, false);   // fill S, P, A
+    int rc = WT2_OK;
+    if (is_mixed(c))    // stage 1 reads the FP32 mirror of the state
+        rc = pass_rc(launch_mirror32(u, ud, c->Yf, c->Yf + c->npts, c->npts, c->stream), "state mirror");
+    if (!rc) rc = run_stage(c, 1, nullptr, nullptr, nullptr, nullptr, u, ud, dt, -2, false);   // fill S, P, A
     if (rc) return rc;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));

@@ -36,6 +36,7 @@ __device__ __forceinline__ void flat_pass_body(const KArgs<N, TC>& a, const int 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     TS* lds = reinterpret_cast<TS*>(smem_raw);
     constexpr int N3 = N * N * N;
+    constexpr bool MIXED = std::is_same<TP, float>::value;
 
     // ---- tile decode: two fixed element coordinates and a line block
     int t = tile;
@@ -152,12 +153,17 @@ __device__ __forceinline__ void flat_pass_body(const KArgs<N, TC>& a, const int 
         const double kv = double(TC(P[g]) + x);
         if (epi == EPI_OUT_RHS) { a.Ou[g] = ku; a.Ov[g] = kv; continue; }
         if (epi == EPI_RK4) {
-            a.Yu[g] = a.Yu[g] + dt * ku;
-            a.Yv[g] = a.Yv[g] + dt * kv;
+            const double nu = a.Yu[g] + dt * ku, nv = a.Yv[g] + dt * kv;
+            a.Yu[g] = nu;
+            a.Yv[g] = nv;
+            if constexpr (MIXED) { a.Yfu[g] = float(nu); a.Yfv[g] = float(nv); }
         } else {
+            // mixed: the FP32 stage input needs Y only to FP32 precision
             const double cs = rk_horner_c(epi, dt);
-            static_cast<TP*>(a.So_u)[g] = TP(a.Yu[g] + cs * ku);
-            static_cast<TP*>(a.So_v)[g] = TP(a.Yv[g] + cs * kv);
+            const double yu = MIXED ? double(a.Yfu[g]) : a.Yu[g];
+            const double yv = MIXED ? double(a.Yfv[g]) : a.Yv[g];
+            static_cast<TP*>(a.So_u)[g] = TP(yu + cs * ku);
+            static_cast<TP*>(a.So_v)[g] = TP(yv + cs * kv);
         }
     }
 }
@@ -257,15 +263,23 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
         return;
     }
     // RK4 stage update (Horner form, rk_horner_c), k = (u_t, P + c); nu, nv:
-    // the next stage's input (S or, after stage 4, the new Y) for the fold
+    // the next stage's input (S or, after stage 4, the new Y) for the fold.
+    // Mixed: stages 1-3 form an FP32 stage input, for which Y is needed only
+    // to FP32 precision, so they read the FP32 mirror of Y (8 B per point
+    // instead of 16); stage 4 reads and writes Y in FP64 and rewrites the
+    // mirror.
     const double dt = a.dt;
     double nu[N], nv[N];
     {
         double yu[N], yv[N];
-        if (!MIXED && epi == EPI_RK1 && a.lap) {
-            // stage 1 reads Y itself as its input: already in registers
+        if (epi == EPI_RK1 && a.lap) {
+            // stage 1 reads Y (mixed: its FP32 mirror) as its input: already
+            // in registers
 #pragma unroll
             for (int r = 0; r < N; ++r) { yu[r] = double(u[r]); yv[r] = double(v[r]); }
+        } else if (MIXED && epi != EPI_RK4) {
+            load_seg_s<N, float, double>(a.Yfu + g0, gs, yu);
+            load_seg_s<N, float, double>(a.Yfv + g0, gs, yv);
         } else {
             load_seg_s<N, double, double>(a.Yu + g0, gs, yu);
             load_seg_s<N, double, double>(a.Yv + g0, gs, yv);
@@ -279,6 +293,10 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
         if (epi == EPI_RK4) {
             if (active) store_seg_s<N, double, double>(a.Yu + g0, gs, nu);
             if (active) store_seg_s<N, double, double>(a.Yv + g0, gs, nv);
+            if constexpr (MIXED) {
+                if (active) store_seg_s<N, float, double>(a.Yfu + g0, gs, nu);
+                if (active) store_seg_s<N, float, double>(a.Yfv + g0, gs, nv);
+            }
         } else {
             if (active) store_seg_s<N, TP, double>(static_cast<TP*>(a.So_u) + g0, gs, nu);
             if (active) store_seg_s<N, TP, double>(static_cast<TP*>(a.So_v) + g0, gs, nv);
@@ -429,7 +447,7 @@ static hipError_t make_kargs(const PassDesc& p, KArgs<N, TC>& a, int* tiles_out,
     a.k.somL = p.ops.somL; a.k.somR = p.ops.somR;
     if ((p.ops.somL || p.ops.somR) && p.lap && !p.needv) return hipErrorInvalidValue;   // the drag reads u_t
     a.Su = p.Su; a.Sv = p.Sv; a.P = p.P;
-    a.So_u = p.So_u; a.So_v = p.So_v; a.Yu = p.Yu; a.Yv = p.Yv;
+    a.So_u = p.So_u; a.So_v = p.So_v; a.Yu = p.Yu; a.Yv = p.Yv; a.Yfu = p.Yfu; a.Yfv = p.Yfv;
     a.Ou = p.Ou; a.Ov = p.Ov;
     a.dt = p.dt;
     a.periodic = p.periodic; a.epi = p.epi;
@@ -984,6 +1002,29 @@ void flat_batch_free(FlatBatchPlan& plan)
     (void)hipFree(plan.tab);
     (void)hipFree(plan.work);
     plan = FlatBatchPlan();
+}
+
+// ------------------------------------------------------------------ FP32 mirror of the state
+
+// Mixed contexts: Yf = float(Y) for both fields (round to nearest), the stage
+// input of stage 1 and the Y of the stage-1..3 updates.
+__global__ void __launch_bounds__(256) mirror32_kernel(const double* __restrict__ u, const double* __restrict__ v,
+                                                       float* __restrict__ fu, float* __restrict__ fv, int64_t n)
+{
+    const int64_t step = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        fu[i] = float(u[i]);
+        fv[i] = float(v[i]);
+    }
+}
+
+hipError_t launch_mirror32(const double* u, const double* v, float* fu, float* fv, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t want = (n + 255) / 256;
+    const int blocks = int(want < 8192 ? want : 8192);
+    hipLaunchKernelGGL(mirror32_kernel, dim3(blocks), dim3(256), 0, s, u, v, fu, fv, n);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ energy

@@ -69,6 +69,7 @@ struct PassDesc {
     void* P;                          // partial sum of the u_t equation (variant partial type)
     void* So_u; void* So_v;           // next stage input (partial type)
     double* Yu; double* Yv;           // RK state
+    float* Yfu; float* Yfv;           // mixed: FP32 mirror of Y (read at stages 1-3, rewritten at stage 4)
     double* Ou; double* Ov;           // outputs
     double dt;
     AxisOps ops;
@@ -77,6 +78,9 @@ struct PassDesc {
 
 // Launch one pencil pass on stream s.  Returns hipError_t.
 hipError_t launch_flat_pass(const PassDesc& p, hipStream_t s);
+
+// Mixed contexts: the FP32 mirror (fu, fv) = float(u, v) of the RK state.
+hipError_t launch_mirror32(const double* u, const double* v, float* fu, float* fv, int64_t n, hipStream_t s);
 
 // The flat SIPG Laplacian out = L u in one launch (no KO term): a workgroup
 // per block of elements, the element staged in LDS, the face neighbours'

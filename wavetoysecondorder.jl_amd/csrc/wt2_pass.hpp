@@ -32,6 +32,7 @@ struct KArgs {
     void* P;
     void* So_u; void* So_v;
     double* Yu; double* Yv;
+    float* Yfu; float* Yfv;   // mixed: FP32 mirror of Y
     double* Ou; double* Ov;
     double dt;
     // per-axis geometry, precomputed on the host (the kernel has no branch on
