@@ -184,7 +184,11 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
     constexpr int N3 = N * N * N;
     constexpr bool MIXED = std::is_same<TP, float>::value;
     __shared__ TC op[OpLayout<N>::SIZE];
-    __shared__ TC xbuf[XCH ? 4 * 512 : 1];
+    // dynamic LDS: the exchange buffers (kind 2), then with ypre the Y slots
+    extern __shared__ __align__(16) unsigned char xdyn[];
+    const int nthr = blockDim.x;
+    TC* xbuf = reinterpret_cast<TC*>(xdyn);
+    uint32_t* ydyn = reinterpret_cast<uint32_t*>(xdyn + xpass_xbuf_bytes<N, TC>(XCH, nthr));
     for (int q = threadIdx.x; q < OpLayout<N>::SIZE; q += blockDim.x) op[q] = a.op[q];
     __syncthreads();
 
@@ -217,7 +221,7 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
     }
     typename std::conditional<XCH, LdsX<TC>, ShflX<TC>>::type xch;
     if constexpr (XCH) {
-        xch.buf = xbuf; xch.n = 512; xch.me = tid; xch.idxL = lidxL; xch.idxR = lidxR; xch.k = 0;
+        xch.buf = xbuf; xch.n = nthr; xch.me = tid; xch.idxL = lidxL; xch.idxR = lidxR; xch.k = 0;
     } else {
         xch.srcL = lidxL; xch.srcR = lidxR;
     }
@@ -239,6 +243,20 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
     const int epi = a.epi;
     TC pv[N];
     if (epi != EPI_FIRST) load_seg_s<N, TP, TC>(P + g0, gs, pv);
+    // the RK update's Y (mixed stages 1-3: its FP32 mirror), loaded straight
+    // into LDS behind the stage input: in flight while the operator runs,
+    // holding no registers
+    const bool y32 = MIXED && epi != EPI_RK4;
+    const bool ypre = a.ypre && epi >= EPI_RK1 && !(epi == EPI_RK1 && a.lap);
+    if (ypre) {
+        if (y32) {
+            y_to_lds<N, float>(a.Yfu + g0, gs, ydyn, nthr, tid);
+            y_to_lds<N, float>(a.Yfv + g0, gs, ydyn + N * nthr, nthr, tid);
+        } else {
+            y_to_lds<N, double>(a.Yu + g0, gs, ydyn, nthr, tid);
+            y_to_lds<N, double>(a.Yv + g0, gs, ydyn + 2 * N * nthr, nthr, tid);
+        }
+    }
     if (a.ko) {
         TC x[N];
 #pragma unroll
@@ -249,6 +267,9 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
     }
     if (a.lap) sipg_add<N, TC>(u, cv, op, a.k, xch, hasL, hasR, TC(v[0]), TC(v[N - 1]));
     const bool fold = a.fold && epi >= EPI_RK1;
+    // no wavefront leaves with its LDS loads in flight (the workgroup's LDS
+    // could be handed to the next workgroup)
+    if (ypre) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     // a folding pass with the LDS exchange needs every work-item at its barriers
     if (!active && !(XCH && fold)) return;
 
@@ -277,6 +298,14 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
             // in registers
 #pragma unroll
             for (int r = 0; r < N; ++r) { yu[r] = double(u[r]); yv[r] = double(v[r]); }
+        } else if (ypre) {
+            if (y32) {
+                y_from_lds<N, float>(ydyn, gs, nthr, tid, yu);
+                y_from_lds<N, float>(ydyn + N * nthr, gs, nthr, tid, yv);
+            } else {
+                y_from_lds<N, double>(ydyn, gs, nthr, tid, yu);
+                y_from_lds<N, double>(ydyn + 2 * N * nthr, gs, nthr, tid, yv);
+            }
         } else if (MIXED && epi != EPI_RK4) {
             load_seg_s<N, float, double>(a.Yfu + g0, gs, yu);
             load_seg_s<N, float, double>(a.Yfv + g0, gs, yv);
@@ -433,6 +462,33 @@ static constexpr size_t static_lds_bound()
     return (sizeof(TC) * OpLayout<N>::SIZE + 255) / 256 * 256 + 256;
 }
 
+// Dynamic LDS of an LDS-free pass launched with nthr work-items (*dyn) and
+// whether its RK update's Y goes straight to LDS (the return value): for the
+// FP32 mirror read at the mixed stages 1-3, when the exchange buffers, the Y
+// slots and the static table fit 64 KB.  Measured at 32^3 x 8^3: mixed z pass
+// 163 -> 143 us, curved 24^3 x 10^3 mixed step 2.034 -> 2.016 ms; for FP64 Y
+// (two 4-byte loads per value where the segment is strided) the FP64 step
+// went 1.265 -> 1.312 ms, so FP64 Y keeps its register loads.  WT2_YPRE = 0
+// switches the LDS loads off, 2 also uses them for FP64 Y.
+template <int N, typename TC, typename TP>
+static int xpass_lds(const PassDesc& p, int nthr, size_t* dyn)
+{
+    static const int ypre_env = [] {
+        const char* ev = std::getenv("WT2_YPRE");
+        return ev ? std::atoi(ev) : 1;
+    }();
+    *dyn = 0;
+    if (!p.direct || p.epi < EPI_RK1) return 0;
+    const bool needy = !(p.epi == EPI_RK1 && p.lap);
+    const bool y32 = std::is_same<TP, float>::value && p.epi != EPI_RK4;
+    *dyn = xpass_xbuf_bytes<N, TC>(p.direct == 2, nthr);
+    if (!ypre_env || !needy || (!y32 && ypre_env < 2) ||
+        static_lds_bound<N, TC>() + *dyn + xpass_y_bytes<N>(y32, nthr) > 64 * 1024)
+        return 0;
+    *dyn += xpass_y_bytes<N>(y32, nthr);
+    return 1;
+}
+
 template <int N, typename TC, typename TS, typename TP>
 static hipError_t make_kargs(const PassDesc& p, KArgs<N, TC>& a, int* tiles_out, int* nthr_out,
                              size_t* shmem_out)
@@ -510,9 +566,14 @@ static hipError_t make_kargs(const PassDesc& p, KArgs<N, TC>& a, int* tiles_out,
     const size_t static_lds = static_lds_bound<N, TC>();
     if (shmem + static_lds > 160 * 1024) return hipErrorInvalidConfiguration;
     a.GS = p.axis == 0 ? 1 : (p.axis == 1 ? N : N * N);
+    // LDS-free RK passes: Y straight to LDS (ypre) when the exchange buffers,
+    // the Y slots and the static table fit 64 KB (three workgroups of 256
+    // work-items per CU at N = 8 FP64).  WT2_YPRE = 0 switches it off.
+    size_t dyn = 0;
+    a.ypre = xpass_lds<N, TC, TP>(p, nthr, &dyn);
     *tiles_out = tiles;
     *nthr_out = nthr;
-    *shmem_out = p.direct ? 0 : shmem;
+    *shmem_out = p.direct ? dyn : shmem;
     return hipSuccess;
 }
 
@@ -532,9 +593,9 @@ static hipError_t launch_t(const PassDesc& p, hipStream_t s)
         if (p.epi < EPI_RK1) return launch_xpass_lean<N, TC, TS, TP>(a, tiles, nthr, p.direct == 2, s);
         (void)hipGetLastError();
         if (p.direct == 2)
-            hipLaunchKernelGGL((flat_xpass_kernel<N, TC, TS, TP, true>), dim3(tiles), dim3(nthr), 0, s, a);
+            hipLaunchKernelGGL((flat_xpass_kernel<N, TC, TS, TP, true>), dim3(tiles), dim3(nthr), shmem, s, a);
         else
-            hipLaunchKernelGGL((flat_xpass_kernel<N, TC, TS, TP, false>), dim3(tiles), dim3(nthr), 0, s, a);
+            hipLaunchKernelGGL((flat_xpass_kernel<N, TC, TS, TP, false>), dim3(tiles), dim3(nthr), shmem, s, a);
         return hipGetLastError();
     }
     auto kern = flat_pass_kernel<N, TC, TS, TP>;
@@ -877,6 +938,12 @@ static hipError_t batch_build_t(const PassDesc* d, int n, FlatBatchPlan& plan)
         shmem = std::max(shmem, sh);
         for (int t = 0; t < tiles; ++t) work.push_back(make_int2(i, t));
     }
+    if (n > 0 && d[0].direct) {
+        // the group's launch has the largest workgroup of its instances: its
+        // LDS layout (and whether Y goes to LDS) follows from that size
+        const int yp = xpass_lds<N, TC, TP>(d[0], nthr, &shmem);
+        for (auto& k : tab) k.ypre = yp;
+    }
     if (n > 0 && tab[0].xcd) {
         // the XCD-contiguous order of the single-context z passes: the
         // workgroup with index b (XCD b % 8) takes entry (b % 8) W/8 + b / 8
@@ -908,7 +975,7 @@ static hipError_t batch_build_t(const PassDesc* d, int n, FlatBatchPlan& plan)
     plan.lean = d[0].epi < EPI_RK1;
     plan.ntiles = int(work.size());
     plan.nthr = nthr;
-    plan.shmem = plan.direct ? 0 : shmem;
+    plan.shmem = shmem;
     return e;
 }
 
@@ -923,10 +990,10 @@ static hipError_t batch_launch_t(const FlatBatchPlan& plan, hipStream_t s)
         (void)hipGetLastError();
         if (plan.direct == 2)
             hipLaunchKernelGGL((flat_xpass_batch_kernel<N, TC, TS, TP, true>), dim3(plan.ntiles), dim3(plan.nthr),
-                               0, s, tab, plan.work);
+                               plan.shmem, s, tab, plan.work);
         else
             hipLaunchKernelGGL((flat_xpass_batch_kernel<N, TC, TS, TP, false>), dim3(plan.ntiles), dim3(plan.nthr),
-                               0, s, tab, plan.work);
+                               plan.shmem, s, tab, plan.work);
         return hipGetLastError();
     }
     auto kern = flat_pass_batch_kernel<N, TC, TS, TP>;

@@ -49,6 +49,7 @@ struct KArgs {
     int periodic, epi;
     int lap, ko, needv, keepv, fold;
     int xcd;           // 1: each XCD takes a contiguous range of tiles
+    int ypre;          // RK passes: Y (or its FP32 mirror) goes straight to LDS at the start
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs; with xcd set, XCD x
@@ -202,6 +203,82 @@ __device__ __forceinline__ void sipg_add(const TC (&u)[N], TC (&out)[N], const T
     xch(u[N - 1], u[0], uL, uR);
     xch(g[N - 1], g[0], gnL, gnR);
     sipg_seg<N, TC>(u, g, out, op, k, uL, gnL, uR, gnR, hasL, hasR, v0, vN);
+}
+
+// Dynamic LDS of the LDS-free RK-pass kernel (flat_xpass_*): the neighbour
+// exchange buffers (kind 2: 4 values per work-item) and, with ypre, the
+// stage's Y segments of both fields (FP64, or the FP32 mirror at the mixed
+// stages 1-3), each work-item's values in slots of its own.
+template <int N, typename TC>
+__host__ __device__ constexpr size_t xpass_xbuf_bytes(bool xch, int nthr)
+{
+    return xch ? (size_t(4) * size_t(nthr) * sizeof(TC) + 15) / 16 * 16 : 0;
+}
+template <int N>
+__host__ __device__ constexpr size_t xpass_y_bytes(bool y32, int nthr)
+{
+    return size_t(2) * N * (y32 ? 4 : 8) * size_t(nthr);
+}
+
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+
+// One field's Y segment (N values of type TY at stride gs from p) loaded
+// straight from memory into LDS (no registers), chunk q of the field at
+// ybase + q * nthr * W, work-item t's W dwords at + t * W: 16-byte chunks
+// where the segment is contiguous and 16-byte aligned, else one dword per
+// chunk (an FP64 value as two).  The LDS address of these loads is the
+// wavefront's base (each lane's data lands at base + lane * size).
+template <int N, typename TY>
+__device__ __forceinline__ void y_to_lds(const TY* p, int gs, uint32_t* ybase, int nthr, int tid)
+{
+    const int wb = tid & ~63;
+    constexpr int PER16 = 16 / int(sizeof(TY));
+    if (gs == 1 && N % PER16 == 0) {
+#pragma unroll
+        for (int c = 0; c < N / PER16; ++c)
+            __builtin_amdgcn_global_load_lds(static_cast<const void*>(p + PER16 * c),
+                                             (lds_void_ptr)(ybase + (c * nthr + wb) * 4), 16, 0, 0);
+    } else if constexpr (sizeof(TY) == 8) {
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                __builtin_amdgcn_global_load_lds(static_cast<const void*>(reinterpret_cast<const uint32_t*>(p + r * gs) + h),
+                                                 (lds_void_ptr)(ybase + (2 * r + h) * nthr + wb), 4, 0, 0);
+    } else {
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+            __builtin_amdgcn_global_load_lds(static_cast<const void*>(p + r * gs),
+                                             (lds_void_ptr)(ybase + r * nthr + wb), 4, 0, 0);
+    }
+}
+
+// This work-item's values of a segment placed by y_to_lds (after the loads
+// have completed: s_waitcnt vmcnt(0)).
+template <int N, typename TY>
+__device__ __forceinline__ void y_from_lds(const uint32_t* ybase, int gs, int nthr, int tid, double (&y)[N])
+{
+    constexpr int PER16 = 16 / int(sizeof(TY));
+    if (gs == 1 && N % PER16 == 0) {
+#pragma unroll
+        for (int c = 0; c < N / PER16; ++c) {
+            const uint4 q = *reinterpret_cast<const uint4*>(ybase + (c * nthr + tid) * 4);
+            if constexpr (sizeof(TY) == 8) {
+                y[2 * c] = __hiloint2double(int(q.y), int(q.x));
+                y[2 * c + 1] = __hiloint2double(int(q.w), int(q.z));
+            } else {
+                y[4 * c] = double(__uint_as_float(q.x)); y[4 * c + 1] = double(__uint_as_float(q.y));
+                y[4 * c + 2] = double(__uint_as_float(q.z)); y[4 * c + 3] = double(__uint_as_float(q.w));
+            }
+        }
+    } else if constexpr (sizeof(TY) == 8) {
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+            y[r] = __hiloint2double(int(ybase[(2 * r + 1) * nthr + tid]), int(ybase[2 * r * nthr + tid]));
+    } else {
+#pragma unroll
+        for (int r = 0; r < N; ++r) y[r] = double(__uint_as_float(ybase[r * nthr + tid]));
+    }
 }
 
 // Segment load / store: N values at stride gs (gs == 1: widest aligned vector).
