@@ -248,10 +248,18 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
     // holding no registers
     const bool y32 = MIXED && epi != EPI_RK4;
     const bool ypre = a.ypre && epi >= EPI_RK1 && !(epi == EPI_RK1 && a.lap);
+    // FP64 Y with strided segments in kind 2: 16-byte loads shared by pairs
+    // of work-items (adjacent lines)
+    constexpr bool PAIRABLE = XCH && N % 2 == 0;
+    const bool ypair = PAIRABLE && !y32 && gs != 1;
     if (ypre) {
         if (y32) {
             y_to_lds<N, float>(a.Yfu + g0, gs, ydyn, nthr, tid);
             y_to_lds<N, float>(a.Yfv + g0, gs, ydyn + N * nthr, nthr, tid);
+        } else if (ypair) {
+            const int64_t pb = active ? g0 - (tid & 1) : g0;
+            y_to_lds_pair<N>(a.Yu + pb, gs, ydyn, nthr, tid);
+            y_to_lds_pair<N>(a.Yv + pb, gs, ydyn + 2 * N * nthr, nthr, tid);
         } else {
             y_to_lds<N, double>(a.Yu + g0, gs, ydyn, nthr, tid);
             y_to_lds<N, double>(a.Yv + g0, gs, ydyn + 2 * N * nthr, nthr, tid);
@@ -302,6 +310,11 @@ __device__ __forceinline__ void flat_xpass_body(const KArgs<N, TC>& a, const int
             if (y32) {
                 y_from_lds<N, float>(ydyn, gs, nthr, tid, yu);
                 y_from_lds<N, float>(ydyn + N * nthr, gs, nthr, tid, yv);
+            } else if (ypair) {
+                if constexpr (PAIRABLE) {
+                    y_from_lds_pair<N>(ydyn, nthr, tid, yu);
+                    y_from_lds_pair<N>(ydyn + 2 * N * nthr, nthr, tid, yv);
+                }
             } else {
                 y_from_lds<N, double>(ydyn, gs, nthr, tid, yu);
                 y_from_lds<N, double>(ydyn + 2 * N * nthr, gs, nthr, tid, yv);
@@ -468,8 +481,10 @@ static constexpr size_t static_lds_bound()
 // slots and the static table fit 64 KB.  Measured at 32^3 x 8^3: mixed z pass
 // 163 -> 143 us, curved 24^3 x 10^3 mixed step 2.034 -> 2.016 ms; for FP64 Y
 // (two 4-byte loads per value where the segment is strided) the FP64 step
-// went 1.265 -> 1.312 ms, so FP64 Y keeps its register loads.  WT2_YPRE = 0
-// switches the LDS loads off, 2 also uses them for FP64 Y.
+// went 1.265 -> 1.312 ms, so FP64 Y keeps its register loads by default.
+// WT2_YPRE = 0 switches the LDS loads off; 2 also uses them for FP64 Y where
+// they can be 16-byte loads (contiguous x segments, or pairs of adjacent
+// lines in kind 2).
 template <int N, typename TC, typename TP>
 static int xpass_lds(const PassDesc& p, int nthr, size_t* dyn)
 {
@@ -482,7 +497,8 @@ static int xpass_lds(const PassDesc& p, int nthr, size_t* dyn)
     const bool needy = !(p.epi == EPI_RK1 && p.lap);
     const bool y32 = std::is_same<TP, float>::value && p.epi != EPI_RK4;
     *dyn = xpass_xbuf_bytes<N, TC>(p.direct == 2, nthr);
-    if (!ypre_env || !needy || (!y32 && ypre_env < 2) ||
+    const bool y64ok = N % 2 == 0 && (p.axis == 0 || p.direct == 2);   // 16-byte loads
+    if (!ypre_env || !needy || (!y32 && (ypre_env < 2 || !y64ok)) ||
         static_lds_bound<N, TC>() + *dyn + xpass_y_bytes<N>(y32, nthr) > 64 * 1024)
         return 0;
     *dyn += xpass_y_bytes<N>(y32, nthr);
@@ -849,6 +865,135 @@ __global__ void __launch_bounds__(256) flat_lap1_kernel(Lap1Args<N, TC> a)
     }
 }
 
+// The same Laplacian with the three axes in separate wavefronts: a workgroup
+// takes one element, WA = ceil(N^2/64) wavefronts per axis (z, then y, then
+// x), a work-item per line.  Every wavefront issues its two neighbour lines'
+// loads at the start -- the x wavefronts also the element's own x-lines, which
+// they stage in LDS -- so a wavefront waits on memory once instead of once per
+// axis.  The z and y contributions go to LDS; the x wavefronts add them to
+// theirs, (c_z + c_y) + c_x as the one-wavefront form and the pencil passes
+// round, and write whole x-lines.
+template <int N>
+struct Lap3Shape {
+    static constexpr int L = N * N;
+    static constexpr int WA = (L + 63) / 64;     // wavefronts per axis
+    static constexpr int NT = 3 * WA * 64;
+    static constexpr int RS = N + 1;             // padded x-line stride
+    static constexpr int ES = RS * N * N;
+};
+
+// c = this axis's contribution on the line (la, lb) of element e, u the
+// line's own values (registers), nb its neighbours' lines.
+template <int N, typename TC, int AX>
+__device__ __forceinline__ void lap3_axis(const Lap1Args<N, TC>& a, const TC* __restrict__ op, const TC (&u)[N],
+                                          int e, int la, int lb, const Lap1Nbr<N, TC>& nb, TC (&cv)[N])
+{
+    constexpr int N3 = N * N * N;
+    constexpr int GS = AX == 0 ? 1 : (AX == 1 ? N : N * N);
+    const int gbase = AX == 0 ? N * (la + N * lb) : (AX == 1 ? la + N * N * lb : la + N * lb);
+    const bool hasL = nb.hasL, hasR = nb.hasR;
+    TC uL = TC(0), gnL = TC(0), uR = TC(0), gnR = TC(0);
+    if (hasL) {
+        TC y0;
+        apply_G_ends<N, TC>(op, nb.xl, y0, gnL);
+        uL = nb.xl[N - 1];
+    }
+    if (hasR) {
+        TC yN;
+        apply_G_ends<N, TC>(op, nb.xr, gnR, yN);
+        uR = nb.xr[0];
+    }
+    TC v0 = TC(0), vN = TC(0);
+    if (a.v) {
+        const double* vl = a.v + int64_t(e) * N3 + gbase;
+        if (!hasL && a.k[AX].somL) v0 = TC(vl[0]);
+        if (!hasR && a.k[AX].somR) vN = TC(vl[(N - 1) * GS]);
+    }
+    TC g[N];
+    apply_G<N, TC>(op, u, g);
+#pragma unroll
+    for (int m = 0; m < N; ++m) cv[m] = TC(0);
+    sipg_seg<N, TC>(u, g, cv, op, a.k[AX], uL, gnL, uR, gnR, hasL, hasR, v0, vN);
+}
+
+// The z or y wavefronts (AX = 2, 1): own line from LDS (staged by the x
+// wavefronts), contribution to C.
+template <int N, typename TC, int AX>
+__device__ __forceinline__ void lap3_zy(const Lap1Args<N, TC>& a, const TC* __restrict__ op, const TC* U, TC* C,
+                                        int e, int la, int lb, const Lap1Nbr<N, TC>& nb)
+{
+    constexpr int RS = Lap3Shape<N>::RS;
+    constexpr int LS = AX == 1 ? RS : RS * N;
+    const int lbase = AX == 1 ? la + RS * N * lb : la + RS * lb;
+    TC u[N], cv[N];
+#pragma unroll
+    for (int m = 0; m < N; ++m) u[m] = U[lbase + m * LS];
+    lap3_axis<N, TC, AX>(a, op, u, e, la, lb, nb, cv);
+#pragma unroll
+    for (int m = 0; m < N; ++m) C[lbase + m * LS] = cv[m];
+}
+
+// (The z and y wavefronts reading their own lines from global memory instead
+// of the x wavefronts' LDS copy measured 165 against 83 us at 32^3 x 8^3.)
+template <int N, typename TC>
+__global__ void __launch_bounds__(Lap3Shape<N>::NT) flat_lap3_kernel(Lap1Args<N, TC> a)
+{
+    using S = Lap3Shape<N>;
+    constexpr int N3 = N * N * N;
+    __shared__ TC U[S::ES], Cz[S::ES], Cy[S::ES];
+    const TC* __restrict__ op = a.op;     // wave-uniform reads (SGPRs)
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int ax = 2 - wave / S::WA;      // wave-uniform
+    const int line = (wave % S::WA) * 64 + (tid & 63);
+    const bool act = line < S::L;
+    const int ln = act ? line : 0;
+    const int la = ln % N, lb = ln / N;
+    // the one-wavefront form's element order and XCD ranges (one element per
+    // workgroup)
+    int wg = blockIdx.x;
+    const int full = (a.nwg / 8) * 8;
+    if (a.xcd && wg < full) wg = (wg % 8) * (a.nwg / 8) + wg / 8;
+    const int e = lap1_elem(a, wg);
+    const int ec0 = e % a.M0, ec1 = (e / a.M0) % a.M1, ec2 = e / (a.M0 * a.M1);
+    Lap1Nbr<N, TC> nb;
+    TC own[N];
+    if (ax == 0) {
+        // own x-line (contiguous) and the x neighbours' lines, then stage the
+        // own line for the z and y wavefronts
+        lap1_load<N, 1, TC>(a.u + int64_t(e) * N3 + N * ln, own);
+        lap1_load_nbrs<N, TC, 0>(a, e, ec0, la, lb, nb);
+        if (act) {
+#pragma unroll
+            for (int m = 0; m < N; ++m) U[S::RS * ln + m] = own[m];
+        }
+    } else if (ax == 1) {
+        lap1_load_nbrs<N, TC, 1>(a, e, ec1, la, lb, nb);
+    } else {
+        lap1_load_nbrs<N, TC, 2>(a, e, ec2, la, lb, nb);
+    }
+    __syncthreads();
+    TC cx[N];
+    if (act) {
+        if (ax == 2) lap3_zy<N, TC, 2>(a, op, U, Cz, e, la, lb, nb);
+        else if (ax == 1) lap3_zy<N, TC, 1>(a, op, U, Cy, e, la, lb, nb);
+        else lap3_axis<N, TC, 0>(a, op, own, e, la, lb, nb, cx);
+    }
+    __syncthreads();
+    if (ax == 0 && act) {
+        double o[N];
+#pragma unroll
+        for (int m = 0; m < N; ++m) o[m] = double((Cz[S::RS * ln + m] + Cy[S::RS * ln + m]) + cx[m]);
+        double* dst = a.out + int64_t(e) * N3 + N * ln;
+        if constexpr (N % 2 == 0) {
+#pragma unroll
+            for (int m = 0; m < N; m += 2) *reinterpret_cast<double2*>(dst + m) = make_double2(o[m], o[m + 1]);
+        } else {
+#pragma unroll
+            for (int m = 0; m < N; ++m) dst[m] = o[m];
+        }
+    }
+}
+
 template <int N, typename TC>
 static hipError_t launch_lap1_t(const Lap1Desc& d, hipStream_t s)
 {
@@ -882,6 +1027,20 @@ static hipError_t launch_lap1_t(const Lap1Desc& d, hipStream_t s)
         a.Ty = t <= 0 ? d.M[1] : std::min(t, d.M[1]);
     }
     if (const char* ev = std::getenv("WT2_LAP1_XCD")) a.xcd = std::atoi(ev) != 0;
+    // N = 8: the three-wavefront form (32^3 x 8^3 FP64: 89 -> 83 us; at N = 10,
+    // 24^3 elements, it measured 123 against 109 us, two wavefronts per axis
+    // with 100 of 128 lanes busy).  WT2_LAP1_FORM = 1 / 3 forces the one- /
+    // three-wavefront form.
+    static const int form = [] {
+        const char* ev = std::getenv("WT2_LAP1_FORM");
+        return ev ? std::atoi(ev) : 0;
+    }();
+    if ((form == 0 && N == 8) || form == 3) {
+        a.nwg = int(Ne);
+        (void)hipGetLastError();
+        hipLaunchKernelGGL((flat_lap3_kernel<N, TC>), dim3(a.nwg), dim3(Lap3Shape<N>::NT), 0, s, a);
+        return hipGetLastError();
+    }
     const size_t shmem = size_t(2) * S::EW * S::ES * sizeof(TC);
     auto kern = flat_lap1_kernel<N, TC>;
     static bool raised = false;

@@ -281,6 +281,36 @@ __device__ __forceinline__ void y_from_lds(const uint32_t* ybase, int gs, int nt
     }
 }
 
+// FP64 Y of a kind-2 pass with strided segments (y, z) and even N: adjacent
+// work-items t, t ^ 1 hold adjacent lines of one element, whose values at a
+// node are adjacent in memory, so one 16-byte load covers both.  Load q of
+// the field covers nodes 2q (even work-items load their pair's two values at
+// node 2q) and 2q + 1 (odd work-items, at node 2q + 1); work-item t's 16
+// bytes land at ybase + (q * nthr + t) * 4.  pbase: the pair's first value
+// (this work-item's line minus t & 1), any in-bounds line for inactive pairs.
+template <int N>
+__device__ __forceinline__ void y_to_lds_pair(const double* pbase, int gs, uint32_t* ybase, int nthr, int tid)
+{
+    const int wb = tid & ~63, h = tid & 1;
+#pragma unroll
+    for (int q = 0; q < N / 2; ++q)
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(pbase + (2 * q + h) * gs),
+                                         (lds_void_ptr)(ybase + (q * nthr + wb) * 4), 16, 0, 0);
+}
+
+template <int N>
+__device__ __forceinline__ void y_from_lds_pair(const uint32_t* ybase, int nthr, int tid, double (&y)[N])
+{
+    const int pb = tid & ~1, h = tid & 1;
+#pragma unroll
+    for (int q = 0; q < N / 2; ++q)
+#pragma unroll
+        for (int hr = 0; hr < 2; ++hr) {
+            const uint32_t* w = ybase + (q * nthr + pb + hr) * 4 + 2 * h;
+            y[2 * q + hr] = __hiloint2double(int(w[1]), int(w[0]));
+        }
+}
+
 // Segment load / store: N values at stride gs (gs == 1: widest aligned vector).
 template <int N, typename TS, typename TC>
 __device__ __forceinline__ void load_seg_s(const TS* __restrict__ p, int gs, TC (&x)[N]);
