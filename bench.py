@@ -929,7 +929,7 @@ def main():
                      "algorithmic_bytes_per_point_step": FUSED_MIN_BYTES["fp64"],
                      "pass_design_bytes_per_point_step": step_bytes("fp64", lps),
                      "pass_design_GBs": design_GBs},
-        "rhs_roofline": {"bound": "hbm", "kernel": "flat_lap1_kernel<8,double> (wt2_rhs3d)",
+        "rhs_roofline": {"bound": "hbm", "kernel": "flat_lap3_kernel<8,double> (wt2_rhs3d)",
                          "ms": r64["rhs_ms"], "achieved": 24 * npts / rhs_s / 1e9, "peak": PEAK_HBM_GBS,
                          "unit": "GB/s", "frac": 24 * npts / rhs_s / 1e9 / PEAK_HBM_GBS,
                          "algorithmic_bytes_per_point": 24, "compulsory_bytes_per_point": 16,

@@ -181,6 +181,47 @@ def test_mixed_rk4_hnorm_error():
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("warp", [0.0, 0.05])
+def test_mixed_state_mirror_follows_the_state(warp):
+    """Mixed stages 1-3 read an FP32 mirror of the state that each call makes
+    afresh: overwriting the state in place between two calls (same device
+    pointers, so the captured step graph is replayed) must give the oracle's
+    evolution of the new state; energy sampling between steps leaves the
+    mirror alone (bitwise the same state as without sampling)."""
+    import torch
+    _say(f"mixed state mirror warp={warp}")
+    N, M = 6, 3
+    mesh = wh.make_warped_uniform_hex(M, M, M, 0.0, 1.0, warp) if warp else wh.make_uniform_hex(M, 0.0, 1.0)
+    elem = wh.make_element(N)
+    tau = wh.tau_effective(None, N)
+    ws = wh.Workspace(mesh, elem, tau=tau, eps_ko=0.1, precision="mixed")
+    o = wo.Oracle(N, M, tau=tau, eps_ko=0.1, warp_A=warp)
+    x = o.coords()
+    uA = np.exp(-np.sum((x - 0.4) ** 2, axis=1) / 0.05)
+    uB = np.sin(math.pi * x[:, 0]) * np.sin(math.pi * x[:, 1]) * np.sin(math.pi * x[:, 2])
+    vB = 0.3 * np.cos(math.pi * x[:, 2])
+    dt = 0.2 * ws.h_min
+    ns = 12
+    u = torch.from_numpy(uA.copy()).cuda()
+    v = torch.zeros_like(u)
+    ws.rk4_steps(u, v, dt, ns)
+    u.copy_(torch.from_numpy(uB))
+    v.copy_(torch.from_numpy(vB))
+    ws.rk4_steps(u, v, dt, ns)
+    uo, vo = o.rk4(uB, vB, dt, ns)
+    H = o.hphys()
+    err = math.sqrt(np.sum((u.cpu().numpy() - uo) ** 2 * H)) / math.sqrt(np.sum(uo ** 2 * H))
+    assert err <= 1e-5, err
+    # the same evolution of B with energies sampled every 3 steps
+    u2 = torch.from_numpy(uB.copy()).cuda()
+    v2 = torch.from_numpy(vB.copy()).cuda()
+    ws.rk4_steps(u2, v2, dt, ns, energy_every=3)
+    u3 = torch.from_numpy(uB.copy()).cuda()
+    v3 = torch.from_numpy(vB.copy()).cuda()
+    ws.rk4_steps(u3, v3, dt, ns)
+    assert torch.equal(u2, u3) and torch.equal(v2, v3)
+
+
 # -------------------------------------------- full-size properties (16.8M points)
 
 def test_full_size_linearity_symmetry_energy():

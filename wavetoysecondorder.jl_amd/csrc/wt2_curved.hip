@@ -902,8 +902,11 @@ __device__ __forceinline__ void volume_node_body(const CArgs<N, TC>& a, const in
         // value is used for both as soon as it is read (unrolled, or as two
         // loops, the compiler kept the loop's 3N G and 6N field values live:
         // N = 10 FP64 118 VGPRs, one workgroup per CU, against 64 and two)
+        // (FP32: unrolled by two, which fits the same 64 VGPRs and issues the
+        // address arithmetic once per two steps)
+        constexpr int GUNROLL = sizeof(TC) == 4 ? 2 : 1;
         TC g0 = TC(0), g1 = TC(0), g2 = TC(0), h0 = TC(0), h1 = TC(0), h2 = TC(0);
-#pragma unroll 1
+#pragma unroll GUNROLL
         for (int p = 0; p < N; ++p) {
             const TC gi = T.G[i * N + p], gj = T.G[j * N + p], gk = T.G[k * N + p];
             g0 = fma(gi, A[p + N * j + N2 * k], g0);
