@@ -51,6 +51,8 @@ CASES = [
     (10, 3, 1.0, False, (0.0,) * 6),
     (4, (16, 2, 3), 1.0, False, (0.0,) * 6),   # 16 elements on one axis: 4 lines per wave
     (3, (33, 2, 2), 1.0, False, (0.0,) * 6),   # 33: one line per wave, partial lanes
+    # 512 element columns: the drop-in RHS marches up the columns along z
+    (8, (32, 16, 3), 1.0, (True, False, False), (0.1, -0.2, 0.3, 0.4, -0.5, 0.6)),
 ]
 
 
@@ -163,6 +165,22 @@ def test_mixed_rhs_close_to_fp64():
     ws.rhs_full(du, dv, u, v)
     assert relerr(dv, dv_ref) <= 2e-5
     assert relerr(du, du_ref) == 0.0
+
+
+@pytest.mark.parametrize("M", [(32, 16, 3), (4, 3, 5)])
+def test_mixed_rhs3d_close_to_oracle(M):
+    """The drop-in RHS in FP32 (at N = 8 on 512 element columns the
+    column-marching kernel, else three wavefronts per element) against the
+    FP64 oracle: relative error <= 2e-5."""
+    _say(f"mixed rhs3d M={M}")
+    bv = (0.1, -0.2, 0.3, 0.4, -0.5, 0.6)
+    ws, o = _pair(8, M, bv=bv, precision="mixed")
+    rng = np.random.default_rng(77)
+    u = rng.standard_normal(o.n)
+    ud = rng.standard_normal(o.n)
+    out = np.zeros(o.n)
+    ws.rhs(out, u, ud, bv)
+    assert relerr(out, o.lap(u)) <= 2e-5
 
 
 def test_mixed_rk4_hnorm_error():

@@ -882,27 +882,42 @@ struct Lap3Shape {
     static constexpr int ES = RS * N * N;
 };
 
+// What the SIPG face terms need of the two neighbour lines: the face value and
+// the face derivative on each side.
+template <typename TC>
+struct Lap3Face {
+    TC uL, gnL, uR, gnR;
+    bool hasL, hasR;
+};
+
+template <int N, typename TC>
+__device__ __forceinline__ Lap3Face<TC> lap3_face(const TC* __restrict__ op, const Lap1Nbr<N, TC>& nb)
+{
+    Lap3Face<TC> f{TC(0), TC(0), TC(0), TC(0), nb.hasL, nb.hasR};
+    if (nb.hasL) {
+        TC y0;
+        apply_G_ends<N, TC>(op, nb.xl, y0, f.gnL);
+        f.uL = nb.xl[N - 1];
+    }
+    if (nb.hasR) {
+        TC yN;
+        apply_G_ends<N, TC>(op, nb.xr, f.gnR, yN);
+        f.uR = nb.xr[0];
+    }
+    return f;
+}
+
 // c = this axis's contribution on the line (la, lb) of element e, u the
-// line's own values (registers), nb its neighbours' lines.
+// line's own values (registers), f the neighbours' face values.
 template <int N, typename TC, int AX>
-__device__ __forceinline__ void lap3_axis(const Lap1Args<N, TC>& a, const TC* __restrict__ op, const TC (&u)[N],
-                                          int e, int la, int lb, const Lap1Nbr<N, TC>& nb, TC (&cv)[N])
+__device__ __forceinline__ void lap3_axis_f(const Lap1Args<N, TC>& a, const TC* __restrict__ op, const TC (&u)[N],
+                                            int e, int la, int lb, const Lap3Face<TC>& f, TC (&cv)[N])
 {
     constexpr int N3 = N * N * N;
     constexpr int GS = AX == 0 ? 1 : (AX == 1 ? N : N * N);
     const int gbase = AX == 0 ? N * (la + N * lb) : (AX == 1 ? la + N * N * lb : la + N * lb);
-    const bool hasL = nb.hasL, hasR = nb.hasR;
-    TC uL = TC(0), gnL = TC(0), uR = TC(0), gnR = TC(0);
-    if (hasL) {
-        TC y0;
-        apply_G_ends<N, TC>(op, nb.xl, y0, gnL);
-        uL = nb.xl[N - 1];
-    }
-    if (hasR) {
-        TC yN;
-        apply_G_ends<N, TC>(op, nb.xr, gnR, yN);
-        uR = nb.xr[0];
-    }
+    const bool hasL = f.hasL, hasR = f.hasR;
+    const TC uL = f.uL, gnL = f.gnL, uR = f.uR, gnR = f.gnR;
     TC v0 = TC(0), vN = TC(0);
     if (a.v) {
         const double* vl = a.v + int64_t(e) * N3 + gbase;
@@ -914,6 +929,14 @@ __device__ __forceinline__ void lap3_axis(const Lap1Args<N, TC>& a, const TC* __
 #pragma unroll
     for (int m = 0; m < N; ++m) cv[m] = TC(0);
     sipg_seg<N, TC>(u, g, cv, op, a.k[AX], uL, gnL, uR, gnR, hasL, hasR, v0, vN);
+}
+
+// The same with the neighbours' lines.
+template <int N, typename TC, int AX>
+__device__ __forceinline__ void lap3_axis(const Lap1Args<N, TC>& a, const TC* __restrict__ op, const TC (&u)[N],
+                                          int e, int la, int lb, const Lap1Nbr<N, TC>& nb, TC (&cv)[N])
+{
+    lap3_axis_f<N, TC, AX>(a, op, u, e, la, lb, lap3_face<N, TC>(op, nb), cv);
 }
 
 // The z or y wavefronts (AX = 2, 1): own line from LDS (staged by the x
@@ -994,6 +1017,148 @@ __global__ void __launch_bounds__(Lap3Shape<N>::NT) flat_lap3_kernel(Lap1Args<N,
     }
 }
 
+// apply_G_ends of a line held in LDS (node m at p[m * stride]), read a pair
+// of nodes at a time in apply_G_ends' order (the same rounding), so that only
+// two of its values are live at once; also its first and last value.
+template <int N, typename TC>
+__device__ __forceinline__ void ring_G_ends(const TC* __restrict__ op, const TC* p, int stride, TC& y0, TC& yN,
+                                            TC& first, TC& last)
+{
+    using L = OpLayout<N>;
+    constexpr int H = L::H, E = L::E;
+    TC b = TC(0), av = TC(0);
+    first = p[0];
+    last = p[(N - 1) * stride];
+#pragma unroll 1
+    for (int q = 0; q < H; ++q) {
+        const TC xa = p[q * stride], xb = p[(N - 1 - q) * stride];
+        b = fma(op[L::AO + q], xa - xb, b);
+        if (q < E) av = fma(op[L::AE + q], xa + xb, av);
+    }
+    if (E > H) av = fma(op[L::AE + E - 1], p[H * stride], av);
+    y0 = b + av;
+    yN = b - av;
+}
+
+// The three-wavefront Laplacian marching up element columns: a workgroup
+// takes the column (ex, ey) and steps through its elements along z.  The z
+// wavefront stages each element's x-lines in a ring of four LDS slots two
+// levels ahead, so it finds its own and both z neighbours' lines in LDS (no
+// z-neighbour loads), and the x and y wavefronts load the next level's
+// neighbour lines while they work on the current one.  The z wavefront also
+// writes the output along its z-lines (a k-plane of the element per store
+// across the wavefront).  Columns are taken XCD by XCD in contiguous ranges,
+// so x and (mostly) y neighbour columns run on the same XCD and share its L2.
+// Same arithmetic and summation order as flat_lap3_kernel, so the results are
+// bitwise equal.  N = 8 (one wavefront per axis), z not periodic.
+template <int N, typename TC>
+__global__ void __launch_bounds__(3 * 64) flat_lapz_kernel(Lap1Args<N, TC> a)
+{
+    static_assert(N * N == 64, "one wavefront per axis");
+    constexpr int N3 = N * N * N;
+    constexpr int RS = N + 1, ES = RS * N * N;
+    __shared__ TC R[4][ES];
+    __shared__ TC Cx[ES], Cy[ES];
+    const TC* __restrict__ op = a.op;      // wave-uniform reads (SGPRs)
+    const int tid = threadIdx.x;
+    // z, y, x by wavefront; readfirstlane makes the branches on it uniform,
+    // and each wavefront kind runs a loop of its own (with the same two
+    // barriers per level), so their registers do not add up
+    const int ax = 2 - __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ln = tid & 63, la = ln % N, lb = ln / N;
+    int wg = blockIdx.x;
+    const int full = (a.nwg / 8) * 8;
+    if (a.xcd && wg < full) wg = (wg % 8) * (a.nwg / 8) + wg / 8;
+    const int ex = wg % a.M0, ey = wg / a.M0;
+    const int M2 = a.M2, lstride = a.M0 * a.M1;
+    const int col = ex + a.M0 * ey;
+    if (ax == 2) {
+        // z: stages levels 0 and 1 (x-lines, one per work-item), then per
+        // level: loads the x-lines two levels up, c_z on its line (i, j) =
+        // (la, lb) from the ring (own and both z neighbours), and after the
+        // first barrier writes (c_z + c_y) + c_x along its line (a k-plane per
+        // store across the wavefront) and the new level into the ring
+        TC own[N];
+        lap1_load<N, 1, TC>(a.u + int64_t(col) * N3 + N * ln, own);
+#pragma unroll
+        for (int m = 0; m < N; ++m) R[0][RS * ln + m] = own[m];
+        if (M2 > 1) {
+            lap1_load<N, 1, TC>(a.u + int64_t(col + lstride) * N3 + N * ln, own);
+#pragma unroll
+            for (int m = 0; m < N; ++m) R[1][RS * ln + m] = own[m];
+        }
+        const int lbase = la + RS * lb;
+        __syncthreads();
+#pragma unroll 1
+        for (int ez = 0; ez < M2; ++ez) {
+            const int e = col + lstride * ez;
+            const bool more = ez + 1 < M2, stage = ez + 2 < M2;
+            TC ownp[N];
+            if (stage) lap1_load<N, 1, TC>(a.u + int64_t(e + 2 * lstride) * N3 + N * ln, ownp);
+            Lap3Face<TC> fz{TC(0), TC(0), TC(0), TC(0), ez > 0, more};
+            if (fz.hasL) {
+                TC y0, first;
+                ring_G_ends<N, TC>(op, R[(ez + 3) & 3] + lbase, RS * N, y0, fz.gnL, first, fz.uL);
+            }
+            if (fz.hasR) {
+                TC yN, last;
+                ring_G_ends<N, TC>(op, R[(ez + 1) & 3] + lbase, RS * N, fz.gnR, yN, fz.uR, last);
+            }
+            const TC* cur = R[ez & 3];
+            TC u[N], cv[N];
+#pragma unroll
+            for (int m = 0; m < N; ++m) u[m] = cur[lbase + m * RS * N];
+            lap3_axis_f<N, TC, 2>(a, op, u, e, la, lb, fz, cv);
+            __syncthreads();
+            double* dst = a.out + int64_t(e) * N3 + la + N * lb;
+#pragma unroll
+            for (int m = 0; m < N; ++m)
+                dst[m * N * N] = double((cv[m] + Cy[lbase + m * RS * N]) + Cx[lbase + m * RS * N]);
+            if (stage) {
+#pragma unroll
+                for (int m = 0; m < N; ++m) R[(ez + 2) & 3][RS * ln + m] = ownp[m];
+            }
+            __syncthreads();
+        }
+    } else {
+        // x (line (j, k)) and y (line (i, k)): per level, load the next level's
+        // neighbour lines while computing this level's contribution into C
+        const bool isx = ax == 0;
+        const int lbase = isx ? RS * ln : la + RS * N * lb;
+        const int ls = isx ? 1 : RS;
+        TC* C = isx ? Cx : Cy;
+        Lap3Face<TC> fc;
+        {
+            Lap1Nbr<N, TC> nb;
+            if (isx) lap1_load_nbrs<N, TC, 0>(a, col, ex, la, lb, nb);
+            else lap1_load_nbrs<N, TC, 1>(a, col, ey, la, lb, nb);
+            fc = lap3_face<N, TC>(op, nb);
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int ez = 0; ez < M2; ++ez) {
+            const int e = col + lstride * ez;
+            const bool more = ez + 1 < M2;
+            Lap1Nbr<N, TC> nbn;
+            if (more) {
+                if (isx) lap1_load_nbrs<N, TC, 0>(a, e + lstride, ex, la, lb, nbn);
+                else lap1_load_nbrs<N, TC, 1>(a, e + lstride, ey, la, lb, nbn);
+            }
+            const TC* cur = R[ez & 3];
+            TC u[N], cv[N];
+#pragma unroll
+            for (int m = 0; m < N; ++m) u[m] = cur[lbase + m * ls];
+            if (isx) lap3_axis_f<N, TC, 0>(a, op, u, e, la, lb, fc, cv);
+            else lap3_axis_f<N, TC, 1>(a, op, u, e, la, lb, fc, cv);
+#pragma unroll
+            for (int m = 0; m < N; ++m) C[lbase + m * ls] = cv[m];
+            __syncthreads();
+            __syncthreads();
+            if (more) fc = lap3_face<N, TC>(op, nbn);
+        }
+    }
+}
+
 template <int N, typename TC>
 static hipError_t launch_lap1_t(const Lap1Desc& d, hipStream_t s)
 {
@@ -1035,6 +1200,22 @@ static hipError_t launch_lap1_t(const Lap1Desc& d, hipStream_t s)
         const char* ev = std::getenv("WT2_LAP1_FORM");
         return ev ? std::atoi(ev) : 0;
     }();
+    // FP32 at N = 8 with z not periodic and at least 512 element columns: the
+    // column-marching form (32^3 x 8^3: 72 against 78-81 us for three
+    // wavefronts per element).  In FP64 it held 139 VGPRs, and with one
+    // workgroup per column only 12 wavefronts per CU: 111 against 89 us, so
+    // FP64 keeps the three-wavefront form.  WT2_LAP1_FORM = 5 forces the
+    // marching form wherever it applies (z not periodic).
+    if constexpr (N == 8) {
+        const bool zok = !d.periodic[2];
+        const bool pick = std::is_same<TC, float>::value && int64_t(d.M[0]) * d.M[1] >= 512;
+        if (zok && ((form == 0 && pick) || form == 5)) {
+            a.nwg = d.M[0] * d.M[1];
+            (void)hipGetLastError();
+            hipLaunchKernelGGL((flat_lapz_kernel<N, TC>), dim3(a.nwg), dim3(3 * 64), 0, s, a);
+            return hipGetLastError();
+        }
+    }
     if ((form == 0 && N == 8) || form == 3) {
         a.nwg = int(Ne);
         (void)hipGetLastError();
