@@ -951,7 +951,10 @@ def main():
                         "max_rel_err_vs_fp64": maxerr, "dtype": "f32 RHS / f64 state",
                         "pass_ms": rmx["pass_ms"], "ms_per_step_repeats": repeat_summary(rmx["rep_ms"]),
                         "tiles": rmx["ws"].tiles(),
-                        "roofline_achieved_GBs": FUSED_MIN_BYTES["mixed"] * npts * K / (rmx["ev_ms"] * 1e-3) / 1e9}
+                        "roofline_achieved_GBs": FUSED_MIN_BYTES["mixed"] * npts * K / (rmx["ev_ms"] * 1e-3) / 1e9,
+                        # the drop-in RHS of the mixed context (FP32 arithmetic, FP64 in and out)
+                        "rhs_ms": rmx["rhs_ms"],
+                        "rhs_frac_24B": 24 * npts / (rmx["rhs_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS}
         del rmx
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(u0, v0, r64["dt"])
