@@ -464,6 +464,8 @@ struct CArgs {
     const TC* coef;                 // 13 x npts SoA
     TC* fbuf;                       // 6 N^2 per element
     const TC* fcoef;                // 5 x 6 N^2 per element: face-node coefficients
+    int fcnode;                     // element kernels: the face node's own K row and c^a from
+                                    // the node coefficients (cache hits), not the face-node table
     const TC* op;                   // even/odd operator table (OpLayout<N>)
     void* out;                      // output (TO): wgt (...) or unweighted
     int64_t npts;
@@ -579,6 +581,19 @@ __device__ __forceinline__ void curved_face_node(const CArgs<N, TC>& a, const in
     const int n = fidx<N>(ax, r, p, qq);
     const int64_t gn = eb + n;
     const TC* fc = a.fcoef + e * 5 * F2 + q;   // K^{a0}, K^{a1}, K^{a2}, kappa, c^a
+    // this side's K row and c^a: equal copies sit in the face-node table (read
+    // in face-loop order) and in the node coefficients (which this workgroup's
+    // node work-items have just read, so these gathers are cache hits)
+    TC kr[3], ca;
+    if (a.fcnode) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) kr[b] = a.coef[kidx(ax, b) * np + gn];
+        ca = a.coef[(6 + ax) * np + gn];
+    } else {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) kr[b] = fc[b * F2];
+        ca = fc[4 * F2];
+    }
     // neighbour element en (-1: outer face), its face fn, the neighbour
     // face's node (pn, qn) that meets this face's (p, qq), and whether
     // this outer face is a Sommerfeld face; bvf: its Dirichlet value
@@ -618,7 +633,7 @@ __device__ __forceinline__ void curved_face_node(const CArgs<N, TC>& a, const in
         // wF/Hphys = |n_u| / (w_r |J|).  B holds u_t (zero without it).
 #pragma unroll
         for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = TC(0);
-        const TC nu = sqrt(fc[ax * F2] / a.coef[9 * np + gn]);
+        const TC nu = sqrt(kr[ax] / a.coef[9 * np + gn]);
         const TC drag = fma(nrm, a.fbuf[e * F2 + q], nu * fma(a.k.iR, A[n], B[n]));
         Pt[q] = -drag * winv[r];
         return;
@@ -643,9 +658,9 @@ __device__ __forceinline__ void curved_face_node(const CArgs<N, TC>& a, const in
     }
     const TC an = alpha * nrm;
 #pragma unroll
-    for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = an * fc[b * F2] * du;
+    for (int b = 0; b < 3; ++b) Sf[(b * 6 + f) * N2 + pq] = an * kr[b] * du;
     TC pt = TC(-0.5) * nrm * dF - a.k.tau * kap * du;
-    if (adv) pt -= nrm * fc[4 * F2] * dv;
+    if (adv) pt -= nrm * ca * dv;
     Pt[q] = pt * winv[r];
 }
 
@@ -1103,6 +1118,13 @@ static void make_cargs(const CurvedData& cd, const CurvedCall& cc, CArgs<N, TC>&
     a.fbuf = reinterpret_cast<TC*>(cd.fbuf);
     a.fcoef = reinterpret_cast<const TC*>(std::is_same<TC, float>::value ? static_cast<const void*>(cd.fc32)
                                                                            : static_cast<const void*>(cd.fc64));
+    // WT2_CURVED_FCNODE = 0: the element kernels read the face node's own K
+    // row and c^a from the face-node table (the earlier form) instead
+    static const int fcnode = [] {
+        const char* ev = std::getenv("WT2_CURVED_FCNODE");
+        return ev ? std::atoi(ev) : 1;
+    }();
+    a.fcnode = fcnode;
     a.op = static_cast<const TC*>(std::is_same<TC, float>::value ? cd.op32 : cd.op64);
     a.out = cc.out ? static_cast<void*>(cc.out) : cc.Pv;
     a.npts = cd.npts;
