@@ -404,17 +404,22 @@ struct StageItems {
 // the fold: e_s's contribution of stage s+1, computed on the new stage input
 // in registers.  f_s = e_(s-1), f_1 = e_4 (the previous step's stage 4, or
 // the prologue at the start of a run).  8 passes per step instead of 12.
-// The default m = x, e = (y, z, y, z): at 32^3 x 8^3 the x pass is the
-// cheapest middle pass (106 us against 125 for z, FP64) and the folding
-// passes cost about the same along z as along x, so x never carries the
-// fold: 1.306 -> 1.276 ms per FP64 step, 0.900 -> 0.833 ms mixed.
-// WT2_FOLD_ORDER = "zyx" selects m = z, e = (y, x, y, x).
+// Two orders: "xyz", m = x, e = (y, z, y, z) (z never the middle pass), and
+// "zyx", m = z, e = (y, x, y, x) (x never carries the fold).  Measured at
+// 32^3 x 8^3 with the current kernels (two runs each, same box): FP64
+// 1.242-1.257 ms per step with zyx against 1.288-1.290 with xyz; mixed
+// 0.783 ms with xyz against 0.803 with zyx (its z passes move FP32 quarter
+// lines).  So FP64 contexts take zyx and mixed ones xyz; WT2_FOLD_ORDER =
+// "xyz" / "zyx" forces one order for both.
 static void fold_order(wt2_ctx* c)
 {
-    static const int order = [] {
+    static const int forced = [] {
         const char* ev = std::getenv("WT2_FOLD_ORDER");
-        return ev && std::strcmp(ev, "zyx") == 0 ? 1 : 0;
+        if (ev && std::strcmp(ev, "zyx") == 0) return 1;
+        if (ev && std::strcmp(ev, "xyz") == 0) return 0;
+        return -1;
     }();
+    const int order = forced >= 0 ? forced : (is_mixed(c) ? 0 : 1);
     static const int tab[2][5] = {{0, 1, 2, 1, 2}, {2, 1, 0, 1, 0}};
     c->fmid = tab[order][0];
     for (int s = 0; s < 4; ++s) c->fe[s] = tab[order][1 + s];
