@@ -417,10 +417,13 @@ static void fold_order(wt2_ctx* c)
         const char* ev = std::getenv("WT2_FOLD_ORDER");
         if (ev && std::strcmp(ev, "zyx") == 0) return 1;
         if (ev && std::strcmp(ev, "xyz") == 0) return 0;
+        if (ev && std::strcmp(ev, "yxz") == 0) return 2;
+        if (ev && std::strcmp(ev, "yzx") == 0) return 3;
         return -1;
     }();
     const int order = forced >= 0 ? forced : (is_mixed(c) ? 0 : 1);
-    static const int tab[2][5] = {{0, 1, 2, 1, 2}, {2, 1, 0, 1, 0}};
+    // m, then e_1..e_4 (e alternates between the two other axes)
+    static const int tab[4][5] = {{0, 1, 2, 1, 2}, {2, 1, 0, 1, 0}, {1, 0, 2, 0, 2}, {1, 2, 0, 2, 0}};
     c->fmid = tab[order][0];
     for (int s = 0; s < 4; ++s) c->fe[s] = tab[order][1 + s];
 }
