@@ -404,13 +404,12 @@ struct StageItems {
 // the fold: e_s's contribution of stage s+1, computed on the new stage input
 // in registers.  f_s = e_(s-1), f_1 = e_4 (the previous step's stage 4, or
 // the prologue at the start of a run).  8 passes per step instead of 12.
-// Two orders: "xyz", m = x, e = (y, z, y, z) (z never the middle pass), and
-// "zyx", m = z, e = (y, x, y, x) (x never carries the fold).  Measured at
-// 32^3 x 8^3 with the current kernels (two runs each, same box): FP64
-// 1.242-1.257 ms per step with zyx against 1.288-1.290 with xyz; mixed
-// 0.783 ms with xyz against 0.803 with zyx (its z passes move FP32 quarter
-// lines).  So FP64 contexts take zyx and mixed ones xyz; WT2_FOLD_ORDER =
-// "xyz" / "zyx" forces one order for both.
+// Orders named m e_1 e_2 (e alternates): measured at 32^3 x 8^3 with the
+// current kernels (300 steps, same box): FP64 zyx 1.219, zxy 1.227, xzy
+// 1.266, xyz 1.268-1.284, yzx 1.303, yxz 1.318 ms per step; mixed xzy 0.769,
+// xyz 0.784, zxy 0.789, yxz 0.797, yzx 0.802, zyx 0.803.  So FP64 contexts
+// take zyx (m = z, e = y, x, y, x) and mixed ones xzy (m = x, e = z, y, z,
+// y); WT2_FOLD_ORDER = one of the six names forces that order for both.
 static void fold_order(wt2_ctx* c)
 {
     static const int forced = [] {
@@ -419,11 +418,14 @@ static void fold_order(wt2_ctx* c)
         if (ev && std::strcmp(ev, "xyz") == 0) return 0;
         if (ev && std::strcmp(ev, "yxz") == 0) return 2;
         if (ev && std::strcmp(ev, "yzx") == 0) return 3;
+        if (ev && std::strcmp(ev, "zxy") == 0) return 4;
+        if (ev && std::strcmp(ev, "xzy") == 0) return 5;
         return -1;
     }();
-    const int order = forced >= 0 ? forced : (is_mixed(c) ? 0 : 1);
+    const int order = forced >= 0 ? forced : (is_mixed(c) ? 5 : 1);
     // m, then e_1..e_4 (e alternates between the two other axes)
-    static const int tab[4][5] = {{0, 1, 2, 1, 2}, {2, 1, 0, 1, 0}, {1, 0, 2, 0, 2}, {1, 2, 0, 2, 0}};
+    static const int tab[6][5] = {{0, 1, 2, 1, 2}, {2, 1, 0, 1, 0}, {1, 0, 2, 0, 2},
+                                  {1, 2, 0, 2, 0}, {2, 0, 1, 0, 1}, {0, 2, 1, 2, 1}};
     c->fmid = tab[order][0];
     for (int s = 0; s < 4; ++s) c->fe[s] = tab[order][1 + s];
 }
