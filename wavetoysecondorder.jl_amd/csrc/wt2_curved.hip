@@ -466,6 +466,7 @@ struct CArgs {
     const TC* fcoef;                // 5 x 6 N^2 per element: face-node coefficients
     int fcnode;                     // element kernels: the face node's own K row and c^a from
                                     // the node coefficients (cache hits), not the face-node table
+    int xcd;                        // element kernel (one context): contiguous element ranges per XCD
     const TC* op;                   // even/odd operator table (OpLayout<N>)
     void* out;                      // output (TO): wgt (...) or unweighted
     int64_t npts;
@@ -1009,7 +1010,14 @@ template <int N, typename TC, typename TS, typename TO, bool GEN>
 __global__ void __launch_bounds__(VolNodeShape<N>::NT) __attribute__((amdgpu_waves_per_eu(VolNodeWaves<N>::value)))
 curved_volnode_kernel(CArgs<N, TC> a, int64_t Ne)
 {
-    const int64_t e0 = int64_t(blockIdx.x) * VolNodeShape<N>::EW;
+    // workgroups are dispatched round-robin over the 8 XCDs: with xcd set, XCD x
+    // takes a contiguous range of element blocks, so the face neighbours whose
+    // u, u_t and face fluxes an element gathers (e +- 1 along x: node stride N)
+    // are read by the same XCD and meet in its L2
+    int64_t wg = blockIdx.x;
+    const int64_t nwg = gridDim.x, full = (nwg / 8) * 8;
+    if (a.xcd && wg < full) wg = (wg % 8) * (nwg / 8) + wg / 8;
+    const int64_t e0 = wg * VolNodeShape<N>::EW;
     volume_node_body<N, TC, TS, TO, GEN>(a, e0, int(min<int64_t>(VolNodeShape<N>::EW, Ne - e0)));
 }
 
@@ -1125,6 +1133,12 @@ static void make_cargs(const CurvedData& cd, const CurvedCall& cc, CArgs<N, TC>&
         return ev ? std::atoi(ev) : 1;
     }();
     a.fcnode = fcnode;
+    // WT2_CURVED_XCD = 0: the element kernel takes element blocks in dispatch order
+    static const int xcd = [] {
+        const char* ev = std::getenv("WT2_CURVED_XCD");
+        return ev ? std::atoi(ev) : 1;
+    }();
+    a.xcd = xcd;
     a.op = static_cast<const TC*>(std::is_same<TC, float>::value ? cd.op32 : cd.op64);
     a.out = cc.out ? static_cast<void*>(cc.out) : cc.Pv;
     a.npts = cd.npts;
