@@ -507,13 +507,28 @@ __device__ __forceinline__ void grad_lds(const TC* f, const CConst<N, TC>& k, in
 template <int N, typename TC, typename TS>
 __device__ __forceinline__ void faceflux_body(const CArgs<N, TC>& a, const int64_t e)
 {
+    // launched with FT work-items; work-item t takes face nodes t, t + FT, ...
+    constexpr int FT = 256, F2 = 6 * N * N, NQ = (F2 + FT - 1) / FT;
     __shared__ TC su[N * N * N];
     __shared__ CConst<N, TC> kl;     // G indexed per lane: from LDS, not kernel arguments
     const TS* u = static_cast<const TS*>(a.u);
-    for (int q = threadIdx.x; q < N * N * N; q += blockDim.x) su[q] = TC(u[e * N * N * N + q]);
-    for (int q = threadIdx.x; q < N * N; q += blockDim.x) kl.G[q] = a.k.G[q];
+    for (int q = threadIdx.x; q < N * N * N; q += FT) su[q] = TC(u[e * N * N * N + q]);
+    for (int q = threadIdx.x; q < N * N; q += FT) kl.G[q] = a.k.G[q];
+    // this work-item's K^{a0}, K^{a1}, K^{a2} are loaded before the barrier, so
+    // their latency overlaps the element's u
+    const TC* fc = a.fcoef + e * 5 * F2;
+    TC K0[NQ], K1[NQ], K2[NQ];
+#pragma unroll
+    for (int t = 0; t < NQ; ++t) {
+        const int q = int(threadIdx.x) + FT * t;
+        K0[t] = K1[t] = K2[t] = TC(0);
+        if (q < F2) { K0[t] = fc[q]; K1[t] = fc[F2 + q]; K2[t] = fc[2 * F2 + q]; }
+    }
     __syncthreads();
-    for (int q = threadIdx.x; q < 6 * N * N; q += blockDim.x) {
+#pragma unroll
+    for (int t = 0; t < NQ; ++t) {
+        const int q = int(threadIdx.x) + FT * t;
+        if (q >= F2) break;
         const int f = q / (N * N), pq = q - f * N * N;
         const int p = pq % N, qq = pq / N;
         const int ax = f >> 1, r = (f & 1) ? N - 1 : 0;
@@ -521,11 +536,10 @@ __device__ __forceinline__ void faceflux_body(const CArgs<N, TC>& a, const int64
         const int i = n % N, j = (n / N) % N, l = n / (N * N);
         TC g[3];
         grad_lds<N, TC>(su, kl, i, j, l, g);
-        const TC* fc = a.fcoef + e * 5 * 6 * N * N + q;
-        TC F = fc[0] * g[0];
-        F = fma(fc[6 * N * N], g[1], F);
-        F = fma(fc[2 * 6 * N * N], g[2], F);
-        a.fbuf[e * 6 * N * N + q] = F;
+        TC F = K0[t] * g[0];
+        F = fma(K1[t], g[1], F);
+        F = fma(K2[t], g[2], F);
+        a.fbuf[e * F2 + q] = F;
     }
 }
 
